@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""TT rounding benchmark on one MI355X: d=32, n=8, r=256 TT (sum of two rank-128 TTs
+with graded core spectra), eps=1e-6.
+
+One step = one ``round(X, 1e-6)`` of that tensor with its cores already resident
+in HBM, in the mixed form (FP32 MFMA contractions/Grams, FP64 r x r Cholesky
+and Jacobi SVD).  The FP64 plain form is timed on the same input for the
+speedup, both results are compared with the FP64 CPU oracle (the
+restatement of @tt_tensor/round.m), and the oracle itself is timed on the host
+cores as the CPU baseline.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-steps P] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "tt-toolbox_amd"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+PEAK = {  # MI355X published dense peaks
+    "fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12,
+}
+CLASS_NAMES = ["gemm", "jacobi", "cholesky", "trtri", "householder"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-rounds", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the TT rounding path has no multi-GPU form in this "
+              "build (see DESIGN.md for how it would shard); run with --gpus 1", flush=True)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import ttgpu
+    from ttgpu import workloads as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    log("generating input (d=32, n=8, r=256)")
+    c = W.mixed3()
+    n, r, core = c["x"]
+    eps = c["eps"]
+    dev = torch.device("cuda:0")
+    dcore = torch.from_numpy(core).to(dev)
+    ctx = ttgpu.Context(0)
+    stream = torch.cuda.ExternalStream(ctx.stream, device=dev)
+    cap = ttgpu.capacity(n, r)
+    out_mixed = torch.empty(cap, dtype=torch.float64, device=dev)
+    out_plain = torch.empty(cap, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def one(prec, out):
+        return ttgpu.round_dev(ctx, n, r, dcore, eps, precision=prec, out=out)
+
+    # warm-up (also builds the workspace)
+    for i in range(max(args.warmup, 1)):
+        one("fp32", out_mixed)
+        log(f"warmup {i + 1} done")
+    # find the dominant kernel class of the mixed form (one profiled round per class)
+    per_class = {}
+    for cls in CLASS_NAMES:
+        ctx.profile_begin(cls)
+        one("fp32", out_mixed)
+        per_class[cls] = ctx.profile_end()
+    dominant = max(per_class, key=lambda k: per_class[k]["ms"])
+    log("per-class ms per round:", {k: round(v["ms"], 3) for k, v in per_class.items()}, "dominant:", dominant)
+
+    # timed mixed rounds, dominant class bracketed by HIP events on the ctx stream
+    torch.cuda.synchronize()
+    ctx.profile_begin(dominant)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    info = {}
+    for i in range(args.steps):
+        r_mixed, y_mixed = ttgpu.round_dev(ctx, n, r, dcore, eps, precision="fp32", out=out_mixed, info=info)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = ctx.profile_end()
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    log(f"mixed: {ms_step:.3f} ms/round (host clock {1e3 * (t1 - t0) / args.steps:.3f}), info {info}")
+
+    # plain FP64 form on the same input
+    one("fp64", out_plain)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(args.plain_steps):
+        r_plain, y_plain = one("fp64", out_plain)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_plain = e0.elapsed_time(e1) / args.plain_steps
+    log(f"plain: {ms_plain:.3f} ms/round")
+
+    # algorithmic work per round at the actual ranks (QR + apply + SVD + apply)
+    d = len(n)
+    rq = list(map(int, r))
+    for i in range(d - 1):
+        rq[i + 1] = min(rq[i] * int(n[i]), rq[i + 1])
+    ro = list(map(int, r_mixed))
+    flops = 0.0
+    byt = 0.0
+    for i in range(d - 1):
+        m_, k_ = rq[i] * int(n[i]), rq[i + 1]
+        if m_ >= k_:
+            flops += 4 * m_ * k_ * k_ - 4 * k_ ** 3 / 3
+        flops += 2 * k_ * r[i + 1] * n[i + 1] * r[i + 2] if i + 2 <= d else 0
+    for i in range(d - 1, 0, -1):
+        M_, N_ = max(rq[i], int(n[i]) * ro[i + 1]), min(rq[i], int(n[i]) * ro[i + 1])
+        flops += 6 * M_ * N_ ** 2 + 20 * N_ ** 3
+        flops += 2 * rq[i - 1] * int(n[i - 1]) * rq[i] * ro[i]
+    size_in = sum(int(r[i]) * int(n[i]) * int(r[i + 1]) for i in range(d))
+    size_q = sum(rq[i] * int(n[i]) * rq[i + 1] for i in range(d))
+    size_out = sum(ro[i] * int(n[i]) * ro[i + 1] for i in range(d))
+    byt = 4.0 * (size_in + 2 * size_q + size_out)  # FP32 storage in the mixed form
+
+    result = {
+        "metric": "TT rounds/sec d=32 n=8 r=256: mixed vs FP64 on GPU at rel err <=1e-6",
+        "value": 1000.0 / ms_step,
+        "unit": "rounds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic (seeded: sum of two rank-128 TTs, core spectra 2^(-j/16))",
+        "config": {"workload": "round d=32 n=8 r=256 (ranks [1,8,64,256x27,64,8,1] after QR caps) eps=1e-6",
+                   "d": 32, "n": 8, "r": 256, "eps": eps, "form": "mixed"},
+        "plain_fp64": {"rounds_per_s": 1000.0 / ms_plain, "ms_per_round": ms_plain,
+                       "speedup_mixed_vs_plain": ms_plain / ms_step},
+        "algorithmic_gflop_per_round": flops / 1e9,
+        "achieved_round_tflops": flops / (ms_step * 1e-3) / 1e12,
+        "ranks_out": ro,
+        "diag": {"qr_fallbacks": info.get("qr_fallbacks"), "jacobi_sweeps": info.get("jacobi_sweeps"),
+                 "jacobi_maxsweeps": info.get("jacobi_maxsweeps"),
+                 "per_class_ms_per_round": {k: v["ms"] for k, v in per_class.items()}},
+    }
+    # roofline of the dominant kernel class
+    avg_ms = prof["ms"] / max(prof["launches"], 1)
+    if dominant == "gemm":
+        peak, unit, bound = PEAK["fp32"], "TFLOP/s", "mfma"
+    else:
+        peak, unit, bound = PEAK["fp64"], "TFLOP/s", "mfma"
+    achieved = (prof["flops"] / max(prof["launches"], 1)) / (avg_ms * 1e-3)
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get(dominant)
+        except Exception:
+            traffic = None
+    result["roofline"] = {"bound": bound, "kernel": dominant, "achieved": achieved / 1e12, "peak": peak / 1e12,
+                          "unit": unit, "frac": achieved / peak, "traffic": traffic,
+                          "launches_per_round": prof["launches"] / args.steps, "avg_launch_ms": avg_ms}
+
+    # parity against the FP64 oracle (CPU) + CPU baseline timing
+    if not args.no_cpu:
+        from oracle import tt_oracle as O
+        try:
+            from threadpoolctl import threadpool_info
+            cores = max([t.get("num_threads", 1) for t in threadpool_info()] or [1])
+        except Exception:
+            cores = os.cpu_count() or 1
+        X = O.TT(n, r, core)
+        times = []
+        ref = None
+        for i in range(args.cpu_rounds):
+            t0 = time.perf_counter()
+            ref = O.tt_round(X, eps)
+            times.append(time.perf_counter() - t0)
+            log(f"cpu oracle round {i + 1}: {times[-1]:.2f} s")
+        med = sorted(times)[len(times) // 2]
+        result["cpu_baseline"] = {"value": 1.0 / med, "unit": "rounds/s", "cores": int(cores), "kind": "port",
+                                  "sample": f"{args.cpu_rounds} full rounds of the same d=32 n=8 r=256 tensor "
+                                            f"(numpy/LAPACK restatement of round.m), median"}
+        ym = O.TT(n, r_mixed, y_mixed.cpu().numpy())
+        yp = O.TT(n, r_plain, y_plain.cpu().numpy())
+        result["rel_err_mixed_vs_oracle"] = O.tt_rel_diff(ym, ref)
+        result["rel_err_plain_vs_oracle"] = O.tt_rel_diff(yp, ref)
+        result["plain_ranks_match_oracle"] = list(map(int, r_plain)) == ref.r.tolist()
+        result["mixed_ranks_match_oracle"] = list(map(int, r_mixed)) == ref.r.tolist()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

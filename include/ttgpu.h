@@ -1,0 +1,163 @@
+/*
+ * ttgpu — C ABI of the MI355X (gfx950) TT rounding / TT-matvec library.
+ *
+ * Layout contract (identical to TT-Toolbox's tt_tensor, @tt_tensor/tt_tensor.m:33-41):
+ *   a TT tensor is d, mode sizes n[d], ranks r[d+1] and ONE flat column-major
+ *   core vector; core k is r[k] x n[k] x r[k+1] (r fastest) and starts at
+ *   ps[k] = sum_{j<k} r[j]*n[j]*r[j+1].  A tt_matrix (@tt_matrix/tt_matrix.m:33-39)
+ *   is the same with core k shaped r[k] x n[k] x m[k] x r[k+1].
+ *
+ * All sizes are int64.  Inputs are borrowed read-only, outputs are written
+ * into caller memory; nothing allocated by the library crosses the ABI.
+ * Functions without the _dev suffix take HOST pointers for core data; the
+ * _dev variants take DEVICE pointers (the ctx's device) and never copy the
+ * cores through the host.  Size/rank arrays are always host memory.
+ *
+ * Threading: one context per host thread; calls on one context are
+ * serialised on its stream.  Every function returns a ttg_status; on error
+ * ttg_last_error(ctx) holds a message.
+ */
+#ifndef TTGPU_H
+#define TTGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  TTG_OK = 0,
+  TTG_BAD_ARG = 1,        /* inconsistent sizes/ranks (cf. @tt_tensor/plus.m:50-59 checks) */
+  TTG_NEED_SPACE = 2,     /* output capacity too small; *len_out holds the required length */
+  TTG_NONFINITE = 3,      /* NaN/Inf met in the input or produced by a factorisation */
+  TTG_NO_CONVERGENCE = 4, /* Jacobi sweep cap reached (analogue of the svd->svds warning, round.m:64-74) */
+  TTG_HIP_ERROR = 5,      /* a HIP runtime call failed */
+  TTG_NO_DEVICE = 6       /* no usable gfx950 device */
+} ttg_status;
+
+typedef enum {
+  TTG_PREC_FP64 = 0,      /* plain form: FP64 throughout */
+  TTG_PREC_FP32 = 1       /* mixed form: FP32 MFMA contractions/Grams, FP64 r x r Cholesky and Jacobi SVD */
+} ttg_precision;
+
+typedef struct ttg_ctx ttg_ctx;
+
+/* Per-call diagnostics (optional out-parameter, may be NULL). */
+typedef struct {
+  int64_t qr_fallbacks;     /* bonds where CholeskyQR was rejected and Householder QR ran */
+  int64_t jacobi_sweeps;    /* total one-sided Jacobi sweeps over all bonds */
+  int64_t jacobi_maxsweeps; /* largest sweep count of one SVD */
+  double  seconds;          /* wall time of the call on the host clock */
+} ttg_info;
+
+/* Context: owns a HIP stream (or borrows `stream` when non-NULL), device workspace.
+ * No reference counterpart: MATLAB holds this state implicitly (single interpreter thread). */
+ttg_status ttg_ctx_create(int device, void *stream, ttg_ctx **out);
+ttg_status ttg_ctx_destroy(ttg_ctx *ctx);
+const char *ttg_last_error(const ttg_ctx *ctx);
+void *ttg_ctx_stream(const ttg_ctx *ctx);
+const char *ttg_version(void);
+
+/* Required output capacity (doubles) for round: the QR-capped input size.
+ * Replaces the storage bookkeeping of @tt_tensor/round.m:48,54-57. */
+ttg_status ttg_round_capacity(int64_t d, const int64_t *n, const int64_t *r, int64_t *cap);
+
+/* Y = round(X, eps, rmax)  —  replaces @tt_tensor/round.m:1-111 (with core/my_chop2.m:18-34).
+ * rmax: NULL (no cap, round.m:22) or d+1 entries (rmax[i] caps r[i], round.m:78).
+ * Output: r_out[d+1] and core_out (capacity `cap` doubles); *len_out = used length.
+ * prec selects the plain FP64 form or the mixed FP32/FP64 form. */
+ttg_status ttg_round(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
+                     const double *core, double eps, const int64_t *rmax, int prec,
+                     int64_t *r_out, double *core_out, int64_t cap, int64_t *len_out,
+                     ttg_info *info);
+ttg_status ttg_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
+                         const double *dcore, double eps, const int64_t *rmax, int prec,
+                         int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
+                         ttg_info *info);
+
+/* round on a tt_matrix — replaces @tt_matrix/round.m:23-27 (rounds tt with modes n.*m). */
+ttg_status ttg_round_matrix(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
+                            const int64_t *r, const double *core, double eps,
+                            const int64_t *rmax, int prec, int64_t *r_out,
+                            double *core_out, int64_t cap, int64_t *len_out, ttg_info *info);
+
+/* y = A*x (tt_matrix times tt_tensor, no rounding) — replaces @tt_matrix/mtimes.m:38-65.
+ * Output ranks ry = rA.*rx, core layout (rA rx) x n x (rA' rx') with rank index a + rA*alpha.
+ * cap >= sum_k ry[k]*n[k]*ry[k+1]. */
+ttg_status ttg_mtimes(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
+                      const int64_t *ra, const double *acore, const int64_t *rx,
+                      const double *xcore, int64_t *ry, double *ycore, int64_t cap,
+                      int64_t *len_out);
+ttg_status ttg_mtimes_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
+                          const int64_t *ra, const double *dacore, const int64_t *rx,
+                          const double *dxcore, int64_t *ry, double *dycore, int64_t cap,
+                          int64_t *len_out);
+
+/* y = round(A*x, eps, rmax) fused: product cores are generated on the device
+ * and consumed by the rounding sweep without being stored as a TT —
+ * replaces the composition @tt_matrix/mtimes.m:38-65 -> @tt_tensor/round.m. */
+ttg_status ttg_mtimes_round(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
+                            const int64_t *ra, const double *acore, const int64_t *rx,
+                            const double *xcore, double eps, const int64_t *rmax, int prec,
+                            int64_t *r_out, double *core_out, int64_t cap, int64_t *len_out,
+                            ttg_info *info);
+ttg_status ttg_mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
+                                const int64_t *ra, const double *dacore, const int64_t *rx,
+                                const double *dxcore, double eps, const int64_t *rmax, int prec,
+                                int64_t *r_out, double *dcore_out, int64_t cap,
+                                int64_t *len_out, ttg_info *info);
+
+/* Batched rounding of `count` independent TT tensors (uneven d, n, r).
+ * Tensor t: d[t], n/r at offsets nofs[t]/rofs[t] of the packed n/r arrays,
+ * cores at device offset cofs[t] of dcore.  Outputs: r_out packed like r,
+ * cores at device offset cofs_out[t] of dcore_out (capacity from
+ * ttg_round_capacity per tensor).  Replaces a MATLAB loop of round() calls. */
+ttg_status ttg_round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d,
+                                 const int64_t *nofs, const int64_t *n, const int64_t *rofs,
+                                 const int64_t *r, const int64_t *cofs, const double *dcore,
+                                 double eps, int prec, int64_t *r_out,
+                                 const int64_t *cofs_out, double *dcore_out, ttg_info *info);
+
+/* Adjoint of the sampled-entry completion loss 0.5*||P_Omega(X) - y||^2.
+ * No reference counterpart (TT-Toolbox has no gradients); entry evaluation
+ * follows @tt_tensor/subsref.m:116-122 and the interface products of
+ * tt_regression/reg_als.m:22-39,66-92.  idx: S x d int32 (row-major, 0-based).
+ * grad: laid out like core.  Device pointers; f32 and f64 forms. */
+ttg_status ttg_adjoint_completion_f64(ttg_ctx *ctx, int64_t d, const int64_t *n,
+                                      const int64_t *r, const double *dcore, int64_t S,
+                                      const int32_t *didx, const double *dvals,
+                                      double *loss, double *dgrad);
+ttg_status ttg_adjoint_completion_f32(ttg_ctx *ctx, int64_t d, const int64_t *n,
+                                      const int64_t *r, const float *dcore, int64_t S,
+                                      const int32_t *didx, const float *dvals,
+                                      double *loss, float *dgrad);
+
+/* Adjoint of the bilinear form <y, A x>: value and gradients w.r.t. the cores
+ * of x and y (either gradient pointer may be NULL).  Interface chain of
+ * core/tt_mvdot.m:16-57.  Device pointers. */
+ttg_status ttg_adjoint_bilinear_f64(ttg_ctx *ctx, int64_t d, const int64_t *n,
+                                    const int64_t *m, const int64_t *ry, const double *dycore,
+                                    const int64_t *ra, const double *dacore,
+                                    const int64_t *rx, const double *dxcore, double *value,
+                                    double *dgx, double *dgy);
+ttg_status ttg_adjoint_bilinear_f32(ttg_ctx *ctx, int64_t d, const int64_t *n,
+                                    const int64_t *m, const int64_t *ry, const float *dycore,
+                                    const int64_t *ra, const float *dacore,
+                                    const int64_t *rx, const float *dxcore, double *value,
+                                    float *dgx, float *dgy);
+
+/* Kernel-class timing (diagnostic; no reference counterpart).  Between
+ * begin and end, every launch of the chosen class on this context is
+ * bracketed by HIP events on the context's stream; end synchronises and
+ * returns the launch count, summed kernel time and the algorithmic flops and
+ * bytes of those launches.  Classes: 1 GEMM, 2 Jacobi SVD, 3 Cholesky,
+ * 4 triangular inverse, 5 Householder QR. */
+ttg_status ttg_profile_begin(ttg_ctx *ctx, int kernel_class);
+ttg_status ttg_profile_end(ttg_ctx *ctx, int64_t *launches, double *ms_total,
+                           double *flops_total, double *bytes_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TTGPU_H */

@@ -1,0 +1,25 @@
+/*
+ * Kernel-level test hooks of libttgpu.so.  No reference counterpart: they
+ * expose one building block each (GEMM, Householder QR, Cholesky + triangular
+ * inverse, Jacobi SVD) so the GPU tests can compare it with LAPACK.  Host
+ * arrays, column-major, FP64 on the boundary.
+ */
+#ifndef TTGPU_TESTING_H
+#define TTGPU_TESTING_H
+#include "ttgpu.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+ttg_status ttg_test_gemm(ttg_ctx *ctx, int prec, int ta, int tb, int64_t M, int64_t N, int64_t K,
+                         const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                         int64_t ldc);
+ttg_status ttg_test_householder(ttg_ctx *ctx, int64_t m, int64_t n, const double *A, double *Q,
+                                double *R);
+ttg_status ttg_test_chol(ttg_ctx *ctx, int64_t k, const double *G, double shift_rel, double *R,
+                         double *Rinv, int *flag, double *stats);
+ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS, double *Y,
+                           int64_t ycap, int64_t *kpad_out, int *sweeps, int *err);
+#ifdef __cplusplus
+}
+#endif
+#endif

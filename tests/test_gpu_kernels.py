@@ -1,0 +1,103 @@
+"""Building blocks of the rounding sweep against numpy/LAPACK (kernel-level hooks)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import ttgpu
+
+pytestmark = pytest.mark.gpu
+
+
+def f64p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@pytest.mark.parametrize("prec,ta,tb,M,N,K", [
+    (0, 0, 0, 37, 51, 29), (0, 1, 0, 64, 64, 2048), (0, 0, 1, 70, 33, 100), (0, 1, 1, 17, 90, 45),
+    (1, 0, 0, 65, 130, 40), (1, 1, 0, 256, 256, 2048),
+])
+def test_gemm(gpu_ctx, prec, ta, tb, M, N, K):
+    rng = np.random.default_rng(M * 7 + N)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    Af, Bf = np.asfortranarray(A), np.asfortranarray(B)
+    Cm = np.zeros((M, N), order="F")
+    s = gpu_ctx.lib.ttg_test_gemm(gpu_ctx.h, prec, ta, tb, M, N, K, f64p(Af), Af.shape[0], f64p(Bf),
+                                  Bf.shape[0], f64p(Cm), M)
+    gpu_ctx.check(s)
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    tol = 1e-12 if prec == 0 else 1e-5
+    assert np.max(np.abs(Cm - ref)) <= tol * np.max(np.abs(ref)) * np.sqrt(K), np.max(np.abs(Cm - ref))
+
+
+@pytest.mark.parametrize("m,n,rank", [(40, 7, 7), (160, 40, 20), (8, 30, 8), (300, 64, 64), (64, 64, 10)])
+def test_householder(gpu_ctx, m, n, rank):
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, rank)) @ rng.standard_normal((rank, n))
+    A = np.asfortranarray(A)
+    kq = min(m, n)
+    Q = np.zeros((m, kq), order="F")
+    R = np.zeros((kq, n), order="F")
+    gpu_ctx.check(gpu_ctx.lib.ttg_test_householder(gpu_ctx.h, m, n, f64p(A), f64p(Q), f64p(R)))
+    assert np.max(np.abs(Q.T @ Q - np.eye(kq))) <= 1e-13
+    assert np.max(np.abs(Q @ R - A)) <= 1e-13 * np.max(np.abs(A)) * m
+    assert np.max(np.abs(np.tril(R, -1))) == 0.0
+    # LAPACK agrees up to column signs
+    q2, r2 = np.linalg.qr(A)
+    np.testing.assert_allclose(np.abs(np.diag(R)[:rank]), np.abs(np.diag(r2)[:rank]), rtol=1e-10)
+
+
+@pytest.mark.parametrize("k", [5, 40, 100, 256])
+def test_cholesky_and_inverse(gpu_ctx, k):
+    rng = np.random.default_rng(k)
+    A = rng.standard_normal((3 * k, k))
+    G = np.asfortranarray(A.T @ A)
+    R = np.zeros((k, k), order="F")
+    X = np.zeros((k, k), order="F")
+    flag = np.zeros(1, dtype=np.int32)
+    stats = np.zeros(2)
+    gpu_ctx.check(gpu_ctx.lib.ttg_test_chol(gpu_ctx.h, k, f64p(G), 0.0, f64p(R), f64p(X), f64p(flag), f64p(stats)))
+    assert flag[0] == 0
+    ref = np.linalg.cholesky(G).T
+    np.testing.assert_allclose(R, ref, rtol=1e-10, atol=1e-10 * np.max(np.abs(ref)))
+    np.testing.assert_allclose(X @ R, np.eye(k), atol=1e-10)
+
+
+def test_cholesky_flags_indefinite(gpu_ctx):
+    k = 6
+    G = np.asfortranarray(np.diag([4.0, 1.0, -1.0, 2.0, 1.0, 1.0]))
+    R = np.zeros((k, k), order="F"); X = np.zeros((k, k), order="F")
+    flag = np.zeros(1, dtype=np.int32); stats = np.zeros(2)
+    gpu_ctx.check(gpu_ctx.lib.ttg_test_chol(gpu_ctx.h, k, f64p(G), 0.0, f64p(R), f64p(X), f64p(flag), f64p(stats)))
+    assert flag[0] == 3
+
+
+@pytest.mark.parametrize("k,trans,deficient", [(6, 0, 0), (40, 1, 20), (64, 0, 0), (128, 1, 0), (256, 0, 100)])
+def test_jacobi_svd(gpu_ctx, k, trans, deficient):
+    rng = np.random.default_rng(k + trans)
+    S = rng.standard_normal((k, k))
+    if deficient:
+        S = S[:, :k - deficient] @ rng.standard_normal((k - deficient, k))
+    S = np.asfortranarray(S)
+    kpad = C.c_int64()
+    cap = 2 * k * (k + 64)
+    Y = np.zeros(cap)
+    sweeps = np.zeros(1, dtype=np.int32)
+    err = np.zeros(1, dtype=np.int32)
+    gpu_ctx.check(gpu_ctx.lib.ttg_test_jacobi(gpu_ctx.h, k, f64p(S), trans, f64p(Y), cap, C.byref(kpad),
+                                              f64p(sweeps), f64p(err)))
+    kp = kpad.value
+    Ym = Y[:2 * k * kp].reshape((2 * k, kp), order="F")
+    top, J = Ym[:k, :k], Ym[k:, :k]
+    M = S.T if trans else S
+    print("sweeps", sweeps[0], "err", err[0], "kpad", kp, flush=True)
+    assert err[0] == 0
+    np.testing.assert_allclose(M @ J, top, atol=1e-12 * np.linalg.norm(M))
+    np.testing.assert_allclose(J.T @ J, np.eye(k), atol=1e-12)
+    sig = np.sort(np.linalg.norm(top, axis=0))[::-1]
+    ref = np.linalg.svd(M, compute_uv=False)
+    np.testing.assert_allclose(sig, ref, atol=1e-13 * ref[0] * k)
+    G = top.T @ top
+    off = G - np.diag(np.diag(G))
+    assert np.max(np.abs(off)) <= 1e-12 * ref[0] ** 2

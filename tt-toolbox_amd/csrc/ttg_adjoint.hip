@@ -1,0 +1,458 @@
+// Adjoint kernels (gradients) — TT-Toolbox has no gradients; the formulas
+// come from its interface products:
+//   * completion loss 0.5*||P_Omega(X) - y||^2: entry = product of r x r
+//     slices (@tt_tensor/subsref.m:116-122); per-sample left/right interface
+//     vectors as in tt_regression/reg_als.m:22-39,66-92;
+//   * bilinear form <y, A x>: left/right interface chains of
+//     core/tt_mvdot.m:16-57 (and dot.m:92-111 for two vectors).
+#include "ttg_internal.h"
+#include "ttg_device.h"
+#include <algorithm>
+
+namespace ttg {
+
+// ---------------------------------------------------------------------------
+// completion
+// ---------------------------------------------------------------------------
+// Group of G threads per sample (G >= max rank, power of two, <= 256).
+// Lbuf[k] (S x r_k): left interface of core k; the chain is stored for all k.
+template <class T>
+__global__ void k_completion_left(int d, const long *__restrict__ n, const long *__restrict__ r,
+                                  const long *__restrict__ cofs, const long *__restrict__ lofs,
+                                  const T *__restrict__ core, long S, const int *__restrict__ idx,
+                                  const T *__restrict__ vals, int G, T *__restrict__ Lbuf,
+                                  T *__restrict__ res, double *__restrict__ lpart) {
+  extern __shared__ char smem[];
+  T *buf = reinterpret_cast<T *>(smem);  // per group: 2 x G
+  __shared__ double red[16];
+  const int groups = blockDim.x / G;
+  const int gid = threadIdx.x / G, t = threadIdx.x % G;
+  const long s = (long)blockIdx.x * groups + gid;
+  T *cur = buf + gid * 2 * G, *nxt = cur + G;
+  const bool active = s < S;
+  if (t == 0) cur[0] = T(1);
+  __syncthreads();
+  for (int k = 0; k < d; ++k) {
+    const long rk = r[k], rk1 = r[k + 1], nk = n[k];
+    const int ik = active ? idx[s * d + k] : 0;
+    if (active && t < rk) Lbuf[lofs[k] + s * rk + t] = cur[t];
+    if (active && t < rk1) {
+      const T *g = core + cofs[k] + rk * (ik + nk * t);
+      T acc = T(0);
+      for (long a = 0; a < rk; ++a) acc += cur[a] * g[a];
+      nxt[t] = acc;
+    }
+    __syncthreads();
+    T *tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  double l = 0.0;
+  if (active && t == 0) {
+    const T rs = cur[0] - vals[s];
+    res[s] = rs;
+    l = 0.5 * (double)rs * (double)rs;
+  }
+  l = block_sum(l, red);
+  if (threadIdx.x == 0) lpart[blockIdx.x] = l;
+}
+
+// Right interfaces R_k (S x r_{k+1}), R_{d-1} = 1.
+template <class T>
+__global__ void k_completion_right(int d, const long *__restrict__ n, const long *__restrict__ r,
+                                   const long *__restrict__ cofs, const long *__restrict__ rofs,
+                                   const T *__restrict__ core, long S, const int *__restrict__ idx,
+                                   int G, T *__restrict__ Rbuf) {
+  extern __shared__ char smem[];
+  T *buf = reinterpret_cast<T *>(smem);
+  const int groups = blockDim.x / G;
+  const int gid = threadIdx.x / G, t = threadIdx.x % G;
+  const long s = (long)blockIdx.x * groups + gid;
+  T *cur = buf + gid * 2 * G, *nxt = cur + G;
+  const bool active = s < S;
+  if (t == 0) cur[0] = T(1);
+  __syncthreads();
+  for (int k = d - 1; k >= 0; --k) {
+    const long rk = r[k], rk1 = r[k + 1], nk = n[k];
+    if (active && t < rk1) Rbuf[rofs[k] + s * rk1 + t] = cur[t];
+    const int ik = active ? idx[s * d + k] : 0;
+    if (active && t < rk) {
+      const T *g = core + cofs[k] + t + rk * ik;
+      T acc = T(0);
+      for (long b = 0; b < rk1; ++b) acc += g[rk * nk * b] * cur[b];
+      nxt[t] = acc;
+    }
+    __syncthreads();
+    T *tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+}
+
+// Partial gradient of core k over a chunk of samples.  Thread t owns the
+// entries e = t, t + blockDim, ... of the n_k x r_k x r_{k+1} core; no races.
+template <class T>
+__global__ __launch_bounds__(256) void k_completion_grad(int d, int k, const long *__restrict__ n,
+                                                         const long *__restrict__ r, long S,
+                                                         const int *__restrict__ idx,
+                                                         const T *__restrict__ Lbuf, long lofs_k,
+                                                         const T *__restrict__ Rbuf, long rofs_k,
+                                                         const T *__restrict__ res, int chunk,
+                                                         T *__restrict__ part) {
+  const long rk = r[k], rk1 = r[k + 1], nk = n[k];
+  const long csize = rk * nk * rk1;
+  const long s0 = (long)blockIdx.x * chunk;
+  const long s1 = min(S, s0 + chunk);
+  T *out = part + (long)blockIdx.x * csize;
+  for (long e = threadIdx.x; e < csize; e += blockDim.x) {
+    const long a = e % rk;
+    const long rest = e / rk;
+    const long i = rest % nk, b = rest / nk;
+    T acc = T(0);
+    for (long s = s0; s < s1; ++s) {
+      if (idx[s * d + k] != i) continue;
+      acc += res[s] * Lbuf[lofs_k + s * rk + a] * Rbuf[rofs_k + s * rk1 + b];
+    }
+    out[e] = acc;
+  }
+}
+
+template <class T>
+__global__ void k_sum_parts(const T *__restrict__ part, int nparts, long size, T *__restrict__ out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < size; e += (long)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int p = 0; p < nparts; ++p) acc += (double)part[(long)p * size + e];
+    out[e] = (T)acc;
+  }
+}
+
+template <class T>
+ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
+                              const T *dcore, int64_t S, const int32_t *didx, const T *dvals,
+                              double *loss, T *dgrad) {
+  TTG_CHECK(d >= 1 && n && r && dcore && didx && dvals && dgrad && S >= 1, TTG_BAD_ARG,
+            "bad completion arguments");
+  TTG_CHECK(r[0] == 1 && r[d] == 1, TTG_BAD_ARG, "completion needs boundary ranks 1");
+  int64_t rmax = 1;
+  for (int64_t i = 0; i <= d; ++i) rmax = std::max(rmax, r[i]);
+  TTG_CHECK(rmax <= 256, TTG_BAD_ARG, "completion supports ranks up to 256");
+  int G = 1;
+  while (G < rmax) G <<= 1;
+  const int block = std::max(G, 256);
+  const int groups = block / G;
+  hipStream_t st = ctx->stream;
+  std::vector<int64_t> cofs(d + 1, 0), lofs(d + 1, 0), rofs(d + 1, 0);
+  for (int64_t k = 0; k < d; ++k) {
+    cofs[k + 1] = cofs[k] + r[k] * n[k] * r[k + 1];
+    lofs[k + 1] = lofs[k] + S * r[k];
+    rofs[k + 1] = rofs[k] + S * r[k + 1];
+  }
+  const int chunk = 256;
+  const int64_t nblk = (S + groups - 1) / groups;
+  const int64_t gblk = (S + chunk - 1) / chunk;
+  int64_t maxcore = 1;
+  for (int64_t k = 0; k < d; ++k) maxcore = std::max(maxcore, r[k] * n[k] * r[k + 1]);
+  ArenaSizer z;
+  z.take<long>(5 * (d + 1));
+  z.take<T>(lofs[d]);
+  z.take<T>(rofs[d]);
+  z.take<T>(S);
+  z.take<double>(nblk);
+  z.take<T>(gblk * maxcore);
+  ctx->arena.reserve(z.used + 4096);
+  ctx->arena.reset();
+  long *tab = ctx->arena.take<long>(5 * (d + 1));
+  T *Lbuf = ctx->arena.take<T>(lofs[d]);
+  T *Rbuf = ctx->arena.take<T>(rofs[d]);
+  T *res = ctx->arena.take<T>(S);
+  double *lpart = ctx->arena.take<double>(nblk);
+  T *part = ctx->arena.take<T>(gblk * maxcore);
+  std::vector<long> htab(5 * (d + 1));
+  for (int64_t k = 0; k < d; ++k) {
+    htab[k] = n[k];
+    htab[(d + 1) + k] = r[k];
+    htab[2 * (d + 1) + k] = cofs[k];
+    htab[3 * (d + 1) + k] = lofs[k];
+  }
+  for (int64_t k = 0; k <= d; ++k) htab[4 * (d + 1) + k] = rofs[k];
+  htab[(d + 1) + d] = r[d];
+  TTG_HIP(hipMemcpyAsync(tab, htab.data(), htab.size() * sizeof(long), hipMemcpyHostToDevice, st));
+  const long *dn = tab, *dr = tab + (d + 1), *dco = tab + 2 * (d + 1), *dlo = tab + 3 * (d + 1);
+  const size_t shm = (size_t)groups * 2 * G * sizeof(T);
+  hipLaunchKernelGGL((k_completion_left<T>), dim3((unsigned)nblk), dim3(block), shm, st, (int)d, dn,
+                     dr, dco, dlo, dcore, (long)S, didx, dvals, G, Lbuf, res, lpart);
+  TTG_HIP(hipGetLastError());
+  const long *dro = tab + 4 * (d + 1);
+  hipLaunchKernelGGL((k_completion_right<T>), dim3((unsigned)nblk), dim3(block), shm, st, (int)d,
+                     dn, dr, dco, dro, dcore, (long)S, didx, G, Rbuf);
+  TTG_HIP(hipGetLastError());
+  for (int64_t k = 0; k < d; ++k) {
+    const long csize = r[k] * n[k] * r[k + 1];
+    hipLaunchKernelGGL((k_completion_grad<T>), dim3((unsigned)gblk), dim3(256), 0, st, (int)d, (int)k,
+                       dn, dr, (long)S, didx, Lbuf, (long)lofs[k], Rbuf, (long)rofs[k], res, chunk,
+                       part);
+    TTG_HIP(hipGetLastError());
+    const int blocks = (int)std::min<long>((csize + 255) / 256, 1024);
+    hipLaunchKernelGGL((k_sum_parts<T>), dim3(blocks), dim3(256), 0, st, part, (int)gblk, csize,
+                       dgrad + cofs[k]);
+    TTG_HIP(hipGetLastError());
+  }
+  std::vector<double> hl(nblk);
+  TTG_HIP(hipMemcpyAsync(hl.data(), lpart, nblk * sizeof(double), hipMemcpyDeviceToHost, st));
+  TTG_HIP(hipStreamSynchronize(st));
+  if (loss) {
+    double s = 0.0;
+    for (auto v : hl) s += v;
+    *loss = s;
+  }
+  return TTG_OK;
+}
+template ttg_status adjoint_completion<double>(ttg_ctx *, int64_t, const int64_t *, const int64_t *, const double *, int64_t, const int32_t *, const double *, double *, double *);
+template ttg_status adjoint_completion<float>(ttg_ctx *, int64_t, const int64_t *, const int64_t *, const float *, int64_t, const int32_t *, const float *, double *, float *);
+
+// ---------------------------------------------------------------------------
+// bilinear <y, A x>
+// ---------------------------------------------------------------------------
+// Core layouts: y_k (ry, n, ry'), A_k (ra, n, m, ra'), x_k (rx, m, rx').
+// Interfaces: Phi_k (ry_k, ra_k, rx_k) left, Psi_k (ry_k, ra_k, rx_k) right.
+struct BiShape {
+  long ry0, ry1, ra0, ra1, rx0, rx1, n, m;
+};
+
+// T2[al, e, j, b] = sum_{c,a,i} Phi[c,a,al] y[c,i,e] A[a,i,j,b]
+template <class T>
+__global__ void k_bi_t2(BiShape s, const T *__restrict__ phi, const T *__restrict__ y,
+                        const T *__restrict__ A, T *__restrict__ t2) {
+  const long total = s.rx0 * s.ry1 * s.m * s.ra1;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x) {
+    const long al = o % s.rx0;
+    long q = o / s.rx0;
+    const long e = q % s.ry1;
+    q /= s.ry1;
+    const long j = q % s.m, b = q / s.m;
+    double acc = 0.0;
+    for (long i = 0; i < s.n; ++i)
+      for (long a = 0; a < s.ra0; ++a) {
+        const double av = (double)A[a + s.ra0 * (i + s.n * (j + s.m * b))];
+        if (av == 0.0) continue;
+        double inner = 0.0;
+        for (long c = 0; c < s.ry0; ++c)
+          inner += (double)phi[c + s.ry0 * (a + s.ra0 * al)] * (double)y[c + s.ry0 * (i + s.n * e)];
+        acc += inner * av;
+      }
+    t2[o] = (T)acc;
+  }
+}
+// Phi'[e,b,f] = sum_{al,j} T2[al,e,j,b] x[al,j,f]
+template <class T>
+__global__ void k_bi_phi_next(BiShape s, const T *__restrict__ t2, const T *__restrict__ x,
+                              T *__restrict__ phi1) {
+  const long total = s.ry1 * s.ra1 * s.rx1;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x) {
+    const long e = o % s.ry1;
+    long q = o / s.ry1;
+    const long b = q % s.ra1, f = q / s.ra1;
+    double acc = 0.0;
+    for (long j = 0; j < s.m; ++j)
+      for (long al = 0; al < s.rx0; ++al)
+        acc += (double)t2[al + s.rx0 * (e + s.ry1 * (j + s.m * b))] * (double)x[al + s.rx0 * (j + s.m * f)];
+    phi1[o] = (T)acc;
+  }
+}
+// gx[al,j,f] = sum_{e,b} T2[al,e,j,b] Psi'[e,b,f]
+template <class T>
+__global__ void k_bi_gx(BiShape s, const T *__restrict__ t2, const T *__restrict__ psi1,
+                        T *__restrict__ gx) {
+  const long total = s.rx0 * s.m * s.rx1;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x) {
+    const long al = o % s.rx0;
+    long q = o / s.rx0;
+    const long j = q % s.m, f = q / s.m;
+    double acc = 0.0;
+    for (long b = 0; b < s.ra1; ++b)
+      for (long e = 0; e < s.ry1; ++e)
+        acc += (double)t2[al + s.rx0 * (e + s.ry1 * (j + s.m * b))] * (double)psi1[e + s.ry1 * (b + s.ra1 * f)];
+    gx[o] = (T)acc;
+  }
+}
+// V[al,j,e,b] = sum_f x[al,j,f] Psi'[e,b,f]
+template <class T>
+__global__ void k_bi_v(BiShape s, const T *__restrict__ x, const T *__restrict__ psi1, T *__restrict__ v) {
+  const long total = s.rx0 * s.m * s.ry1 * s.ra1;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x) {
+    const long al = o % s.rx0;
+    long q = o / s.rx0;
+    const long j = q % s.m;
+    q /= s.m;
+    const long e = q % s.ry1, b = q / s.ry1;
+    double acc = 0.0;
+    for (long f = 0; f < s.rx1; ++f)
+      acc += (double)x[al + s.rx0 * (j + s.m * f)] * (double)psi1[e + s.ry1 * (b + s.ra1 * f)];
+    v[o] = (T)acc;
+  }
+}
+// W[a,al,i,e] = sum_{j,b} A[a,i,j,b] V[al,j,e,b]
+template <class T>
+__global__ void k_bi_w(BiShape s, const T *__restrict__ A, const T *__restrict__ v, T *__restrict__ w) {
+  const long total = s.ra0 * s.rx0 * s.n * s.ry1;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x) {
+    const long a = o % s.ra0;
+    long q = o / s.ra0;
+    const long al = q % s.rx0;
+    q /= s.rx0;
+    const long i = q % s.n, e = q / s.n;
+    double acc = 0.0;
+    for (long b = 0; b < s.ra1; ++b)
+      for (long j = 0; j < s.m; ++j)
+        acc += (double)A[a + s.ra0 * (i + s.n * (j + s.m * b))] *
+               (double)v[al + s.rx0 * (j + s.m * (e + s.ry1 * b))];
+    w[o] = (T)acc;
+  }
+}
+// Psi[c,a,al] = sum_{i,e} y[c,i,e] W[a,al,i,e];  gy[c,i,e] = sum_{a,al} Phi[c,a,al] W[a,al,i,e]
+template <class T>
+__global__ void k_bi_psi(BiShape s, const T *__restrict__ y, const T *__restrict__ w, T *__restrict__ psi) {
+  const long total = s.ry0 * s.ra0 * s.rx0;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x) {
+    const long c = o % s.ry0;
+    long q = o / s.ry0;
+    const long a = q % s.ra0, al = q / s.ra0;
+    double acc = 0.0;
+    for (long e = 0; e < s.ry1; ++e)
+      for (long i = 0; i < s.n; ++i)
+        acc += (double)y[c + s.ry0 * (i + s.n * e)] * (double)w[a + s.ra0 * (al + s.rx0 * (i + s.n * e))];
+    psi[o] = (T)acc;
+  }
+}
+template <class T>
+__global__ void k_bi_gy(BiShape s, const T *__restrict__ phi, const T *__restrict__ w, T *__restrict__ gy) {
+  const long total = s.ry0 * s.n * s.ry1;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x) {
+    const long c = o % s.ry0;
+    long q = o / s.ry0;
+    const long i = q % s.n, e = q / s.n;
+    double acc = 0.0;
+    for (long al = 0; al < s.rx0; ++al)
+      for (long a = 0; a < s.ra0; ++a)
+        acc += (double)phi[c + s.ry0 * (a + s.ra0 * al)] * (double)w[a + s.ra0 * (al + s.rx0 * (i + s.n * e))];
+    gy[o] = (T)acc;
+  }
+}
+template <class T> __global__ void k_fill_one(T *p) { p[0] = T(1); }
+
+static dim3 grid_for(long total) { return dim3((unsigned)std::min<long>((total + 255) / 256, 4096)); }
+
+template <class T>
+ttg_status adjoint_bilinear(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
+                            const int64_t *ry, const T *dycore, const int64_t *ra,
+                            const T *dacore, const int64_t *rx, const T *dxcore, double *value,
+                            T *dgx, T *dgy) {
+  TTG_CHECK(d >= 1 && n && m && ry && ra && rx && dycore && dacore && dxcore, TTG_BAD_ARG,
+            "bad bilinear arguments");
+  TTG_CHECK(ry[0] == 1 && ra[0] == 1 && rx[0] == 1 && ry[d] == 1 && ra[d] == 1 && rx[d] == 1,
+            TTG_BAD_ARG, "bilinear form needs boundary ranks 1");
+  hipStream_t st = ctx->stream;
+  std::vector<int64_t> oy(d + 1, 0), oa(d + 1, 0), ox(d + 1, 0), oi(d + 2, 0);
+  int64_t tmax = 1;
+  for (int64_t k = 0; k < d; ++k) {
+    oy[k + 1] = oy[k] + ry[k] * n[k] * ry[k + 1];
+    oa[k + 1] = oa[k] + ra[k] * n[k] * m[k] * ra[k + 1];
+    ox[k + 1] = ox[k] + rx[k] * m[k] * rx[k + 1];
+    tmax = std::max(tmax, rx[k] * ry[k + 1] * m[k] * ra[k + 1]);
+    tmax = std::max(tmax, rx[k] * m[k] * ry[k + 1] * ra[k + 1]);
+    tmax = std::max(tmax, ra[k] * rx[k] * n[k] * ry[k + 1]);
+  }
+  for (int64_t k = 0; k <= d; ++k) oi[k + 1] = oi[k] + ry[k] * ra[k] * rx[k];
+  ArenaSizer z;
+  z.take<T>(oi[d + 1]);
+  z.take<T>(oi[d + 1]);
+  z.take<T>(tmax);
+  z.take<T>(tmax);
+  ctx->arena.reserve(z.used + 4096);
+  ctx->arena.reset();
+  T *phi = ctx->arena.take<T>(oi[d + 1]);
+  T *psi = ctx->arena.take<T>(oi[d + 1]);
+  T *t2 = ctx->arena.take<T>(tmax);
+  T *w = ctx->arena.take<T>(tmax);
+  auto shape = [&](int64_t k) {
+    BiShape s;
+    s.ry0 = ry[k]; s.ry1 = ry[k + 1]; s.ra0 = ra[k]; s.ra1 = ra[k + 1];
+    s.rx0 = rx[k]; s.rx1 = rx[k + 1]; s.n = n[k]; s.m = m[k];
+    return s;
+  };
+  hipLaunchKernelGGL((k_fill_one<T>), dim3(1), dim3(1), 0, st, phi);
+  hipLaunchKernelGGL((k_fill_one<T>), dim3(1), dim3(1), 0, st, psi + oi[d]);
+  for (int64_t k = 0; k < d; ++k) {  // left chain (tt_mvdot.m:16-57)
+    BiShape s = shape(k);
+    hipLaunchKernelGGL((k_bi_t2<T>), grid_for(s.rx0 * s.ry1 * s.m * s.ra1), dim3(256), 0, st, s,
+                       phi + oi[k], dycore + oy[k], dacore + oa[k], t2);
+    hipLaunchKernelGGL((k_bi_phi_next<T>), grid_for(s.ry1 * s.ra1 * s.rx1), dim3(256), 0, st, s, t2,
+                       dxcore + ox[k], phi + oi[k + 1]);
+  }
+  for (int64_t k = d - 1; k >= 0; --k) {  // right chain
+    BiShape s = shape(k);
+    hipLaunchKernelGGL((k_bi_v<T>), grid_for(s.rx0 * s.m * s.ry1 * s.ra1), dim3(256), 0, st, s,
+                       dxcore + ox[k], psi + oi[k + 1], t2);
+    hipLaunchKernelGGL((k_bi_w<T>), grid_for(s.ra0 * s.rx0 * s.n * s.ry1), dim3(256), 0, st, s,
+                       dacore + oa[k], t2, w);
+    hipLaunchKernelGGL((k_bi_psi<T>), grid_for(s.ry0 * s.ra0 * s.rx0), dim3(256), 0, st, s,
+                       dycore + oy[k], w, psi + oi[k]);
+    if (dgy)
+      hipLaunchKernelGGL((k_bi_gy<T>), grid_for(s.ry0 * s.n * s.ry1), dim3(256), 0, st, s,
+                         phi + oi[k], w, dgy + oy[k]);
+  }
+  if (dgx) {
+    for (int64_t k = 0; k < d; ++k) {
+      BiShape s = shape(k);
+      hipLaunchKernelGGL((k_bi_t2<T>), grid_for(s.rx0 * s.ry1 * s.m * s.ra1), dim3(256), 0, st, s,
+                         phi + oi[k], dycore + oy[k], dacore + oa[k], t2);
+      hipLaunchKernelGGL((k_bi_gx<T>), grid_for(s.rx0 * s.m * s.rx1), dim3(256), 0, st, s, t2,
+                         psi + oi[k + 1], dgx + ox[k]);
+    }
+  }
+  TTG_HIP(hipGetLastError());
+  T hv = T(0);
+  TTG_HIP(hipMemcpyAsync(&hv, phi + oi[d], sizeof(T), hipMemcpyDeviceToHost, st));
+  TTG_HIP(hipStreamSynchronize(st));
+  if (value) *value = (double)hv;
+  return TTG_OK;
+}
+template ttg_status adjoint_bilinear<double>(ttg_ctx *, int64_t, const int64_t *, const int64_t *, const int64_t *, const double *, const int64_t *, const double *, const int64_t *, const double *, double *, double *, double *);
+template ttg_status adjoint_bilinear<float>(ttg_ctx *, int64_t, const int64_t *, const int64_t *, const int64_t *, const float *, const int64_t *, const float *, const int64_t *, const float *, double *, float *, float *);
+
+// ---------------------------------------------------------------------------
+// batched rounding: per-tensor sweep on the shared stream (first version)
+// ---------------------------------------------------------------------------
+ttg_status round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
+                     const double *dcore, double eps, const int64_t *rmax, int prec,
+                     int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
+                     ttg_info *info);
+ttg_status round_capacity(int64_t d, const int64_t *n, const int64_t *r, int64_t *cap);
+
+ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, const int64_t *nofs,
+                             const int64_t *n, const int64_t *rofs, const int64_t *r,
+                             const int64_t *cofs, const double *dcore, double eps, int prec,
+                             int64_t *r_out, const int64_t *cofs_out, double *dcore_out,
+                             ttg_info *info) {
+  ttg_status worst = TTG_OK;
+  ttg_info acc{0, 0, 0, 0.0};
+  for (int64_t t = 0; t < count; ++t) {
+    int64_t cap = 0;
+    ttg_status s = round_capacity(d[t], n + nofs[t], r + rofs[t], &cap);
+    if (s != TTG_OK) return s;
+    ttg_info one{0, 0, 0, 0.0};
+    int64_t used = 0;
+    s = round_dev(ctx, d[t], n + nofs[t], r + rofs[t], dcore + cofs[t], eps, nullptr, prec,
+                  r_out + rofs[t], dcore_out + cofs_out[t], cap, &used, &one);
+    if (s != TTG_OK && s != TTG_NO_CONVERGENCE) return s;
+    if (s == TTG_NO_CONVERGENCE) worst = s;
+    acc.qr_fallbacks += one.qr_fallbacks;
+    acc.jacobi_sweeps += one.jacobi_sweeps;
+    acc.jacobi_maxsweeps = std::max(acc.jacobi_maxsweeps, one.jacobi_maxsweeps);
+    acc.seconds += one.seconds;
+  }
+  if (info) *info = acc;
+  return worst;
+}
+
+}  // namespace ttg

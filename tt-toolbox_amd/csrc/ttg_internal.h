@@ -1,0 +1,187 @@
+// Internal declarations shared by the ttgpu HIP sources (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include "../../include/ttgpu.h"
+
+namespace ttg {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  ttg_status status;
+  Error(ttg_status s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+#define TTG_HIP(call)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      throw ::ttg::Error(TTG_HIP_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TTG_CHECK(cond, st, msg)                                                    \
+  do {                                                                              \
+    if (!(cond)) throw ::ttg::Error(st, msg);                                       \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device workspace: one growable arena, carved per call
+// ---------------------------------------------------------------------------
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  void reserve(size_t bytes) {
+    if (bytes <= cap) return;
+    if (base) TTG_HIP(hipFree(base));
+    base = nullptr;
+    cap = 0;
+    TTG_HIP(hipMalloc(&base, bytes));
+    cap = bytes;
+  }
+  void reset() { used = 0; }
+  template <class T> T *take(size_t count) {
+    size_t bytes = ((count * sizeof(T)) + 255) & ~size_t(255);
+    if (used + bytes > cap) throw Error(TTG_HIP_ERROR, "workspace arena exhausted");
+    T *p = reinterpret_cast<T *>(base + used);
+    used += bytes;
+    return p;
+  }
+  ~Arena() {
+    if (base) (void)hipFree(base);
+  }
+};
+
+// Size-only twin of Arena used to compute the workspace requirement.
+struct ArenaSizer {
+  size_t used = 0;
+  template <class T> void take(size_t count) { used += ((count * sizeof(T)) + 255) & ~size_t(255); }
+};
+
+// Kernel-class timing: events around every launch of one class.
+enum KernelClass { KC_NONE = 0, KC_GEMM = 1, KC_JACOBI = 2, KC_CHOL = 3, KC_TRTRI = 4, KC_HQR = 5 };
+struct Profiler {
+  int kind = KC_NONE;
+  std::vector<hipEvent_t> pool;  // pairs
+  size_t used = 0;               // pairs in use
+  std::vector<double> flops, bytes;
+  void begin(hipStream_t s);
+  void end(hipStream_t s, double fl, double by);
+};
+// the profiler of the context whose call is running on this thread (or null)
+extern thread_local Profiler *g_prof;
+struct ProfScope {
+  Profiler *p;
+  hipStream_t s;
+  double fl, by;
+  ProfScope(int kind, hipStream_t st, double f, double b) : p(nullptr), s(st), fl(f), by(b) {
+    if (g_prof && g_prof->kind == kind) {
+      p = g_prof;
+      p->begin(s);
+    }
+  }
+  ~ProfScope() {
+    if (p) p->end(s, fl, by);
+  }
+};
+
+}  // namespace ttg
+
+struct ttg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int num_cus = 256;
+  ttg::Arena arena;       // per-call scratch
+  ttg::Arena persist;     // small persistent device scalars (barrier counters, flags)
+  void *host_pinned = nullptr;
+  size_t host_pinned_cap = 0;
+  std::string last_error;
+  // diagnostics of the current call
+  int64_t qr_fallbacks = 0, jacobi_sweeps = 0, jacobi_maxsweeps = 0;
+  ttg::Profiler prof;
+};
+
+namespace ttg {
+
+// ---------------------------------------------------------------------------
+// kernel launchers (implemented in ttg_kernels.hip)
+// ---------------------------------------------------------------------------
+// Column-major GEMM C = alpha*op(A)*op(B) + beta*C.  ws (may be null) is used
+// for split-K partials; ws_elems is its capacity in T elements.
+template <class T>
+void gemm(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, T alpha,
+          const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
+          T *ws, int64_t ws_elems, int num_cus);
+// bytes of split-K workspace gemm() may use for the given shape
+int64_t gemm_ws_elems(int64_t M, int64_t N, int64_t K, int num_cus);
+
+// y[i] = (T)x[i]
+template <class TO, class TI> void convert(hipStream_t s, TO *y, const TI *x, int64_t count);
+// B (cols x rows) = A^T where A is rows x cols, both column-major
+template <class T> void transpose(hipStream_t s, const T *A, int64_t rows, int64_t cols, T *B);
+// Frobenius norm of `count` contiguous values -> *out (device), fixed-order reduction
+template <class T> void fro_norm(hipStream_t s, const T *x, int64_t count, double *out);
+// x[i] *= 1/ *nrm if *nrm != 0 (nrm device scalar)
+template <class T> void scale_by_inv(hipStream_t s, T *x, int64_t count, const double *nrm);
+
+// One-workgroup Householder QR (LAPACK dgeqr2+dorg2r semantics): A (m x n, lda=m)
+// -> Q (m x kq, explicit) and R (kq x n), kq = min(m,n).  A is overwritten.
+void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, double *R,
+                    double *tau);
+
+// One-workgroup Cholesky G = R^T R (upper R) and its inverse, FP64, k x k.
+// G is read from the upper triangle.  flag[0] = 0 ok, j+1 if pivot j failed.
+// cond[0] = ||R||_F * ||R^{-1}||_F.  shift is added to the diagonal first.
+void chol_inv(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
+              double *R, double *Rinv, int *flag, double *cond);
+
+// One-sided block Jacobi SVD on the stacked matrix Y = [S; I] (FP64).
+// Setup from S (k x k, column-major, optionally transposed) then rotate
+// until converged.  Returns sweeps via *sweeps (device int).
+struct JacobiPlan {
+  int64_t k = 0;        // matrix order
+  int64_t kpad = 0;     // padded column count (2*b*p)
+  int b = 0, p = 0;     // block size and number of workgroups
+  int64_t rows = 0;     // stacked rows = 2*k
+};
+JacobiPlan jacobi_plan(int64_t k);
+size_t jacobi_ws_bytes(const JacobiPlan &pl);
+void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
+                int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps);
+
+// Sort the column norms of Y's top k rows (descending) and apply my_chop2.
+// sv[k], perm[k] written; out_i[0] = r1 (after rmax cap), out_d[0] = ||s||.
+void sort_chop(hipStream_t s, const double *Y, int64_t ldy, int64_t k, int64_t kpad,
+               double ep, int64_t rmax, double *sv, int *perm, int *out_i, double *out_d);
+
+// Gather columns perm[0..r1) of Y (rows x *) with row offset into dst (rows_out x r1).
+template <class T>
+void gather_cols(hipStream_t s, const double *Y, int64_t ldy, int64_t row0, int64_t rows,
+                 const int *perm, int64_t r1, T *dst, int64_t ldd);
+// Like gather_cols but writes the transpose: dst (r1 x rows), ld = r1.
+template <class T>
+void gather_cols_t(hipStream_t s, const double *Y, int64_t ldy, int64_t row0, int64_t rows,
+                   const int *perm, int64_t r1, T *dst);
+
+// balance: nrm[0] = ||core0||, core0 /= nrm[0]; nrm0 = exp(mean log|nrm|); all *= nrm0.
+// Cores are gathered from src pointers (device array of pointers) into a packed output.
+template <class TI>
+void finalize_balance(hipStream_t s, int64_t d, TI *const *src_dev, const int64_t *sizes_dev,
+                      const int64_t *ofs_dev, int64_t size0, double *nrm_dev, double *out,
+                      int64_t total);
+
+// mtimes: one product core (rA*rx) x n x (rA'*rx') (mtimes.m:38-65 layout)
+template <class T>
+void mtimes_core(hipStream_t s, int64_t ra0, int64_t n, int64_t m, int64_t ra1, const double *A,
+                 int64_t rx0, int64_t rx1, const double *X, T *Y);
+
+// check for non-finite values; sets *flag nonzero if any
+void check_finite(hipStream_t s, const double *x, int64_t count, int *flag);
+
+}  // namespace ttg

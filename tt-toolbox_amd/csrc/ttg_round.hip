@@ -1,0 +1,551 @@
+// Host orchestration of TT rounding on one GPU (sweeps of @tt_tensor/round.m).
+//
+// Sweep 1 (round.m:39-53), bond i = 0..d-2, left unfolding A = core_i as
+// (r_i n_i) x r_{i+1}:
+//   tall  -> CholeskyQR2 (plain FP64) / shifted CholeskyQR3 (mixed: FP32
+//            Grams and TRSM-GEMMs, FP64 Cholesky); rejected factorizations
+//            (Cholesky breakdown, diag(R) spread > 1e7, second-pass R far from
+//            I) are redone by one-workgroup Householder QR;
+//   wide  -> Householder QR (Q square, rank capped at r_i n_i, round.m:48);
+//   nrm[i+1] = ||R||_F, R /= nrm, core_{i+1} <- R core_{i+1} (MFMA GEMM).
+// Sweep 2 (round.m:59-89), bond i = d-1..1, W = core_i as r_i x n_i r_{i+1}:
+//   QR of the tall orientation, one-sided block Jacobi SVD of the square
+//   factor stacked over the identity, descending sort + my_chop2 on the
+//   device, then core_i <- V(:,1:r1)^T and core_{i-1} <- core_{i-1} U S.
+// Finalise (round.m:94-107): normalise core_1, balance all cores by
+// exp(mean(log|nrm|)).
+//
+// The first sweep runs speculatively (one host synchronisation at its end;
+// a rejected bond and everything after it is recomputed).  The second sweep
+// synchronises once per bond to learn r1, which sizes the next launches.
+#include "ttg_internal.h"
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <algorithm>
+
+namespace ttg {
+
+void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
+                     const float *A, int64_t lda, const float *B, int64_t ldb, double *C,
+                     int64_t ldc, float *ws, int64_t wse, int cus);
+void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
+                 double *R, int *flag, double *stats);
+void tri_inverse(hipStream_t s, int64_t k, const double *R, double *X);
+
+static constexpr int kMaxRank = 512;
+static constexpr int kMaxSweeps = 60;
+
+// capped ranks of the first sweep (round.m:48)
+static std::vector<int64_t> capped_ranks(int64_t d, const int64_t *n, const int64_t *r) {
+  std::vector<int64_t> rr(r, r + d + 1);
+  for (int64_t i = 0; i + 1 < d; ++i) rr[i + 1] = std::min(rr[i] * n[i], rr[i + 1]);
+  return rr;
+}
+
+static void validate_tt(int64_t d, const int64_t *n, const int64_t *r) {
+  TTG_CHECK(d >= 1, TTG_BAD_ARG, "d must be >= 1");
+  TTG_CHECK(n != nullptr && r != nullptr, TTG_BAD_ARG, "n and r are required");
+  for (int64_t i = 0; i < d; ++i) TTG_CHECK(n[i] >= 1, TTG_BAD_ARG, "mode sizes must be >= 1");
+  for (int64_t i = 0; i <= d; ++i) TTG_CHECK(r[i] >= 1, TTG_BAD_ARG, "ranks must be >= 1");
+}
+
+template <class T> struct Rounder {
+  ttg_ctx *ctx;
+  hipStream_t st;
+  int cus;
+  int64_t d;
+  std::vector<int64_t> n, r, ofs;
+  // buf[0]: input cores (sweep 1 never writes it); buf[1]: outputs of the
+  // R*core GEMMs; buf[2]: Q factors.  Every sweep-1 bond therefore keeps its
+  // inputs and can be recomputed after a rejected Cholesky-QR.
+  T *buf[3];
+  std::vector<T *> cp;
+  // scratch
+  double *G, *R1, *R2, *R3, *Rinv, *Rprod, *Rtmp, *Y, *sv, *nrm, *stats, *bstats;
+  T *RT, *Q1, *Q2, *Tt, *Qs, *US, *gws;
+  int64_t gws_elems;
+  double *hA, *hQ, *htau;
+  int *perm, *iflag, *scal_i, *jerr, *jsweeps;
+  unsigned *bar;
+  double *scal_d;
+  int64_t max_core, kmax, rsq;
+  bool mixed;
+  int64_t fallbacks = 0, sweeps_total = 0, sweeps_max = 0;
+  bool noconv = false;
+
+  // a buffer region for core k that is not the current one
+  T *other(int64_t k) const {
+    for (int b = 0; b < 3; ++b)
+      if (cp[k] != buf[b] + ofs[k]) return buf[b] + ofs[k];
+    return buf[0] + ofs[k];
+  }
+
+  static size_t jac_elems(int64_t kmax) { return (size_t)(2 * kmax) * (size_t)(kmax + 64); }
+
+  template <class A> static void layout(A &a, int64_t d, int64_t total, int64_t max_core, int64_t kmax, int64_t rsq) {
+    for (int b = 0; b < 3; ++b) a.template take<T>(total);
+    for (int q = 0; q < 7; ++q) a.template take<double>(rsq);
+    a.template take<double>(jac_elems(kmax));
+    a.template take<double>(kmax + 64);      // sv
+    a.template take<double>(d + 8);          // nrm (+ balance scales)
+    a.template take<double>(64);             // stats
+    a.template take<double>(8 * (d + 8));    // per-bond stats
+    a.template take<T>(rsq);                 // RT
+    for (int q = 0; q < 5; ++q) a.template take<T>(max_core);  // Q1 Q2 Tt Qs US
+    a.template take<T>(std::max<int64_t>(32 * kmax * kmax, 1 << 16));  // gemm split-K
+    a.template take<double>(max_core);       // hA
+    a.template take<double>(max_core);       // hQ
+    a.template take<double>(kmax + 64);      // htau
+    a.template take<int>(kmax + 64);         // perm
+    a.template take<int>(4 * (d + 8));       // iflag (per bond)
+    a.template take<int>(16);                // scal_i
+    a.template take<int>(16);                // jerr
+    a.template take<int>(16);                // jsweeps
+    a.template take<unsigned>(kMaxSweeps + 16);
+    a.template take<double>(16);             // scal_d
+  }
+
+  void carve(Arena &a, int64_t total) {
+    for (int b = 0; b < 3; ++b) buf[b] = a.take<T>(total);
+    G = a.take<double>(rsq);
+    R1 = a.take<double>(rsq);
+    R2 = a.take<double>(rsq);
+    R3 = a.take<double>(rsq);
+    Rinv = a.take<double>(rsq);
+    Rprod = a.take<double>(rsq);
+    Rtmp = a.take<double>(rsq);
+    Y = a.take<double>(jac_elems(kmax));
+    sv = a.take<double>(kmax + 64);
+    nrm = a.take<double>(d + 8);
+    stats = a.take<double>(64);
+    bstats = a.take<double>(8 * (d + 8));
+    RT = a.take<T>(rsq);
+    Q1 = a.take<T>(max_core);
+    Q2 = a.take<T>(max_core);
+    Tt = a.take<T>(max_core);
+    Qs = a.take<T>(max_core);
+    US = a.take<T>(max_core);
+    gws_elems = std::max<int64_t>(32 * kmax * kmax, 1 << 16);
+    gws = a.take<T>(gws_elems);
+    hA = a.take<double>(max_core);
+    hQ = a.take<double>(max_core);
+    htau = a.take<double>(kmax + 64);
+    perm = a.take<int>(kmax + 64);
+    iflag = a.take<int>(4 * (d + 8));
+    scal_i = a.take<int>(16);
+    jerr = a.take<int>(16);
+    jsweeps = a.take<int>(16);
+    bar = a.take<unsigned>(kMaxSweeps + 16);
+    scal_d = a.take<double>(16);
+  }
+
+  // ---- small helpers ------------------------------------------------------
+  void g(bool ta, bool tb, int64_t M, int64_t N, int64_t K, const T *A, int64_t lda, const T *B,
+         int64_t ldb, T *C, int64_t ldc) {
+    gemm<T>(st, ta, tb, M, N, K, T(1), A, lda, B, ldb, T(0), C, ldc, gws, gws_elems, cus);
+  }
+  void gram(const T *A, int64_t m, int64_t k, double *Gout) {
+    if constexpr (std::is_same<T, double>::value) {
+      gemm<double>(st, true, false, k, k, m, 1.0, A, m, A, m, 0.0, Gout, k, gws, gws_elems, cus);
+    } else {
+      gemm_f32_f64out(st, true, false, k, k, m, A, m, A, m, Gout, k, gws, gws_elems, cus);
+    }
+  }
+  // dst (T) = src (double)
+  void to_T(T *dst, const double *src, int64_t count) {
+    if constexpr (std::is_same<T, double>::value) {
+      TTG_HIP(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+      convert<float, double>(st, dst, src, count);
+    }
+  }
+
+  // One Cholesky-QR pass: Qout = A R^{-1}, R = chol(A^T A (+ shift)).
+  void cholqr_pass(const T *A, int64_t m, int64_t k, double shift_rel, T *Qout, double *Rout,
+                   int *flag, double *st2) {
+    gram(A, m, k, G);
+    chol_factor(st, k, G, k, shift_rel, Rout, flag, st2);
+    tri_inverse(st, k, Rout, Rinv);
+    to_T(RT, Rinv, k * k);
+    g(false, false, m, k, k, A, m, RT, k, Qout, m);
+  }
+
+  // Tall QR (m >= k): Qout (m x k, T), Rout (k x k, FP64).  flags[0..2] and
+  // stt[0..5] feed the acceptance test (qr_accept), checked by the caller.
+  void qr_tall_fast(const T *A, int64_t m, int64_t k, T *Qout, double *Rout, int *flags,
+                    double *stt) {
+    if (!mixed) {
+      cholqr_pass(A, m, k, 0.0, Q1, R1, flags + 0, stt + 0);
+      cholqr_pass(Q1, m, k, 0.0, Qout, R2, flags + 1, stt + 2);
+      gemm<double>(st, false, false, k, k, k, 1.0, R2, k, R1, k, 0.0, Rout, k, nullptr, 0, cus);
+    } else {
+      const double u32 = 5.96e-8;
+      const double shift_rel = 8.0 * u32 * std::sqrt((double)m) * (double)k;
+      cholqr_pass(A, m, k, shift_rel, Q1, R1, flags + 0, stt + 0);
+      cholqr_pass(Q1, m, k, 0.0, Q2, R2, flags + 1, stt + 2);
+      cholqr_pass(Q2, m, k, 0.0, Qout, R3, flags + 2, stt + 4);
+      gemm<double>(st, false, false, k, k, k, 1.0, R3, k, R2, k, 0.0, Rtmp, k, nullptr, 0, cus);
+      gemm<double>(st, false, false, k, k, k, 1.0, Rtmp, k, R1, k, 0.0, Rout, k, nullptr, 0, cus);
+    }
+  }
+  bool qr_accept(const int *hflags, const double *hstats) const {
+    if (!mixed) {
+      if (hflags[0] || hflags[1]) return false;
+      if (!(hstats[0] < 1e7)) return false;   // diag(R1) spread: kappa estimate
+      if (!(hstats[3] < 0.1)) return false;   // R2 ~ I: Q1 was nearly orthonormal
+      return true;
+    }
+    if (hflags[0] || hflags[1] || hflags[2]) return false;
+    if (!(hstats[2] < 1e7)) return false;     // second pass well conditioned
+    if (!(hstats[5] < 0.1)) return false;     // third pass R ~ I
+    return true;
+  }
+  // Householder fallback (FP64) for the tall/wide QR of A (m x k).
+  void qr_householder(const T *A, int64_t m, int64_t k, T *Qout, double *Rout) {
+    const int64_t kq = std::min(m, k);
+    if constexpr (std::is_same<T, double>::value) {
+      TTG_HIP(hipMemcpyAsync(hA, A, m * k * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+      convert<double, float>(st, hA, A, m * k);
+    }
+    householder_qr(st, m, k, hA, hQ, Rout, htau);
+    to_T(Qout, hQ, m * kq);
+    ++fallbacks;
+  }
+
+  // ---- sweep 1 -----------------------------------------------------------
+  // Bond i reads core i from (i == 0 ? buf0 : buf1) and core i+1 from buf0;
+  // writes Q to buf2 and R*core_{i+1} to buf1.  Returns true when the
+  // Cholesky-QR acceptance check is pending.
+  bool sweep1_bond(int64_t i, bool force_householder,
+                   const std::function<void(int64_t, T *)> &gen) {
+    const int64_t m = r[i] * n[i], k = r[i + 1];
+    const int64_t kk = std::min(m, k);
+    const T *A = (i == 0 ? buf[0] : buf[1]) + ofs[i];
+    T *Qdst = buf[2] + ofs[i];
+    double *R = Rprod;
+    bool pending = false;
+    if (m >= k && !force_householder) {
+      qr_tall_fast(A, m, k, Qdst, R, iflag + 4 * i, bstats + 8 * i);
+      pending = true;
+    } else {
+      qr_householder(A, m, k, Qdst, R);
+    }
+    cp[i] = Qdst;
+    // nrm(i+1) = ||R||_F ; R /= nrm   (round.m:41-44)
+    fro_norm<double>(st, R, kk * k, nrm + i + 1);
+    scale_by_inv<double>(st, R, kk * k, nrm + i + 1);
+    // core_{i+1} <- R core_{i+1}   (round.m:45-47)
+    const int64_t N1 = n[i + 1] * r[i + 2];
+    T *src = buf[0] + ofs[i + 1];
+    if (gen) gen(i + 1, src);
+    to_T(RT, R, kk * k);
+    T *dst = buf[1] + ofs[i + 1];
+    g(false, false, kk, N1, k, RT, kk, src, k, dst, kk);
+    cp[i + 1] = dst;
+    r[i + 1] = kk;  // round.m:48
+    return pending;
+  }
+
+  // ---- sweep 2 -----------------------------------------------------------
+  void svd_bond(int64_t i, double ep, int64_t rmax_i, bool force_householder, int64_t &r1,
+                bool &qr_ok, int64_t &kj, bool &wide) {
+    const int64_t kw = r[i], N = n[i] * r[i + 1];
+    wide = (N >= kw);
+    const T *W = cp[i];
+    int *fl = iflag + 4 * (d + 4);
+    double *R = Rprod;
+    kj = wide ? kw : N;
+    const int64_t mt = wide ? N : kw;  // rows of the tall orientation
+    const T *tallA = W;
+    if (wide) {
+      transpose<T>(st, W, kw, N, Tt);
+      tallA = Tt;
+    }
+    TTG_HIP(hipMemsetAsync(fl, 0, 4 * sizeof(int), st));
+    if (!force_householder) {
+      qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats);
+    } else {
+      qr_householder(tallA, mt, kj, Qs, R);
+    }
+    JacobiPlan pl = jacobi_plan(kj);
+    jacobi_svd(st, pl, R, /*transS=*/wide, Y, jsweeps, bar, jerr, kMaxSweeps);
+    sort_chop(st, Y, pl.rows, kj, pl.kpad, ep, rmax_i, sv, perm, scal_i, scal_d + 8);
+    // one synchronisation: r1, Jacobi status, QR acceptance data
+    int h_i[4] = {0, 0, 0, 0};
+    int h_fl[4] = {0, 0, 0, 0};
+    double h_st[6] = {0, 0, 0, 0, 0, 0};
+    TTG_HIP(hipMemcpyAsync(h_i, scal_i, sizeof(int), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipMemcpyAsync(h_i + 1, jerr, sizeof(int), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipMemcpyAsync(h_i + 2, jsweeps, sizeof(int), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipMemcpyAsync(h_fl, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipMemcpyAsync(h_st, stats, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipStreamSynchronize(st));
+    qr_ok = force_householder || qr_accept(h_fl, h_st);
+    r1 = h_i[0];
+    if (qr_ok) {
+      if (h_i[1] == 2) throw Error(TTG_HIP_ERROR, "Jacobi grid barrier timed out");
+      if (h_i[1] == 1) noconv = true;
+      sweeps_total += h_i[2];
+      sweeps_max = std::max<int64_t>(sweeps_max, h_i[2]);
+    }
+  }
+
+  void sweep2_bond(int64_t i, double ep, int64_t rmax_i) {
+    int64_t r1, kj;
+    bool ok, wide;
+    svd_bond(i, ep, rmax_i, false, r1, ok, kj, wide);
+    if (!ok) svd_bond(i, ep, rmax_i, true, r1, ok, kj, wide);
+    const int64_t kw = r[i], N = n[i] * r[i + 1];
+    JacobiPlan pl = jacobi_plan(kj);
+    T *core_i = other(i);
+    if (wide) {
+      // W^T = Qs R, R^T J = Y:  U S = Y[0:kw, perm],  V = Qs J[:, perm]
+      gather_cols<T>(st, Y, pl.rows, 0, kw, perm, r1, US, kw);
+      gather_cols<T>(st, Y, pl.rows, kw, kw, perm, r1, Q1, kw);  // J(:, perm(1:r1))
+      // core_i (r1 x N) = J_r^T Qs^T
+      g(true, true, r1, N, kw, Q1, kw, Qs, N, core_i, r1);
+    } else {
+      // W = Qs R, R J = Y:  U S = Qs Y[0:N, perm],  V = J[:, perm]
+      gather_cols<T>(st, Y, pl.rows, 0, N, perm, r1, Q1, N);
+      g(false, false, kw, r1, N, Qs, kw, Q1, N, US, kw);
+      gather_cols_t<T>(st, Y, pl.rows, N, N, perm, r1, core_i);
+    }
+    cp[i] = core_i;
+    // core_{i-1} <- core_{i-1} * (U S)   (round.m:81-83)
+    const int64_t mp = r[i - 1] * n[i - 1];
+    T *dst = other(i - 1);
+    g(false, false, mp, r1, kw, cp[i - 1], mp, US, kw, dst, mp);
+    cp[i - 1] = dst;
+    r[i] = r1;
+  }
+
+  // ---- driver --------------------------------------------------------------
+  void run(double eps, const int64_t *rmax, const std::function<void(int64_t, T *)> &gen) {
+    // sweep 1, speculative: one synchronisation at its end; a rejected bond
+    // is recomputed with Householder QR together with every bond after it.
+    std::vector<char> force(d, 0);
+    const std::vector<int64_t> r_in = r;
+    int64_t start = 0;
+    while (d > 1) {
+      std::vector<int64_t> pending;
+      for (int64_t i = start; i + 1 < d; ++i) {
+        TTG_HIP(hipMemsetAsync(iflag + 4 * i, 0, 4 * sizeof(int), st));
+        if (sweep1_bond(i, force[i] != 0, gen)) pending.push_back(i);
+      }
+      if (pending.empty()) break;
+      std::vector<int> hfl(4 * (d + 8));
+      std::vector<double> hst(8 * (d + 8));
+      TTG_HIP(hipMemcpyAsync(hfl.data(), iflag, hfl.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+      TTG_HIP(hipMemcpyAsync(hst.data(), bstats, hst.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      TTG_HIP(hipStreamSynchronize(st));
+      int64_t bad = -1;
+      for (int64_t i : pending)
+        if (!qr_accept(&hfl[4 * i], &hst[8 * i])) {
+          bad = i;
+          break;
+        }
+      if (bad < 0) break;
+      force[bad] = 1;
+      for (int64_t j = bad + 1; j <= d; ++j) r[j] = r_in[j];
+      start = bad;
+    }
+    const double ep = d > 1 ? eps / std::sqrt((double)(d - 1)) : INFINITY;
+    for (int64_t i = d - 1; i >= 1; --i) {
+      const int64_t rm = rmax ? rmax[i] : INT64_MAX;
+      sweep2_bond(i, ep, rm);
+    }
+  }
+};
+
+// dst/ofs of the packed output with final ranks
+static int64_t packed_size(int64_t d, const int64_t *n, const int64_t *r) {
+  int64_t s = 0;
+  for (int64_t i = 0; i < d; ++i) s += r[i] * n[i] * r[i + 1];
+  return s;
+}
+
+template <class T>
+static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
+                             const double *dcore, double eps, const int64_t *rmax, int64_t *r_out,
+                             double *dcore_out, int64_t cap, int64_t *len_out, ttg_info *info,
+                             const std::function<void(int64_t, T *)> &gen,
+                             const std::vector<int64_t> *gen_ranks) {
+  auto t0 = std::chrono::steady_clock::now();
+  validate_tt(d, n, r);
+  if (rmax)
+    for (int64_t i = 1; i < d; ++i) TTG_CHECK(rmax[i] >= 1, TTG_BAD_ARG, "rmax entries must be >= 1");
+  TTG_CHECK(eps >= 0.0 && std::isfinite(eps), TTG_BAD_ARG, "eps must be finite and >= 0");
+  hipStream_t st = ctx->stream;
+  Rounder<T> R;
+  R.ctx = ctx;
+  R.st = st;
+  R.cus = ctx->num_cus;
+  R.d = d;
+  R.n.assign(n, n + d);
+  R.r.assign(r, r + d + 1);
+  R.mixed = !std::is_same<T, double>::value;
+  std::vector<int64_t> sizes(d);
+  R.ofs.assign(d + 1, 0);
+  int64_t max_core = 1, kmax = 1;
+  for (int64_t i = 0; i < d; ++i) {
+    sizes[i] = r[i] * n[i] * r[i + 1];
+    R.ofs[i + 1] = R.ofs[i] + sizes[i];
+    max_core = std::max(max_core, sizes[i]);
+  }
+  std::vector<int64_t> rc = capped_ranks(d, n, r);
+  for (int64_t i = 0; i <= d; ++i) kmax = std::max(kmax, rc[i]);
+  for (int64_t i = 0; i < d; ++i) kmax = std::max(kmax, std::min(rc[i + 1], n[i] * rc[i + 1]));
+  TTG_CHECK(kmax <= kMaxRank, TTG_BAD_ARG, "TT ranks above 512 are not supported");
+  const int64_t cap_needed = packed_size(d, n, rc.data());
+  if (cap < cap_needed) {
+    if (len_out) *len_out = cap_needed;
+    return TTG_NEED_SPACE;
+  }
+  int64_t rsq = kmax * kmax;
+  for (int64_t i = 0; i + 1 < d; ++i) rsq = std::max(rsq, std::min(rc[i] * n[i], r[i + 1]) * r[i + 1]);
+  R.max_core = max_core;
+  R.kmax = kmax;
+  R.rsq = rsq;
+  ArenaSizer z;
+  Rounder<T>::layout(z, d, R.ofs[d], max_core, kmax, rsq);
+  z.take<int>(16);                       // nonfinite flag
+  z.take<int64_t>(3 * (d + 1));          // pack tables
+  z.take<void *>(d + 1);
+  ctx->arena.reserve(z.used + 4096);
+  ctx->arena.reset();
+  R.carve(ctx->arena, R.ofs[d]);
+  int *nf = ctx->arena.take<int>(16);
+  int64_t *dtab = ctx->arena.take<int64_t>(3 * (d + 1));
+  void **dsrc = ctx->arena.take<void *>(d + 1);
+  // input -> working buffer 0 (converted for the mixed form)
+  if (!gen) {
+    TTG_HIP(hipMemsetAsync(nf, 0, sizeof(int), st));
+    check_finite(st, dcore, R.ofs[d], nf);
+    int hnf = 0;
+    TTG_HIP(hipMemcpyAsync(&hnf, nf, sizeof(int), hipMemcpyDeviceToHost, st));
+    if constexpr (std::is_same<T, double>::value) {
+      TTG_HIP(hipMemcpyAsync(R.buf[0], dcore, R.ofs[d] * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+      convert<float, double>(st, R.buf[0], dcore, R.ofs[d]);
+    }
+    TTG_HIP(hipStreamSynchronize(st));
+    if (hnf) throw Error(TTG_NONFINITE, "input core contains NaN or Inf");
+  } else {
+    (void)gen_ranks;
+  }
+  R.cp.resize(d);
+  for (int64_t i = 0; i < d; ++i) R.cp[i] = R.buf[0] + R.ofs[i];
+  if (gen) gen(0, R.cp[0]);
+  R.run(eps, rmax, gen);
+  // finalise: nrm(1) = ||core_1||, balance, pack   (round.m:90-107)
+  const int64_t s0 = R.r[0] * R.n[0] * R.r[1];
+  fro_norm<T>(st, R.cp[0], s0, R.nrm + 0);
+  std::vector<int64_t> htab(3 * (d + 1));
+  int64_t o = 0;
+  for (int64_t i = 0; i < d; ++i) {
+    htab[i] = R.r[i] * R.n[i] * R.r[i + 1];
+    htab[(d + 1) + i] = o;
+    o += htab[i];
+  }
+  std::vector<void *> hsrc(d);
+  for (int64_t i = 0; i < d; ++i) hsrc[i] = R.cp[i];
+  TTG_HIP(hipMemcpyAsync(dtab, htab.data(), htab.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  TTG_HIP(hipMemcpyAsync(dsrc, hsrc.data(), d * sizeof(void *), hipMemcpyHostToDevice, st));
+  finalize_balance<T>(st, d, reinterpret_cast<T *const *>(dsrc), dtab, dtab + (d + 1), s0, R.nrm,
+                      dcore_out, o);
+  TTG_HIP(hipStreamSynchronize(st));
+  for (int64_t i = 0; i <= d; ++i) r_out[i] = R.r[i];
+  if (len_out) *len_out = o;
+  if (info) {
+    info->qr_fallbacks = R.fallbacks;
+    info->jacobi_sweeps = R.sweeps_total;
+    info->jacobi_maxsweeps = R.sweeps_max;
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return R.noconv ? TTG_NO_CONVERGENCE : TTG_OK;
+}
+
+ttg_status round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
+                     const double *dcore, double eps, const int64_t *rmax, int prec,
+                     int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
+                     ttg_info *info) {
+  if (prec == TTG_PREC_FP64)
+    return round_impl<double>(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info,
+                              nullptr, nullptr);
+  TTG_CHECK(prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  return round_impl<float>(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info,
+                           nullptr, nullptr);
+}
+
+ttg_status round_capacity(int64_t d, const int64_t *n, const int64_t *r, int64_t *cap) {
+  validate_tt(d, n, r);
+  std::vector<int64_t> rc = capped_ranks(d, n, r);
+  *cap = packed_size(d, n, rc.data());
+  return TTG_OK;
+}
+
+// ---- mtimes (unfused) and fused mtimes + round -----------------------------
+static void validate_mv(int64_t d, const int64_t *n, const int64_t *m, const int64_t *ra,
+                        const int64_t *rx) {
+  TTG_CHECK(d >= 1, TTG_BAD_ARG, "d must be >= 1");
+  for (int64_t i = 0; i < d; ++i) TTG_CHECK(n[i] >= 1 && m[i] >= 1, TTG_BAD_ARG, "mode sizes must be >= 1");
+  for (int64_t i = 0; i <= d; ++i) TTG_CHECK(ra[i] >= 1 && rx[i] >= 1, TTG_BAD_ARG, "ranks must be >= 1");
+}
+
+ttg_status mtimes_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
+                      const int64_t *ra, const double *dacore, const int64_t *rx,
+                      const double *dxcore, int64_t *ry, double *dycore, int64_t cap,
+                      int64_t *len_out) {
+  validate_mv(d, n, m, ra, rx);
+  std::vector<int64_t> rp(d + 1);
+  for (int64_t i = 0; i <= d; ++i) rp[i] = ra[i] * rx[i];
+  const int64_t need = packed_size(d, n, rp.data());
+  if (len_out) *len_out = need;
+  if (cap < need) return TTG_NEED_SPACE;
+  int64_t oa = 0, ox = 0, oy = 0;
+  for (int64_t i = 0; i < d; ++i) {
+    mtimes_core<double>(ctx->stream, ra[i], n[i], m[i], ra[i + 1], dacore + oa, rx[i], rx[i + 1],
+                        dxcore + ox, dycore + oy);
+    oa += ra[i] * n[i] * m[i] * ra[i + 1];
+    ox += rx[i] * m[i] * rx[i + 1];
+    oy += rp[i] * n[i] * rp[i + 1];
+  }
+  TTG_HIP(hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i <= d; ++i) ry[i] = rp[i];
+  return TTG_OK;
+}
+
+ttg_status mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
+                            const int64_t *ra, const double *dacore, const int64_t *rx,
+                            const double *dxcore, double eps, const int64_t *rmax, int prec,
+                            int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
+                            ttg_info *info) {
+  validate_mv(d, n, m, ra, rx);
+  std::vector<int64_t> rp(d + 1), oa(d + 1, 0), ox(d + 1, 0);
+  for (int64_t i = 0; i <= d; ++i) rp[i] = ra[i] * rx[i];
+  for (int64_t i = 0; i < d; ++i) {
+    oa[i + 1] = oa[i] + ra[i] * n[i] * m[i] * ra[i + 1];
+    ox[i + 1] = ox[i] + rx[i] * m[i] * rx[i + 1];
+  }
+  hipStream_t st = ctx->stream;
+  std::vector<int64_t> nn(n, n + d), mm(m, m + d), raa(ra, ra + d + 1), rxx(rx, rx + d + 1);
+  if (prec == TTG_PREC_FP64) {
+    std::function<void(int64_t, double *)> gen = [&](int64_t i, double *dst) {
+      mtimes_core<double>(st, raa[i], nn[i], mm[i], raa[i + 1], dacore + oa[i], rxx[i], rxx[i + 1],
+                          dxcore + ox[i], dst);
+    };
+    return round_impl<double>(ctx, d, n, rp.data(), nullptr, eps, rmax, r_out, dcore_out, cap,
+                              len_out, info, gen, &rp);
+  }
+  TTG_CHECK(prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  std::function<void(int64_t, float *)> gen = [&](int64_t i, float *dst) {
+    mtimes_core<float>(st, raa[i], nn[i], mm[i], raa[i + 1], dacore + oa[i], rxx[i], rxx[i + 1],
+                       dxcore + ox[i], dst);
+  };
+  return round_impl<float>(ctx, d, n, rp.data(), nullptr, eps, rmax, r_out, dcore_out, cap, len_out,
+                           info, gen, &rp);
+}
+
+}  // namespace ttg

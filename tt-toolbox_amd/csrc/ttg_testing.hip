@@ -1,0 +1,131 @@
+// Kernel-level test hooks (declared in include/ttgpu_testing.h).  Host
+// arrays in and out; each hook runs exactly one building block so that the
+// GPU tests can compare it with numpy/LAPACK.  Not part of the TT API.
+#include "ttg_internal.h"
+#include <algorithm>
+
+namespace ttg {
+void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
+                 double *R, int *flag, double *stats);
+void tri_inverse(hipStream_t s, int64_t k, const double *R, double *X);
+void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
+                     const float *A, int64_t lda, const float *B, int64_t ldb, double *C,
+                     int64_t ldc, float *ws, int64_t wse, int cus);
+}  // namespace ttg
+using namespace ttg;
+
+namespace {
+template <class T> T *up(const T *h, size_t n, hipStream_t s) {
+  T *d = nullptr;
+  TTG_HIP(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)));
+  if (h && n) TTG_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return d;
+}
+template <class T> void down(T *h, const T *d, size_t n, hipStream_t s) {
+  if (n) TTG_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+struct Frees {
+  std::vector<void *> p;
+  ~Frees() {
+    for (auto q : p) (void)hipFree(q);
+  }
+  template <class T> T *add(T *q) {
+    p.push_back((void *)q);
+    return q;
+  }
+};
+}  // namespace
+
+#define GUARD(ctx, ...)                                    \
+  if (!ctx) return TTG_BAD_ARG;                            \
+  try {                                                    \
+    TTG_HIP(hipSetDevice(ctx->device));                    \
+    __VA_ARGS__;                                           \
+    TTG_HIP(hipStreamSynchronize(ctx->stream));            \
+    return TTG_OK;                                         \
+  } catch (const ttg::Error &e) {                          \
+    ctx->last_error = e.what();                            \
+    return e.status;                                       \
+  }
+
+extern "C" {
+
+// C = op(A) op(B), prec 0 = FP64, 1 = FP32 operands (A, B, C given as double)
+ttg_status ttg_test_gemm(ttg_ctx *ctx, int prec, int ta, int tb, int64_t M, int64_t N, int64_t K,
+                         const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                         int64_t ldc) {
+  GUARD(ctx, {
+    hipStream_t s = ctx->stream;
+    Frees f;
+    const int64_t na = ta ? lda * M : lda * K, nb = tb ? ldb * K : ldb * N, nc = ldc * N;
+    const int64_t wse = 32 * M * N + 1024;
+    if (prec == 0) {
+      double *dA = f.add(up(A, na, s)), *dB = f.add(up(B, nb, s)), *dC = f.add(up<double>(nullptr, nc, s));
+      double *ws = f.add(up<double>(nullptr, wse, s));
+      TTG_HIP(hipMemsetAsync(dC, 0, nc * sizeof(double), s));
+      gemm<double>(s, ta, tb, M, N, K, 1.0, dA, lda, dB, ldb, 0.0, dC, ldc, ws, wse, ctx->num_cus);
+      down(C, dC, nc, s);
+    } else {
+      double *dA = f.add(up(A, na, s)), *dB = f.add(up(B, nb, s)), *dC = f.add(up<double>(nullptr, nc, s));
+      float *fA = f.add(up<float>(nullptr, na, s)), *fB = f.add(up<float>(nullptr, nb, s));
+      float *ws = f.add(up<float>(nullptr, wse, s));
+      convert<float, double>(s, fA, dA, na);
+      convert<float, double>(s, fB, dB, nb);
+      TTG_HIP(hipMemsetAsync(dC, 0, nc * sizeof(double), s));
+      gemm_f32_f64out(s, ta, tb, M, N, K, fA, lda, fB, ldb, dC, ldc, ws, wse, ctx->num_cus);
+      down(C, dC, nc, s);
+    }
+  })
+}
+
+ttg_status ttg_test_householder(ttg_ctx *ctx, int64_t m, int64_t n, const double *A, double *Q,
+                                double *R) {
+  GUARD(ctx, {
+    hipStream_t s = ctx->stream;
+    Frees f;
+    const int64_t kq = std::min(m, n);
+    double *dA = f.add(up(A, m * n, s)), *dQ = f.add(up<double>(nullptr, m * kq, s));
+    double *dR = f.add(up<double>(nullptr, kq * n, s)), *tau = f.add(up<double>(nullptr, kq + 1, s));
+    householder_qr(s, m, n, dA, dQ, dR, tau);
+    down(Q, dQ, m * kq, s);
+    down(R, dR, kq * n, s);
+  })
+}
+
+ttg_status ttg_test_chol(ttg_ctx *ctx, int64_t k, const double *G, double shift_rel, double *R,
+                         double *Rinv, int *flag, double *stats) {
+  GUARD(ctx, {
+    hipStream_t s = ctx->stream;
+    Frees f;
+    double *dG = f.add(up(G, k * k, s)), *dR = f.add(up<double>(nullptr, k * k, s));
+    double *dX = f.add(up<double>(nullptr, k * k, s)), *dst = f.add(up<double>(nullptr, 8, s));
+    int *dfl = f.add(up<int>(nullptr, 4, s));
+    chol_factor(s, k, dG, k, shift_rel, dR, dfl, dst);
+    tri_inverse(s, k, dR, dX);
+    down(R, dR, k * k, s);
+    down(Rinv, dX, k * k, s);
+    down(flag, dfl, 1, s);
+    down(stats, dst, 2, s);
+  })
+}
+
+// Jacobi SVD of S (k x k); returns Y (2k x kpad, column-major), sweeps, kpad.
+ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS, double *Y,
+                           int64_t ycap, int64_t *kpad_out, int *sweeps, int *err) {
+  GUARD(ctx, {
+    hipStream_t s = ctx->stream;
+    Frees f;
+    JacobiPlan pl = jacobi_plan(k);
+    TTG_CHECK(pl.rows * pl.kpad <= ycap, TTG_NEED_SPACE, "Y capacity");
+    double *dS = f.add(up(S, k * k, s)), *dY = f.add(up<double>(nullptr, pl.rows * pl.kpad, s));
+    int *dsw = f.add(up<int>(nullptr, 4, s)), *derr = f.add(up<int>(nullptr, 4, s));
+    unsigned *bar = f.add(up<unsigned>(nullptr, 128, s));
+    jacobi_svd(s, pl, dS, transS != 0, dY, dsw, bar, derr, 60);
+    down(Y, dY, pl.rows * pl.kpad, s);
+    down(sweeps, dsw, 1, s);
+    down(err, derr, 1, s);
+    *kpad_out = pl.kpad;
+  })
+}
+
+}  // extern "C"

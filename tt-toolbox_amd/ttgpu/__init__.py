@@ -1,0 +1,17 @@
+"""ttgpu: MI355X (gfx950) TT rounding and TT-matrix x TT-tensor contraction.
+
+Drop-in for TT-Toolbox's ``round(X,eps,rmax)`` and ``A*x`` on the
+``tt_tensor``/``tt_matrix`` core/ps/r/n layout.  The compute path is
+libttgpu.so (hand-written HIP kernels behind a C ABI, include/ttgpu.h).
+"""
+from ._lib import (LIB_PATH, EXPORTS, Context, TTGError, TTGInfo, default_context, load,
+                   PREC_FP32, PREC_FP64, STATUS_NAMES)
+from .tt import (TTTensor, TTMatrix, capacity, mtimes, mtimes_round, mtimes_round_dev, round,
+                 round_batched_dev, round_dev, round_matrix, torch_context)
+
+__all__ = [
+    "LIB_PATH", "EXPORTS", "Context", "TTGError", "TTGInfo", "default_context", "load",
+    "PREC_FP32", "PREC_FP64", "STATUS_NAMES", "TTTensor", "TTMatrix", "capacity", "mtimes",
+    "mtimes_round", "mtimes_round_dev", "round", "round_batched_dev", "round_dev",
+    "round_matrix", "torch_context",
+]
