@@ -16,7 +16,9 @@ ttg_status ttg_test_gemm(ttg_ctx *ctx, int prec, int ta, int tb, int64_t M, int6
 ttg_status ttg_test_householder(ttg_ctx *ctx, int64_t m, int64_t n, const double *A, double *Q,
                                 double *R);
 ttg_status ttg_test_chol(ttg_ctx *ctx, int64_t k, const double *G, double shift_rel, double *R,
-                         double *Rinv, int *flag, double *stats);
+                         int *flag, double *stats);
+ttg_status ttg_test_trsm(ttg_ctx *ctx, int prec, int64_t m, int64_t k, const double *A,
+                         const double *R, double *X);
 ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS, double *Y,
                            int64_t ycap, int64_t *kpad_out, int *sweeps, int *err);
 #ifdef __cplusplus

@@ -48,29 +48,51 @@ def test_householder(gpu_ctx, m, n, rank):
     np.testing.assert_allclose(np.abs(np.diag(R)[:rank]), np.abs(np.diag(r2)[:rank]), rtol=1e-10)
 
 
-@pytest.mark.parametrize("k", [5, 40, 100, 256])
-def test_cholesky_and_inverse(gpu_ctx, k):
-    rng = np.random.default_rng(k)
-    A = rng.standard_normal((3 * k, k))
-    G = np.asfortranarray(A.T @ A)
+def _chol(gpu_ctx, G, shift=0.0):
+    k = G.shape[0]
+    G = np.asfortranarray(G)
     R = np.zeros((k, k), order="F")
-    X = np.zeros((k, k), order="F")
     flag = np.zeros(1, dtype=np.int32)
     stats = np.zeros(2)
-    gpu_ctx.check(gpu_ctx.lib.ttg_test_chol(gpu_ctx.h, k, f64p(G), 0.0, f64p(R), f64p(X), f64p(flag), f64p(stats)))
-    assert flag[0] == 0
+    gpu_ctx.check(gpu_ctx.lib.ttg_test_chol(gpu_ctx.h, k, f64p(G), shift, f64p(R), f64p(flag), f64p(stats)))
+    return R, int(flag[0]), stats
+
+
+@pytest.mark.parametrize("k", [5, 40, 100, 256, 320])
+def test_cholesky(gpu_ctx, k):
+    rng = np.random.default_rng(k)
+    A = rng.standard_normal((3 * k, k))
+    G = A.T @ A
+    R, flag, stats = _chol(gpu_ctx, G)
+    assert flag == 0
     ref = np.linalg.cholesky(G).T
     np.testing.assert_allclose(R, ref, rtol=1e-10, atol=1e-10 * np.max(np.abs(ref)))
-    np.testing.assert_allclose(X @ R, np.eye(k), atol=1e-10)
+    assert np.max(np.abs(np.tril(R, -1))) == 0.0
+    np.testing.assert_allclose(stats[0], np.max(np.abs(np.diag(ref))) / np.min(np.abs(np.diag(ref))), rtol=1e-8)
 
 
 def test_cholesky_flags_indefinite(gpu_ctx):
-    k = 6
-    G = np.asfortranarray(np.diag([4.0, 1.0, -1.0, 2.0, 1.0, 1.0]))
-    R = np.zeros((k, k), order="F"); X = np.zeros((k, k), order="F")
-    flag = np.zeros(1, dtype=np.int32); stats = np.zeros(2)
-    gpu_ctx.check(gpu_ctx.lib.ttg_test_chol(gpu_ctx.h, k, f64p(G), 0.0, f64p(R), f64p(X), f64p(flag), f64p(stats)))
-    assert flag[0] == 3
+    G = np.diag([4.0, 1.0, -1.0, 2.0, 1.0, 1.0])
+    _, flag, _ = _chol(gpu_ctx, G)
+    assert flag == 3
+
+
+@pytest.mark.parametrize("prec,m,k,logcond", [(0, 2048, 256, 8), (0, 300, 40, 12), (1, 2048, 256, 3), (0, 1000, 320, 6), (0, 50, 7, 10)])
+def test_trsm_backward_stable(gpu_ctx, prec, m, k, logcond):
+    """X = A R^{-1}: the residual ||X R - A|| stays O(u ||X|| ||R||) even for ill-conditioned R."""
+    rng = np.random.default_rng(m + k)
+    U, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    Vt, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    B = (U * np.logspace(0, -logcond, k)) @ Vt
+    R = np.linalg.qr(B)[1]
+    R = np.asfortranarray(R * np.sign(np.diag(R))[:, None])
+    A = np.asfortranarray(rng.standard_normal((m, k)) @ R)
+    X = np.zeros((m, k), order="F")
+    gpu_ctx.check(gpu_ctx.lib.ttg_test_trsm(gpu_ctx.h, prec, m, k, f64p(A), f64p(R), f64p(X)))
+    u = 1.1e-16 if prec == 0 else 6e-8
+    res = np.linalg.norm(X @ R - A) / (np.linalg.norm(X) * np.linalg.norm(R))
+    print("trsm residual", res, "cond", np.linalg.cond(R), flush=True)
+    assert res <= 10 * u * np.sqrt(k)
 
 
 @pytest.mark.parametrize("k,trans,deficient", [(6, 0, 0), (40, 1, 20), (64, 0, 0), (128, 1, 0), (256, 0, 100)])

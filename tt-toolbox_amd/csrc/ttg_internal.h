@@ -135,11 +135,15 @@ template <class T> void scale_by_inv(hipStream_t s, T *x, int64_t count, const d
 void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, double *R,
                     double *tau);
 
-// One-workgroup Cholesky G = R^T R (upper R) and its inverse, FP64, k x k.
-// G is read from the upper triangle.  flag[0] = 0 ok, j+1 if pivot j failed.
-// cond[0] = ||R||_F * ||R^{-1}||_F.  shift is added to the diagonal first.
-void chol_inv(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-              double *R, double *Rinv, int *flag, double *cond);
+// One-workgroup Cholesky G + shift = R^T R (upper R), FP64, k x k (k <= 512).
+// G is read from its upper triangle; shift = shift_rel * trace(G) / k.
+// flag[0] = 0 ok, j+1 if pivot j failed; stats[0] = diag spread, stats[1] = max|R - I|.
+void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
+                 double *R, int *flag, double *stats);
+// X (m x k) = A R^{-1}, R upper triangular (FP64), blocked substitution.
+template <class T>
+void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t lda, const double *R,
+                      T *X, int64_t ldx);
 
 // One-sided block Jacobi SVD on the stacked matrix Y = [S; I] (FP64).
 // Setup from S (k x k, column-major, optionally transposed) then rotate
@@ -152,8 +156,10 @@ struct JacobiPlan {
 };
 JacobiPlan jacobi_plan(int64_t k);
 size_t jacobi_ws_bytes(const JacobiPlan &pl);
+// ws: jacobi_polar_ws_elems(pl) doubles of scratch for the final polar correction.
+size_t jacobi_polar_ws_elems(const JacobiPlan &pl);
 void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
-                int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps);
+                int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws);
 
 // Sort the column norms of Y's top k rows (descending) and apply my_chop2.
 // sv[k], perm[k] written; out_i[0] = r1 (after rmax cap), out_d[0] = ||s||.

@@ -423,395 +423,6 @@ void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, d
 }
 
 // ---------------------------------------------------------------------------
-// Cholesky G + shift*trace(G)/k*I = R^T R (upper R), one workgroup, nb = 32.
-// stats[0] = max|r_jj| / min|r_jj|; stats[1] = max_ij |R_ij - delta_ij|.
-// ---------------------------------------------------------------------------
-constexpr int CNB = 32;
-constexpr int CHOL_MAXK = 512;
-__global__ __launch_bounds__(1024) void k_chol(int k, const double *__restrict__ G, long ldg,
-                                               double shift_rel, double *__restrict__ R,
-                                               int *flag, double *stats) {
-  __shared__ double D[CNB][CNB + 1];
-  __shared__ double red[16];
-  __shared__ int sh_fail;
-  extern __shared__ double Ps[];  // CNB x (k) panel
-  const int tid = threadIdx.x;
-  double tr = 0.0;
-  for (int j = tid; j < k; j += blockDim.x) tr += G[j + (long)j * ldg];
-  tr = block_sum(tr, red);
-  const double shift = shift_rel * tr / (k > 0 ? k : 1);
-  for (long idx = tid; idx < (long)k * k; idx += blockDim.x) {
-    const int i = (int)(idx % k), j = (int)(idx / k);
-    double v = 0.0;
-    if (i <= j) v = G[i + (long)j * ldg] + (i == j ? shift : 0.0);
-    R[idx] = v;
-  }
-  if (tid == 0) sh_fail = 0;
-  __syncthreads();
-  for (int J0 = 0; J0 < k; J0 += CNB) {
-    const int jb = min(CNB, k - J0);
-    for (int idx = tid; idx < CNB * CNB; idx += blockDim.x) {
-      const int i = idx % CNB, j = idx / CNB;
-      D[i][j] = (i < jb && j < jb && i <= j) ? R[(J0 + i) + (long)(J0 + j) * k] : 0.0;
-    }
-    __syncthreads();
-    for (int c = 0; c < jb; ++c) {
-      if (tid == 0) {
-        const double p = D[c][c];
-        if (!(p > 0.0) || !isfinite(p)) {
-          sh_fail = J0 + c + 1;
-        } else {
-          D[c][c] = sqrt(p);
-        }
-      }
-      __syncthreads();
-      if (sh_fail) break;
-      const double dc = D[c][c];
-      if (tid > c && tid < jb) D[c][tid] /= dc;
-      __syncthreads();
-      for (int idx = tid; idx < CNB * CNB; idx += blockDim.x) {
-        const int i = idx % CNB, j = idx / CNB;
-        if (i > c && i <= j && j < jb) D[i][j] -= D[c][i] * D[c][j];
-      }
-      __syncthreads();
-    }
-    if (sh_fail) break;
-    for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
-      const int i = idx % jb, j = idx / jb;
-      if (i <= j) R[(J0 + i) + (long)(J0 + j) * k] = D[i][j];
-    }
-    const int w = k - J0 - jb;
-    if (w > 0) {
-      // panel: solve D^T P = R[J0:J0+jb, J0+jb:k]  (forward substitution per column)
-      for (int c = tid; c < w; c += blockDim.x) {
-        const long col = (long)(J0 + jb + c) * k;
-        for (int t = 0; t < jb; ++t) {
-          double v = R[J0 + t + col];
-          for (int l = 0; l < t; ++l) v -= D[l][t] * Ps[l * w + c];
-          Ps[t * w + c] = v / D[t][t];
-        }
-      }
-      __syncthreads();
-      for (int idx = tid; idx < jb * w; idx += blockDim.x) {
-        const int t = idx % jb, c = idx / jb;
-        R[J0 + t + (long)(J0 + jb + c) * k] = Ps[t * w + c];
-      }
-      // trailing update of the upper triangle
-      const long tri = (long)w * (w + 1) / 2;
-      for (long idx = tid; idx < tri; idx += blockDim.x) {
-        // map idx -> (i, j) with i <= j, column-major over the upper triangle
-        int j = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
-        while ((long)j * (j + 1) / 2 > idx) --j;
-        while ((long)(j + 1) * (j + 2) / 2 <= idx) ++j;
-        const int i = (int)(idx - (long)j * (j + 1) / 2);
-        double s = 0.0;
-        for (int t = 0; t < jb; ++t) s += Ps[t * w + i] * Ps[t * w + j];
-        R[(J0 + jb + i) + (long)(J0 + jb + j) * k] -= s;
-      }
-    }
-    __syncthreads();
-  }
-  __syncthreads();
-  double dmax = 0.0, dmin = INFINITY, dev = 0.0;
-  if (!sh_fail) {
-    for (int j = tid; j < k; j += blockDim.x) {
-      const double v = fabs(R[j + (long)j * k]);
-      dmax = fmax(dmax, v);
-      dmin = fmin(dmin, v);
-    }
-    for (long idx = tid; idx < (long)k * k; idx += blockDim.x) {
-      const int i = (int)(idx % k), j = (int)(idx / k);
-      dev = fmax(dev, fabs(R[idx] - (i == j ? 1.0 : 0.0)));
-    }
-  }
-  dmax = block_max(dmax, red);
-  dmin = -block_max(-dmin, red);
-  dev = block_max(dev, red);
-  if (tid == 0) {
-    flag[0] = sh_fail;
-    stats[0] = sh_fail ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
-    stats[1] = sh_fail ? INFINITY : dev;
-  }
-}
-
-// Upper-triangular inverse: one wave per column c solves R x = e_c by
-// column-oriented back substitution (x held in registers, 8 rows per lane).
-constexpr int TRI_ROWS_PER_LANE = CHOL_MAXK / 64;
-__global__ __launch_bounds__(256) void k_trtri(int k, const double *__restrict__ R,
-                                               double *__restrict__ X) {
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (c >= k) return;
-  double x[TRI_ROWS_PER_LANE];
-#pragma unroll
-  for (int q = 0; q < TRI_ROWS_PER_LANE; ++q) {
-    const int i = lane + 64 * q;
-    x[q] = (i == c) ? 1.0 : 0.0;
-  }
-  for (int j = c; j >= 0; --j) {
-    const double *rj = R + (long)j * k;
-    const double rjj = rj[j];
-    // broadcast x[j] from its owner lane
-    const int q = j >> 6, owner = j & 63;
-    double bj = 0.0;
-#pragma unroll
-    for (int qq = 0; qq < TRI_ROWS_PER_LANE; ++qq)
-      if (qq == q) bj = x[qq];
-    bj = __shfl(bj, owner, 64);
-    const double xj = bj / rjj;
-#pragma unroll
-    for (int qq = 0; qq < TRI_ROWS_PER_LANE; ++qq) {
-      const int i = lane + 64 * qq;
-      if (i < j) x[qq] -= xj * rj[i];
-      else if (i == j) x[qq] = xj;
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < TRI_ROWS_PER_LANE; ++q) {
-    const int i = lane + 64 * q;
-    if (i < k) X[i + (long)c * k] = (i <= c) ? x[q] : 0.0;
-  }
-}
-
-void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats) {
-  TTG_CHECK(k <= CHOL_MAXK, TTG_BAD_ARG, "rank above 512 not supported by the Cholesky kernel");
-  const size_t lds = (size_t)CNB * (size_t)std::max<int64_t>(k, 1) * sizeof(double);
-  ProfScope prof(KC_CHOL, s, (double)k * k * k / 3.0, 8.0 * 2.0 * (double)k * k);
-  TTG_HIP(hipFuncSetAttribute((const void *)k_chol, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_chol, dim3(1), dim3(1024), lds, s, (int)k, G, (long)ldg, shift_rel, R,
-                     flag, stats);
-  TTG_HIP(hipGetLastError());
-}
-void tri_inverse(hipStream_t s, int64_t k, const double *R, double *X) {
-  TTG_CHECK(k <= CHOL_MAXK, TTG_BAD_ARG, "rank above 512 not supported by the triangular inverse");
-  const int waves = 4;
-  ProfScope prof(KC_TRTRI, s, (double)k * k * k / 3.0, 8.0 * 2.0 * (double)k * k);
-  hipLaunchKernelGGL(k_trtri, dim3((unsigned)((k + waves - 1) / waves)), dim3(64 * waves), 0, s,
-                     (int)k, R, X);
-  TTG_HIP(hipGetLastError());
-}
-
-// ---------------------------------------------------------------------------
-// One-sided block Jacobi SVD (cooperative; p workgroups, 2p column blocks)
-// ---------------------------------------------------------------------------
-// Y is rows x kpad (rows = 2k): rows [0,k) hold the matrix S being
-// orthogonalised, rows [k,2k) accumulate the rotations (start = identity).
-// Each outer step gives workgroup w the block pair (round-robin "circle"
-// schedule over 2p blocks); the 2b columns sit in LDS while one inner sweep
-// of scalar rotations runs (b pairs per round, one wave per pair).  A sweep
-// is 2p-1 outer steps; iteration stops after a sweep without rotations.
-__device__ __forceinline__ void circle_pair(int players, int round, int idx, int &a, int &b) {
-  const int P1 = players - 1;
-  if (idx == 0) {
-    a = P1;
-    b = round;
-  } else {
-    a = (round + idx) % P1;
-    b = (round - idx + P1) % P1;
-  }
-}
-
-// Software grid barrier for a cooperative launch.  bar[0] = arrivals,
-// bar[1] = generation.  Bounded spin: after ~4 s a timeout flag is raised
-// and every workgroup leaves the kernel at its next check.
-__device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen, int *err) {
-  __syncthreads();
-  __shared__ int sh_ok;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned target = gen + 1;
-    const unsigned prev = atomicAdd(&bar[0], 1u);
-    int ok = 1;
-    if (prev == nblocks - 1) {
-      atomicExch(&bar[0], 0u);
-      __threadfence();
-      atomicExch(&bar[1], target);
-    } else {
-      const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != target) {
-        __builtin_amdgcn_s_sleep(2);
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          ok = 0;
-          break;
-        }
-        if (__builtin_amdgcn_s_memrealtime() - t0 > 400000000ull) {  // 100 MHz clock: 4 s
-          atomicExch(err, 2);
-          ok = 0;
-          break;
-        }
-      }
-    }
-    __threadfence();
-    gen = target;
-    sh_ok = ok;
-  }
-  __syncthreads();
-  return sh_ok != 0;
-}
-
-__global__ __launch_bounds__(1024) void k_jacobi_setup(int k, int kpad, const double *__restrict__ S,
-                                                       int transS, double *__restrict__ Y) {
-  const int rows = 2 * k;
-  const long total = (long)rows * kpad;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
-       idx += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % rows), j = (int)(idx / rows);
-    double v = 0.0;
-    if (j < k) {
-      if (i < k) v = transS ? S[j + (long)i * k] : S[i + (long)j * k];
-      else v = (i - k == j) ? 1.0 : 0.0;
-    }
-    Y[idx] = v;
-  }
-}
-
-__global__ __launch_bounds__(1024) void k_jacobi(int k, int b, int p, double *__restrict__ Y,
-                                                 double tol, int max_sweeps, int *sweeps_out,
-                                                 unsigned *bar, int *err) {
-  extern __shared__ double L[];  // 2b columns x rows
-  const int rows = 2 * k;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
-  const int w = blockIdx.x;
-  const int nblk = 2 * p;
-  const int lc = 2 * b;
-  __shared__ int sh_rot;
-  __shared__ int sh_total;
-  unsigned gen = 0;
-  int sweep = 0;
-  bool converged = false;
-  for (sweep = 0; sweep < max_sweeps && !converged; ++sweep) {
-    if (tid == 0) sh_rot = 0;
-    __syncthreads();
-    const int outer = (nblk > 1) ? nblk - 1 : 1;
-    for (int step = 0; step < outer; ++step) {
-      int bA = 0, bB = 1;
-      if (nblk > 1) circle_pair(nblk, step, w, bA, bB);
-      // load the two blocks
-      for (long idx = tid; idx < (long)lc * rows; idx += blockDim.x) {
-        const int c = (int)(idx / rows), i = (int)(idx % rows);
-        const int gcol = (c < b) ? bA * b + c : bB * b + (c - b);
-        L[idx] = Y[i + (long)gcol * rows];
-      }
-      __syncthreads();
-      // one inner sweep over the 2b local columns
-      for (int t = 0; t < lc - 1; ++t) {
-        for (int pr = wv; pr < b; pr += nw) {
-          int pa, pb;
-          circle_pair(lc, t, pr, pa, pb);
-          double *cp = L + (long)pa * rows;
-          double *cq = L + (long)pb * rows;
-          double al = 0.0, be = 0.0, ga = 0.0;
-          for (int i = lane; i < k; i += 64) {
-            const double x = cp[i], y = cq[i];
-            al += x * x;
-            be += y * y;
-            ga += x * y;
-          }
-          al = wave_sum(al);
-          be = wave_sum(be);
-          ga = wave_sum(ga);
-          if (fabs(ga) > tol * sqrt(al) * sqrt(be)) {
-            const double zeta = (be - al) / (2.0 * ga);
-            double tt;
-            if (fabs(zeta) > 1e150) tt = 0.5 / zeta;
-            else tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-            const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
-            for (int i = lane; i < rows; i += 64) {
-              const double x = cp[i], y = cq[i];
-              cp[i] = cs * x - sn * y;
-              cq[i] = sn * x + cs * y;
-            }
-            if (lane == 0) atomicAdd(&sh_rot, 1);
-          }
-        }
-        __syncthreads();
-      }
-      // store back
-      for (long idx = tid; idx < (long)lc * rows; idx += blockDim.x) {
-        const int c = (int)(idx / rows), i = (int)(idx % rows);
-        const int gcol = (c < b) ? bA * b + c : bB * b + (c - b);
-        Y[i + (long)gcol * rows] = L[idx];
-      }
-      if (p > 1) {
-        if (!grid_sync(bar, p, gen, err)) return;
-      } else {
-        __syncthreads();
-      }
-    }
-    // sweep-level convergence: sum rotation counts over workgroups
-    if (p > 1) {
-      if (tid == 0) atomicAdd(&bar[2 + sweep], (unsigned)sh_rot);
-      if (!grid_sync(bar, p, gen, err)) return;
-      if (tid == 0)
-        sh_total = (int)__hip_atomic_load(&bar[2 + sweep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      __syncthreads();
-    } else {
-      if (tid == 0) sh_total = sh_rot;
-      __syncthreads();
-    }
-    converged = (sh_total == 0);
-  }
-  if (w == 0 && tid == 0) {
-    sweeps_out[0] = sweep;
-    if (!converged) atomicExch(err, 1);
-  }
-}
-
-JacobiPlan jacobi_plan(int64_t k) {
-  JacobiPlan pl;
-  pl.k = k;
-  pl.rows = 2 * k;
-  const size_t budget = 144 * 1024;
-  // largest block size that fits LDS, capped at 16 columns per wave-pair slot
-  int64_t bmax = (int64_t)(budget / (sizeof(double) * 2 * (size_t)pl.rows));
-  if (bmax < 1) bmax = 1;
-  int64_t b;
-  if (2 * bmax >= k) {
-    b = (k + 1) / 2;  // single workgroup
-    if (b < 1) b = 1;
-  } else {
-    b = std::min<int64_t>(bmax, 16);
-  }
-  pl.b = (int)b;
-  pl.p = (int)((k + 2 * b - 1) / (2 * b));
-  if (pl.p < 1) pl.p = 1;
-  pl.kpad = 2 * (int64_t)pl.b * pl.p;
-  return pl;
-}
-size_t jacobi_ws_bytes(const JacobiPlan &pl) { return (size_t)pl.rows * pl.kpad * sizeof(double); }
-
-void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
-                int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps) {
-  const int k = (int)pl.k, kpad = (int)pl.kpad;
-  {
-    const long total = (long)pl.rows * kpad;
-    const int blocks = (int)std::min<long>((total + 1023) / 1024, 2048);
-    hipLaunchKernelGGL(k_jacobi_setup, dim3(blocks), dim3(1024), 0, s, k, kpad, S, transS ? 1 : 0, Y);
-    TTG_HIP(hipGetLastError());
-  }
-  TTG_HIP(hipMemsetAsync(barrier_dev, 0, sizeof(unsigned) * (2 + max_sweeps), s));
-  TTG_HIP(hipMemsetAsync(err_dev, 0, sizeof(int), s));
-  // algorithmic flops of the small SVD (R-SVD of the k x k factor, GVL Fig. 8.6.1: 20k^3)
-  ProfScope prof(KC_JACOBI, s, 20.0 * (double)k * k * k, 8.0 * 4.0 * (double)k * k);
-  const double tol = sqrt((double)std::max(k, 1)) * DBL_EPSILON;
-  const size_t lds = (size_t)2 * pl.b * pl.rows * sizeof(double);
-  int b = pl.b, p = pl.p;
-  int kk = k;
-  double *Yp = Y;
-  int ms = max_sweeps;
-  void *args[] = {&kk, &b, &p, &Yp, (void *)&tol, &ms, &sweeps_dev, &barrier_dev, &err_dev};
-  TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  if (p > 1) {
-    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_jacobi, dim3(p), dim3(1024), args, lds, s));
-  } else {
-    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(1024), lds, s, kk, b, p, Yp, tol, ms, sweeps_dev,
-                       barrier_dev, err_dev);
-    TTG_HIP(hipGetLastError());
-  }
-}
-
-// ---------------------------------------------------------------------------
 // singular values: column norms, descending rank sort, my_chop2
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_sort_chop(const double *__restrict__ Y, long ldy, int k,

@@ -29,9 +29,6 @@ namespace ttg {
 void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
                      const float *A, int64_t lda, const float *B, int64_t ldb, double *C,
                      int64_t ldc, float *ws, int64_t wse, int cus);
-void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats);
-void tri_inverse(hipStream_t s, int64_t k, const double *R, double *X);
 
 static constexpr int kMaxRank = 512;
 static constexpr int kMaxSweeps = 60;
@@ -62,7 +59,7 @@ template <class T> struct Rounder {
   T *buf[3];
   std::vector<T *> cp;
   // scratch
-  double *G, *R1, *R2, *R3, *Rinv, *Rprod, *Rtmp, *Y, *sv, *nrm, *stats, *bstats;
+  double *G, *R1, *R2, *R3, *Rinv, *Rprod, *Rtmp, *Y, *sv, *nrm, *stats, *bstats, *Yws;
   T *RT, *Q1, *Q2, *Tt, *Qs, *US, *gws;
   int64_t gws_elems;
   double *hA, *hQ, *htau;
@@ -87,6 +84,7 @@ template <class T> struct Rounder {
     for (int b = 0; b < 3; ++b) a.template take<T>(total);
     for (int q = 0; q < 7; ++q) a.template take<double>(rsq);
     a.template take<double>(jac_elems(kmax));
+    a.template take<double>(jac_elems(kmax) + (kmax + 64) * (kmax + 64));  // polar correction
     a.template take<double>(kmax + 64);      // sv
     a.template take<double>(d + 8);          // nrm (+ balance scales)
     a.template take<double>(64);             // stats
@@ -116,6 +114,7 @@ template <class T> struct Rounder {
     Rprod = a.take<double>(rsq);
     Rtmp = a.take<double>(rsq);
     Y = a.take<double>(jac_elems(kmax));
+    Yws = a.take<double>(jac_elems(kmax) + (kmax + 64) * (kmax + 64));
     sv = a.take<double>(kmax + 64);
     nrm = a.take<double>(d + 8);
     stats = a.take<double>(64);
@@ -161,14 +160,12 @@ template <class T> struct Rounder {
     }
   }
 
-  // One Cholesky-QR pass: Qout = A R^{-1}, R = chol(A^T A (+ shift)).
+  // One Cholesky-QR pass: R = chol(A^T A (+ shift)), Qout = A R^{-1} by triangular solve.
   void cholqr_pass(const T *A, int64_t m, int64_t k, double shift_rel, T *Qout, double *Rout,
                    int *flag, double *st2) {
     gram(A, m, k, G);
     chol_factor(st, k, G, k, shift_rel, Rout, flag, st2);
-    tri_inverse(st, k, Rout, Rinv);
-    to_T(RT, Rinv, k * k);
-    g(false, false, m, k, k, A, m, RT, k, Qout, m);
+    trsm_right_upper<T>(st, m, k, A, m, Rout, Qout, m);
   }
 
   // Tall QR (m >= k): Qout (m x k, T), Rout (k x k, FP64).  flags[0..2] and
@@ -270,7 +267,7 @@ template <class T> struct Rounder {
       qr_householder(tallA, mt, kj, Qs, R);
     }
     JacobiPlan pl = jacobi_plan(kj);
-    jacobi_svd(st, pl, R, /*transS=*/wide, Y, jsweeps, bar, jerr, kMaxSweeps);
+    jacobi_svd(st, pl, R, /*transS=*/wide, Y, jsweeps, bar, jerr, kMaxSweeps, Yws);
     sort_chop(st, Y, pl.rows, kj, pl.kpad, ep, rmax_i, sv, perm, scal_i, scal_d + 8);
     // one synchronisation: r1, Jacobi status, QR acceptance data
     int h_i[4] = {0, 0, 0, 0};

@@ -5,9 +5,6 @@
 #include <algorithm>
 
 namespace ttg {
-void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats);
-void tri_inverse(hipStream_t s, int64_t k, const double *R, double *X);
 void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
                      const float *A, int64_t lda, const float *B, int64_t ldb, double *C,
                      int64_t ldc, float *ws, int64_t wse, int cus);
@@ -92,20 +89,39 @@ ttg_status ttg_test_householder(ttg_ctx *ctx, int64_t m, int64_t n, const double
   })
 }
 
+// R (k x k) upper Cholesky factor of G + shift
 ttg_status ttg_test_chol(ttg_ctx *ctx, int64_t k, const double *G, double shift_rel, double *R,
-                         double *Rinv, int *flag, double *stats) {
+                         int *flag, double *stats) {
   GUARD(ctx, {
     hipStream_t s = ctx->stream;
     Frees f;
     double *dG = f.add(up(G, k * k, s)), *dR = f.add(up<double>(nullptr, k * k, s));
-    double *dX = f.add(up<double>(nullptr, k * k, s)), *dst = f.add(up<double>(nullptr, 8, s));
+    double *dst = f.add(up<double>(nullptr, 8, s));
     int *dfl = f.add(up<int>(nullptr, 4, s));
     chol_factor(s, k, dG, k, shift_rel, dR, dfl, dst);
-    tri_inverse(s, k, dR, dX);
     down(R, dR, k * k, s);
-    down(Rinv, dX, k * k, s);
     down(flag, dfl, 1, s);
     down(stats, dst, 2, s);
+  })
+}
+
+// X = A R^{-1}, R upper triangular; prec 0 = FP64, 1 = FP32 data
+ttg_status ttg_test_trsm(ttg_ctx *ctx, int prec, int64_t m, int64_t k, const double *A,
+                         const double *R, double *X) {
+  GUARD(ctx, {
+    hipStream_t s = ctx->stream;
+    Frees f;
+    double *dA = f.add(up(A, m * k, s)), *dR = f.add(up(R, k * k, s));
+    double *dX = f.add(up<double>(nullptr, m * k, s));
+    if (prec == 0) {
+      trsm_right_upper<double>(s, m, k, dA, m, dR, dX, m);
+    } else {
+      float *fA = f.add(up<float>(nullptr, m * k, s)), *fX = f.add(up<float>(nullptr, m * k, s));
+      convert<float, double>(s, fA, dA, m * k);
+      trsm_right_upper<float>(s, m, k, fA, m, dR, fX, m);
+      convert<double, float>(s, dX, fX, m * k);
+    }
+    down(X, dX, m * k, s);
   })
 }
 
@@ -120,7 +136,8 @@ ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS,
     double *dS = f.add(up(S, k * k, s)), *dY = f.add(up<double>(nullptr, pl.rows * pl.kpad, s));
     int *dsw = f.add(up<int>(nullptr, 4, s)), *derr = f.add(up<int>(nullptr, 4, s));
     unsigned *bar = f.add(up<unsigned>(nullptr, 128, s));
-    jacobi_svd(s, pl, dS, transS != 0, dY, dsw, bar, derr, 60);
+    double *ws = f.add(up<double>(nullptr, jacobi_polar_ws_elems(pl), s));
+    jacobi_svd(s, pl, dS, transS != 0, dY, dsw, bar, derr, 60, ws);
     down(Y, dY, pl.rows * pl.kpad, s);
     down(sweeps, dsw, 1, s);
     down(err, derr, 1, s);

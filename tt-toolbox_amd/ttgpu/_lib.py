@@ -36,7 +36,7 @@ EXPORTS = [
     "ttg_profile_begin", "ttg_profile_end",
 ]
 # kernel-level test hooks (include/ttgpu_testing.h)
-TEST_EXPORTS = ["ttg_test_gemm", "ttg_test_householder", "ttg_test_chol", "ttg_test_jacobi"]
+TEST_EXPORTS = ["ttg_test_gemm", "ttg_test_householder", "ttg_test_chol", "ttg_test_trsm", "ttg_test_jacobi"]
 KERNEL_CLASSES = {"gemm": 1, "jacobi": 2, "cholesky": 3, "trtri": 4, "householder": 5}
 
 
@@ -99,8 +99,9 @@ def load(path: Path | str | None = None):
         "ttg_profile_end": (st, [P, I64P, DP, DP, DP]),
         "ttg_test_gemm": (st, [P, C.c_int, C.c_int, C.c_int, I64, I64, I64, P, I64, P, I64, P, I64]),
         "ttg_test_householder": (st, [P, I64, I64, P, P, P]),
-        "ttg_test_chol": (st, [P, I64, P, C.c_double, P, P, P, P]),
+        "ttg_test_chol": (st, [P, I64, P, C.c_double, P, P, P]),
         "ttg_test_jacobi": (st, [P, I64, P, C.c_int, P, I64, I64P, P, P]),
+        "ttg_test_trsm": (st, [P, C.c_int, I64, I64, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
