@@ -21,6 +21,8 @@ ttg_status ttg_test_trsm(ttg_ctx *ctx, int prec, int64_t m, int64_t k, const dou
                          const double *R, double *X);
 ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS, double *Y,
                            int64_t ycap, int64_t *kpad_out, int *sweeps, int *err);
+ttg_status ttg_test_jacobi_timing(ttg_ctx *ctx, int64_t k, const double *S, int64_t *times,
+                                  int64_t ntimes);
 #ifdef __cplusplus
 }
 #endif

@@ -157,9 +157,11 @@ struct JacobiPlan {
 JacobiPlan jacobi_plan(int64_t k);
 size_t jacobi_ws_bytes(const JacobiPlan &pl);
 // ws: jacobi_polar_ws_elems(pl) doubles of scratch for the final polar correction.
+// dbg (optional): 4 timestamps (100 MHz clock) per outer step of the first sweep.
 size_t jacobi_polar_ws_elems(const JacobiPlan &pl);
 void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
-                int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws);
+                int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws,
+                unsigned long long *dbg = nullptr);
 
 // Sort the column norms of Y's top k rows (descending) and apply my_chop2.
 // sv[k], perm[k] written; out_i[0] = r1 (after rmax cap), out_d[0] = ||s||.

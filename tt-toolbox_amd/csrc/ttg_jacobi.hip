@@ -82,21 +82,32 @@ __device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen, int *e
   return sh_ok != 0;
 }
 
-constexpr int JKL = 8;  // rows per lane held in registers: k <= 512
+// Three independent wave sums, interleaved so their DPP chains overlap.
+__device__ __forceinline__ void wave_sum3(double &a, double &b, double &c) {
+  a += dpp_d<0xB1>(a); b += dpp_d<0xB1>(b); c += dpp_d<0xB1>(c);
+  a += dpp_d<0x4E>(a); b += dpp_d<0x4E>(b); c += dpp_d<0x4E>(c);
+  a += dpp_d<0x141>(a); b += dpp_d<0x141>(b); c += dpp_d<0x141>(c);
+  a += dpp_d<0x140>(a); b += dpp_d<0x140>(b); c += dpp_d<0x140>(c);
+  a = (readlane_d(a, 0) + readlane_d(a, 16)) + (readlane_d(a, 32) + readlane_d(a, 48));
+  b = (readlane_d(b, 0) + readlane_d(b, 16)) + (readlane_d(b, 32) + readlane_d(b, 48));
+  c = (readlane_d(c, 0) + readlane_d(c, 16)) + (readlane_d(c, 32) + readlane_d(c, 48));
+}
 
-template <class TS>
-__global__ __launch_bounds__(1024) void k_jacobi2(int k, int b, int p, TS *__restrict__ Sg, long lds,
+// KL: rows of S per lane (k <= 64*KL); LC = 2b local columns (multiple of 16, <= 64).
+template <int KL, int LC>
+__global__ __launch_bounds__(1024) void k_jacobi2(int k, int p, double *__restrict__ Sg, long lds,
                                                   double *__restrict__ Jg, long ldj, double tol,
                                                   int max_sweeps, int *sweeps_out, unsigned *bar,
-                                                  int *err) {
+                                                  int *err, unsigned long long *dbg) {
+  constexpr int B = LC / 2;
   extern __shared__ char smem[];
-  const int lc = 2 * b;
-  TS *Sl = reinterpret_cast<TS *>(smem);                        // lc x k, column-major
-  double *Vl = reinterpret_cast<double *>(smem + (((size_t)lc * k * sizeof(TS) + 15) & ~size_t(15)));
+  double *Sl = reinterpret_cast<double *>(smem);  // LC x k, column-major
+  double *Vl = Sl + (size_t)LC * k;               // LC x LC, column-major
   __shared__ int sh_steprot, sh_rot, sh_total;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   const int w = blockIdx.x;
   const int nblk = 2 * p;
+  const int total = LC * k;
   unsigned gen = 0;
   bool converged = false;
   int sweep = 0;
@@ -106,65 +117,77 @@ __global__ __launch_bounds__(1024) void k_jacobi2(int k, int b, int p, TS *__res
     for (int step = 0; step < outer; ++step) {
       int bA, bB;
       circle_pair(nblk, step, w, bA, bB);
-      for (int idx = tid; idx < lc * k; idx += blockDim.x) {
-        const int c = idx / k, i = idx - c * k;
-        const int gcol = (c < b) ? bA * b + c : bB * b + (c - b);
-        Sl[idx] = Sg[i + (long)gcol * lds];
+      // load the S columns of the block pair (batched, 8 independent loads per pass)
+      for (int base = 0; base < total; base += 8 * 1024) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int idx = base + tid + j * 1024;
+          v[j] = 0.0;
+          if (idx < total) {
+            const int c = idx / k, i = idx - c * k;
+            const int gcol = (c < B) ? bA * B + c : bB * B + (c - B);
+            v[j] = Sg[i + (long)gcol * lds];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int idx = base + tid + j * 1024;
+          if (idx < total) Sl[idx] = v[j];
+        }
       }
-      for (int idx = tid; idx < lc * lc; idx += blockDim.x) Vl[idx] = (idx % lc == idx / lc) ? 1.0 : 0.0;
+      for (int idx = tid; idx < LC * LC; idx += blockDim.x) Vl[idx] = (idx % LC == idx / LC) ? 1.0 : 0.0;
       if (tid == 0) sh_steprot = 0;
       __syncthreads();
-      const int rounds = (step == 0) ? lc - 1 : b;
+      const bool rec = dbg && sweep == 0 && w == 0 && tid == 0;
+      if (rec) dbg[step * 4 + 0] = __builtin_amdgcn_s_memrealtime();
+      const int rounds = (step == 0) ? LC - 1 : B;
       for (int t = 0; t < rounds; ++t) {
-        for (int pr = wv; pr < b; pr += nw) {
+        for (int pr = wv; pr < B; pr += nw) {
           int pa, pb;
           if (step == 0) {
-            circle_pair(lc, t, pr, pa, pb);
+            circle_pair(LC, t, pr, pa, pb);
           } else {
             pa = pr;
-            pb = b + (pr + t) % b;
+            pb = B + (pr + t) % B;
           }
-          TS *cp = Sl + (long)pa * k;
-          TS *cq = Sl + (long)pb * k;
-          TS x[JKL], y[JKL];
-          TS al = 0, be = 0, ga = 0;
+          double *cp = Sl + (long)pa * k;
+          double *cq = Sl + (long)pb * k;
+          double x[KL], y[KL];
+          double al = 0.0, be = 0.0, ga = 0.0;
 #pragma unroll
-          for (int q = 0; q < JKL; ++q) {
+          for (int q = 0; q < KL; ++q) {
             const int i = lane + 64 * q;
-            if (i < k) {
-              x[q] = cp[i];
-              y[q] = cq[i];
-              al += x[q] * x[q];
-              be += y[q] * y[q];
-              ga += x[q] * y[q];
-            }
+            const int ic = i < k ? i : k - 1;  // branch-free: clamp, then zero
+            const double xv = cp[ic], yv = cq[ic];
+            x[q] = i < k ? xv : 0.0;
+            y[q] = i < k ? yv : 0.0;
           }
-          al = wave_sum(al);
-          be = wave_sum(be);
-          ga = wave_sum(ga);
-          const double dal = (double)al, dbe = (double)be, dga = (double)ga;
-          if (dga * dga > tol * tol * dal * dbe) {
-            const double zeta = (dbe - dal) / (2.0 * dga);
+#pragma unroll
+          for (int q = 0; q < KL; ++q) {
+            al += x[q] * x[q];
+            be += y[q] * y[q];
+            ga += x[q] * y[q];
+          }
+          wave_sum3(al, be, ga);
+          if (ga * ga > tol * tol * al * be) {
+            const double zeta = (be - al) / (2.0 * ga);
             double tt;
             if (fabs(zeta) > 1e150) tt = 0.5 / zeta;
             else tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-            double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
-            const double nr = 1.0 / sqrt(cs * cs + sn * sn);  // c^2 + s^2 = 1 to the last bit
-            cs *= nr;
-            sn *= nr;
-            const TS c_ = (TS)cs, s_ = (TS)sn;
+            const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
 #pragma unroll
-            for (int q = 0; q < JKL; ++q) {
+            for (int q = 0; q < KL; ++q) {
               const int i = lane + 64 * q;
               if (i < k) {
-                cp[i] = c_ * x[q] - s_ * y[q];
-                cq[i] = s_ * x[q] + c_ * y[q];
+                cp[i] = cs * x[q] - sn * y[q];
+                cq[i] = sn * x[q] + cs * y[q];
               }
             }
-            for (int i = lane; i < lc; i += 64) {
-              const double vp = Vl[pa * lc + i], vq = Vl[pb * lc + i];
-              Vl[pa * lc + i] = cs * vp - sn * vq;
-              Vl[pb * lc + i] = sn * vp + cs * vq;
+            if (lane < LC) {
+              const double vp = Vl[pa * LC + lane], vq = Vl[pb * LC + lane];
+              Vl[pa * LC + lane] = cs * vp - sn * vq;
+              Vl[pb * LC + lane] = sn * vp + cs * vq;
             }
             if (lane == 0) atomicAdd(&sh_steprot, 1);
           }
@@ -172,45 +195,61 @@ __global__ __launch_bounds__(1024) void k_jacobi2(int k, int b, int p, TS *__res
         __syncthreads();
       }
       const int rot = sh_steprot;
+      if (rec) dbg[step * 4 + 1] = __builtin_amdgcn_s_memrealtime();
       if (rot > 0) {
-        for (int idx = tid; idx < lc * k; idx += blockDim.x) {
-          const int c = idx / k, i = idx - c * k;
-          const int gcol = (c < b) ? bA * b + c : bB * b + (c - b);
-          Sg[i + (long)gcol * lds] = Sl[idx];
+        for (int base = 0; base < total; base += 8 * 1024) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int idx = base + tid + j * 1024;
+            if (idx < total) {
+              const int c = idx / k, i = idx - c * k;
+              const int gcol = (c < B) ? bA * B + c : bB * B + (c - B);
+              Sg[i + (long)gcol * lds] = Sl[idx];
+            }
+          }
         }
         // J(:, pair) <- J(:, pair) * V with FP64 MFMA; each wave owns 16-row bands
-        const int nrt = (k + 15) / 16, nct = lc / 16;
+        constexpr int NCT = LC / 16;
+        const int nrt = (k + 15) / 16;
         for (int rt = wv; rt < nrt; rt += nw) {
-          v4d_j acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
           const int i = rt * 16 + (lane & 15);
-          for (int kk = 0; kk < lc; kk += 4) {
-            const int l = kk + (lane >> 4);
-            const int gl = (l < b) ? bA * b + l : bB * b + (l - b);
-            const double a = (i < k) ? Jg[i + (long)gl * ldj] : 0.0;
+          double a[LC / 4];
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct)
-              if (ct < nct) acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Vl[l + (ct * 16 + (lane & 15)) * lc], acc[ct], 0, 0, 0);
+          for (int kk = 0; kk < LC / 4; ++kk) {
+            const int l = 4 * kk + (lane >> 4);
+            const int gl = (l < B) ? bA * B + l : bB * B + (l - B);
+            a[kk] = (i < k) ? Jg[i + (long)gl * ldj] : 0.0;
+          }
+          v4d_j acc[NCT];
+#pragma unroll
+          for (int ct = 0; ct < NCT; ++ct) acc[ct] = v4d_j{0, 0, 0, 0};
+#pragma unroll
+          for (int kk = 0; kk < LC / 4; ++kk) {
+            const int l = 4 * kk + (lane >> 4);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+              acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], Vl[l + (ct * 16 + (lane & 15)) * LC], acc[ct], 0, 0, 0);
           }
 #pragma unroll
-          for (int ct = 0; ct < 4; ++ct) {
-            if (ct < nct) {
+          for (int ct = 0; ct < NCT; ++ct) {
 #pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                const int row = rt * 16 + (lane >> 4) + 4 * v;
-                const int c = ct * 16 + (lane & 15);
-                const int gcol = (c < b) ? bA * b + c : bB * b + (c - b);
-                if (row < k) Jg[row + (long)gcol * ldj] = acc[ct][v];
-              }
+            for (int v = 0; v < 4; ++v) {
+              const int row = rt * 16 + (lane >> 4) + 4 * v;
+              const int c = ct * 16 + (lane & 15);
+              const int gcol = (c < B) ? bA * B + c : bB * B + (c - B);
+              if (row < k) Jg[row + (long)gcol * ldj] = acc[ct][v];
             }
           }
         }
       }
       if (tid == 0) sh_rot += rot;
+      if (rec) dbg[step * 4 + 2] = __builtin_amdgcn_s_memrealtime();
       if (p > 1) {
         if (!grid_sync(bar, p, gen, err)) return;
       } else {
         __syncthreads();
       }
+      if (rec) dbg[step * 4 + 3] = __builtin_amdgcn_s_memrealtime();
     }
     if (p > 1) {
       if (tid == 0) atomicAdd(&bar[2 + sweep], (unsigned)sh_rot);
@@ -263,23 +302,47 @@ JacobiPlan jacobi_plan(int64_t k) {
 }
 size_t jacobi_ws_bytes(const JacobiPlan &pl) { return (size_t)pl.rows * pl.kpad * sizeof(double); }
 
-template <class TS>
-static void launch_jacobi(hipStream_t s, const JacobiPlan &pl, TS *Sg, long lds, double *Jg, long ldj,
-                          double tol, int max_sweeps, int *sweeps_dev, unsigned *bar, int *err) {
-  const int lc = 2 * pl.b;
-  const size_t lsz = (((size_t)lc * pl.k * sizeof(TS) + 15) & ~size_t(15)) + (size_t)lc * lc * sizeof(double);
+template <int KL, int LC>
+static void launch_jacobi_t(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg,
+                            long ldj, double tol, int max_sweeps, int *sweeps_dev, unsigned *bar,
+                            int *err, unsigned long long *dbg) {
+  const size_t lsz = ((size_t)LC * pl.k + (size_t)LC * LC) * sizeof(double);
   TTG_CHECK(lsz <= 160 * 1024, TTG_BAD_ARG, "Jacobi block does not fit in LDS");
-  TTG_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * (2 + max_sweeps), s));
-  TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi2<TS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
-  int k = (int)pl.k, b = pl.b, p = pl.p, ms = max_sweeps;
+  TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi2<KL, LC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
+  int k = (int)pl.k, p = pl.p, ms = max_sweeps;
   if (p > 1) {
-    void *args[] = {&k, &b, &p, &Sg, &lds, &Jg, &ldj, &tol, &ms, &sweeps_dev, &bar, &err};
-    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_jacobi2<TS>, dim3(p), dim3(1024), args, lsz, s));
+    void *args[] = {&k, &p, &Sg, &lds, &Jg, &ldj, &tol, &ms, &sweeps_dev, &bar, &err, &dbg};
+    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_jacobi2<KL, LC>, dim3(p), dim3(1024), args, lsz, s));
   } else {
-    hipLaunchKernelGGL((k_jacobi2<TS>), dim3(1), dim3(1024), lsz, s, k, b, p, Sg, lds, Jg, ldj, tol, ms,
-                       sweeps_dev, bar, err);
+    hipLaunchKernelGGL((k_jacobi2<KL, LC>), dim3(1), dim3(1024), lsz, s, k, p, Sg, lds, Jg, ldj, tol, ms,
+                       sweeps_dev, bar, err, dbg);
     TTG_HIP(hipGetLastError());
   }
+}
+
+template <int KL>
+static void launch_jacobi_kl(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg,
+                             long ldj, double tol, int ms, int *sw, unsigned *bar, int *err,
+                             unsigned long long *dbg) {
+  switch (2 * pl.b) {
+    case 16: launch_jacobi_t<KL, 16>(s, pl, Sg, lds, Jg, ldj, tol, ms, sw, bar, err, dbg); break;
+    case 32: launch_jacobi_t<KL, 32>(s, pl, Sg, lds, Jg, ldj, tol, ms, sw, bar, err, dbg); break;
+    case 48: launch_jacobi_t<KL, 48>(s, pl, Sg, lds, Jg, ldj, tol, ms, sw, bar, err, dbg); break;
+    case 64: launch_jacobi_t<KL, 64>(s, pl, Sg, lds, Jg, ldj, tol, ms, sw, bar, err, dbg); break;
+    default: throw Error(TTG_BAD_ARG, "unsupported Jacobi block size");
+  }
+}
+
+static void launch_jacobi(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg, long ldj,
+                          double tol, int max_sweeps, int *sweeps_dev, unsigned *bar, int *err,
+                          unsigned long long *dbg) {
+  TTG_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * (2 + max_sweeps), s));
+  const int64_t k = pl.k;
+  if (k <= 64) launch_jacobi_kl<1>(s, pl, Sg, lds, Jg, ldj, tol, max_sweeps, sweeps_dev, bar, err, dbg);
+  else if (k <= 128) launch_jacobi_kl<2>(s, pl, Sg, lds, Jg, ldj, tol, max_sweeps, sweeps_dev, bar, err, dbg);
+  else if (k <= 256) launch_jacobi_kl<4>(s, pl, Sg, lds, Jg, ldj, tol, max_sweeps, sweeps_dev, bar, err, dbg);
+  else if (k <= 512) launch_jacobi_kl<8>(s, pl, Sg, lds, Jg, ldj, tol, max_sweeps, sweeps_dev, bar, err, dbg);
+  else throw Error(TTG_BAD_ARG, "Jacobi SVD supports k <= 512");
 }
 
 __global__ void k_polar_m(int n, double *M) {
@@ -296,7 +359,8 @@ size_t jacobi_polar_ws_elems(const JacobiPlan &pl) {
 }
 
 void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
-                int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws) {
+                int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws,
+                unsigned long long *dbg) {
   const int k = (int)pl.k, kpad = (int)pl.kpad;
   const long total = (long)pl.rows * kpad;
   const int blocks = (int)std::min<long>((total + 255) / 256, 2048);
@@ -307,8 +371,8 @@ void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool trans
   // algorithmic flops of the small SVD (R-SVD of the k x k factor, GVL Fig. 8.6.1: 20k^3)
   ProfScope prof(KC_JACOBI, s, 20.0 * (double)k * k * k, 8.0 * 4.0 * (double)k * k);
   const double tol = sqrt((double)std::max(k, 1)) * DBL_EPSILON;
-  launch_jacobi<double>(s, pl, Y, (long)pl.rows, Y + k, (long)pl.rows, tol, max_sweeps, sweeps_dev,
-                        barrier_dev, err_dev);
+  launch_jacobi(s, pl, Y, (long)pl.rows, Y + k, (long)pl.rows, tol, max_sweeps, sweeps_dev,
+                barrier_dev, err_dev, dbg);
   // polar correction of J (and of S J with it)
   double *M = ws, *Y2 = ws + (size_t)kpad * kpad;
   gemm<double>(s, true, false, kpad, kpad, k, 1.0, Y + k, pl.rows, Y + k, pl.rows, 0.0, M, kpad,

@@ -145,4 +145,23 @@ ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS,
   })
 }
 
+// Jacobi with timestamps of the first sweep: times[4*step + e], e = after load,
+// after inner rounds, after the J update, after the barrier (100 MHz ticks).
+ttg_status ttg_test_jacobi_timing(ttg_ctx *ctx, int64_t k, const double *S, int64_t *times,
+                                  int64_t ntimes) {
+  GUARD(ctx, {
+    hipStream_t s = ctx->stream;
+    Frees f;
+    JacobiPlan pl = jacobi_plan(k);
+    double *dS = f.add(up(S, k * k, s)), *dY = f.add(up<double>(nullptr, pl.rows * pl.kpad, s));
+    int *dsw = f.add(up<int>(nullptr, 4, s)), *derr = f.add(up<int>(nullptr, 4, s));
+    unsigned *bar = f.add(up<unsigned>(nullptr, 128, s));
+    double *ws = f.add(up<double>(nullptr, jacobi_polar_ws_elems(pl), s));
+    unsigned long long *dbg = f.add(up<unsigned long long>(nullptr, ntimes, s));
+    TTG_HIP(hipMemsetAsync(dbg, 0, ntimes * 8, s));
+    jacobi_svd(s, pl, dS, true, dY, dsw, bar, derr, 60, ws, dbg);
+    down(reinterpret_cast<unsigned long long *>(times), dbg, ntimes, s);
+  })
+}
+
 }  // extern "C"
