@@ -23,6 +23,8 @@ ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS,
                            int64_t ycap, int64_t *kpad_out, int *sweeps, int *err);
 ttg_status ttg_test_jacobi_timing(ttg_ctx *ctx, int64_t k, const double *S, int64_t *times,
                                   int64_t ntimes);
+ttg_status ttg_test_chol_timing(ttg_ctx *ctx, int64_t k, const double *G, int64_t *times,
+                                int64_t ntimes);
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,7 @@ __global__ void k_completion_left(int d, const long *__restrict__ n, const long 
                                   const T *__restrict__ core, long S, const int *__restrict__ idx,
                                   const T *__restrict__ vals, int G, T *__restrict__ Lbuf,
                                   T *__restrict__ res, double *__restrict__ lpart) {
-  extern __shared__ char smem[];
+  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   T *buf = reinterpret_cast<T *>(smem);  // per group: 2 x G
   __shared__ double red[16];
   const int groups = blockDim.x / G;
@@ -63,7 +63,7 @@ __global__ void k_completion_right(int d, const long *__restrict__ n, const long
                                    const long *__restrict__ cofs, const long *__restrict__ rofs,
                                    const T *__restrict__ core, long S, const int *__restrict__ idx,
                                    int G, T *__restrict__ Rbuf) {
-  extern __shared__ char smem[];
+  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   T *buf = reinterpret_cast<T *>(smem);
   const int groups = blockDim.x / G;
   const int gid = threadIdx.x / G, t = threadIdx.x % G;

@@ -1,9 +1,8 @@
 // Cholesky factorisation and triangular solve for Cholesky-QR (gfx950).
 //
-//   k_chol2 : one workgroup, G + shift = R^T R (upper R), blocked by 32.
-//             The 32x32 diagonal block is factorised by wave 0 alone (LDS,
-//             wave-level ordering, no workgroup barriers), the panel is a
-//             forward substitution and the trailing update an MFMA SYRK.
+//   k_chol3 : one workgroup, G + shift = R^T R (upper R), blocked by 32:
+//             register-resident diagonal block (wave 0), forward-substitution
+//             panel staged in LDS, MFMA SYRK trailing update.
 //   k_trsm  : X = A R^{-1} (R upper, right side), 32 rows per workgroup held
 //             in LDS; per 32-column block J: B = A_J - X_{<J} R_{<J,J}
 //             (MFMA), then row-wise substitution with R_JJ (8 lanes per
@@ -46,138 +45,181 @@ constexpr int CB = 32;  // block size
 // ---------------------------------------------------------------------------
 // Cholesky
 // ---------------------------------------------------------------------------
-// R (k x k, ld k) receives the upper factor (lower triangle zero).
-// stats[0] = max|r_jj| / min|r_jj|, stats[1] = max_ij |R_ij - delta_ij|.
-__global__ __launch_bounds__(256) void k_chol2(int k, const double *__restrict__ G, long ldg,
-                                               double shift_rel, double *__restrict__ R, int *flag,
-                                               double *stats) {
-  __shared__ double D[CB][CB + 1];
-  __shared__ double red[8];
+// One workgroup of 1024 threads; R (k x k, ld k) lives in global memory (it
+// stays in L2) and receives the upper factor with a zero lower triangle.
+// Per 32-column block J0:
+//   diag : wave 0 factorises R_JJ with lane j holding column j in registers;
+//          row c of the factor is published through LDS (Dl, row-major) and
+//          every lane applies the rank-1 update to its column.
+//   panel: R_J,rest <- R_JJ^{-T} R_J,rest by forward substitution, one column
+//          per thread, staged in LDS (Ps) with coalesced global traffic.
+//   syrk : R_rest,rest -= Ps^T Ps on 16x16 upper tiles (FP64 MFMA).
+// stats[0] = max|r_jj| / min|r_jj|, stats[1] = max_ij |R_ij - delta_ij|,
+// flag = 1-based index of the first non-positive pivot (0 = success).
+constexpr int CHOL_THREADS = 512;
+
+__global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__restrict__ G, long ldg,
+                                                        double shift_rel, double *__restrict__ R,
+                                                        int *flag, double *stats,
+                                                        unsigned long long *dbg) {
+  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
+  double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
+  double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
+  __shared__ double red[CHOL_THREADS / 64];
+  __shared__ double dinv[CB];
   __shared__ int sh_fail;
-  extern __shared__ double Ps[];  // CB x w panel, Ps[t*w + c]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int NW = CHOL_THREADS / 64;
   double tr = 0.0;
-  for (int j = tid; j < k; j += blockDim.x) tr += G[j + (long)j * ldg];
+  for (int j = tid; j < k; j += CHOL_THREADS) tr += G[j + (long)j * ldg];
   tr = block_sum(tr, red);
   const double shift = shift_rel * tr / (k > 0 ? k : 1);
-  for (long idx = tid; idx < (long)k * k; idx += blockDim.x) {
-    const int i = (int)(idx % k), j = (int)(idx / k);
-    R[idx] = (i <= j) ? G[i + (long)j * ldg] + (i == j ? shift : 0.0) : 0.0;
-  }
+  for (int j = wv; j < k; j += NW)
+    for (int i = lane; i < k; i += 64)
+      R[i + (long)j * k] = (i <= j) ? G[i + (long)j * ldg] + (i == j ? shift : 0.0) : 0.0;
   if (tid == 0) sh_fail = 0;
-  __syncthreads();
+  double dmax = 0.0, dmin = INFINITY, dev = 0.0;
   for (int J0 = 0; J0 < k; J0 += CB) {
     const int jb = min(CB, k - J0);
-    for (int idx = tid; idx < CB * CB; idx += blockDim.x) {
-      const int i = idx % CB, j = idx / CB;
-      D[i][j] = (i < jb && j < jb && i <= j) ? R[(J0 + i) + (long)(J0 + j) * k] : (i == j ? 1.0 : 0.0);
+    __syncthreads();
+    if (dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    // stage the diagonal block column-major in Dl (identity-padded past jb)
+    for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
+      const int i = idx & (CB - 1), j = idx >> 5;
+      Dl[idx] = (i < jb && j < jb) ? (i <= j ? R[(J0 + i) + (long)(J0 + j) * k] : 0.0) : (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
     if (wv == 0) {
-      // wave 0: unblocked upper Cholesky of the diagonal block
-      int fail = 0;
-      for (int c = 0; c < jb; ++c) {
-        const double p = D[c][c];
-        if (!(p > 0.0) || !isfinite(p)) {
-          fail = c + 1;
-          break;
-        }
+      const int j = lane & (CB - 1);
+      double col[CB];
+#pragma unroll
+      for (int i = 0; i < CB; ++i) col[i] = Dl[j * CB + i];
+      wave_sync_lds();
+      unsigned failmask = 0;
+      double myinv = 1.0;
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        double p = readlane_d(col[c], c);
+        // straight-line code only (branches here push col[] into scratch): a
+        // failed pivot is recorded in a bit mask and replaced by a harmless 1
+        const bool bad = !(p > 0.0) || !isfinite(p);
+        failmask |= bad ? (1u << c) : 0u;
+        p = bad ? 1.0 : p;
         const double dc = sqrt(p);
+        const double inv = 1.0 / dc;
+        const double rc = (j == c) ? dc : (j > c ? col[c] * inv : 0.0);
+        col[c] = rc;
+        myinv = (j == c) ? inv : myinv;
+        // lanes 32..63 mirror lanes 0..31 and store identical values (no exec branch
+        // inside the unrolled loop: branches there push col[] into scratch)
+        Dl[c * CB + j] = rc;
         wave_sync_lds();
-        if (lane == 0) D[c][c] = dc;
-        if (lane > c && lane < jb) D[c][lane] /= dc;
-        wave_sync_lds();
-        const int rem = jb - c - 1;
-        for (int e = lane; e < rem * rem; e += 64) {
-          const int i = c + 1 + e % rem, j = c + 1 + e / rem;
-          if (i <= j) D[i][j] -= D[c][i] * D[c][j];
-        }
-        wave_sync_lds();
+#pragma unroll
+        for (int i = c + 1; i < CB; ++i) col[i] -= Dl[c * CB + i] * rc;
+        __builtin_amdgcn_sched_barrier(0);
       }
-      if (fail && lane == 0) sh_fail = J0 + fail;
+      if (lane < CB) dinv[j] = myinv;
+      if (lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
     }
     __syncthreads();
+    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
     if (sh_fail) break;
-    for (int idx = tid; idx < jb * jb; idx += blockDim.x) {
-      const int i = idx % jb, j = idx / jb;
-      if (i <= j) R[(J0 + i) + (long)(J0 + j) * k] = D[i][j];
+    // diagonal block to R (+ statistics)
+    for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), l = idx >> 5;
+      if (t < jb && l < jb && t <= l) {
+        const double v = Dl[t * CB + l];
+        R[(J0 + t) + (long)(J0 + l) * k] = v;
+        if (t == l) {
+          dmax = fmax(dmax, fabs(v));
+          dmin = fmin(dmin, fabs(v));
+          dev = fmax(dev, fabs(v - 1.0));
+        } else {
+          dev = fmax(dev, fabs(v));
+        }
+      }
     }
     const int w = k - J0 - jb;
-    if (w > 0) {
-      // panel: D^T P = R[J0:J0+jb, J0+jb:k], one column per thread, P kept in LDS
-      for (int c = tid; c < w; c += blockDim.x) {
-        const long col = (long)(J0 + jb + c) * k;
-        for (int t = 0; t < jb; ++t) {
-          double v = R[J0 + t + col];
-          for (int l = 0; l < t; ++l) v -= D[l][t] * Ps[l * w + c];
-          Ps[t * w + c] = v / D[t][t];
-        }
-        for (int t = jb; t < CB; ++t) Ps[t * w + c] = 0.0;
-      }
-      __syncthreads();
-      for (int idx = tid; idx < jb * w; idx += blockDim.x) {
-        const int t = idx % jb, c = idx / jb;
-        R[J0 + t + (long)(J0 + jb + c) * k] = Ps[t * w + c];
-      }
-      // trailing SYRK on 16x16 tiles of the upper triangle (MFMA FP64)
-      const int nt = (w + 15) / 16;
-      const int ntiles = nt * (nt + 1) / 2;
-      for (int tile = wv; tile < ntiles; tile += 4) {
-        int tj = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
-        while (tj * (tj + 1) / 2 > tile) --tj;
-        while ((tj + 1) * (tj + 2) / 2 <= tile) ++tj;
-        const int ti = tile - tj * (tj + 1) / 2;
-        v4d_f acc = {0, 0, 0, 0};
-        const int ia = ti * 16 + (lane & 15), jbcol = tj * 16 + (lane & 15);
+    if (w <= 0) continue;
+    const int wpad = (w + 15) & ~15, wps = wpad + 1;
+    // stage the panel rows (coalesced: 32 consecutive threads read one column)
+    for (int idx = tid; idx < CB * wpad; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), c = idx >> 5;
+      Ps[t * wps + c] = (t < jb && c < w) ? R[(J0 + t) + (long)(J0 + jb + c) * k] : 0.0;
+    }
+    __syncthreads();
+    for (int c = tid; c < wpad; c += CHOL_THREADS) {
+      double b[CB];
 #pragma unroll
-        for (int kk = 0; kk < CB; kk += 4) {
-          const int t = kk + (lane >> 4);
-          const double a = (ia < w) ? Ps[t * w + ia] : 0.0;
-          const double b = (jbcol < w) ? Ps[t * w + jbcol] : 0.0;
-          acc = MFf<double>::op(a, b, acc);
-        }
+      for (int t = 0; t < CB; ++t) b[t] = Ps[t * wps + c];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = ti * 16 + MFf<double>::row(lane, v);
-          const int j = tj * 16 + (lane & 15);
-          if (i < w && j < w && i <= j) R[(J0 + jb + i) + (long)(J0 + jb + j) * k] -= acc[v];
-        }
+      for (int t = 0; t < CB; ++t) {
+        const double x = b[t] * dinv[t];
+        b[t] = x;
+#pragma unroll
+        for (int l = t + 1; l < CB; ++l) b[l] -= Dl[t * CB + l] * x;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int t = 0; t < CB; ++t) {
+        const double v = (t < jb && c < w) ? b[t] : 0.0;
+        Ps[t * wps + c] = v;
+        dev = fmax(dev, fabs(v));
       }
     }
     __syncthreads();
+    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
+    for (int idx = tid; idx < CB * w; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), c = idx >> 5;
+      if (t < jb) R[(J0 + t) + (long)(J0 + jb + c) * k] = Ps[t * wps + c];
+    }
+    // trailing SYRK on the upper 16x16 tiles
+    const int nt = wpad >> 4;
+    const int ntiles = nt * (nt + 1) / 2;
+    for (int tile = wv; tile < ntiles; tile += NW) {
+      int tj = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+      while (tj * (tj + 1) / 2 > tile) --tj;
+      while ((tj + 1) * (tj + 2) / 2 <= tile) ++tj;
+      const int ti = tile - tj * (tj + 1) / 2;
+      v4d_f acc = {0, 0, 0, 0};
+      const int ia = ti * 16 + (lane & 15), ja = tj * 16 + (lane & 15);
+#pragma unroll
+      for (int kk = 0; kk < CB; kk += 4) {
+        const int t = kk + (lane >> 4);
+        acc = MFf<double>::op(Ps[t * wps + ia], Ps[t * wps + ja], acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = ti * 16 + MFf<double>::row(lane, v);
+        const int jj = tj * 16 + (lane & 15);
+        if (i < w && jj < w && i <= jj) R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] -= acc[v];
+      }
+    }
   }
   __syncthreads();
-  double dmax = 0.0, dmin = INFINITY, dev = 0.0;
-  if (!sh_fail) {
-    for (int j = tid; j < k; j += blockDim.x) {
-      const double v = fabs(R[j + (long)j * k]);
-      dmax = fmax(dmax, v);
-      dmin = fmin(dmin, v);
-    }
-    for (long idx = tid; idx < (long)k * k; idx += blockDim.x) {
-      const int i = (int)(idx % k), j = (int)(idx / k);
-      dev = fmax(dev, fabs(R[idx] - (i == j ? 1.0 : 0.0)));
-    }
-  }
+  if (dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
+  const int failed = sh_fail;
   dmax = block_max(dmax, red);
   dmin = -block_max(-dmin, red);
   dev = block_max(dev, red);
   if (tid == 0) {
-    flag[0] = sh_fail;
-    stats[0] = sh_fail ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
-    stats[1] = sh_fail ? INFINITY : dev;
+    flag[0] = failed;
+    stats[0] = failed ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
+    stats[1] = failed ? INFINITY : dev;
   }
 }
 
-constexpr int CHOL2_MAXK = 512;
+constexpr int CHOL_MAXK = 512;
 
 void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats) {
-  TTG_CHECK(k <= CHOL2_MAXK, TTG_BAD_ARG, "rank above 512 not supported by the Cholesky kernel");
-  const size_t lds = (size_t)CB * (size_t)std::max<int64_t>(k, 1) * sizeof(double);
+                 double *R, int *flag, double *stats, unsigned long long *dbg) {
+  TTG_CHECK(k >= 1 && k <= CHOL_MAXK, TTG_BAD_ARG, "Cholesky kernel supports 1 <= k <= 512");
+  const int64_t wmax = std::max<int64_t>(k - CB, 0);
+  const int64_t wps = ((wmax + 15) & ~(int64_t)15) + 1;
+  const size_t lds = ((size_t)CB * CB + (size_t)CB * wps) * sizeof(double);
   ProfScope prof(KC_CHOL, s, (double)k * k * k / 3.0, 8.0 * 2.0 * (double)k * k);
-  TTG_HIP(hipFuncSetAttribute((const void *)k_chol2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_chol2, dim3(1), dim3(256), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats);
+  TTG_HIP(hipFuncSetAttribute((const void *)k_chol3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_chol3, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
   TTG_HIP(hipGetLastError());
 }
 
@@ -188,7 +230,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict__ A, long lda,
                                               const double *__restrict__ R, T *__restrict__ X,
                                               long ldx) {
-  extern __shared__ char smem[];
+  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   typedef typename MFf<T>::V V;
   const int kp = (k + CB - 1) / CB * CB;
   T *Xs = reinterpret_cast<T *>(smem);   // 32 x kp, column-major (Xs[col*32 + row])

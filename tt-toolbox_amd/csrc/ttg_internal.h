@@ -139,7 +139,7 @@ void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, d
 // G is read from its upper triangle; shift = shift_rel * trace(G) / k.
 // flag[0] = 0 ok, j+1 if pivot j failed; stats[0] = diag spread, stats[1] = max|R - I|.
 void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats);
+                 double *R, int *flag, double *stats, unsigned long long *dbg = nullptr);
 // X (m x k) = A R^{-1}, R upper triangular (FP64), blocked substitution.
 template <class T>
 void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t lda, const double *R,
@@ -159,6 +159,8 @@ size_t jacobi_ws_bytes(const JacobiPlan &pl);
 // ws: jacobi_polar_ws_elems(pl) doubles of scratch for the final polar correction.
 // dbg (optional): 4 timestamps (100 MHz clock) per outer step of the first sweep.
 size_t jacobi_polar_ws_elems(const JacobiPlan &pl);
+// unsigned words of the barrier/statistics buffer jacobi_svd needs
+int jacobi_bar_words(int max_sweeps);
 void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
                 int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws,
                 unsigned long long *dbg = nullptr);

@@ -82,165 +82,323 @@ __device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen, int *e
   return sh_ok != 0;
 }
 
-// Three independent wave sums, interleaved so their DPP chains overlap.
-__device__ __forceinline__ void wave_sum3(double &a, double &b, double &c) {
-  a += dpp_d<0xB1>(a); b += dpp_d<0xB1>(b); c += dpp_d<0xB1>(c);
-  a += dpp_d<0x4E>(a); b += dpp_d<0x4E>(b); c += dpp_d<0x4E>(c);
-  a += dpp_d<0x141>(a); b += dpp_d<0x141>(b); c += dpp_d<0x141>(c);
-  a += dpp_d<0x140>(a); b += dpp_d<0x140>(b); c += dpp_d<0x140>(c);
-  a = (readlane_d(a, 0) + readlane_d(a, 16)) + (readlane_d(a, 32) + readlane_d(a, 48));
-  b = (readlane_d(b, 0) + readlane_d(b, 16)) + (readlane_d(b, 32) + readlane_d(b, 48));
-  c = (readlane_d(c, 0) + readlane_d(c, 16)) + (readlane_d(c, 32) + readlane_d(c, 48));
+// Sums of three values over each 16-lane row; afterwards every lane of the
+// row holds the bitwise-identical row total (each step adds a lane to its
+// mirror partner, and FP addition is commutative).
+__device__ __forceinline__ void row16_sum3(double &a, double &b, double &c) {
+  a += dpp_d<0xB1>(a); b += dpp_d<0xB1>(b); c += dpp_d<0xB1>(c);     // quad_perm [1,0,3,2]
+  a += dpp_d<0x4E>(a); b += dpp_d<0x4E>(b); c += dpp_d<0x4E>(c);     // quad_perm [2,3,0,1]
+  a += dpp_d<0x141>(a); b += dpp_d<0x141>(b); c += dpp_d<0x141>(c);  // row_half_mirror
+  a += dpp_d<0x140>(a); b += dpp_d<0x140>(b); c += dpp_d<0x140>(c);  // row_mirror
 }
 
-// KL: rows of S per lane (k <= 64*KL); LC = 2b local columns (multiple of 16, <= 64).
-template <int KL, int LC>
-__global__ __launch_bounds__(1024) void k_jacobi2(int k, int p, double *__restrict__ Sg, long lds,
-                                                  double *__restrict__ Jg, long ldj, double tol,
-                                                  int max_sweeps, int *sweeps_out, unsigned *bar,
-                                                  int *err, unsigned long long *dbg) {
+// 1/x and 1/sqrt(x) from the hardware estimates (about 23 correct bits) and
+// two Newton steps each: full double precision for the normal, positive
+// arguments that occur in a rotation, without the special-case handling of a
+// general division or square root.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-x, y, 1.0);
+  return __builtin_fma(y, e, y);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double e = __builtin_fma(-h * y, y, 0.5);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-h * y, y, 0.5);
+  return __builtin_fma(y, e, y);
+}
+
+// Rotation that makes columns with squared norms al, be and inner product ga
+// orthogonal (Rutishauser): with d = be - al, t = sign(d) 2ga / (|d| + sqrt(d^2
+// + 4ga^2)), the smaller root of t^2 + 2 zeta t - 1 = 0, zeta = d / (2ga); t
+// only needs ~46 bits (one Newton step on each estimate), c = 1/sqrt(1+t^2)
+// is refined to full precision so that c^2 + s^2 = 1 to rounding.
+__device__ __forceinline__ void jrot(double al, double be, double ga, double &cs, double &sn) {
+  const double d = be - al, g2 = 2.0 * ga;
+  const double q = __builtin_fma(d, d, g2 * g2);
+  double y = __builtin_amdgcn_rsq(q);
+  y = y * __builtin_fma(-0.5 * q * y, y, 1.5);
+  const double den = fabs(d) + q * y;
+  double inv = __builtin_amdgcn_rcp(den);
+  inv = inv * __builtin_fma(-den, inv, 2.0);
+  const double t = (d >= 0.0 ? g2 : -g2) * inv;
+  cs = rsq_nr(__builtin_fma(t, t, 1.0));
+  sn = cs * t;
+}
+
+// Workgroup barrier that orders LDS only: waits for this wave's LDS traffic
+// (lgkmcnt = 0) but not for its outstanding global loads, so a wave can keep
+// loads in flight across it.  Encoding of s_waitcnt on gfx9: vmcnt = 63,
+// expcnt = 7, lgkmcnt = 0.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// Barrier among the rotating waves only, through an LDS counter (the helper
+// waves of the workgroup never arrive here).  All rotating waves execute the
+// same number of rounds, so the count is always reached; the spin cap only
+// turns a logic error into a reported failure instead of a hang.
+__device__ __forceinline__ void rot_sync(unsigned *cnt, unsigned target, int *err) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if ((threadIdx.x & 63) == 0) atomicAdd(cnt, 1u);
+  unsigned spins = 0;
+  while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
+    if (++spins > (1u << 24)) {
+      atomicExch(err, 3);
+      break;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Layout of the barrier/statistics buffer: [0] arrivals, [1] generation,
+// [2, 2+ms) rotations per sweep, then per sweep the number of rotations whose
+// cos^2 exceeded the stopping threshold.
+static __host__ __device__ inline int jacobi_mc_offset(int max_sweeps) { return (2 + max_sweeps + 1) & ~1; }
+int jacobi_bar_words(int max_sweeps) { return jacobi_mc_offset(max_sweeps) + 2 * max_sweeps; }
+
+// J(:, pair) <- J(:, pair) * V for the 16-row band rt (FP64 MFMA, one wave),
+// split into the load of the band's J entries and the multiply-store.
+template <int LC>
+__device__ __forceinline__ void jacobi_j_band_load(int rt, int k, const double *__restrict__ Jg, long ldj,
+                                                   int bA, int bB, int lane, double *a) {
   constexpr int B = LC / 2;
-  extern __shared__ char smem[];
-  double *Sl = reinterpret_cast<double *>(smem);  // LC x k, column-major
-  double *Vl = Sl + (size_t)LC * k;               // LC x LC, column-major
-  __shared__ int sh_steprot, sh_rot, sh_total;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
-  const int w = blockIdx.x;
-  const int nblk = 2 * p;
-  const int total = LC * k;
+  const int i = rt * 16 + (lane & 15);
+#pragma unroll
+  for (int kk = 0; kk < LC / 4; ++kk) {
+    const int l = 4 * kk + (lane >> 4);
+    const int gl = (l < B) ? bA * B + l : bB * B + (l - B);
+    a[kk] = (i < k) ? Jg[i + (long)gl * ldj] : 0.0;
+  }
+}
+template <int LC>
+__device__ __forceinline__ void jacobi_j_band_mul(int rt, int k, double *__restrict__ Jg, long ldj,
+                                                  const double *V, int bA, int bB, int lane,
+                                                  const double *a) {
+  constexpr int B = LC / 2, NCT = LC / 16;
+  v4d_j acc[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) acc[ct] = v4d_j{0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < LC / 4; ++kk) {
+    const int l = 4 * kk + (lane >> 4);
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct)
+      acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], V[l + (ct * 16 + (lane & 15)) * LC], acc[ct], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);  // keep the V loads from all being hoisted (register pressure)
+  }
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rt * 16 + (lane >> 4) + 4 * v;
+      const int c = ct * 16 + (lane & 15);
+      const int gcol = (c < B) ? bA * B + c : bB * B + (c - B);
+      if (row < k) Jg[row + (long)gcol * ldj] = acc[ct][v];
+    }
+  }
+}
+template <int LC>
+__device__ __forceinline__ void jacobi_j_band(int rt, int k, double *__restrict__ Jg, long ldj,
+                                              const double *V, int bA, int bB, int lane) {
+  double a[LC / 4];
+  jacobi_j_band_load<LC>(rt, k, Jg, ldj, bA, bB, lane, a);
+  jacobi_j_band_mul<LC>(rt, k, Jg, ldj, V, bA, bB, lane, a);
+}
+
+// KL: rows per lane of a 16-lane group (k <= 16*KL); LC = 2b local columns
+// (multiple of 16, <= 64); NT threads.  Each 16-lane group owns one column
+// pair per round, so a wave rotates four pairs at once and the dot products
+// need only row-local DPP sums.  Waves not needed for the B = LC/2 pairs of a
+// round apply the previous step's accumulated rotations to J meanwhile (the
+// J columns of that pair are not touched by anyone else until the next grid
+// barrier, which orders them before their next owner's update).
+// Convergence: a sweep without rotations, or a sweep none of whose rotations
+// had cos^2 > stop_cos2 (they leave cosines of O(cos^2) behind).
+template <int KL, int LC, int NT>
+__global__ __launch_bounds__(NT) void k_jacobi3(int k, int p, double *__restrict__ Sg, long lds,
+                                                double *__restrict__ Jg, long ldj, double tol,
+                                                double stop_cos2, int max_sweeps, int *sweeps_out,
+                                                unsigned *bar, int *err, unsigned long long *dbg) {
+  constexpr int B = LC / 2;
+  constexpr int LDC = 16 * KL + 16;  // column stride: 128 B mod 256 B, groups alternate bank halves
+  constexpr int NW = NT / 64;
+  constexpr int WR = B / 4;
+  constexpr int NH = NW - WR;
+  static_assert(B % 4 == 0 && WR < NW, "block pair does not fit the workgroup");
+  constexpr int TPC = NT / LC;                 // threads per column for loads and stores
+  constexpr int NJ = (LDC + TPC - 1) / TPC;    // rows per thread
+  static_assert(NT % LC == 0, "threads must divide evenly over the columns");
+  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
+  double *Sl = reinterpret_cast<double *>(smem);  // LC x LDC
+  double *Vl = Sl + LC * LDC;                     // LC x LC, column-major (this step)
+  double *Vp = Vl + LC * LC;                      // LC x LC (previous step, pending for J)
+  __shared__ int sh_steprot, sh_rot;
+  __shared__ unsigned sh_rsync;
+  __shared__ int sh_big, sh_total, sh_bigtot;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int grp = tid >> 4, li = tid & 15;
+  const int cc = tid / TPC, i0 = tid - cc * TPC;
+  const int w = blockIdx.x, nblk = 2 * p;
+  const int nband = (k + 15) >> 4;
+  unsigned *rotcnt = bar + 2;
+  unsigned *bigcnt = bar + jacobi_mc_offset(max_sweeps);
+  const double tol2 = tol * tol;
   unsigned gen = 0;
   bool converged = false;
-  int sweep = 0;
+  int pend = 0, pA = 0, pB = 0;
+  int sweep;
   for (sweep = 0; sweep < max_sweeps && !converged; ++sweep) {
-    if (tid == 0) sh_rot = 0;
-    const int outer = nblk - 1;
-    for (int step = 0; step < outer; ++step) {
+    if (tid == 0) {
+      sh_rot = 0;
+      sh_big = 0;
+    }
+    int mybig = 0;  // rotations with cos^2 > stop_cos2 (this lane's groups)
+    for (int step = 0; step < nblk - 1; ++step) {
       int bA, bB;
       circle_pair(nblk, step, w, bA, bB);
-      // load the S columns of the block pair (batched, 8 independent loads per pass)
-      for (int base = 0; base < total; base += 8 * 1024) {
-        double v[8];
+      {
+        // thread -> (column c, rows i0 + TPC*j): invariants stay few, offsets immediate
+        const int gcol = (cc < B) ? bA * B + cc : bB * B + (cc - B);
+        const double *src = Sg + (long)gcol * lds;
+        double v[NJ];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int idx = base + tid + j * 1024;
-          v[j] = 0.0;
-          if (idx < total) {
-            const int c = idx / k, i = idx - c * k;
-            const int gcol = (c < B) ? bA * B + c : bB * B + (c - B);
-            v[j] = Sg[i + (long)gcol * lds];
-          }
+        for (int j = 0; j < NJ; ++j) {
+          const int i = i0 + TPC * j;
+          v[j] = (i < k) ? src[i] : 0.0;
         }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int idx = base + tid + j * 1024;
-          if (idx < total) Sl[idx] = v[j];
-        }
+        for (int j = 0; j < NJ; ++j)
+          if (LDC % TPC == 0 || i0 + TPC * j < LDC) Sl[cc * LDC + i0 + TPC * j] = v[j];
       }
-      for (int idx = tid; idx < LC * LC; idx += blockDim.x) Vl[idx] = (idx % LC == idx / LC) ? 1.0 : 0.0;
-      if (tid == 0) sh_steprot = 0;
+      for (int idx = tid; idx < LC * LC; idx += NT) Vl[idx] = ((idx % LC) == (idx / LC)) ? 1.0 : 0.0;
+      if (tid == 0) {
+        sh_steprot = 0;
+        sh_rsync = 0u;
+      }
       __syncthreads();
       const bool rec = dbg && sweep == 0 && w == 0 && tid == 0;
       if (rec) dbg[step * 4 + 0] = __builtin_amdgcn_s_memrealtime();
       const int rounds = (step == 0) ? LC - 1 : B;
-      for (int t = 0; t < rounds; ++t) {
-        for (int pr = wv; pr < B; pr += nw) {
-          int pa, pb;
-          if (step == 0) {
-            circle_pair(LC, t, pr, pa, pb);
-          } else {
-            pa = pr;
-            pb = B + (pr + t) % B;
-          }
-          double *cp = Sl + (long)pa * k;
-          double *cq = Sl + (long)pb * k;
-          double x[KL], y[KL];
-          double al = 0.0, be = 0.0, ga = 0.0;
+      int myrot = 0;
+      if (wv < WR) {
+        // Rotating waves, synchronised per round through an LDS counter so the
+        // helper waves never hold them up.  After the first step of a sweep the
+        // pairs are (g, B + (g + t) mod B): column g stays in registers for the
+        // whole step and only the partner column moves through LDS.
+        constexpr int NV = LC / 16;
+        const bool xs = step > 0;
+        double x[KL], vx[NV];
+        if (xs) {
 #pragma unroll
-          for (int q = 0; q < KL; ++q) {
-            const int i = lane + 64 * q;
-            const int ic = i < k ? i : k - 1;  // branch-free: clamp, then zero
-            const double xv = cp[ic], yv = cq[ic];
-            x[q] = i < k ? xv : 0.0;
-            y[q] = i < k ? yv : 0.0;
-          }
+          for (int q = 0; q < KL; ++q) x[q] = Sl[grp * LDC + li + 16 * q];
 #pragma unroll
-          for (int q = 0; q < KL; ++q) {
-            al += x[q] * x[q];
-            be += y[q] * y[q];
-            ga += x[q] * y[q];
-          }
-          wave_sum3(al, be, ga);
-          if (ga * ga > tol * tol * al * be) {
-            const double zeta = (be - al) / (2.0 * ga);
-            double tt;
-            if (fabs(zeta) > 1e150) tt = 0.5 / zeta;
-            else tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-            const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
-#pragma unroll
-            for (int q = 0; q < KL; ++q) {
-              const int i = lane + 64 * q;
-              if (i < k) {
-                cp[i] = cs * x[q] - sn * y[q];
-                cq[i] = sn * x[q] + cs * y[q];
-              }
-            }
-            if (lane < LC) {
-              const double vp = Vl[pa * LC + lane], vq = Vl[pb * LC + lane];
-              Vl[pa * LC + lane] = cs * vp - sn * vq;
-              Vl[pb * LC + lane] = sn * vp + cs * vq;
-            }
-            if (lane == 0) atomicAdd(&sh_steprot, 1);
-          }
+          for (int r = 0; r < NV; ++r) vx[r] = Vl[grp * LC + li + 16 * r];
         }
-        __syncthreads();
+        for (int t = 0; t < rounds; ++t) {
+          int pa, pb;
+          if (!xs) {
+            circle_pair(LC, t, grp, pa, pb);
+          } else {
+            pa = grp;
+            const int q = grp + t;
+            pb = B + (q >= B ? q - B : q);
+          }
+          double *cp = Sl + pa * LDC + li;
+          double *cq = Sl + pb * LDC + li;
+          double y[KL], vy[NV];
+          if (!xs) {
+#pragma unroll
+            for (int q = 0; q < KL; ++q) x[q] = cp[16 * q];
+#pragma unroll
+            for (int r = 0; r < NV; ++r) vx[r] = Vl[pa * LC + li + 16 * r];
+          }
+#pragma unroll
+          for (int q = 0; q < KL; ++q) y[q] = cq[16 * q];
+#pragma unroll
+          for (int r = 0; r < NV; ++r) vy[r] = Vl[pb * LC + li + 16 * r];
+          double al0 = 0.0, al1 = 0.0, be0 = 0.0, be1 = 0.0, ga0 = 0.0, ga1 = 0.0;
+#pragma unroll
+          for (int q = 0; q < KL; q += 2) {
+            al0 = __builtin_fma(x[q], x[q], al0);
+            be0 = __builtin_fma(y[q], y[q], be0);
+            ga0 = __builtin_fma(x[q], y[q], ga0);
+            al1 = __builtin_fma(x[q + 1], x[q + 1], al1);
+            be1 = __builtin_fma(y[q + 1], y[q + 1], be1);
+            ga1 = __builtin_fma(x[q + 1], y[q + 1], ga1);
+          }
+          double al = al0 + al1, be = be0 + be1, ga = ga0 + ga1;
+          row16_sum3(al, be, ga);
+          const double g2 = ga * ga, ab = al * be;
+          const bool rot = g2 > tol2 * ab && ab >= DBL_MIN;
+          // branch-free: a pair that does not rotate gets c = 1, s = 0 (exact identity)
+          double cs, sn;
+          jrot(al, be, ga, cs, sn);
+          cs = rot ? cs : 1.0;
+          sn = rot ? sn : 0.0;
+          mybig += (rot && g2 > stop_cos2 * ab) ? 1 : 0;
+          myrot += rot ? 1 : 0;
+#pragma unroll
+          for (int q = 0; q < KL; ++q) {
+            const double xn = cs * x[q] - sn * y[q];
+            y[q] = sn * x[q] + cs * y[q];
+            x[q] = xn;
+            cq[16 * q] = y[q];
+          }
+#pragma unroll
+          for (int r = 0; r < NV; ++r) {
+            const double vn = cs * vx[r] - sn * vy[r];
+            vy[r] = sn * vx[r] + cs * vy[r];
+            vx[r] = vn;
+            Vl[pb * LC + li + 16 * r] = vy[r];
+          }
+          if (!xs) {
+#pragma unroll
+            for (int q = 0; q < KL; ++q) cp[16 * q] = x[q];
+#pragma unroll
+            for (int r = 0; r < NV; ++r) Vl[pa * LC + li + 16 * r] = vx[r];
+          }
+          rot_sync(&sh_rsync, (unsigned)(WR * (t + 1)), err);
+        }
+        if (xs) {
+#pragma unroll
+          for (int q = 0; q < KL; ++q) Sl[grp * LDC + li + 16 * q] = x[q];
+#pragma unroll
+          for (int r = 0; r < NV; ++r) Vl[grp * LC + li + 16 * r] = vx[r];
+        }
+      } else if (pend) {
+        // helper waves: the previous step's rotations applied to J
+        for (int rt = wv - WR; rt < nband; rt += NH) jacobi_j_band<LC>(rt, k, Jg, ldj, Vp, pA, pB, lane);
       }
+      if (li == 0 && myrot > 0) atomicAdd(&sh_steprot, myrot);
+      if (li == 0 && mybig > 0) atomicAdd(&sh_big, mybig);
+      mybig = 0;
+      __syncthreads();
       const int rot = sh_steprot;
       if (rec) dbg[step * 4 + 1] = __builtin_amdgcn_s_memrealtime();
       if (rot > 0) {
-        for (int base = 0; base < total; base += 8 * 1024) {
+        {
+          const int gcol = (cc < B) ? bA * B + cc : bB * B + (cc - B);
+          double *dst = Sg + (long)gcol * lds;
 #pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const int idx = base + tid + j * 1024;
-            if (idx < total) {
-              const int c = idx / k, i = idx - c * k;
-              const int gcol = (c < B) ? bA * B + c : bB * B + (c - B);
-              Sg[i + (long)gcol * lds] = Sl[idx];
-            }
+          for (int j = 0; j < NJ; ++j) {
+            const int i = i0 + TPC * j;
+            if (i < k) dst[i] = Sl[cc * LDC + i];
           }
         }
-        // J(:, pair) <- J(:, pair) * V with FP64 MFMA; each wave owns 16-row bands
-        constexpr int NCT = LC / 16;
-        const int nrt = (k + 15) / 16;
-        for (int rt = wv; rt < nrt; rt += nw) {
-          const int i = rt * 16 + (lane & 15);
-          double a[LC / 4];
-#pragma unroll
-          for (int kk = 0; kk < LC / 4; ++kk) {
-            const int l = 4 * kk + (lane >> 4);
-            const int gl = (l < B) ? bA * B + l : bB * B + (l - B);
-            a[kk] = (i < k) ? Jg[i + (long)gl * ldj] : 0.0;
-          }
-          v4d_j acc[NCT];
-#pragma unroll
-          for (int ct = 0; ct < NCT; ++ct) acc[ct] = v4d_j{0, 0, 0, 0};
-#pragma unroll
-          for (int kk = 0; kk < LC / 4; ++kk) {
-            const int l = 4 * kk + (lane >> 4);
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct)
-              acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], Vl[l + (ct * 16 + (lane & 15)) * LC], acc[ct], 0, 0, 0);
-          }
-#pragma unroll
-          for (int ct = 0; ct < NCT; ++ct) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const int row = rt * 16 + (lane >> 4) + 4 * v;
-              const int c = ct * 16 + (lane & 15);
-              const int gcol = (c < B) ? bA * B + c : bB * B + (c - B);
-              if (row < k) Jg[row + (long)gcol * ldj] = acc[ct][v];
-            }
-          }
-        }
+        for (int idx = tid; idx < LC * LC; idx += NT) Vp[idx] = Vl[idx];
+        pend = 1;
+        pA = bA;
+        pB = bB;
+      } else {
+        pend = 0;
       }
       if (tid == 0) sh_rot += rot;
       if (rec) dbg[step * 4 + 2] = __builtin_amdgcn_s_memrealtime();
@@ -251,15 +409,27 @@ __global__ __launch_bounds__(1024) void k_jacobi2(int k, int p, double *__restri
       }
       if (rec) dbg[step * 4 + 3] = __builtin_amdgcn_s_memrealtime();
     }
+    // sweep totals: rotations, and rotations above the stopping cosine
+    __syncthreads();
     if (p > 1) {
-      if (tid == 0) atomicAdd(&bar[2 + sweep], (unsigned)sh_rot);
+      if (tid == 0) {
+        atomicAdd(&rotcnt[sweep], (unsigned)sh_rot);
+        atomicAdd(&bigcnt[sweep], (unsigned)sh_big);
+      }
       if (!grid_sync(bar, p, gen, err)) return;
-      if (tid == 0) sh_total = (int)__hip_atomic_load(&bar[2 + sweep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      if (tid == 0) sh_total = sh_rot;
+      if (tid == 0) {
+        sh_total = (int)__hip_atomic_load(&rotcnt[sweep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        sh_bigtot = (int)__hip_atomic_load(&bigcnt[sweep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    } else if (tid == 0) {
+      sh_total = sh_rot;
+      sh_bigtot = sh_big;
     }
     __syncthreads();
-    converged = (sh_total == 0);
+    converged = (sh_total == 0) || (sh_bigtot == 0);
+  }
+  if (pend) {
+    for (int rt = wv; rt < nband; rt += NW) jacobi_j_band<LC>(rt, k, Jg, ldj, Vp, pA, pB, lane);
   }
   if (w == 0 && tid == 0) {
     sweeps_out[0] += sweep;
@@ -284,14 +454,14 @@ __global__ void k_jacobi_setup(int k, int kpad, const double *__restrict__ S, in
   }
 }
 
-// Local column count 2b is a multiple of 16 (MFMA tiles of the J update) and
-// at most 64 (four accumulator tiles per wave).
+// Local column count 2b is 16 or 32 (MFMA tiles of the J update); above k = 32
+// the matrix is split over p = ceil(k/32) cooperating workgroups.
 JacobiPlan jacobi_plan(int64_t k) {
   JacobiPlan pl;
   pl.k = k;
   pl.rows = 2 * k;
-  if (k <= 64) {
-    pl.b = (int)(8 * ((std::max<int64_t>(k, 1) + 15) / 16));
+  if (k <= 32) {
+    pl.b = k <= 16 ? 8 : 16;
     pl.p = 1;
   } else {
     pl.b = 16;
@@ -302,47 +472,47 @@ JacobiPlan jacobi_plan(int64_t k) {
 }
 size_t jacobi_ws_bytes(const JacobiPlan &pl) { return (size_t)pl.rows * pl.kpad * sizeof(double); }
 
-template <int KL, int LC>
+template <int KL, int LC, int NT>
 static void launch_jacobi_t(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg,
-                            long ldj, double tol, int max_sweeps, int *sweeps_dev, unsigned *bar,
-                            int *err, unsigned long long *dbg) {
-  const size_t lsz = ((size_t)LC * pl.k + (size_t)LC * LC) * sizeof(double);
-  TTG_CHECK(lsz <= 160 * 1024, TTG_BAD_ARG, "Jacobi block does not fit in LDS");
-  TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi2<KL, LC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
+                            long ldj, double tol, double stop_cos2, int max_sweeps, int *sweeps_dev,
+                            unsigned *bar, int *err, unsigned long long *dbg) {
+  const size_t lsz = ((size_t)LC * (16 * KL + 16) + 2 * (size_t)LC * LC) * sizeof(double);
+  TTG_CHECK(lsz <= 160 * 1024 - 256, TTG_BAD_ARG, "Jacobi block does not fit in LDS");
+  TTG_CHECK(pl.k <= 16 * KL && 2 * pl.b == LC, TTG_BAD_ARG, "Jacobi plan/kernel mismatch");
+  TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi3<KL, LC, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
   int k = (int)pl.k, p = pl.p, ms = max_sweeps;
   if (p > 1) {
-    void *args[] = {&k, &p, &Sg, &lds, &Jg, &ldj, &tol, &ms, &sweeps_dev, &bar, &err, &dbg};
-    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_jacobi2<KL, LC>, dim3(p), dim3(1024), args, lsz, s));
+    void *args[] = {&k, &p, &Sg, &lds, &Jg, &ldj, &tol, &stop_cos2, &ms, &sweeps_dev, &bar, &err, &dbg};
+    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_jacobi3<KL, LC, NT>, dim3(p), dim3(NT), args, lsz, s));
   } else {
-    hipLaunchKernelGGL((k_jacobi2<KL, LC>), dim3(1), dim3(1024), lsz, s, k, p, Sg, lds, Jg, ldj, tol, ms,
-                       sweeps_dev, bar, err, dbg);
+    hipLaunchKernelGGL((k_jacobi3<KL, LC, NT>), dim3(1), dim3(NT), lsz, s, k, p, Sg, lds, Jg, ldj, tol,
+                       stop_cos2, ms, sweeps_dev, bar, err, dbg);
     TTG_HIP(hipGetLastError());
   }
 }
 
-template <int KL>
-static void launch_jacobi_kl(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg,
-                             long ldj, double tol, int ms, int *sw, unsigned *bar, int *err,
-                             unsigned long long *dbg) {
-  switch (2 * pl.b) {
-    case 16: launch_jacobi_t<KL, 16>(s, pl, Sg, lds, Jg, ldj, tol, ms, sw, bar, err, dbg); break;
-    case 32: launch_jacobi_t<KL, 32>(s, pl, Sg, lds, Jg, ldj, tol, ms, sw, bar, err, dbg); break;
-    case 48: launch_jacobi_t<KL, 48>(s, pl, Sg, lds, Jg, ldj, tol, ms, sw, bar, err, dbg); break;
-    case 64: launch_jacobi_t<KL, 64>(s, pl, Sg, lds, Jg, ldj, tol, ms, sw, bar, err, dbg); break;
-    default: throw Error(TTG_BAD_ARG, "unsupported Jacobi block size");
-  }
-}
-
 static void launch_jacobi(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg, long ldj,
-                          double tol, int max_sweeps, int *sweeps_dev, unsigned *bar, int *err,
-                          unsigned long long *dbg) {
-  TTG_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * (2 + max_sweeps), s));
+                          double tol, double stop_cos2, int max_sweeps, int *sweeps_dev, unsigned *bar,
+                          int *err, unsigned long long *dbg) {
+  TTG_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * jacobi_bar_words(max_sweeps), s));
   const int64_t k = pl.k;
-  if (k <= 64) launch_jacobi_kl<1>(s, pl, Sg, lds, Jg, ldj, tol, max_sweeps, sweeps_dev, bar, err, dbg);
-  else if (k <= 128) launch_jacobi_kl<2>(s, pl, Sg, lds, Jg, ldj, tol, max_sweeps, sweeps_dev, bar, err, dbg);
-  else if (k <= 256) launch_jacobi_kl<4>(s, pl, Sg, lds, Jg, ldj, tol, max_sweeps, sweeps_dev, bar, err, dbg);
-  else if (k <= 512) launch_jacobi_kl<8>(s, pl, Sg, lds, Jg, ldj, tol, max_sweeps, sweeps_dev, bar, err, dbg);
-  else throw Error(TTG_BAD_ARG, "Jacobi SVD supports k <= 512");
+  const int lc = 2 * pl.b;
+#define TTG_JAC(KL, LC, NT) launch_jacobi_t<KL, LC, NT>(s, pl, Sg, lds, Jg, ldj, tol, stop_cos2, max_sweeps, sweeps_dev, bar, err, dbg)
+  if (k <= 32) {
+    switch (lc) {
+      case 16: TTG_JAC(4, 16, 512); break;
+      case 32: TTG_JAC(4, 32, 512); break;
+      default: throw Error(TTG_BAD_ARG, "unsupported Jacobi block size");
+    }
+  } else {
+    TTG_CHECK(lc == 32, TTG_BAD_ARG, "unsupported Jacobi block size");
+    if (k <= 64) TTG_JAC(4, 32, 512);
+    else if (k <= 128) TTG_JAC(8, 32, 512);
+    else if (k <= 256) TTG_JAC(16, 32, 512);
+    else if (k <= 512) TTG_JAC(32, 32, 512);
+    else throw Error(TTG_BAD_ARG, "Jacobi SVD supports k <= 512");
+  }
+#undef TTG_JAC
 }
 
 __global__ void k_polar_m(int n, double *M) {
@@ -371,7 +541,10 @@ void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool trans
   // algorithmic flops of the small SVD (R-SVD of the k x k factor, GVL Fig. 8.6.1: 20k^3)
   ProfScope prof(KC_JACOBI, s, 20.0 * (double)k * k * k, 8.0 * 4.0 * (double)k * k);
   const double tol = sqrt((double)std::max(k, 1)) * DBL_EPSILON;
-  launch_jacobi(s, pl, Y, (long)pl.rows, Y + k, (long)pl.rows, tol, max_sweeps, sweeps_dev,
+  // stop after a sweep whose largest pair cosine is <= 1e-8: its rotations
+  // leave cosines of O(1e-16) behind
+  const double stop_cos2 = 1e-16;
+  launch_jacobi(s, pl, Y, (long)pl.rows, Y + k, (long)pl.rows, tol, stop_cos2, max_sweeps, sweeps_dev,
                 barrier_dev, err_dev, dbg);
   // polar correction of J (and of S J with it)
   double *M = ws, *Y2 = ws + (size_t)kpad * kpad;

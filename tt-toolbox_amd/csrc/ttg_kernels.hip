@@ -429,7 +429,7 @@ __global__ __launch_bounds__(1024) void k_sort_chop(const double *__restrict__ Y
                                                     int kpad, double ep, long rmax,
                                                     double *__restrict__ sv, int *__restrict__ perm,
                                                     int *out_i, double *out_d) {
-  extern __shared__ double sig[];  // kpad
+  extern __shared__ __align__(16) double sig[];  // kpad
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   for (int j = wv; j < kpad; j += nw) {
     const double *col = Y + (long)j * ldy;

@@ -100,7 +100,7 @@ template <class T> struct Rounder {
     a.template take<int>(16);                // scal_i
     a.template take<int>(16);                // jerr
     a.template take<int>(16);                // jsweeps
-    a.template take<unsigned>(kMaxSweeps + 16);
+    a.template take<unsigned>(jacobi_bar_words(kMaxSweeps));
     a.template take<double>(16);             // scal_d
   }
 
@@ -135,7 +135,7 @@ template <class T> struct Rounder {
     scal_i = a.take<int>(16);
     jerr = a.take<int>(16);
     jsweeps = a.take<int>(16);
-    bar = a.take<unsigned>(kMaxSweeps + 16);
+    bar = a.take<unsigned>(jacobi_bar_words(kMaxSweeps));
     scal_d = a.take<double>(16);
   }
 
@@ -283,6 +283,7 @@ template <class T> struct Rounder {
     r1 = h_i[0];
     if (qr_ok) {
       if (h_i[1] == 2) throw Error(TTG_HIP_ERROR, "Jacobi grid barrier timed out");
+      if (h_i[1] == 3) throw Error(TTG_HIP_ERROR, "Jacobi round barrier did not complete");
       if (h_i[1] == 1) noconv = true;
       sweeps_total += h_i[2];
       sweeps_max = std::max<int64_t>(sweeps_max, h_i[2]);

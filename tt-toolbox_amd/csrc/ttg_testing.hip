@@ -135,7 +135,7 @@ ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS,
     TTG_CHECK(pl.rows * pl.kpad <= ycap, TTG_NEED_SPACE, "Y capacity");
     double *dS = f.add(up(S, k * k, s)), *dY = f.add(up<double>(nullptr, pl.rows * pl.kpad, s));
     int *dsw = f.add(up<int>(nullptr, 4, s)), *derr = f.add(up<int>(nullptr, 4, s));
-    unsigned *bar = f.add(up<unsigned>(nullptr, 128, s));
+    unsigned *bar = f.add(up<unsigned>(nullptr, jacobi_bar_words(60), s));
     double *ws = f.add(up<double>(nullptr, jacobi_polar_ws_elems(pl), s));
     jacobi_svd(s, pl, dS, transS != 0, dY, dsw, bar, derr, 60, ws);
     down(Y, dY, pl.rows * pl.kpad, s);
@@ -155,11 +155,29 @@ ttg_status ttg_test_jacobi_timing(ttg_ctx *ctx, int64_t k, const double *S, int6
     JacobiPlan pl = jacobi_plan(k);
     double *dS = f.add(up(S, k * k, s)), *dY = f.add(up<double>(nullptr, pl.rows * pl.kpad, s));
     int *dsw = f.add(up<int>(nullptr, 4, s)), *derr = f.add(up<int>(nullptr, 4, s));
-    unsigned *bar = f.add(up<unsigned>(nullptr, 128, s));
+    unsigned *bar = f.add(up<unsigned>(nullptr, jacobi_bar_words(60), s));
     double *ws = f.add(up<double>(nullptr, jacobi_polar_ws_elems(pl), s));
     unsigned long long *dbg = f.add(up<unsigned long long>(nullptr, ntimes, s));
     TTG_HIP(hipMemsetAsync(dbg, 0, ntimes * 8, s));
     jacobi_svd(s, pl, dS, true, dY, dsw, bar, derr, 60, ws, dbg);
+    down(reinterpret_cast<unsigned long long *>(times), dbg, ntimes, s);
+  })
+}
+
+// Cholesky with timestamps: times[3b] block b start, [3b+1] after the diagonal
+// factor, [3b+2] after the panel solve, [3 nblocks] end (100 MHz ticks).
+ttg_status ttg_test_chol_timing(ttg_ctx *ctx, int64_t k, const double *G, int64_t *times,
+                                int64_t ntimes) {
+  GUARD(ctx, {
+    hipStream_t s = ctx->stream;
+    Frees f;
+    TTG_CHECK(ntimes >= 3 * ((k + 31) / 32) + 1, TTG_BAD_ARG, "times too short");
+    double *dG = f.add(up(G, k * k, s)), *dR = f.add(up<double>(nullptr, k * k, s));
+    double *dst = f.add(up<double>(nullptr, 8, s));
+    int *dfl = f.add(up<int>(nullptr, 4, s));
+    unsigned long long *dbg = f.add(up<unsigned long long>(nullptr, ntimes, s));
+    TTG_HIP(hipMemsetAsync(dbg, 0, ntimes * 8, s));
+    chol_factor(s, k, dG, k, 0.0, dR, dfl, dst, dbg);
     down(reinterpret_cast<unsigned long long *>(times), dbg, ntimes, s);
   })
 }
