@@ -35,3 +35,24 @@ def test_fused_round_matches_oracle(gpu_ctx):
     print("fused ranks", y.r.tolist(), info, flush=True)
     assert y.r.tolist() == ref.r.tolist()
     assert O.tt_rel_diff(O.TT(y.n, y.r, y.core), ref) <= 1e-12
+
+
+def test_fused_round_config2_full_size(gpu_ctx):
+    """Config 2 at its real size: d=12, A rank 5 (identity Kronecker + 0.1 N(0,1)),
+    x rank 64, product rank 320, eps 1e-6 — ranks identical to the oracle's
+    mtimes-then-round and relative difference <= 1e-12, plain form."""
+    c = O.make_config(2)
+    a, x = c["a"], c["x"]
+    ref = O.tt_round(O.ttm_mtimes(a, x), c["eps"])
+    A = ttgpu.TTMatrix(a.n, a.m, a.tt.r, a.tt.core)
+    X = ttgpu.TTTensor(x.n, x.r, x.core)
+    y = ttgpu.mtimes_round(A, X, c["eps"])
+    print("config 2 ranks", y.r.tolist(), flush=True)
+    assert y.r.tolist() == ref.r.tolist()
+    assert O.tt_rel_diff(O.TT(y.n, y.r, y.core), ref) <= 1e-12
+    # unfused path: A*x on the GPU, then round on the GPU
+    p = ttgpu.mtimes(A, X)
+    assert p.r.tolist() == (a.tt.r * x.r).tolist()
+    y2 = ttgpu.round(p, c["eps"])
+    assert y2.r.tolist() == ref.r.tolist()
+    assert O.tt_rel_diff(O.TT(y2.n, y2.r, y2.core), ref) <= 1e-12
