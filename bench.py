@@ -3,8 +3,8 @@
 with graded core spectra), eps=1e-6.
 
 One step = one ``round(X, 1e-6)`` of that tensor with its cores already resident
-in HBM, in the mixed form (FP32 MFMA contractions/Grams, FP64 r x r Cholesky
-and Jacobi SVD).  The FP64 plain form is timed on the same input for the
+in HBM, in the mixed form (FP32 MFMA Gram products; FP64 storage, solves,
+contractions, r x r Cholesky and Jacobi SVD).  The FP64 plain form is timed on the same input for the
 speedup, both results are compared with the FP64 CPU oracle (the
 restatement of @tt_tensor/round.m), and the oracle itself is timed on the host
 cores as the CPU baseline.  Prints one JSON line.
@@ -145,7 +145,7 @@ def main():
     size_in = sum(int(r[i]) * int(n[i]) * int(r[i + 1]) for i in range(d))
     size_q = sum(rq[i] * int(n[i]) * rq[i + 1] for i in range(d))
     size_out = sum(ro[i] * int(n[i]) * ro[i + 1] for i in range(d))
-    byt = 4.0 * (size_in + 2 * size_q + size_out)  # FP32 storage in the mixed form
+    byt = 8.0 * (size_in + 2 * size_q + size_out)  # FP64 storage in both forms
 
     result = {
         "metric": "TT rounds/sec d=32 n=8 r=256: mixed vs FP64 on GPU at rel err <=1e-6",

@@ -38,7 +38,8 @@ typedef enum {
 
 typedef enum {
   TTG_PREC_FP64 = 0,      /* plain form: FP64 throughout */
-  TTG_PREC_FP32 = 1       /* mixed form: FP32 MFMA contractions/Grams, FP64 r x r Cholesky and Jacobi SVD */
+  TTG_PREC_FP32 = 1       /* mixed form: FP32 MFMA Gram products (shifted CholeskyQR2); FP64 storage,
+                             triangular solves, contractions, r x r Cholesky and Jacobi SVD */
 } ttg_precision;
 
 typedef struct ttg_ctx ttg_ctx;

@@ -107,6 +107,7 @@ def test_mixed_small(gpu_ctx):
     y = ttgpu.round(to_gpu_tt(x), 1e-6, precision="fp32", info=info)
     err = O.tt_rel_diff(to_oracle(y), ref)
     print("mixed ranks", y.r.tolist(), "ref", ref.r.tolist(), "err", err, info, flush=True)
+    assert y.r.tolist() == ref.r.tolist()
     assert err <= 1e-6
 
 

@@ -70,10 +70,13 @@ constexpr int GBM = 64, GBN = 64, GBK = 16;
 // accumulator element v of lane l is D[MF<T>::row(l, v)][l%16].
 // Split-K: blockIdx.z selects K range [z*kchunk, ...); with gridDim.z > 1 the
 // raw partial product goes to P + z*M*N (ld M) and k_splitk_reduce finishes.
-template <class T, class TO, bool TA, bool TB>
+// TI is the storage type of A and B, T the MFMA type: TI = double, T = float
+// rounds the operands to FP32 while staging them (FP32 Gram products of FP64
+// data).
+template <class TI, class T, class TO, bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T alpha,
-                                              const T *__restrict__ A, long lda,
-                                              const T *__restrict__ B, long ldb, T beta,
+                                              const TI *__restrict__ A, long lda,
+                                              const TI *__restrict__ B, long ldb, T beta,
                                               TO *__restrict__ C, long ldc, T *__restrict__ P) {
   __shared__ T As[GBK][GBM + 4];
   __shared__ T Bs[GBK][GBN + 4];
@@ -95,14 +98,14 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int k = kk + 4 * i, gm = m0 + mm, gk = k0 + k;
-        As[k][mm] = (gm < M && gk < kend) ? A[gm + (long)gk * lda] : T(0);
+        As[k][mm] = (gm < M && gk < kend) ? T(A[gm + (long)gk * lda]) : T(0);
       }
     } else {
       const int kk = tid & 15, mm = tid >> 4;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int m = mm + 16 * i, gm = m0 + m, gk = k0 + kk;
-        As[kk][m] = (gm < M && gk < kend) ? A[gk + (long)gm * lda] : T(0);
+        As[kk][m] = (gm < M && gk < kend) ? T(A[gk + (long)gm * lda]) : T(0);
       }
     }
     if (!TB) {
@@ -110,14 +113,14 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int n = nn + 16 * i, gn = n0 + n, gk = k0 + kk;
-        Bs[kk][n] = (gn < N && gk < kend) ? B[gk + (long)gn * ldb] : T(0);
+        Bs[kk][n] = (gn < N && gk < kend) ? T(B[gk + (long)gn * ldb]) : T(0);
       }
     } else {
       const int nn = tid & 63, kk = tid >> 6;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int k = kk + 4 * i, gn = n0 + nn, gk = k0 + k;
-        Bs[k][nn] = (gn < N && gk < kend) ? B[gn + (long)gk * ldb] : T(0);
+        Bs[k][nn] = (gn < N && gk < kend) ? T(B[gn + (long)gk * ldb]) : T(0);
       }
     }
     __syncthreads();
@@ -187,9 +190,9 @@ int64_t gemm_ws_elems(int64_t M, int64_t N, int64_t K, int num_cus) {
   return s > 1 ? (int64_t)s * M * N : 0;
 }
 
-template <class T, class TO>
+template <class TI, class T, class TO>
 static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, int64_t K, T alpha,
-                      const T *A, int64_t lda, const T *B, int64_t ldb, T beta, TO *C, int64_t ldc,
+                      const TI *A, int64_t lda, const TI *B, int64_t ldb, T beta, TO *C, int64_t ldc,
                       T *ws, int64_t ws_elems, int num_cus) {
   if (M <= 0 || N <= 0) return;
   int S = splitk_count(M, N, K, num_cus);
@@ -202,9 +205,9 @@ static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, in
   if (K <= 0) kchunk = 0;
   dim3 grid((unsigned)((M + GBM - 1) / GBM), (unsigned)((N + GBN - 1) / GBN), (unsigned)S);
   ProfScope prof(KC_GEMM, st, 2.0 * (double)M * (double)N * (double)K,
-                 (double)sizeof(T) * ((double)M * K + (double)K * N) + (double)sizeof(TO) * M * N);
+                 (double)sizeof(TI) * ((double)M * K + (double)K * N) + (double)sizeof(TO) * M * N);
 #define TTG_GEMM_LAUNCH(TA_, TB_)                                                            \
-  hipLaunchKernelGGL((k_gemm<T, TO, TA_, TB_>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K, \
+  hipLaunchKernelGGL((k_gemm<TI, T, TO, TA_, TB_>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K, \
                      kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws)
   if (!ta && !tb) TTG_GEMM_LAUNCH(false, false);
   else if (ta && !tb) TTG_GEMM_LAUNCH(true, false);
@@ -225,19 +228,26 @@ template <>
 void gemm<double>(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha,
                   const double *A, int64_t lda, const double *B, int64_t ldb, double beta,
                   double *C, int64_t ldc, double *ws, int64_t wse, int cus) {
-  gemm_impl<double, double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ws, wse, cus);
+  gemm_impl<double, double, double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ws, wse, cus);
 }
 template <>
 void gemm<float>(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha,
                  const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
                  int64_t ldc, float *ws, int64_t wse, int cus) {
-  gemm_impl<float, float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ws, wse, cus);
+  gemm_impl<float, float, float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ws, wse, cus);
 }
 // FP32 operands, FP64 result (Gram matrices of the mixed form).
 void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
                      const float *A, int64_t lda, const float *B, int64_t ldb, double *C,
                      int64_t ldc, float *ws, int64_t wse, int cus) {
-  gemm_impl<float, double>(s, ta, tb, M, N, K, 1.0f, A, lda, B, ldb, 0.0f, C, ldc, ws, wse, cus);
+  gemm_impl<float, float, double>(s, ta, tb, M, N, K, 1.0f, A, lda, B, ldb, 0.0f, C, ldc, ws, wse, cus);
+}
+// FP64 operands rounded to FP32 on the way into LDS, FP32 MFMA, FP64 result
+// (the Gram products of the mixed form; ws holds FP32 split-K partials).
+void gemm_f64_via_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
+                      const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                      int64_t ldc, float *ws, int64_t wse, int cus) {
+  gemm_impl<double, float, double>(s, ta, tb, M, N, K, 1.0f, A, lda, B, ldb, 0.0f, C, ldc, ws, wse, cus);
 }
 
 // ---------------------------------------------------------------------------

@@ -2,8 +2,9 @@
 //
 // Sweep 1 (round.m:39-53), bond i = 0..d-2, left unfolding A = core_i as
 // (r_i n_i) x r_{i+1}:
-//   tall  -> CholeskyQR2 (plain FP64) / shifted CholeskyQR3 (mixed: FP32
-//            Grams and TRSM-GEMMs, FP64 Cholesky); rejected factorizations
+//   tall  -> CholeskyQR2 (plain: FP64 Grams; mixed: FP32 MFMA Grams of the
+//            FP64 data, escalated to FP64 Grams when rejected; FP64 Cholesky
+//            and solves in both); rejected factorizations
 //            (Cholesky breakdown, diag(R) spread > 1e7, second-pass R far from
 //            I) are redone by one-workgroup Householder QR;
 //   wide  -> Householder QR (Q square, rank capped at r_i n_i, round.m:48);
@@ -29,6 +30,9 @@ namespace ttg {
 void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
                      const float *A, int64_t lda, const float *B, int64_t ldb, double *C,
                      int64_t ldc, float *ws, int64_t wse, int cus);
+void gemm_f64_via_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
+                      const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                      int64_t ldc, float *ws, int64_t wse, int cus);
 
 static constexpr int kMaxRank = 512;
 static constexpr int kMaxSweeps = 60;
@@ -144,9 +148,13 @@ template <class T> struct Rounder {
          int64_t ldb, T *C, int64_t ldc) {
     gemm<T>(st, ta, tb, M, N, K, T(1), A, lda, B, ldb, T(0), C, ldc, gws, gws_elems, cus);
   }
-  void gram(const T *A, int64_t m, int64_t k, double *Gout) {
+  void gram(const T *A, int64_t m, int64_t k, double *Gout, bool g32) {
     if constexpr (std::is_same<T, double>::value) {
-      gemm<double>(st, true, false, k, k, m, 1.0, A, m, A, m, 0.0, Gout, k, gws, gws_elems, cus);
+      if (g32)  // FP32 MFMA Gram of FP64 data (its error only perturbs Q's orthogonality)
+        gemm_f64_via_f32(st, true, false, k, k, m, A, m, A, m, Gout, k, reinterpret_cast<float *>(gws),
+                         2 * gws_elems, cus);
+      else
+        gemm<double>(st, true, false, k, k, m, 1.0, A, m, A, m, 0.0, Gout, k, gws, gws_elems, cus);
     } else {
       gemm_f32_f64out(st, true, false, k, k, m, A, m, A, m, Gout, k, gws, gws_elems, cus);
     }
@@ -162,40 +170,29 @@ template <class T> struct Rounder {
 
   // One Cholesky-QR pass: R = chol(A^T A (+ shift)), Qout = A R^{-1} by triangular solve.
   void cholqr_pass(const T *A, int64_t m, int64_t k, double shift_rel, T *Qout, double *Rout,
-                   int *flag, double *st2) {
-    gram(A, m, k, G);
+                   int *flag, double *st2, bool g32) {
+    gram(A, m, k, G, g32);
     chol_factor(st, k, G, k, shift_rel, Rout, flag, st2);
     trsm_right_upper<T>(st, m, k, A, m, Rout, Qout, m);
   }
 
-  // Tall QR (m >= k): Qout (m x k, T), Rout (k x k, FP64).  flags[0..2] and
-  // stt[0..5] feed the acceptance test (qr_accept), checked by the caller.
+  // Tall QR (m >= k): Qout (m x k, T), Rout (k x k, FP64), by CholeskyQR2.
+  // flags[0..1] and stt[0..3] feed the acceptance test (qr_accept), checked
+  // by the caller.  g32: FP32 MFMA Gram products (the mixed form).  Their
+  // error only perturbs the orthogonality of Q (both Q = A R^{-1} solves are
+  // FP64, so A = Q R holds to FP64 accuracy); a bond whose conditioning is
+  // beyond FP32 Grams (Cholesky breakdown or the second pass far from I) is
+  // rejected and redone with FP64 Grams by the caller.
   void qr_tall_fast(const T *A, int64_t m, int64_t k, T *Qout, double *Rout, int *flags,
-                    double *stt) {
-    if (!mixed) {
-      cholqr_pass(A, m, k, 0.0, Q1, R1, flags + 0, stt + 0);
-      cholqr_pass(Q1, m, k, 0.0, Qout, R2, flags + 1, stt + 2);
-      gemm<double>(st, false, false, k, k, k, 1.0, R2, k, R1, k, 0.0, Rout, k, nullptr, 0, cus);
-    } else {
-      const double u32 = 5.96e-8;
-      const double shift_rel = 8.0 * u32 * std::sqrt((double)m) * (double)k;
-      cholqr_pass(A, m, k, shift_rel, Q1, R1, flags + 0, stt + 0);
-      cholqr_pass(Q1, m, k, 0.0, Q2, R2, flags + 1, stt + 2);
-      cholqr_pass(Q2, m, k, 0.0, Qout, R3, flags + 2, stt + 4);
-      gemm<double>(st, false, false, k, k, k, 1.0, R3, k, R2, k, 0.0, Rtmp, k, nullptr, 0, cus);
-      gemm<double>(st, false, false, k, k, k, 1.0, Rtmp, k, R1, k, 0.0, Rout, k, nullptr, 0, cus);
-    }
+                    double *stt, bool g32) {
+    cholqr_pass(A, m, k, 0.0, Q1, R1, flags + 0, stt + 0, g32);
+    cholqr_pass(Q1, m, k, 0.0, Qout, R2, flags + 1, stt + 2, g32);
+    gemm<double>(st, false, false, k, k, k, 1.0, R2, k, R1, k, 0.0, Rout, k, nullptr, 0, cus);
   }
   bool qr_accept(const int *hflags, const double *hstats) const {
-    if (!mixed) {
-      if (hflags[0] || hflags[1]) return false;
-      if (!(hstats[0] < 1e7)) return false;   // diag(R1) spread: kappa estimate
-      if (!(hstats[3] < 0.1)) return false;   // R2 ~ I: Q1 was nearly orthonormal
-      return true;
-    }
-    if (hflags[0] || hflags[1] || hflags[2]) return false;
-    if (!(hstats[2] < 1e7)) return false;     // second pass well conditioned
-    if (!(hstats[5] < 0.1)) return false;     // third pass R ~ I
+    if (hflags[0] || hflags[1]) return false;
+    if (!(hstats[0] < 1e7)) return false;   // diag(R1) spread: kappa estimate
+    if (!(hstats[3] < 0.1)) return false;   // R2 ~ I: Q1 was nearly orthonormal
     return true;
   }
   // Householder fallback (FP64) for the tall/wide QR of A (m x k).
@@ -215,7 +212,8 @@ template <class T> struct Rounder {
   // Bond i reads core i from (i == 0 ? buf0 : buf1) and core i+1 from buf0;
   // writes Q to buf2 and R*core_{i+1} to buf1.  Returns true when the
   // Cholesky-QR acceptance check is pending.
-  bool sweep1_bond(int64_t i, bool force_householder,
+  // level: 0 = the form's CholeskyQR2, 1 = FP64 CholeskyQR2, 2 = Householder
+  bool sweep1_bond(int64_t i, int level,
                    const std::function<void(int64_t, T *)> &gen) {
     const int64_t m = r[i] * n[i], k = r[i + 1];
     const int64_t kk = std::min(m, k);
@@ -223,8 +221,8 @@ template <class T> struct Rounder {
     T *Qdst = buf[2] + ofs[i];
     double *R = Rprod;
     bool pending = false;
-    if (m >= k && !force_householder) {
-      qr_tall_fast(A, m, k, Qdst, R, iflag + 4 * i, bstats + 8 * i);
+    if (m >= k && level < 2) {
+      qr_tall_fast(A, m, k, Qdst, R, iflag + 4 * i, bstats + 8 * i, mixed && level == 0);
       pending = true;
     } else {
       qr_householder(A, m, k, Qdst, R);
@@ -246,7 +244,7 @@ template <class T> struct Rounder {
   }
 
   // ---- sweep 2 -----------------------------------------------------------
-  void svd_bond(int64_t i, double ep, int64_t rmax_i, bool force_householder, int64_t &r1,
+  void svd_bond(int64_t i, double ep, int64_t rmax_i, int level, int64_t &r1,
                 bool &qr_ok, int64_t &kj, bool &wide) {
     const int64_t kw = r[i], N = n[i] * r[i + 1];
     wide = (N >= kw);
@@ -261,8 +259,8 @@ template <class T> struct Rounder {
       tallA = Tt;
     }
     TTG_HIP(hipMemsetAsync(fl, 0, 4 * sizeof(int), st));
-    if (!force_householder) {
-      qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats);
+    if (level < 2) {
+      qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats, mixed && level == 0);
     } else {
       qr_householder(tallA, mt, kj, Qs, R);
     }
@@ -279,7 +277,7 @@ template <class T> struct Rounder {
     TTG_HIP(hipMemcpyAsync(h_fl, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
     TTG_HIP(hipMemcpyAsync(h_st, stats, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
     TTG_HIP(hipStreamSynchronize(st));
-    qr_ok = force_householder || qr_accept(h_fl, h_st);
+    qr_ok = level == 2 || qr_accept(h_fl, h_st);
     r1 = h_i[0];
     if (qr_ok) {
       if (h_i[1] == 2) throw Error(TTG_HIP_ERROR, "Jacobi grid barrier timed out");
@@ -293,8 +291,13 @@ template <class T> struct Rounder {
   void sweep2_bond(int64_t i, double ep, int64_t rmax_i) {
     int64_t r1, kj;
     bool ok, wide;
-    svd_bond(i, ep, rmax_i, false, r1, ok, kj, wide);
-    if (!ok) svd_bond(i, ep, rmax_i, true, r1, ok, kj, wide);
+    // escalate on rejection: (mixed) FP64 CholeskyQR2, then Householder
+    int level = 0;
+    svd_bond(i, ep, rmax_i, level, r1, ok, kj, wide);
+    while (!ok) {
+      level = (level == 0 && mixed) ? 1 : 2;
+      svd_bond(i, ep, rmax_i, level, r1, ok, kj, wide);
+    }
     const int64_t kw = r[i], N = n[i] * r[i + 1];
     JacobiPlan pl = jacobi_plan(kj);
     T *core_i = other(i);
@@ -322,15 +325,16 @@ template <class T> struct Rounder {
   // ---- driver --------------------------------------------------------------
   void run(double eps, const int64_t *rmax, const std::function<void(int64_t, T *)> &gen) {
     // sweep 1, speculative: one synchronisation at its end; a rejected bond
-    // is recomputed with Householder QR together with every bond after it.
-    std::vector<char> force(d, 0);
+    // is recomputed one QR level up (mixed -> FP64 CholeskyQR2 -> Householder)
+    // together with every bond after it.
+    std::vector<int> force(d, 0);
     const std::vector<int64_t> r_in = r;
     int64_t start = 0;
     while (d > 1) {
       std::vector<int64_t> pending;
       for (int64_t i = start; i + 1 < d; ++i) {
         TTG_HIP(hipMemsetAsync(iflag + 4 * i, 0, 4 * sizeof(int), st));
-        if (sweep1_bond(i, force[i] != 0, gen)) pending.push_back(i);
+        if (sweep1_bond(i, force[i], gen)) pending.push_back(i);
       }
       if (pending.empty()) break;
       std::vector<int> hfl(4 * (d + 8));
@@ -345,7 +349,7 @@ template <class T> struct Rounder {
           break;
         }
       if (bad < 0) break;
-      force[bad] = 1;
+      force[bad] = (force[bad] == 0 && mixed) ? 1 : 2;
       for (int64_t j = bad + 1; j <= d; ++j) r[j] = r_in[j];
       start = bad;
     }
@@ -369,7 +373,7 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
                              const double *dcore, double eps, const int64_t *rmax, int64_t *r_out,
                              double *dcore_out, int64_t cap, int64_t *len_out, ttg_info *info,
                              const std::function<void(int64_t, T *)> &gen,
-                             const std::vector<int64_t> *gen_ranks) {
+                             const std::vector<int64_t> *gen_ranks, bool mixed) {
   auto t0 = std::chrono::steady_clock::now();
   validate_tt(d, n, r);
   if (rmax)
@@ -383,7 +387,7 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
   R.d = d;
   R.n.assign(n, n + d);
   R.r.assign(r, r + d + 1);
-  R.mixed = !std::is_same<T, double>::value;
+  R.mixed = mixed;
   std::vector<int64_t> sizes(d);
   R.ofs.assign(d + 1, 0);
   int64_t max_core = 1, kmax = 1;
@@ -469,12 +473,9 @@ ttg_status round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r
                      const double *dcore, double eps, const int64_t *rmax, int prec,
                      int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
                      ttg_info *info) {
-  if (prec == TTG_PREC_FP64)
-    return round_impl<double>(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info,
-                              nullptr, nullptr);
-  TTG_CHECK(prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
-  return round_impl<float>(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info,
-                           nullptr, nullptr);
+  TTG_CHECK(prec == TTG_PREC_FP64 || prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  return round_impl<double>(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info,
+                            nullptr, nullptr, prec == TTG_PREC_FP32);
 }
 
 ttg_status round_capacity(int64_t d, const int64_t *n, const int64_t *r, int64_t *cap) {
@@ -529,21 +530,13 @@ ttg_status mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int
   }
   hipStream_t st = ctx->stream;
   std::vector<int64_t> nn(n, n + d), mm(m, m + d), raa(ra, ra + d + 1), rxx(rx, rx + d + 1);
-  if (prec == TTG_PREC_FP64) {
-    std::function<void(int64_t, double *)> gen = [&](int64_t i, double *dst) {
-      mtimes_core<double>(st, raa[i], nn[i], mm[i], raa[i + 1], dacore + oa[i], rxx[i], rxx[i + 1],
-                          dxcore + ox[i], dst);
-    };
-    return round_impl<double>(ctx, d, n, rp.data(), nullptr, eps, rmax, r_out, dcore_out, cap,
-                              len_out, info, gen, &rp);
-  }
-  TTG_CHECK(prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
-  std::function<void(int64_t, float *)> gen = [&](int64_t i, float *dst) {
-    mtimes_core<float>(st, raa[i], nn[i], mm[i], raa[i + 1], dacore + oa[i], rxx[i], rxx[i + 1],
-                       dxcore + ox[i], dst);
+  TTG_CHECK(prec == TTG_PREC_FP64 || prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  std::function<void(int64_t, double *)> gen = [&](int64_t i, double *dst) {
+    mtimes_core<double>(st, raa[i], nn[i], mm[i], raa[i + 1], dacore + oa[i], rxx[i], rxx[i + 1],
+                        dxcore + ox[i], dst);
   };
-  return round_impl<float>(ctx, d, n, rp.data(), nullptr, eps, rmax, r_out, dcore_out, cap, len_out,
-                           info, gen, &rp);
+  return round_impl<double>(ctx, d, n, rp.data(), nullptr, eps, rmax, r_out, dcore_out, cap,
+                            len_out, info, gen, &rp, prec == TTG_PREC_FP32);
 }
 
 }  // namespace ttg
