@@ -95,7 +95,7 @@ def test_trsm_backward_stable(gpu_ctx, prec, m, k, logcond):
     assert res <= 10 * u * np.sqrt(k)
 
 
-@pytest.mark.parametrize("k,trans,deficient", [(6, 0, 0), (40, 1, 20), (64, 0, 0), (128, 1, 0), (256, 0, 100)])
+@pytest.mark.parametrize("k,trans,deficient", [(6, 0, 0), (16, 1, 3), (40, 1, 20), (64, 0, 0), (128, 1, 0), (256, 0, 100), (300, 1, 0)])
 def test_jacobi_svd(gpu_ctx, k, trans, deficient):
     rng = np.random.default_rng(k + trans)
     S = rng.standard_normal((k, k))

@@ -82,16 +82,6 @@ __device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen, int *e
   return sh_ok != 0;
 }
 
-// Sums of three values over each 16-lane row; afterwards every lane of the
-// row holds the bitwise-identical row total (each step adds a lane to its
-// mirror partner, and FP addition is commutative).
-__device__ __forceinline__ void row16_sum3(double &a, double &b, double &c) {
-  a += dpp_d<0xB1>(a); b += dpp_d<0xB1>(b); c += dpp_d<0xB1>(c);     // quad_perm [1,0,3,2]
-  a += dpp_d<0x4E>(a); b += dpp_d<0x4E>(b); c += dpp_d<0x4E>(c);     // quad_perm [2,3,0,1]
-  a += dpp_d<0x141>(a); b += dpp_d<0x141>(b); c += dpp_d<0x141>(c);  // row_half_mirror
-  a += dpp_d<0x140>(a); b += dpp_d<0x140>(b); c += dpp_d<0x140>(c);  // row_mirror
-}
-
 // 1/x and 1/sqrt(x) from the hardware estimates (about 23 correct bits) and
 // two Newton steps each: full double precision for the normal, positive
 // arguments that occur in a rotation, without the special-case handling of a
@@ -117,7 +107,8 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // + 4ga^2)), the smaller root of t^2 + 2 zeta t - 1 = 0, zeta = d / (2ga); t
 // only needs ~46 bits (one Newton step on each estimate), c = 1/sqrt(1+t^2)
 // is refined to full precision so that c^2 + s^2 = 1 to rounding.
-__device__ __forceinline__ void jrot(double al, double be, double ga, double &cs, double &sn) {
+__device__ __forceinline__ void jrot(double al, double be, double ga, double &cs, double &sn,
+                                     double &tn) {
   const double d = be - al, g2 = 2.0 * ga;
   const double q = __builtin_fma(d, d, g2 * g2);
   double y = __builtin_amdgcn_rsq(q);
@@ -128,34 +119,7 @@ __device__ __forceinline__ void jrot(double al, double be, double ga, double &cs
   const double t = (d >= 0.0 ? g2 : -g2) * inv;
   cs = rsq_nr(__builtin_fma(t, t, 1.0));
   sn = cs * t;
-}
-
-// Workgroup barrier that orders LDS only: waits for this wave's LDS traffic
-// (lgkmcnt = 0) but not for its outstanding global loads, so a wave can keep
-// loads in flight across it.  Encoding of s_waitcnt on gfx9: vmcnt = 63,
-// expcnt = 7, lgkmcnt = 0.
-__device__ __forceinline__ void lds_barrier() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0xC07F);
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
-// Barrier among the rotating waves only, through an LDS counter (the helper
-// waves of the workgroup never arrive here).  All rotating waves execute the
-// same number of rounds, so the count is always reached; the spin cap only
-// turns a logic error into a reported failure instead of a hang.
-__device__ __forceinline__ void rot_sync(unsigned *cnt, unsigned target, int *err) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  if ((threadIdx.x & 63) == 0) atomicAdd(cnt, 1u);
-  unsigned spins = 0;
-  while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
-    if (++spins > (1u << 24)) {
-      atomicExch(err, 3);
-      break;
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  tn = t;
 }
 
 // Layout of the barrier/statistics buffer: [0] arrivals, [1] generation,
@@ -164,110 +128,144 @@ __device__ __forceinline__ void rot_sync(unsigned *cnt, unsigned target, int *er
 static __host__ __device__ inline int jacobi_mc_offset(int max_sweeps) { return (2 + max_sweeps + 1) & ~1; }
 int jacobi_bar_words(int max_sweeps) { return jacobi_mc_offset(max_sweeps) + 2 * max_sweeps; }
 
-// J(:, pair) <- J(:, pair) * V for the 16-row band rt (FP64 MFMA, one wave),
-// split into the load of the band's J entries and the multiply-store.
-template <int LC>
-__device__ __forceinline__ void jacobi_j_band_load(int rt, int k, const double *__restrict__ Jg, long ldj,
-                                                   int bA, int bB, int lane, double *a) {
-  constexpr int B = LC / 2;
-  const int i = rt * 16 + (lane & 15);
+// X(:, pair) <- X(:, pair) * V for the 16-row bands rt = w, w + nw, ... of a
+// k-row matrix (FP64 MFMA).  SRC_LDS: the operand rows come from the LDS
+// copy Xl (column stride ldx) and go to the global columns of the pair in Xg;
+// otherwise they are read from and written back to Xg.  Up to NBW bands are
+// processed together so that 2*NBW independent accumulator chains hide the
+// MFMA latency.
+template <int LC, int NBW, bool SRC_LDS>
+__device__ __forceinline__ void jacobi_pair_mul(int k, int w, int nw, const double *Xl, int ldx,
+                                                double *__restrict__ Xg, long ldg, const double *V,
+                                                int bA, int bB, int lane) {
+  constexpr int B = LC / 2, NCT = LC / 16, NK = LC / 4;
+  const int nband = (k + 15) >> 4;
+  for (int rt0 = w; rt0 < nband; rt0 += NBW * nw) {
+    double a[NBW][NK];
 #pragma unroll
-  for (int kk = 0; kk < LC / 4; ++kk) {
-    const int l = 4 * kk + (lane >> 4);
-    const int gl = (l < B) ? bA * B + l : bB * B + (l - B);
-    a[kk] = (i < k) ? Jg[i + (long)gl * ldj] : 0.0;
-  }
-}
-template <int LC>
-__device__ __forceinline__ void jacobi_j_band_mul(int rt, int k, double *__restrict__ Jg, long ldj,
-                                                  const double *V, int bA, int bB, int lane,
-                                                  const double *a) {
-  constexpr int B = LC / 2, NCT = LC / 16;
-  v4d_j acc[NCT];
+    for (int bb = 0; bb < NBW; ++bb) {
+      const int rt = rt0 + bb * nw;
+      const int i = rt * 16 + (lane & 15);
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) acc[ct] = v4d_j{0, 0, 0, 0};
+      for (int kk = 0; kk < NK; ++kk) {
+        const int l = 4 * kk + (lane >> 4);
+        if (SRC_LDS) {
+          a[bb][kk] = (rt < nband) ? Xl[l * ldx + i] : 0.0;
+        } else {
+          const int gl = (l < B) ? bA * B + l : bB * B + (l - B);
+          a[bb][kk] = (rt < nband && i < k) ? Xg[i + (long)gl * ldg] : 0.0;
+        }
+      }
+    }
+    v4d_j acc[NBW][NCT];
 #pragma unroll
-  for (int kk = 0; kk < LC / 4; ++kk) {
-    const int l = 4 * kk + (lane >> 4);
+    for (int bb = 0; bb < NBW; ++bb)
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct)
-      acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], V[l + (ct * 16 + (lane & 15)) * LC], acc[ct], 0, 0, 0);
-    __builtin_amdgcn_sched_barrier(0);  // keep the V loads from all being hoisted (register pressure)
-  }
+      for (int ct = 0; ct < NCT; ++ct) acc[bb][ct] = v4d_j{0, 0, 0, 0};
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) {
+    for (int kk = 0; kk < NK; ++kk) {
+      const int l = 4 * kk + (lane >> 4);
+      double bv[NCT];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = rt * 16 + (lane >> 4) + 4 * v;
-      const int c = ct * 16 + (lane & 15);
-      const int gcol = (c < B) ? bA * B + c : bB * B + (c - B);
-      if (row < k) Jg[row + (long)gcol * ldj] = acc[ct][v];
+      for (int ct = 0; ct < NCT; ++ct) bv[ct] = V[(ct * 16 + (lane & 15)) * LC + l];
+      // transposed product (X V)^T = V^T X^T: each lane then holds 4 columns of
+      // one row, so 16 consecutive lanes store 16 consecutive rows (128 B)
+#pragma unroll
+      for (int bb = 0; bb < NBW; ++bb)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+          acc[bb][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[ct], a[bb][kk], acc[bb][ct], 0, 0, 0);
+    }
+#pragma unroll
+    for (int bb = 0; bb < NBW; ++bb) {
+      const int rt = rt0 + bb * nw;
+      const int row = rt * 16 + (lane & 15);
+      if (rt >= nband || row >= k) continue;
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = ct * 16 + (lane >> 4) + 4 * v;
+          const int gcol = (c < B) ? bA * B + c : bB * B + (c - B);
+          Xg[row + (long)gcol * ldg] = acc[bb][ct][v];
+        }
+      }
     }
   }
 }
-template <int LC>
-__device__ __forceinline__ void jacobi_j_band(int rt, int k, double *__restrict__ Jg, long ldj,
-                                              const double *V, int bA, int bB, int lane) {
-  double a[LC / 4];
-  jacobi_j_band_load<LC>(rt, k, Jg, ldj, bA, bB, lane, a);
-  jacobi_j_band_mul<LC>(rt, k, Jg, ldj, V, bA, bB, lane, a);
-}
 
-// KL: rows per lane of a 16-lane group (k <= 16*KL); LC = 2b local columns
-// (multiple of 16, <= 64); NT threads.  Each 16-lane group owns one column
-// pair per round, so a wave rotates four pairs at once and the dot products
-// need only row-local DPP sums.  Waves not needed for the B = LC/2 pairs of a
-// round apply the previous step's accumulated rotations to J meanwhile (the
-// J columns of that pair are not touched by anyone else until the next grid
-// barrier, which orders them before their next owner's update).
-// Convergence: a sweep without rotations, or a sweep none of whose rotations
-// had cos^2 > stop_cos2 (they leave cosines of O(cos^2) behind).
+// Block-Gram one-sided Jacobi.  KL: S rows are padded to 16*KL (>= k);
+// LC = 2b local columns (16 or 32); NT threads, of which the first B/4 waves
+// rotate (four column pairs each) and all form the Gram and the products.
+// One outer step, for the block pair (bA, bB) of this workgroup:
+//   1. the LC columns of S are staged in LDS and their Gram G = S_p^T S_p is
+//      formed with FP64 MFMA (upper 16x16 tiles, K split over the waves,
+//      four accumulator chains per wave);
+//   2. the rotation rounds work on G two-sided (G <- J^T G J: a rotation
+//      touches two rows and two columns of G instead of two length-k
+//      columns); a 16-lane group owns one pair per round, with a column
+//      phase and a row phase separated by workgroup barriers; rotations
+//      accumulate in V (LC x LC).  The angle of pair (p, q) comes from G_pp,
+//      G_qq, G_pq, the dot products a one-sided sweep would compute, so the
+//      iteration is the one-sided Jacobi method; G is formed afresh from S
+//      every step, so rounding in the two-sided updates never accumulates
+//      beyond one step;
+//   3. S_p <- S_p V and J_p <- J_p V (FP64 MFMA from LDS) go back to HBM for
+//      the next owners of the blocks.  The J columns are loaded into
+//      registers at the start of the step, so their latency hides behind the
+//      Gram and the rounds, and are staged in LDS (over the Gram partials)
+//      only after the rounds.
+// Convergence: a sweep without rotations, or one none of whose rotations had
+// cos^2 > stop_cos2 (they leave cosines of O(cos^2) behind).
 template <int KL, int LC, int NT>
-__global__ __launch_bounds__(NT) void k_jacobi3(int k, int p, double *__restrict__ Sg, long lds,
+__global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict__ Sg, long lds,
                                                 double *__restrict__ Jg, long ldj, double tol,
                                                 double stop_cos2, int max_sweeps, int *sweeps_out,
                                                 unsigned *bar, int *err, unsigned long long *dbg) {
   constexpr int B = LC / 2;
-  constexpr int LDC = 16 * KL + 16;  // column stride: 128 B mod 256 B, groups alternate bank halves
+  constexpr int ROWS = 16 * KL;            // padded row count of S
+  constexpr int LDC = ROWS + 1;            // odd column stride: conflict-free MFMA operand reads
+  constexpr int LDG = LC + 1;              // odd: row and column accesses of G conflict-free
   constexpr int NW = NT / 64;
-  constexpr int WR = B / 4;
-  constexpr int NH = NW - WR;
-  static_assert(B % 4 == 0 && WR < NW, "block pair does not fit the workgroup");
-  constexpr int TPC = NT / LC;                 // threads per column for loads and stores
-  constexpr int NJ = (LDC + TPC - 1) / TPC;    // rows per thread
-  static_assert(NT % LC == 0, "threads must divide evenly over the columns");
+  constexpr int WR = B / 4;                // rotating waves
+  constexpr int NT16 = LC / 16;
+  constexpr int NTILE = NT16 * (NT16 + 1) / 2;
+  constexpr int KS = NW / NTILE;           // K split of each Gram tile
+  constexpr int KR = ROWS / KS;            // rows per Gram job
+  constexpr int TPC = NT / LC;
+  constexpr int NJ = (LDC + TPC - 1) / TPC;
+  static_assert(B % 4 == 0 && WR <= NW, "block pair does not fit the workgroup");
+  static_assert(NT % LC == 0 && KR % 16 == 0 && KS >= 1, "bad Jacobi tiling");
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   double *Sl = reinterpret_cast<double *>(smem);  // LC x LDC
-  double *Vl = Sl + LC * LDC;                     // LC x LC, column-major (this step)
-  double *Vp = Vl + LC * LC;                      // LC x LC (previous step, pending for J)
-  __shared__ int sh_steprot, sh_rot;
-  __shared__ unsigned sh_rsync;
-  __shared__ int sh_big, sh_total, sh_bigtot;
+  double *Gl = Sl + LC * LDC;                     // LC x LDG, symmetric
+  double *Vl = Gl + LC * LDG;                     // LC x LC
+  double *Jl = Vl + LC * LC;                      // LC x LDC (J columns of the pair)
+  double *Pt = Jl;                                // Gram partial tiles (NTILE * KS * 256), dead before Jl is written
+  __shared__ int sh_steprot, sh_rot, sh_big, sh_total, sh_bigtot;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int grp = tid >> 4, li = tid & 15;
   const int cc = tid / TPC, i0 = tid - cc * TPC;
   const int w = blockIdx.x, nblk = 2 * p;
-  const int nband = (k + 15) >> 4;
   unsigned *rotcnt = bar + 2;
   unsigned *bigcnt = bar + jacobi_mc_offset(max_sweeps);
   const double tol2 = tol * tol;
   unsigned gen = 0;
   bool converged = false;
-  int pend = 0, pA = 0, pB = 0;
   int sweep;
   for (sweep = 0; sweep < max_sweeps && !converged; ++sweep) {
     if (tid == 0) {
       sh_rot = 0;
       sh_big = 0;
     }
-    int mybig = 0;  // rotations with cos^2 > stop_cos2 (this lane's groups)
     for (int step = 0; step < nblk - 1; ++step) {
       int bA, bB;
       circle_pair(nblk, step, w, bA, bB);
+      const int gcol_cc = (cc < B) ? bA * B + cc : bB * B + (cc - B);
+      double jv[NJ];  // J columns of the pair, in flight until after the rounds
       {
-        // thread -> (column c, rows i0 + TPC*j): invariants stay few, offsets immediate
-        const int gcol = (cc < B) ? bA * B + cc : bB * B + (cc - B);
-        const double *src = Sg + (long)gcol * lds;
+        const double *src = Sg + (long)gcol_cc * lds;
+        const double *jsrc = Jg + (long)gcol_cc * ldj;
         double v[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
@@ -275,139 +273,129 @@ __global__ __launch_bounds__(NT) void k_jacobi3(int k, int p, double *__restrict
           v[j] = (i < k) ? src[i] : 0.0;
         }
 #pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int i = i0 + TPC * j;
+          jv[j] = (i < k) ? jsrc[i] : 0.0;
+        }
+#pragma unroll
         for (int j = 0; j < NJ; ++j)
           if (LDC % TPC == 0 || i0 + TPC * j < LDC) Sl[cc * LDC + i0 + TPC * j] = v[j];
       }
       for (int idx = tid; idx < LC * LC; idx += NT) Vl[idx] = ((idx % LC) == (idx / LC)) ? 1.0 : 0.0;
-      if (tid == 0) {
-        sh_steprot = 0;
-        sh_rsync = 0u;
-      }
+      if (tid == 0) sh_steprot = 0;
       __syncthreads();
       const bool rec = dbg && sweep == 0 && w == 0 && tid == 0;
       if (rec) dbg[step * 4 + 0] = __builtin_amdgcn_s_memrealtime();
+      // 1. Gram: wave j < NTILE * KS forms rows [(j % KS) KR, +KR) of upper tile
+      //    j / KS = (ti, tj) (column order); partial tiles are summed in LDS
+      if (wv < NTILE * KS) {
+        const int tile = wv / KS, ks = wv - tile * KS;
+        int tj = 0, ti = tile;
+        while (ti > tj) {
+          ti -= tj + 1;
+          ++tj;
+        }
+        v4d_j acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = v4d_j{0, 0, 0, 0};
+        const double *ca = Sl + (ti * 16 + (lane & 15)) * LDC + ks * KR + (lane >> 4);
+        const double *cb = Sl + (tj * 16 + (lane & 15)) * LDC + ks * KR + (lane >> 4);
+#pragma unroll 4
+        for (int r = 0; r < KR; r += 16) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[r + 4 * q], cb[r + 4 * q], acc[q], 0, 0, 0);
+        }
+        double *pt = Pt + (tile * KS + ks) * 256;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          pt[(lane & 15) * 16 + (lane >> 4) + 4 * v] = (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]);
+      }
+      __syncthreads();
+      for (int idx = tid; idx < LC * LC; idx += NT) {
+        const int i = idx % LC, j = idx / LC;
+        const int a = i <= j ? i : j, c = i <= j ? j : i;  // upper entry (a <= c)
+        const int tile = (c >> 4) * ((c >> 4) + 1) / 2 + (a >> 4);
+        double g = 0.0;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) g += Pt[(tile * KS + ks) * 256 + (c & 15) * 16 + (a & 15)];
+        Gl[j * LDG + i] = g;
+      }
+      __syncthreads();
+      if (rec) dbg[step * 4 + 1] = __builtin_amdgcn_s_memrealtime();
+      // 2. rotation rounds on G
       const int rounds = (step == 0) ? LC - 1 : B;
-      int myrot = 0;
-      if (wv < WR) {
-        // Rotating waves, synchronised per round through an LDS counter so the
-        // helper waves never hold them up.  After the first step of a sweep the
-        // pairs are (g, B + (g + t) mod B): column g stays in registers for the
-        // whole step and only the partner column moves through LDS.
-        constexpr int NV = LC / 16;
-        const bool xs = step > 0;
-        double x[KL], vx[NV];
-        if (xs) {
-#pragma unroll
-          for (int q = 0; q < KL; ++q) x[q] = Sl[grp * LDC + li + 16 * q];
-#pragma unroll
-          for (int r = 0; r < NV; ++r) vx[r] = Vl[grp * LC + li + 16 * r];
+      int myrot = 0, mybig = 0;
+      const bool rw = wv < WR;
+      for (int t = 0; t < rounds; ++t) {
+        if (!rw) {  // non-rotating waves only keep the barrier count
+          __syncthreads();
+          __syncthreads();
+          continue;
         }
-        for (int t = 0; t < rounds; ++t) {
-          int pa, pb;
-          if (!xs) {
-            circle_pair(LC, t, grp, pa, pb);
-          } else {
-            pa = grp;
-            const int q = grp + t;
-            pb = B + (q >= B ? q - B : q);
-          }
-          double *cp = Sl + pa * LDC + li;
-          double *cq = Sl + pb * LDC + li;
-          double y[KL], vy[NV];
-          if (!xs) {
-#pragma unroll
-            for (int q = 0; q < KL; ++q) x[q] = cp[16 * q];
-#pragma unroll
-            for (int r = 0; r < NV; ++r) vx[r] = Vl[pa * LC + li + 16 * r];
-          }
-#pragma unroll
-          for (int q = 0; q < KL; ++q) y[q] = cq[16 * q];
-#pragma unroll
-          for (int r = 0; r < NV; ++r) vy[r] = Vl[pb * LC + li + 16 * r];
-          double al0 = 0.0, al1 = 0.0, be0 = 0.0, be1 = 0.0, ga0 = 0.0, ga1 = 0.0;
-#pragma unroll
-          for (int q = 0; q < KL; q += 2) {
-            al0 = __builtin_fma(x[q], x[q], al0);
-            be0 = __builtin_fma(y[q], y[q], be0);
-            ga0 = __builtin_fma(x[q], y[q], ga0);
-            al1 = __builtin_fma(x[q + 1], x[q + 1], al1);
-            be1 = __builtin_fma(y[q + 1], y[q + 1], be1);
-            ga1 = __builtin_fma(x[q + 1], y[q + 1], ga1);
-          }
-          double al = al0 + al1, be = be0 + be1, ga = ga0 + ga1;
-          row16_sum3(al, be, ga);
-          const double g2 = ga * ga, ab = al * be;
-          const bool rot = g2 > tol2 * ab && ab >= DBL_MIN;
-          // branch-free: a pair that does not rotate gets c = 1, s = 0 (exact identity)
-          double cs, sn;
-          jrot(al, be, ga, cs, sn);
-          cs = rot ? cs : 1.0;
-          sn = rot ? sn : 0.0;
-          mybig += (rot && g2 > stop_cos2 * ab) ? 1 : 0;
-          myrot += rot ? 1 : 0;
-#pragma unroll
-          for (int q = 0; q < KL; ++q) {
-            const double xn = cs * x[q] - sn * y[q];
-            y[q] = sn * x[q] + cs * y[q];
-            x[q] = xn;
-            cq[16 * q] = y[q];
-          }
-#pragma unroll
-          for (int r = 0; r < NV; ++r) {
-            const double vn = cs * vx[r] - sn * vy[r];
-            vy[r] = sn * vx[r] + cs * vy[r];
-            vx[r] = vn;
-            Vl[pb * LC + li + 16 * r] = vy[r];
-          }
-          if (!xs) {
-#pragma unroll
-            for (int q = 0; q < KL; ++q) cp[16 * q] = x[q];
-#pragma unroll
-            for (int r = 0; r < NV; ++r) Vl[pa * LC + li + 16 * r] = vx[r];
-          }
-          rot_sync(&sh_rsync, (unsigned)(WR * (t + 1)), err);
+        int pa, pb;
+        if (step == 0) {
+          circle_pair(LC, t, grp, pa, pb);
+        } else {
+          pa = grp;
+          const int q = grp + t;
+          pb = B + (q >= B ? q - B : q);
         }
-        if (xs) {
+        const double al = Gl[pa * LDG + pa], be = Gl[pb * LDG + pb], ga = Gl[pb * LDG + pa];
+        const double g2 = ga * ga, ab = al * be;
+        const bool rot = g2 > tol2 * ab && ab >= DBL_MIN;
+        double cs, sn, tn;
+        jrot(al, be, ga, cs, sn, tn);
+        cs = rot ? cs : 1.0;
+        sn = rot ? sn : 0.0;
+        mybig += (rot && g2 > stop_cos2 * ab) ? 1 : 0;
+        myrot += rot ? 1 : 0;
+        // column phase: G(:, pa), G(:, pb), V(:, pa), V(:, pb)
 #pragma unroll
-          for (int q = 0; q < KL; ++q) Sl[grp * LDC + li + 16 * q] = x[q];
-#pragma unroll
-          for (int r = 0; r < NV; ++r) Vl[grp * LC + li + 16 * r] = vx[r];
+        for (int r = li; r < LC; r += 16) {
+          const double gp = Gl[pa * LDG + r], gq = Gl[pb * LDG + r];
+          Gl[pa * LDG + r] = cs * gp - sn * gq;
+          Gl[pb * LDG + r] = sn * gp + cs * gq;
+          const double vp = Vl[pa * LC + r], vq = Vl[pb * LC + r];
+          Vl[pa * LC + r] = cs * vp - sn * vq;
+          Vl[pb * LC + r] = sn * vp + cs * vq;
         }
-      } else if (pend) {
-        // helper waves: the previous step's rotations applied to J
-        for (int rt = wv - WR; rt < nband; rt += NH) jacobi_j_band<LC>(rt, k, Jg, ldj, Vp, pA, pB, lane);
+        __syncthreads();
+        // row phase: G(pa, :), G(pb, :), then the exact 2x2 block
+#pragma unroll
+        for (int r = li; r < LC; r += 16) {
+          const double gp = Gl[r * LDG + pa], gq = Gl[r * LDG + pb];
+          Gl[r * LDG + pa] = cs * gp - sn * gq;
+          Gl[r * LDG + pb] = sn * gp + cs * gq;
+        }
+        if (li == 0 && rot) {
+          Gl[pa * LDG + pa] = al - tn * ga;
+          Gl[pb * LDG + pb] = be + tn * ga;
+          Gl[pb * LDG + pa] = 0.0;
+          Gl[pa * LDG + pb] = 0.0;
+        }
+        __syncthreads();
       }
       if (li == 0 && myrot > 0) atomicAdd(&sh_steprot, myrot);
       if (li == 0 && mybig > 0) atomicAdd(&sh_big, mybig);
-      mybig = 0;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+        if (LDC % TPC == 0 || i0 + TPC * j < LDC) Jl[cc * LDC + i0 + TPC * j] = jv[j];
       __syncthreads();
       const int rot = sh_steprot;
-      if (rec) dbg[step * 4 + 1] = __builtin_amdgcn_s_memrealtime();
+      if (rec) dbg[step * 4 + 2] = __builtin_amdgcn_s_memrealtime();
+      // 3. S_p <- S_p V and J_p <- J_p V, straight to HBM
       if (rot > 0) {
-        {
-          const int gcol = (cc < B) ? bA * B + cc : bB * B + (cc - B);
-          double *dst = Sg + (long)gcol * lds;
-#pragma unroll
-          for (int j = 0; j < NJ; ++j) {
-            const int i = i0 + TPC * j;
-            if (i < k) dst[i] = Sl[cc * LDC + i];
-          }
-        }
-        for (int idx = tid; idx < LC * LC; idx += NT) Vp[idx] = Vl[idx];
-        pend = 1;
-        pA = bA;
-        pB = bB;
-      } else {
-        pend = 0;
+        jacobi_pair_mul<LC, 2, true>(k, wv, NW, Sl, LDC, Sg, lds, Vl, bA, bB, lane);
+        jacobi_pair_mul<LC, 2, true>(k, wv, NW, Jl, LDC, Jg, ldj, Vl, bA, bB, lane);
       }
       if (tid == 0) sh_rot += rot;
-      if (rec) dbg[step * 4 + 2] = __builtin_amdgcn_s_memrealtime();
+      if (rec) dbg[step * 4 + 3] = __builtin_amdgcn_s_memrealtime();
       if (p > 1) {
         if (!grid_sync(bar, p, gen, err)) return;
       } else {
         __syncthreads();
       }
-      if (rec) dbg[step * 4 + 3] = __builtin_amdgcn_s_memrealtime();
     }
     // sweep totals: rotations, and rotations above the stopping cosine
     __syncthreads();
@@ -427,9 +415,6 @@ __global__ __launch_bounds__(NT) void k_jacobi3(int k, int p, double *__restrict
     }
     __syncthreads();
     converged = (sh_total == 0) || (sh_bigtot == 0);
-  }
-  if (pend) {
-    for (int rt = wv; rt < nband; rt += NW) jacobi_j_band<LC>(rt, k, Jg, ldj, Vp, pA, pB, lane);
   }
   if (w == 0 && tid == 0) {
     sweeps_out[0] += sweep;
@@ -454,18 +439,22 @@ __global__ void k_jacobi_setup(int k, int kpad, const double *__restrict__ S, in
   }
 }
 
-// Local column count 2b is 16 or 32 (MFMA tiles of the J update); above k = 32
-// the matrix is split over p = ceil(k/32) cooperating workgroups.
+// Local column count 2b is 16 or 32 (MFMA tiles); above k = 32 the matrix is
+// split over p cooperating workgroups, with 16-column blocks pairs above
+// k = 256 (the S block pair must fit in LDS).
 JacobiPlan jacobi_plan(int64_t k) {
   JacobiPlan pl;
   pl.k = k;
   pl.rows = 2 * k;
-  if (k <= 32) {
-    pl.b = k <= 16 ? 8 : 16;
+  if (k <= 16) {
+    pl.b = 8;
     pl.p = 1;
-  } else {
+  } else if (k <= 256) {
     pl.b = 16;
     pl.p = (int)((k + 31) / 32);
+  } else {
+    pl.b = 8;
+    pl.p = (int)((k + 15) / 16);
   }
   pl.kpad = 2 * (int64_t)pl.b * pl.p;
   return pl;
@@ -476,16 +465,18 @@ template <int KL, int LC, int NT>
 static void launch_jacobi_t(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg,
                             long ldj, double tol, double stop_cos2, int max_sweeps, int *sweeps_dev,
                             unsigned *bar, int *err, unsigned long long *dbg) {
-  const size_t lsz = ((size_t)LC * (16 * KL + 16) + 2 * (size_t)LC * LC) * sizeof(double);
+  constexpr int NT16 = LC / 16, NTILE = NT16 * (NT16 + 1) / 2, KS = (NT / 64) / NTILE;
+  static_assert((size_t)NTILE * KS * 256 <= (size_t)LC * (16 * KL + 1), "Gram partials must fit in the J area");
+  const size_t lsz = (2 * (size_t)LC * (16 * KL + 1) + (size_t)LC * (LC + 1) + (size_t)LC * LC) * sizeof(double);
   TTG_CHECK(lsz <= 160 * 1024 - 256, TTG_BAD_ARG, "Jacobi block does not fit in LDS");
   TTG_CHECK(pl.k <= 16 * KL && 2 * pl.b == LC, TTG_BAD_ARG, "Jacobi plan/kernel mismatch");
-  TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi3<KL, LC, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
+  TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi4<KL, LC, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
   int k = (int)pl.k, p = pl.p, ms = max_sweeps;
   if (p > 1) {
     void *args[] = {&k, &p, &Sg, &lds, &Jg, &ldj, &tol, &stop_cos2, &ms, &sweeps_dev, &bar, &err, &dbg};
-    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_jacobi3<KL, LC, NT>, dim3(p), dim3(NT), args, lsz, s));
+    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_jacobi4<KL, LC, NT>, dim3(p), dim3(NT), args, lsz, s));
   } else {
-    hipLaunchKernelGGL((k_jacobi3<KL, LC, NT>), dim3(1), dim3(NT), lsz, s, k, p, Sg, lds, Jg, ldj, tol,
+    hipLaunchKernelGGL((k_jacobi4<KL, LC, NT>), dim3(1), dim3(NT), lsz, s, k, p, Sg, lds, Jg, ldj, tol,
                        stop_cos2, ms, sweeps_dev, bar, err, dbg);
     TTG_HIP(hipGetLastError());
   }
@@ -498,19 +489,17 @@ static void launch_jacobi(hipStream_t s, const JacobiPlan &pl, double *Sg, long 
   const int64_t k = pl.k;
   const int lc = 2 * pl.b;
 #define TTG_JAC(KL, LC, NT) launch_jacobi_t<KL, LC, NT>(s, pl, Sg, lds, Jg, ldj, tol, stop_cos2, max_sweeps, sweeps_dev, bar, err, dbg)
-  if (k <= 32) {
-    switch (lc) {
-      case 16: TTG_JAC(4, 16, 512); break;
-      case 32: TTG_JAC(4, 32, 512); break;
-      default: throw Error(TTG_BAD_ARG, "unsupported Jacobi block size");
-    }
-  } else {
+  if (k <= 16) {
+    TTG_CHECK(lc == 16, TTG_BAD_ARG, "unsupported Jacobi block size");
+    TTG_JAC(4, 16, 256);
+  } else if (k <= 256) {
     TTG_CHECK(lc == 32, TTG_BAD_ARG, "unsupported Jacobi block size");
     if (k <= 64) TTG_JAC(4, 32, 512);
     else if (k <= 128) TTG_JAC(8, 32, 512);
-    else if (k <= 256) TTG_JAC(16, 32, 512);
-    else if (k <= 512) TTG_JAC(32, 32, 512);
-    else throw Error(TTG_BAD_ARG, "Jacobi SVD supports k <= 512");
+    else TTG_JAC(16, 32, 512);
+  } else {
+    TTG_CHECK(lc == 16 && k <= 512, TTG_BAD_ARG, "Jacobi SVD supports k <= 512");
+    TTG_JAC(32, 16, 256);
   }
 #undef TTG_JAC
 }
