@@ -105,8 +105,17 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__r
         const bool bad = !(p > 0.0) || !isfinite(p);
         failmask |= bad ? (1u << c) : 0u;
         p = bad ? 1.0 : p;
-        const double dc = sqrt(p);
-        const double inv = 1.0 / dc;
+        // 1/sqrt(p) from the hardware estimate and two Newton steps (full
+        // precision for the positive normal pivots that reach this point)
+        double inv = __builtin_amdgcn_rsq(p);
+        {
+          const double h = 0.5 * p;
+          double e = __builtin_fma(-h * inv, inv, 0.5);
+          inv = __builtin_fma(inv, e, inv);
+          e = __builtin_fma(-h * inv, inv, 0.5);
+          inv = __builtin_fma(inv, e, inv);
+        }
+        const double dc = p * inv;
         const double rc = (j == c) ? dc : (j > c ? col[c] * inv : 0.0);
         col[c] = rc;
         myinv = (j == c) ? inv : myinv;
@@ -173,27 +182,53 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__r
       const int t = idx & (CB - 1), c = idx >> 5;
       if (t < jb) R[(J0 + t) + (long)(J0 + jb + c) * k] = Ps[t * wps + c];
     }
-    // trailing SYRK on the upper 16x16 tiles
+    // trailing SYRK on the upper 16x16 tiles, four tiles per wave at a time:
+    // their R entries are loaded first and their MFMA chains interleaved, so
+    // the L2 latency of the read-modify-write is paid once per four tiles.
+    // The product is formed transposed (tile^T = Ps_j^T Ps_i) so that 16
+    // consecutive lanes touch 16 consecutive rows of R.
     const int nt = wpad >> 4;
     const int ntiles = nt * (nt + 1) / 2;
-    for (int tile = wv; tile < ntiles; tile += NW) {
-      int tj = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
-      while (tj * (tj + 1) / 2 > tile) --tj;
-      while ((tj + 1) * (tj + 2) / 2 <= tile) ++tj;
-      const int ti = tile - tj * (tj + 1) / 2;
-      v4d_f acc = {0, 0, 0, 0};
-      const int ia = ti * 16 + (lane & 15), ja = tj * 16 + (lane & 15);
+    for (int base = wv; base < ntiles; base += 4 * NW) {
+      int ti[4], tj[4];
+      bool ok[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int tile = base + q * NW;
+        ok[q] = tile < ntiles;
+        int c = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+        while (c * (c + 1) / 2 > tile) --c;
+        while ((c + 1) * (c + 2) / 2 <= tile) ++c;
+        tj[q] = c;
+        ti[q] = tile - c * (c + 1) / 2;
+      }
+      const int i_l = lane & 15, j_l = lane >> 4;
+      double old[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = ti[q] * 16 + i_l, jj = tj[q] * 16 + j_l + 4 * v;
+          old[q][v] = (ok[q] && i < w && jj < w && i <= jj) ? R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] : 0.0;
+        }
+      v4d_f acc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = v4d_f{0, 0, 0, 0};
 #pragma unroll
       for (int kk = 0; kk < CB; kk += 4) {
         const int t = kk + (lane >> 4);
-        acc = MFf<double>::op(Ps[t * wps + ia], Ps[t * wps + ja], acc);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc[q] = MFf<double>::op(Ps[t * wps + tj[q] * 16 + i_l], Ps[t * wps + ti[q] * 16 + i_l], acc[q]);
       }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = ti * 16 + MFf<double>::row(lane, v);
-        const int jj = tj * 16 + (lane & 15);
-        if (i < w && jj < w && i <= jj) R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] -= acc[v];
-      }
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          // acc[q][v] = tile^T[j_l + 4v][i_l] = tile[i_l][j_l + 4v]
+          const int i = ti[q] * 16 + i_l, jj = tj[q] * 16 + j_l + 4 * v;
+          if (ok[q] && i < w && jj < w && i <= jj) R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] = old[q][v] - acc[q][v];
+        }
     }
   }
   __syncthreads();

@@ -350,23 +350,44 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
         sn = rot ? sn : 0.0;
         mybig += (rot && g2 > stop_cos2 * ab) ? 1 : 0;
         myrot += rot ? 1 : 0;
-        // column phase: G(:, pa), G(:, pb), V(:, pa), V(:, pb)
+        // column phase: G(:, pa), G(:, pb), V(:, pa), V(:, pb) -- all loads
+        // first, so one LDS latency covers the whole update
+        constexpr int NR = LC / 16;
+        {
+          double gp[NR], gq[NR], vp[NR], vq[NR];
 #pragma unroll
-        for (int r = li; r < LC; r += 16) {
-          const double gp = Gl[pa * LDG + r], gq = Gl[pb * LDG + r];
-          Gl[pa * LDG + r] = cs * gp - sn * gq;
-          Gl[pb * LDG + r] = sn * gp + cs * gq;
-          const double vp = Vl[pa * LC + r], vq = Vl[pb * LC + r];
-          Vl[pa * LC + r] = cs * vp - sn * vq;
-          Vl[pb * LC + r] = sn * vp + cs * vq;
+          for (int u = 0; u < NR; ++u) {
+            const int r = li + 16 * u;
+            gp[u] = Gl[pa * LDG + r];
+            gq[u] = Gl[pb * LDG + r];
+            vp[u] = Vl[pa * LC + r];
+            vq[u] = Vl[pb * LC + r];
+          }
+#pragma unroll
+          for (int u = 0; u < NR; ++u) {
+            const int r = li + 16 * u;
+            Gl[pa * LDG + r] = cs * gp[u] - sn * gq[u];
+            Gl[pb * LDG + r] = sn * gp[u] + cs * gq[u];
+            Vl[pa * LC + r] = cs * vp[u] - sn * vq[u];
+            Vl[pb * LC + r] = sn * vp[u] + cs * vq[u];
+          }
         }
         __syncthreads();
         // row phase: G(pa, :), G(pb, :), then the exact 2x2 block
+        {
+          double gp[NR], gq[NR];
 #pragma unroll
-        for (int r = li; r < LC; r += 16) {
-          const double gp = Gl[r * LDG + pa], gq = Gl[r * LDG + pb];
-          Gl[r * LDG + pa] = cs * gp - sn * gq;
-          Gl[r * LDG + pb] = sn * gp + cs * gq;
+          for (int u = 0; u < NR; ++u) {
+            const int r = li + 16 * u;
+            gp[u] = Gl[r * LDG + pa];
+            gq[u] = Gl[r * LDG + pb];
+          }
+#pragma unroll
+          for (int u = 0; u < NR; ++u) {
+            const int r = li + 16 * u;
+            Gl[r * LDG + pa] = cs * gp[u] - sn * gq[u];
+            Gl[r * LDG + pb] = sn * gp[u] + cs * gq[u];
+          }
         }
         if (li == 0 && rot) {
           Gl[pa * LDG + pa] = al - tn * ga;
