@@ -429,30 +429,4 @@ ttg_status round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r
                      ttg_info *info);
 ttg_status round_capacity(int64_t d, const int64_t *n, const int64_t *r, int64_t *cap);
 
-ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, const int64_t *nofs,
-                             const int64_t *n, const int64_t *rofs, const int64_t *r,
-                             const int64_t *cofs, const double *dcore, double eps, int prec,
-                             int64_t *r_out, const int64_t *cofs_out, double *dcore_out,
-                             ttg_info *info) {
-  ttg_status worst = TTG_OK;
-  ttg_info acc{0, 0, 0, 0.0};
-  for (int64_t t = 0; t < count; ++t) {
-    int64_t cap = 0;
-    ttg_status s = round_capacity(d[t], n + nofs[t], r + rofs[t], &cap);
-    if (s != TTG_OK) return s;
-    ttg_info one{0, 0, 0, 0.0};
-    int64_t used = 0;
-    s = round_dev(ctx, d[t], n + nofs[t], r + rofs[t], dcore + cofs[t], eps, nullptr, prec,
-                  r_out + rofs[t], dcore_out + cofs_out[t], cap, &used, &one);
-    if (s != TTG_OK && s != TTG_NO_CONVERGENCE) return s;
-    if (s == TTG_NO_CONVERGENCE) worst = s;
-    acc.qr_fallbacks += one.qr_fallbacks;
-    acc.jacobi_sweeps += one.jacobi_sweeps;
-    acc.jacobi_maxsweeps = std::max(acc.jacobi_maxsweeps, one.jacobi_maxsweeps);
-    acc.seconds += one.seconds;
-  }
-  if (info) *info = acc;
-  return worst;
-}
-
 }  // namespace ttg

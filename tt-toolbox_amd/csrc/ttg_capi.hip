@@ -98,6 +98,7 @@ ttg_status ttg_ctx_destroy(ttg_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  ttg::destroy_workers(ctx);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
   delete ctx;

@@ -105,9 +105,16 @@ struct ttg_ctx {
   // diagnostics of the current call
   int64_t qr_fallbacks = 0, jacobi_sweeps = 0, jacobi_maxsweeps = 0;
   ttg::Profiler prof;
+  // batched rounding: worker contexts (own stream and workspace each),
+  // created on first use and owned by this context
+  std::vector<ttg_ctx *> workers;
+  int batch_workers = 0;  // 0: chosen by the tuner on first use
 };
 
 namespace ttg {
+
+// releases the batched-rounding worker contexts of ctx
+void destroy_workers(ttg_ctx *ctx);
 
 // ---------------------------------------------------------------------------
 // kernel launchers (implemented in ttg_kernels.hip)

@@ -126,7 +126,23 @@ __device__ __forceinline__ void jrot(double al, double be, double ga, double &cs
 // [2, 2+ms) rotations per sweep, then per sweep the number of rotations whose
 // cos^2 exceeded the stopping threshold.
 static __host__ __device__ inline int jacobi_mc_offset(int max_sweeps) { return (2 + max_sweeps + 1) & ~1; }
-int jacobi_bar_words(int max_sweeps) { return jacobi_mc_offset(max_sweeps) + 2 * max_sweeps; }
+// then one double: ||S||_F^2 (8-byte aligned: the offset is even)
+static __host__ __device__ inline int jacobi_fro_offset(int max_sweeps) { return jacobi_mc_offset(max_sweeps) + 2 * max_sweeps; }
+int jacobi_bar_words(int max_sweeps) { return jacobi_fro_offset(max_sweeps) + 2; }
+
+// ||S||_F^2 of the top k x k block of Y (one workgroup, fixed summation order)
+__global__ __launch_bounds__(1024) void k_fro2(int k, const double *__restrict__ Y, long ldy, double *out) {
+  __shared__ double red[16];
+  double sum = 0.0;
+  const long total = (long)k * k;
+  for (long idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const long c = idx / k, i = idx - c * k;
+    const double v = Y[i + c * ldy];
+    sum = __builtin_fma(v, v, sum);
+  }
+  sum = block_sum(sum, red);
+  if (threadIdx.x == 0) out[0] = sum;
+}
 
 // X(:, pair) <- X(:, pair) * V for the 16-row bands rt = w, w + nw, ... of a
 // k-row matrix (FP64 MFMA).  SRC_LDS: the operand rows come from the LDS
@@ -250,6 +266,11 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
   unsigned *rotcnt = bar + 2;
   unsigned *bigcnt = bar + jacobi_mc_offset(max_sweeps);
   const double tol2 = tol * tol;
+  // columns with ||s||^2 below k (2u)^2 ||S||_F^2 are numerically zero: they
+  // are not rotated (their directions are rounding noise; rotating them only
+  // stalls convergence on rank-deficient input)
+  const double small2 = (double)k * 4.0 * DBL_EPSILON * DBL_EPSILON *
+                        *reinterpret_cast<const double *>(bar + jacobi_fro_offset(max_sweeps));
   unsigned gen = 0;
   bool converged = false;
   int sweep;
@@ -343,7 +364,7 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
         }
         const double al = Gl[pa * LDG + pa], be = Gl[pb * LDG + pb], ga = Gl[pb * LDG + pa];
         const double g2 = ga * ga, ab = al * be;
-        const bool rot = g2 > tol2 * ab && ab >= DBL_MIN;
+        const bool rot = g2 > tol2 * ab && ab >= DBL_MIN && al > small2 && be > small2;
         double cs, sn, tn;
         jrot(al, be, ga, cs, sn, tn);
         cs = rot ? cs : 1.0;
@@ -507,6 +528,9 @@ static void launch_jacobi(hipStream_t s, const JacobiPlan &pl, double *Sg, long 
                           double tol, double stop_cos2, int max_sweeps, int *sweeps_dev, unsigned *bar,
                           int *err, unsigned long long *dbg) {
   TTG_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * jacobi_bar_words(max_sweeps), s));
+  hipLaunchKernelGGL(k_fro2, dim3(1), dim3(1024), 0, s, (int)pl.k, Sg, lds,
+                     reinterpret_cast<double *>(bar + jacobi_fro_offset(max_sweeps)));
+  TTG_HIP(hipGetLastError());
   const int64_t k = pl.k;
   const int lc = 2 * pl.b;
 #define TTG_JAC(KL, LC, NT) launch_jacobi_t<KL, LC, NT>(s, pl, Sg, lds, Jg, ldj, tol, stop_cos2, max_sweeps, sweeps_dev, bar, err, dbg)

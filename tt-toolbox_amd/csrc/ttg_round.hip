@@ -324,21 +324,24 @@ template <class T> struct Rounder {
 
   // ---- driver --------------------------------------------------------------
   void run(double eps, const int64_t *rmax, const std::function<void(int64_t, T *)> &gen) {
-    // sweep 1, speculative: one synchronisation at its end; a rejected bond
-    // is recomputed one QR level up (mixed -> FP64 CholeskyQR2 -> Householder)
-    // together with every bond after it.
+    // sweep 1, speculative: one synchronisation at its end.  After a rejected
+    // bond the sweep restarts there one QR level up (mixed -> FP64
+    // CholeskyQR2 -> Householder) and continues bond by bond with an
+    // immediate check, so rank-deficient inputs, where many bonds fail,
+    // cost one retry per failing bond instead of a restart each.
     std::vector<int> force(d, 0);
     const std::vector<int64_t> r_in = r;
     int64_t start = 0;
-    while (d > 1) {
+    bool careful = false;
+    std::vector<int> hfl(4 * (d + 8));
+    std::vector<double> hst(8 * (d + 8));
+    while (d > 1 && !careful) {
       std::vector<int64_t> pending;
       for (int64_t i = start; i + 1 < d; ++i) {
         TTG_HIP(hipMemsetAsync(iflag + 4 * i, 0, 4 * sizeof(int), st));
         if (sweep1_bond(i, force[i], gen)) pending.push_back(i);
       }
       if (pending.empty()) break;
-      std::vector<int> hfl(4 * (d + 8));
-      std::vector<double> hst(8 * (d + 8));
       TTG_HIP(hipMemcpyAsync(hfl.data(), iflag, hfl.size() * sizeof(int), hipMemcpyDeviceToHost, st));
       TTG_HIP(hipMemcpyAsync(hst.data(), bstats, hst.size() * sizeof(double), hipMemcpyDeviceToHost, st));
       TTG_HIP(hipStreamSynchronize(st));
@@ -352,6 +355,21 @@ template <class T> struct Rounder {
       force[bad] = (force[bad] == 0 && mixed) ? 1 : 2;
       for (int64_t j = bad + 1; j <= d; ++j) r[j] = r_in[j];
       start = bad;
+      careful = true;
+    }
+    if (careful) {
+      for (int64_t i = start; i + 1 < d; ++i) {
+        for (;;) {
+          TTG_HIP(hipMemsetAsync(iflag + 4 * i, 0, 4 * sizeof(int), st));
+          if (!sweep1_bond(i, force[i], gen)) break;
+          TTG_HIP(hipMemcpyAsync(hfl.data() + 4 * i, iflag + 4 * i, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+          TTG_HIP(hipMemcpyAsync(hst.data() + 8 * i, bstats + 8 * i, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+          TTG_HIP(hipStreamSynchronize(st));
+          if (qr_accept(&hfl[4 * i], &hst[8 * i])) break;
+          force[i] = (force[i] == 0 && mixed) ? 1 : 2;
+          r[i + 1] = r_in[i + 1];
+        }
+      }
     }
     const double ep = d > 1 ? eps / std::sqrt((double)(d - 1)) : INFINITY;
     for (int64_t i = d - 1; i >= 1; --i) {

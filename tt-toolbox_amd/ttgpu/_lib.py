@@ -6,7 +6,9 @@ gfx950 device is missing, every call raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libttgpu.so"
@@ -134,6 +136,7 @@ class Context:
         self.h = h
         self.lib = lib
         self.device = device
+        _live.append(weakref.ref(self))
 
     def check(self, status, allow=(TTG_OK,)):
         if status in allow:
@@ -168,6 +171,24 @@ class Context:
 
 
 _default_ctx = {}
+_live = []
+
+
+def _close_all():
+    """Destroy every context (and its batched-rounding workers) while the HIP
+    runtime is still alive: interpreter-teardown finalisers run too late."""
+    for ref in list(_live):
+        ctx = ref()
+        if ctx is not None:
+            try:
+                ctx.close()
+            except Exception:
+                pass
+    _live.clear()
+    _default_ctx.clear()
+
+
+atexit.register(_close_all)
 
 
 def default_context(device: int = 0) -> Context:
