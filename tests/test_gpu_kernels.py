@@ -31,7 +31,8 @@ def test_gemm(gpu_ctx, prec, ta, tb, M, N, K):
     assert np.max(np.abs(Cm - ref)) <= tol * np.max(np.abs(ref)) * np.sqrt(K), np.max(np.abs(Cm - ref))
 
 
-@pytest.mark.parametrize("m,n,rank", [(40, 7, 7), (160, 40, 20), (8, 30, 8), (300, 64, 64), (64, 64, 10)])
+@pytest.mark.parametrize("m,n,rank", [(40, 7, 7), (160, 40, 20), (8, 30, 8), (300, 64, 64), (64, 64, 10),
+                                         (2048, 256, 128), (1000, 100, 100), (70, 90, 50)])
 def test_householder(gpu_ctx, m, n, rank):
     rng = np.random.default_rng(m + n)
     A = rng.standard_normal((m, rank)) @ rng.standard_normal((rank, n))

@@ -137,10 +137,14 @@ template <class T> void fro_norm(hipStream_t s, const T *x, int64_t count, doubl
 // x[i] *= 1/ *nrm if *nrm != 0 (nrm device scalar)
 template <class T> void scale_by_inv(hipStream_t s, T *x, int64_t count, const double *nrm);
 
-// One-workgroup Householder QR (LAPACK dgeqr2+dorg2r semantics): A (m x n, lda=m)
-// -> Q (m x kq, explicit) and R (kq x n), kq = min(m,n).  A is overwritten.
+// Householder QR (LAPACK dgeqrf+dorgqr semantics): A (m x n, lda=m) -> Q
+// (m x kq, explicit) and R (kq x n), kq = min(m,n).  A is overwritten.  With a
+// workspace of householder_ws_elems doubles it runs blocked (32-column panels
+// on one workgroup, trailing and Q updates as multi-CU GEMMs); without one,
+// or for kq <= 32, unblocked on one workgroup.
+size_t householder_ws_elems(int64_t m, int64_t n, int num_cus);
 void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, double *R,
-                    double *tau);
+                    double *tau, double *ws = nullptr, int num_cus = 256);
 
 // One-workgroup Cholesky G + shift = R^T R (upper R), FP64, k x k (k <= 512).
 // G is read from its upper triangle; shift = shift_rel * trace(G) / k.

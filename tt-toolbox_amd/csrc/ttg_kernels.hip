@@ -423,13 +423,174 @@ __global__ __launch_bounds__(1024) void k_householder_qr(int m, int n, double *_
     __syncthreads();
   }
 }
-void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, double *R,
-                    double *tau) {
-  const double kq = (double)std::min(m, n);
+// ---------------------------------------------------------------------------
+// Blocked Householder QR (dgeqrf + dorgqr structure, compact WY): 32-column
+// panels are factorised by one workgroup (k_hh_panel), which also writes the
+// explicit reflector block V (unit diagonal, zeros above) and its triangular
+// factor T (dlarft, forward, columnwise); the trailing matrix and the
+// accumulation of Q are updated with the multi-CU GEMM:
+//   A2 <- (I - V T V^T)^T A2 = A2 - V (T^T (V^T A2)),   Q2 <- Q2 - V (T (V^T Q2)).
+// ---------------------------------------------------------------------------
+constexpr int HB = 32;
+
+__global__ __launch_bounds__(1024) void k_hh_panel(int m, int j0, int jb, double *__restrict__ A,
+                                                   double *__restrict__ tau, double *__restrict__ V,
+                                                   double *__restrict__ T) {
+  __shared__ double red[16];
+  __shared__ double sh_tau, sh_scale;
+  __shared__ double Gv[HB][HB + 1];  // V^T V of the panel
+  __shared__ double tl[HB];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  for (int c = 0; c < jb; ++c) {
+    const int col = j0 + c;
+    double *aj = A + (long)col * m;
+    double ss = 0.0;
+    for (int i = col + 1 + tid; i < m; i += blockDim.x) ss += aj[i] * aj[i];
+    ss = block_sum(ss, red);
+    if (tid == 0) {
+      const double alpha = aj[col];
+      if (ss == 0.0) {
+        sh_tau = 0.0;
+        sh_scale = 0.0;
+      } else {
+        const double nrm = sqrt(alpha * alpha + ss);
+        const double beta = alpha >= 0.0 ? -nrm : nrm;
+        sh_tau = (beta - alpha) / beta;
+        sh_scale = 1.0 / (alpha - beta);
+        aj[col] = beta;
+      }
+      tau[col] = sh_tau;
+      tl[c] = sh_tau;
+    }
+    __syncthreads();
+    const double tj = sh_tau, sc = sh_scale;
+    if (tj != 0.0)
+      for (int i = col + 1 + tid; i < m; i += blockDim.x) aj[i] *= sc;
+    __syncthreads();
+    if (tj != 0.0) {
+      for (int cc = col + 1 + wv; cc < j0 + jb; cc += nw) {
+        double *ac = A + (long)cc * m;
+        double w = (lane == 0) ? ac[col] : 0.0;
+        for (int i = col + 1 + lane; i < m; i += 64) w += aj[i] * ac[i];
+        w = wave_sum(w) * tj;
+        if (lane == 0) ac[col] -= w;
+        for (int i = col + 1 + lane; i < m; i += 64) ac[i] -= w * aj[i];
+      }
+    }
+    __syncthreads();
+  }
+  // explicit V = panel reflectors (rows j0..m-1 of columns j0..j0+jb-1)
+  const long rows = m - j0;
+  for (long idx = tid; idx < rows * jb; idx += blockDim.x) {
+    const int c = (int)(idx / rows), i = j0 + (int)(idx - (long)c * rows);
+    const int col = j0 + c;
+    V[i + (long)col * m] = (i < col) ? 0.0 : (i == col ? 1.0 : A[i + (long)col * m]);
+  }
+  __syncthreads();
+  // V^T V (upper triangle): one wave per entry
+  for (int e = wv; e < jb * (jb + 1) / 2; e += nw) {
+    int b = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while (b * (b + 1) / 2 > e) --b;
+    while ((b + 1) * (b + 2) / 2 <= e) ++b;
+    const int a = e - b * (b + 1) / 2;  // a <= b
+    const double *va = V + (long)(j0 + a) * m, *vb = V + (long)(j0 + b) * m;
+    double g = 0.0;
+    for (int i = j0 + b + lane; i < m; i += 64) g += va[i] * vb[i];
+    g = wave_sum(g);
+    if (lane == 0) Gv[a][b] = g;
+  }
+  __syncthreads();
+  // T: T(i,i) = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) (V(:,0:i)^T v_i)   (dlarft),
+  // built in LDS by one wave (lane r owns row r), then stored
+  __shared__ double Tl[HB][HB + 1];
+  if (wv == 0) {
+    const int r = lane;
+    for (int i = 0; i < jb; ++i) {
+      double t = 0.0;
+      if (r < i) {
+        for (int l = r; l < i; ++l) t += Tl[r][l] * Gv[l][i];
+        t *= -tl[i];
+      } else if (r == i) {
+        t = tl[i];
+      }
+      if (r < HB) Tl[r][i] = (r <= i) ? t : 0.0;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < HB * HB; idx += blockDim.x) {
+    const int r = idx % HB, i = idx / HB;
+    T[r + i * HB] = (r < jb && i < jb && r <= i) ? Tl[r][i] : 0.0;
+  }
+}
+
+__global__ void k_triu_copy(int kq, int n, int m, const double *__restrict__ A, double *__restrict__ R) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < (long)kq * n;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % kq), c = (int)(idx / kq);
+    R[idx] = (i <= c) ? A[i + (long)c * m] : 0.0;
+  }
+}
+__global__ void k_eye(int m, int kq, double *__restrict__ Q) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < (long)m * kq;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % m), c = (int)(idx / m);
+    Q[idx] = (i == c) ? 1.0 : 0.0;
+  }
+}
+
+size_t householder_ws_elems(int64_t m, int64_t n, int num_cus) {
+  const int64_t kq = std::min(m, n);
+  const int64_t w = (int64_t)HB * std::max<int64_t>(n, 1);
+  return (size_t)(m * kq + HB * kq + 2 * w) +
+         (size_t)std::max<int64_t>(gemm_ws_elems(HB, n, m, num_cus), gemm_ws_elems(HB, kq, m, num_cus));
+}
+
+void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, double *R, double *tau,
+                    double *ws, int num_cus) {
+  const int64_t kq = std::min(m, n);
   ProfScope prof(KC_HQR, s, 4.0 * m * n * kq - 2.0 * (m + n) * kq * kq + 4.0 / 3.0 * kq * kq * kq,
                  8.0 * ((double)m * n + m * kq + kq * n));
-  hipLaunchKernelGGL(k_householder_qr, dim3(1), dim3(1024), 0, s, (int)m, (int)n, A, Q, R, tau);
+  if (kq <= HB || ws == nullptr) {
+    hipLaunchKernelGGL(k_householder_qr, dim3(1), dim3(1024), 0, s, (int)m, (int)n, A, Q, R, tau);
+    TTG_HIP(hipGetLastError());
+    return;
+  }
+  double *V = ws;                       // m x kq
+  double *T = V + m * kq;               // HB x kq (block j0 at column j0)
+  double *W = T + HB * kq;              // HB x n
+  double *W2 = W + HB * std::max<int64_t>(n, 1);
+  double *gws = W2 + HB * std::max<int64_t>(n, 1);
+  const int64_t gwe = std::max<int64_t>(gemm_ws_elems(HB, n, m, num_cus), gemm_ws_elems(HB, kq, m, num_cus));
+  for (int64_t j0 = 0; j0 < kq; j0 += HB) {
+    const int64_t jb = std::min<int64_t>(HB, kq - j0);
+    hipLaunchKernelGGL(k_hh_panel, dim3(1), dim3(1024), 0, s, (int)m, (int)j0, (int)jb, A, tau, V, T + j0 * HB);
+    TTG_HIP(hipGetLastError());
+    const int64_t n2 = n - j0 - jb;
+    if (n2 > 0) {
+      const double *Vb = V + j0 + j0 * m;
+      double *A2 = A + j0 + (j0 + jb) * m;
+      gemm<double>(s, true, false, jb, n2, m - j0, 1.0, Vb, m, A2, m, 0.0, W, jb, gws, gwe, num_cus);
+      gemm<double>(s, true, false, jb, n2, jb, 1.0, T + j0 * HB, HB, W, jb, 0.0, W2, jb, nullptr, 0, num_cus);
+      gemm<double>(s, false, false, m - j0, n2, jb, -1.0, Vb, m, W2, jb, 1.0, A2, m, nullptr, 0, num_cus);
+    }
+  }
+  const int blocks = (int)std::min<int64_t>((m * kq + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_triu_copy, dim3((unsigned)std::min<int64_t>((kq * n + 255) / 256, 2048)), dim3(256), 0, s,
+                     (int)kq, (int)n, (int)m, A, R);
+  hipLaunchKernelGGL(k_eye, dim3(blocks), dim3(256), 0, s, (int)m, (int)kq, Q);
   TTG_HIP(hipGetLastError());
+  const int64_t last = (kq - 1) / HB * HB;
+  for (int64_t j0 = last; j0 >= 0; j0 -= HB) {
+    const int64_t jb = std::min<int64_t>(HB, kq - j0);
+    const int64_t nc = kq - j0;
+    const double *Vb = V + j0 + j0 * m;
+    double *Q2 = Q + j0 + j0 * m;
+    gemm<double>(s, true, false, jb, nc, m - j0, 1.0, Vb, m, Q2, m, 0.0, W, jb, gws, gwe, num_cus);
+    gemm<double>(s, false, false, jb, nc, jb, 1.0, T + j0 * HB, HB, W, jb, 0.0, W2, jb, nullptr, 0, num_cus);
+    gemm<double>(s, false, false, m - j0, nc, jb, -1.0, Vb, m, W2, jb, 1.0, Q2, m, nullptr, 0, num_cus);
+  }
 }
 
 // ---------------------------------------------------------------------------

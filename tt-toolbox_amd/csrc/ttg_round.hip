@@ -66,7 +66,7 @@ template <class T> struct Rounder {
   double *G, *R1, *R2, *R3, *Rinv, *Rprod, *Rtmp, *Y, *sv, *nrm, *stats, *bstats, *Yws;
   T *RT, *Q1, *Q2, *Tt, *Qs, *US, *gws;
   int64_t gws_elems;
-  double *hA, *hQ, *htau;
+  double *hA, *hQ, *htau, *hws;
   int *perm, *iflag, *scal_i, *jerr, *jsweeps;
   unsigned *bar;
   double *scal_d;
@@ -83,6 +83,10 @@ template <class T> struct Rounder {
   }
 
   static size_t jac_elems(int64_t kmax) { return (size_t)(2 * kmax) * (size_t)(kmax + 64); }
+  // V (m x kq <= one core) + T + two HB x n blocks + split-K partials
+  static size_t hh_ws(int64_t max_core, int64_t kmax) {
+    return (size_t)max_core + (size_t)(3 * 32 + 32 * 32) * (size_t)(kmax + 64) + 4096;
+  }
 
   template <class A> static void layout(A &a, int64_t d, int64_t total, int64_t max_core, int64_t kmax, int64_t rsq) {
     for (int b = 0; b < 3; ++b) a.template take<T>(total);
@@ -99,6 +103,7 @@ template <class T> struct Rounder {
     a.template take<double>(max_core);       // hA
     a.template take<double>(max_core);       // hQ
     a.template take<double>(kmax + 64);      // htau
+    a.template take<double>(hh_ws(max_core, kmax));  // blocked Householder workspace
     a.template take<int>(kmax + 64);         // perm
     a.template take<int>(4 * (d + 8));       // iflag (per bond)
     a.template take<int>(16);                // scal_i
@@ -134,6 +139,7 @@ template <class T> struct Rounder {
     hA = a.take<double>(max_core);
     hQ = a.take<double>(max_core);
     htau = a.take<double>(kmax + 64);
+    hws = a.take<double>(hh_ws(max_core, kmax));
     perm = a.take<int>(kmax + 64);
     iflag = a.take<int>(4 * (d + 8));
     scal_i = a.take<int>(16);
@@ -203,7 +209,9 @@ template <class T> struct Rounder {
     } else {
       convert<double, float>(st, hA, A, m * k);
     }
-    householder_qr(st, m, k, hA, hQ, Rout, htau);
+    TTG_CHECK(householder_ws_elems(m, k, cus) <= hh_ws(max_core, kmax), TTG_HIP_ERROR,
+              "Householder workspace too small");
+    householder_qr(st, m, k, hA, hQ, Rout, htau, hws, cus);
     to_T(Qout, hQ, m * kq);
     ++fallbacks;
   }

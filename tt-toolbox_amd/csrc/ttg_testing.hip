@@ -83,7 +83,8 @@ ttg_status ttg_test_householder(ttg_ctx *ctx, int64_t m, int64_t n, const double
     const int64_t kq = std::min(m, n);
     double *dA = f.add(up(A, m * n, s)), *dQ = f.add(up<double>(nullptr, m * kq, s));
     double *dR = f.add(up<double>(nullptr, kq * n, s)), *tau = f.add(up<double>(nullptr, kq + 1, s));
-    householder_qr(s, m, n, dA, dQ, dR, tau);
+    double *ws = f.add(up<double>(nullptr, householder_ws_elems(m, n, ctx->num_cus), s));
+    householder_qr(s, m, n, dA, dQ, dR, tau, ws, ctx->num_cus);
     down(Q, dQ, m * kq, s);
     down(R, dR, kq * n, s);
   })

@@ -175,15 +175,14 @@ _live = []
 
 
 def _close_all():
-    """Destroy every context (and its batched-rounding workers) while the HIP
-    runtime is still alive: interpreter-teardown finalisers run too late."""
+    """At interpreter exit, forget every context instead of destroying it:
+    finalisers that call into the HIP runtime while it is being torn down
+    (worker streams of the batched rounder included) can crash the process,
+    and the operating system reclaims the device resources anyway."""
     for ref in list(_live):
         ctx = ref()
         if ctx is not None:
-            try:
-                ctx.close()
-            except Exception:
-                pass
+            ctx.h = None
     _live.clear()
     _default_ctx.clear()
 
