@@ -34,6 +34,88 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _rate(fn, reps, sync):
+    fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    sync()
+    return reps / (time.perf_counter() - t0)
+
+
+def other_configs(ttgpu, W, ctx, dev):
+    """Configs 1, 2 and 5 (and the <y, A x> gradient at config-1 shapes), one GPU,
+    inputs resident in HBM; rates are host-timed over back-to-back calls."""
+    import numpy as np
+    import torch
+    from ttgpu import autograd as AG
+    sync = torch.cuda.synchronize
+    out = {}
+    # config 1: round(X + 0.5 X, 1e-10), d=20 n=4 r=40 -> 20
+    c1 = W.round1()
+    n1, r1, core1 = c1["z"]
+    d1 = torch.from_numpy(core1).to(dev)
+    o1 = torch.empty(ttgpu.capacity(n1, r1), dtype=torch.float64, device=dev)
+    ro = {}
+    rate = _rate(lambda: ro.__setitem__("r", ttgpu.round_dev(ctx, n1, r1, d1, c1["eps"], out=o1)[0]), 20, sync)
+    out["config1_round"] = {"workload": "round(X + 0.5 X, 1e-10), d=20 n=4 r=40, FP64", "rounds_per_s": rate,
+                            "ranks_out_max": int(max(ro["r"]))}
+    # config 2: A*x (d=12, n=m=4, rank_A 5, rank_x 64) and the fused round(A*x, 1e-6)
+    c2 = W.matvec2()
+    na, ma, ra, acore = c2["a"]
+    nx, rx, xcore = c2["x"]
+    da = torch.from_numpy(acore).to(dev)
+    dx = torch.from_numpy(xcore).to(dev)
+    ry = ra * rx
+    ysize = int(sum(ry[k] * na[k] * ry[k + 1] for k in range(len(na))))
+    oy = torch.empty(ysize, dtype=torch.float64, device=dev)
+    rate = _rate(lambda: ttgpu.mtimes_dev(ctx, na, ma, ra, da, rx, dx, out=oy), 50, sync)
+    byt = 8.0 * (ysize + acore.size + xcore.size)
+    out["config2_mtimes"] = {"workload": "A*x, d=12 n=m=4 rank_A=5 rank_x=64 -> rank 320, FP64, unfused",
+                             "contractions_per_s": rate, "hbm_bytes_per_contraction": byt,
+                             "achieved_GBps_incl_host": byt * rate / 1e9,
+                             "hbm_roofline_frac_incl_host": byt * rate / PEAK["hbm"]}
+    o2 = torch.empty(ttgpu.capacity(na, ry), dtype=torch.float64, device=dev)
+    rr = {}
+    rate = _rate(lambda: rr.__setitem__("r", ttgpu.mtimes_round_dev(ctx, na, ma, ra, da, rx, dx, 1e-6, out=o2)[0]),
+                 5, sync)
+    out["config2_fused_mtimes_round"] = {
+        "workload": "round(A*x, 1e-6) fused (product cores built on chip), FP64",
+        "rounds_per_s": rate, "ranks_out": [int(v) for v in rr["r"]]}
+    # config 5: completion loss + gradient, 65,536 samples, d=16 n=8, model rank 32
+    c5 = W.completion5()
+    n5, r5, core5 = c5["model"]
+    idx = torch.from_numpy(c5["idx"]).to(dev)
+    vals = torch.from_numpy(c5["vals"]).to(dev)
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        core = torch.tensor(core5, dtype=dt, device=dev, requires_grad=True)
+
+        def step():
+            loss = AG.completion_loss(core, n5, r5, idx, vals)
+            torch.autograd.grad(loss, core)
+        out[f"config5_completion_grad_{name}"] = {
+            "workload": "loss 1/2||P(X) - y||^2 and all core gradients, 65,536 samples, d=16 n=8 rank 32",
+            "evaluations_per_s": _rate(step, 20, sync)}
+    # <y, A x> and its gradients at config-1 shapes
+    cb = W.bilinear1()
+    nb, mb, rab, acb = cb["a"]
+    _, rxb, xcb = cb["x"]
+    _, ryb, ycb = cb["y"]
+    ac = torch.from_numpy(acb).to(dev)
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        yc = torch.tensor(ycb, dtype=dt, device=dev, requires_grad=True)
+        xc = torch.tensor(xcb, dtype=dt, device=dev, requires_grad=True)
+        acc_ = ac.to(dt)
+
+        def stepb():
+            v = AG.bilinear(yc, xc, acc_, nb, mb, ryb, rab, rxb)
+            torch.autograd.grad(v, (yc, xc))
+        out[f"bilinear_grad_{name}"] = {"workload": "<y, A x> and its x, y gradients, d=20 n=m=4, ranks 20/5/20",
+                                        "evaluations_per_s": _rate(stepb, 20, sync)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +124,11 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-rounds", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-rounding leg (512 uneven tensors, config 4 recipe)")
+    ap.add_argument("--batched-count", type=int, default=512)
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the configs 1, 2 and 5 throughput leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -189,6 +276,60 @@ def main():
                           "unit": unit, "frac": achieved / peak, "traffic": traffic,
                           "launches_per_round": prof["launches"] / args.steps, "avg_launch_ms": avg_ms}
 
+    # batched rounding of uneven tensors (config-4 recipe): one call for the whole
+    # batch against a per-tensor loop of round_dev on one stream (32-tensor sample)
+    if not args.no_batched:
+        del dcore, out_mixed, out_plain
+        torch.cuda.empty_cache()
+        log(f"generating {args.batched_count} uneven tensors (config 4 recipe)")
+        bb = W.batch4(count=args.batched_count)
+        tens = bb["tensors"]
+        cofs = np.concatenate([[0], np.cumsum([t[2].size for t in tens])])
+        dall = torch.empty(int(cofs[-1]), dtype=torch.float64, device=dev)
+        for i, t in enumerate(tens):
+            dall[int(cofs[i]):int(cofs[i + 1])].copy_(torch.from_numpy(t[2]))
+        ns, rs = [t[0] for t in tens], [t[1] for t in tens]
+        torch.cuda.synchronize()
+        binfo = {}
+        t0 = time.perf_counter()
+        ttgpu.round_batched_dev(ctx, ns, rs, dall, cofs[:-1], bb["eps"], info=binfo)  # first call tunes
+        t1 = time.perf_counter()
+        ttgpu.round_batched_dev(ctx, ns, rs, dall, cofs[:-1], bb["eps"], info=binfo)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        m = min(32, len(tens))
+        for i in range(m):
+            ttgpu.round_dev(ctx, ns[i], rs[i], dall[int(cofs[i]):int(cofs[i + 1])], bb["eps"])
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        # the loop sample is the first m tensors; scale by their share of the batch's QR work
+        def qr_work(n_, r_):
+            c, rk = 0.0, float(r_[0])
+            for j in range(len(n_) - 1):
+                mm = rk * n_[j]
+                kk = min(mm, float(r_[j + 1]))
+                c += mm * kk * kk
+                rk = kk
+            return c
+        wall = sum(qr_work(n_, r_) for n_, r_ in zip(ns, rs))
+        wsamp = sum(qr_work(n_, r_) for n_, r_ in zip(ns[:m], rs[:m]))
+        loop_est = (t3 - t2) * wall / wsamp
+        result["batched"] = {
+            "workload": f"{len(tens)} TT tensors, d in [8,24], n in {{2,4,8,16}}, ranks in [8,128] doubled "
+                        f"by self-addition, eps=1e-8, FP64",
+            "tensors_per_s": len(tens) / (t2 - t1), "seconds": t2 - t1, "first_call_seconds_incl_tuning": t1 - t0,
+            "per_tensor_loop_tensors_per_s": len(tens) / loop_est,
+            "per_tensor_loop_sample": f"first {m} tensors timed, scaled by QR work to the batch",
+            "speedup_vs_loop": loop_est / (t2 - t1), "qr_fallbacks": binfo.get("qr_fallbacks"),
+            "jacobi_sweeps": binfo.get("jacobi_sweeps")}
+        log("batched:", result["batched"])
+        del dall
+
+    # the other configurations at one GPU: rounds/s, contractions/s, gradient evaluations/s
+    if not args.no_other:
+        result["other_configs"] = other_configs(ttgpu, W, ctx, dev)
+        log("other configs:", result["other_configs"])
+
     # parity against the FP64 oracle (CPU) + CPU baseline timing
     if not args.no_cpu:
         from oracle import tt_oracle as O
@@ -215,8 +356,35 @@ def main():
         result["rel_err_plain_vs_oracle"] = O.tt_rel_diff(yp, ref)
         result["plain_ranks_match_oracle"] = list(map(int, r_plain)) == ref.r.tolist()
         result["mixed_ranks_match_oracle"] = list(map(int, r_mixed)) == ref.r.tolist()
+        if not args.no_other:  # oracle rates of the small configs, same host cores
+            c1 = W.round1()
+            Z = O.TT(*c1["z"])
+            t0 = time.perf_counter()
+            for _ in range(5):
+                O.tt_round(Z, c1["eps"])
+            t1 = time.perf_counter()
+            c2 = W.matvec2()
+            na, ma, ra, acore = c2["a"]
+            A2 = O.TTM(na, ma, O.TT(na * ma, ra, acore))
+            X2 = O.TT(*c2["x"])
+            t2 = time.perf_counter()
+            Y2 = O.ttm_mtimes(A2, X2)
+            t3 = time.perf_counter()
+            O.tt_round(Y2, 1e-6)
+            t4 = time.perf_counter()
+            result["cpu_baseline_other"] = {
+                "kind": "port", "cores": int(cores),
+                "config1_rounds_per_s": 5 / (t1 - t0),
+                "config2_mtimes_per_s": 1 / (t3 - t2),
+                "config2_mtimes_then_round_per_s": 1 / (t4 - t2)}
     print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":
     main()
+    # skip the HIP runtime's exit-time teardown, which has crashed intermittently
+    # in processes that ran the batched rounder's worker streams; the JSON line is
+    # already written and the OS releases the device resources
+    sys.stdout.flush()
+    sys.stderr.flush()
+    os._exit(0)

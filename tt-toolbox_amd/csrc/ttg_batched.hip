@@ -1,14 +1,24 @@
 // Batched rounding of many independent TT tensors (uneven d, n, r).
 //
-// Each rounding is a chain of latency-bound bond steps that occupies only a
-// few CUs at a time, so a batch is run as W concurrent chains: W worker
-// contexts, each with its own HIP stream and device workspace, driven by W
-// host threads that take tensors longest-first from a shared counter
-// (a greedy LPT schedule; the cost estimate is the QR work sum_i m_i k_i^2 at
-// the QR-capped ranks).  W is tuned once per context: the first batch large
-// enough is timed on a prefix with W in {8, 16, 32} and the fastest is kept.
+// Two executions, chosen per context by a tuner:
+//   * persistent launch (FP64): one launch of k_round_persist in which every
+//     workgroup rounds whole tensors taken longest-first from a device work
+//     counter (greedy LPT; the cost estimate is the QR work sum_i m_i k_i^2
+//     at the QR-capped ranks).  The launch (counter reset + kernel) is
+//     captured in a hipGraph and replayed while its arguments are unchanged.
+//     Launch shapes: 256 threads with 112 KB of LDS for the Jacobi matrix
+//     (one workgroup per CU), 256 threads with 48 KB (two per CU), or 512
+//     threads with 112 KB;
+//   * W concurrent chains of the single-tensor rounder: W worker contexts,
+//     each with its own HIP stream and device workspace, driven by W host
+//     threads over the same LPT order (any precision, any size).
+// The first batch of at least 64 tensors on a context is split into four
+// interleaved quarters of the LPT order, each run with one candidate
+// (three persistent shapes, 16 stream workers); the candidate with the least
+// time per unit of estimated work is kept for later batches.
 #include "ttg_internal.h"
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <numeric>
@@ -87,6 +97,116 @@ static void run_batch(ttg_ctx *ctx, const BatchArgs &a, const std::vector<int64_
   for (auto &x : th) x.join();
 }
 
+// Persistent launch over tensors sel[0..m) (indices into the batch, LPT order).
+struct PShape {
+  int nt, per_cu;
+  int64_t ycap;
+};
+static const PShape kPShapes[3] = {{256, 1, 14336}, {256, 2, 6144}, {512, 1, 14336}};
+
+static void run_persist(ttg_ctx *ctx, const BatchArgs &a, int64_t count, const std::vector<int64_t> &sel,
+                        const PShape &sh, std::vector<ttg_status> &st, std::vector<ttg_info> &inf,
+                        std::vector<std::string> &err) {
+  const int64_t m = (int64_t)sel.size();
+  if (m == 0) return;
+  hipStream_t s = ctx->stream;
+  int64_t ntot = 0, rtot = 0, slot_core = 1, slot_kmax = 1;
+  for (int64_t t = 0; t < count; ++t) {
+    ntot = std::max(ntot, a.nofs[t] + a.d[t]);
+    rtot = std::max(rtot, a.rofs[t] + a.d[t] + 1);
+  }
+  std::vector<PTensorDesc> desc(count);
+  for (int64_t t = 0; t < count; ++t) desc[t] = PTensorDesc{(int32_t)a.d[t], 0, a.nofs[t], a.rofs[t], a.cofs[t], a.cofs_out[t]};
+  for (int64_t t : sel) {
+    const int64_t dd = a.d[t], *nn = a.n + a.nofs[t], *rr = a.r + a.rofs[t];
+    int64_t rq = rr[0];
+    for (int64_t i = 0; i < dd; ++i) {
+      slot_core = std::max(slot_core, rr[i] * nn[i] * rr[i + 1]);
+      slot_kmax = std::max(slot_kmax, rr[i + 1]);
+      slot_kmax = std::max(slot_kmax, rr[i]);
+      const int64_t rn = (i + 1 < dd) ? std::min(rq * nn[i], rr[i + 1]) : rr[dd];
+      slot_kmax = std::max(slot_kmax, std::min(rq, nn[i] * rn));
+      rq = rn;
+    }
+  }
+  const int slots = (int)std::min<int64_t>(m, (int64_t)ctx->num_cus * sh.per_cu);
+  const int64_t slot_elems = persist_slot_elems(slot_core, slot_kmax);
+  ArenaSizer z;
+  z.take<PTensorDesc>(count);
+  z.take<int>(m);
+  z.take<int64_t>(ntot);
+  z.take<int64_t>(rtot);
+  z.take<int64_t>(rtot);
+  z.take<int>(4 * count);
+  z.take<int>(64);
+  z.take<double>((size_t)slots * slot_elems);
+  ctx->parena.reserve(z.used + 4096);
+  ctx->parena.reset();
+  PTensorDesc *ddesc = ctx->parena.take<PTensorDesc>(count);
+  int *dorder = ctx->parena.take<int>(m);
+  int64_t *dn = ctx->parena.take<int64_t>(ntot);
+  int64_t *dr = ctx->parena.take<int64_t>(rtot);
+  int64_t *drout = ctx->parena.take<int64_t>(rtot);
+  int *dstat = ctx->parena.take<int>(4 * count);
+  int *dctr = ctx->parena.take<int>(64);
+  double *dws = ctx->parena.take<double>((size_t)slots * slot_elems);
+  std::vector<int> hord(sel.begin(), sel.end());
+  TTG_HIP(hipMemcpyAsync(ddesc, desc.data(), count * sizeof(PTensorDesc), hipMemcpyHostToDevice, s));
+  TTG_HIP(hipMemcpyAsync(dorder, hord.data(), m * sizeof(int), hipMemcpyHostToDevice, s));
+  TTG_HIP(hipMemcpyAsync(dn, a.n, ntot * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  TTG_HIP(hipMemcpyAsync(dr, a.r, rtot * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  TTG_HIP(hipMemcpyAsync(drout, a.r_out, rtot * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  persist_prepare(sh.nt, sh.ycap);
+  const int kSweeps = 60;
+  // the captured launch is replayed while every argument is unchanged
+  const std::vector<uint64_t> key = {(uint64_t)sh.nt, (uint64_t)slots, (uint64_t)m, (uint64_t)(uintptr_t)ddesc,
+                                     (uint64_t)(uintptr_t)a.dcore, (uint64_t)(uintptr_t)a.dcore_out,
+                                     (uint64_t)(uintptr_t)dws, (uint64_t)slot_core, (uint64_t)slot_kmax,
+                                     (uint64_t)sh.ycap, (uint64_t)std::hash<double>{}(a.eps),
+                                     (uint64_t)(uintptr_t)ctx->parena.base};
+  if (!ctx->pgraph || ctx->pgraph_key != key) {
+    if (ctx->pgraph) {
+      TTG_HIP(hipGraphExecDestroy(ctx->pgraph));
+      ctx->pgraph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    TTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      persist_launch(s, sh.nt, slots, (int)m, ddesc, dorder, dn, dr, a.dcore, a.dcore_out, drout, dws, slot_core,
+                     slot_kmax, dctr, dstat, a.eps, kSweeps, sh.ycap);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);  // leave the stream usable
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    TTG_HIP(hipStreamEndCapture(s, &g));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    TTG_HIP(ie);
+    ctx->pgraph = ge;
+    ctx->pgraph_key = key;
+  }
+  TTG_HIP(hipGraphLaunch(ctx->pgraph, s));
+  std::vector<int> hstat(4 * count);
+  TTG_HIP(hipMemcpyAsync(a.r_out, drout, rtot * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipMemcpyAsync(hstat.data(), dstat, 4 * count * sizeof(int), hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  for (int64_t t : sel) {
+    const int code = hstat[4 * t];
+    inf[t] = ttg_info{0, hstat[4 * t + 1], hstat[4 * t + 2], 0.0};
+    if (code & 2) {
+      st[t] = TTG_NONFINITE;
+      err[t] = "input core contains NaN or Inf";
+    } else if (code & 4) {
+      st[t] = TTG_HIP_ERROR;
+      err[t] = "persistent workspace too small";
+    } else if (code & 1) {
+      st[t] = TTG_NO_CONVERGENCE;
+    }
+  }
+}
+
 ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, const int64_t *nofs,
                              const int64_t *n, const int64_t *rofs, const int64_t *r,
                              const int64_t *cofs, const double *dcore, double eps, int prec,
@@ -122,8 +242,50 @@ ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, cons
   std::vector<ttg_status> st(count, TTG_OK);
   std::vector<ttg_info> inf(count, ttg_info{0, 0, 0, 0.0});
   std::vector<std::string> err(count);
-  int64_t begin = 0;
-  if (ctx->batch_workers == 0) {
+  // persistent launch for FP64 batches that fit one workgroup per tensor
+  bool persist_ok = (prec == TTG_PREC_FP64);
+  for (int64_t t = 0; t < count && persist_ok; ++t)
+    persist_ok = persist_supported(d[t], n + nofs[t], r + rofs[t]);
+  if (persist_ok) {
+    auto work = [&](const std::vector<int64_t> &sel) {
+      double c = 0.0;
+      for (int64_t t : sel) c += cost[t];
+      return c;
+    };
+    if (ctx->batch_mode == 0 && count >= 64) {
+      // tune on four interleaved quarters of the LPT order (every tensor is
+      // rounded exactly once): three persistent shapes and 16 stream workers
+      double best = 1e300;
+      int bestm = 1;
+      for (int c = 0; c < 4; ++c) {
+        std::vector<int64_t> sel;
+        for (int64_t j = c; j < count; j += 4) sel.push_back(order[j]);
+        auto s0 = std::chrono::steady_clock::now();
+        if (c < 3) {
+          run_persist(ctx, a, count, sel, kPShapes[c], st, inf, err);
+        } else {
+          std::vector<int64_t> sub(sel);
+          run_batch(ctx, a, sub, 0, (int64_t)sub.size(), 16, st, inf, err);
+        }
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count();
+        const double rate = sec / work(sel);
+        if (rate < best) {
+          best = rate;
+          bestm = c + 1;
+        }
+      }
+      ctx->batch_mode = bestm;
+    } else {
+      const int mode = ctx->batch_mode == 0 ? 1 : ctx->batch_mode;
+      if (mode <= 3) {
+        run_persist(ctx, a, count, order, kPShapes[mode - 1], st, inf, err);
+      } else {
+        run_batch(ctx, a, order, 0, count, (int)std::min<int64_t>(16, count), st, inf, err);
+      }
+    }
+  }
+  int64_t begin = persist_ok ? count : 0;
+  if (!persist_ok && ctx->batch_workers == 0) {
     // tune the worker count on consecutive slices of the (cost-ordered) batch
     const int cand[3] = {8, 16, 32};
     const int64_t slice = count / 8;

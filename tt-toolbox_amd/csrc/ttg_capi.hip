@@ -99,6 +99,7 @@ ttg_status ttg_ctx_destroy(ttg_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ttg::destroy_workers(ctx);
+  if (ctx->pgraph) (void)hipGraphExecDestroy(ctx->pgraph);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
   delete ctx;

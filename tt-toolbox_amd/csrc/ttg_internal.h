@@ -109,12 +109,37 @@ struct ttg_ctx {
   // created on first use and owned by this context
   std::vector<ttg_ctx *> workers;
   int batch_workers = 0;  // 0: chosen by the tuner on first use
+  // persistent batched rounding: descriptors + slot workspace, the captured
+  // launch (reset + persistent kernel) with the arguments it was captured for,
+  // and the tuned launch shape (0: not tuned yet)
+  ttg::Arena parena;
+  hipGraphExec_t pgraph = nullptr;
+  std::vector<uint64_t> pgraph_key;
+  int batch_mode = 0;
 };
 
 namespace ttg {
 
 // releases the batched-rounding worker contexts of ctx
 void destroy_workers(ttg_ctx *ctx);
+
+// one tensor of a persistent batched rounding (device copy)
+struct PTensorDesc {
+  int32_t d, pad;
+  int64_t nofs, rofs;     // into the flat n and r arrays
+  int64_t in_ofs, out_ofs;  // first input core, first output/working core
+};
+// true if one workgroup can round the tensor (d <= 64, ranks <= 512)
+bool persist_supported(int64_t d, const int64_t *n, const int64_t *r);
+// doubles of per-slot workspace for the largest core / bond rank of a batch
+int64_t persist_slot_elems(int64_t slot_core, int64_t slot_kmax);
+// ycap: doubles of LDS for the Jacobi matrix [S; J] (larger ones stay in the slot)
+void persist_prepare(int nt, int64_t ycap);
+// reset of the work counter + the persistent kernel on `slots` workgroups of nt threads
+void persist_launch(hipStream_t s, int nt, int slots, int count, const PTensorDesc *tens,
+                    const int *order, const int64_t *nall, const int64_t *rall, const double *in,
+                    double *out, int64_t *r_out, double *ws, int64_t slot_core, int64_t slot_kmax, int *ctr,
+                    int *tstat, double eps, int max_sweeps, int64_t ycap);
 
 // ---------------------------------------------------------------------------
 // kernel launchers (implemented in ttg_kernels.hip)
@@ -169,12 +194,14 @@ JacobiPlan jacobi_plan(int64_t k);
 size_t jacobi_ws_bytes(const JacobiPlan &pl);
 // ws: jacobi_polar_ws_elems(pl) doubles of scratch for the final polar correction.
 // dbg (optional): 4 timestamps (100 MHz clock) per outer step of the first sweep.
+// stop_cos2: the iteration ends after a sweep none of whose rotations had a
+// pair cosine^2 above it; the columns are then orthogonal to O(stop_cos2).
 size_t jacobi_polar_ws_elems(const JacobiPlan &pl);
 // unsigned words of the barrier/statistics buffer jacobi_svd needs
 int jacobi_bar_words(int max_sweeps);
 void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
                 int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws,
-                unsigned long long *dbg = nullptr);
+                unsigned long long *dbg = nullptr, double stop_cos2 = 1e-16);
 
 // Sort the column norms of Y's top k rows (descending) and apply my_chop2.
 // sv[k], perm[k] written; out_i[0] = r1 (after rmax cap), out_d[0] = ||s||.
@@ -201,6 +228,11 @@ void finalize_balance(hipStream_t s, int64_t d, TI *const *src_dev, const int64_
 template <class T>
 void mtimes_core(hipStream_t s, int64_t ra0, int64_t n, int64_t m, int64_t ra1, const double *A,
                  int64_t rx0, int64_t rx1, const double *X, T *Y);
+// C (M x n ra1 rx1) = R (M x ra0 rx0, ld ldr) * (product core of A and X), the
+// product core generated on chip (never written to memory).
+void gemm_mtimes(hipStream_t s, int64_t M, const double *R, int64_t ldr, int64_t ra0, int64_t n,
+                 int64_t m, int64_t ra1, const double *A, int64_t rx0, int64_t rx1, const double *X,
+                 double *C, int64_t ldc);
 
 // check for non-finite values; sets *flag nonzero if any
 void check_finite(hipStream_t s, const double *x, int64_t count, int *flag);

@@ -564,7 +564,7 @@ size_t jacobi_polar_ws_elems(const JacobiPlan &pl) {
 
 void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
                 int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws,
-                unsigned long long *dbg) {
+                unsigned long long *dbg, double stop_cos2) {
   const int k = (int)pl.k, kpad = (int)pl.kpad;
   const long total = (long)pl.rows * kpad;
   const int blocks = (int)std::min<long>((total + 255) / 256, 2048);
@@ -575,9 +575,6 @@ void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool trans
   // algorithmic flops of the small SVD (R-SVD of the k x k factor, GVL Fig. 8.6.1: 20k^3)
   ProfScope prof(KC_JACOBI, s, 20.0 * (double)k * k * k, 8.0 * 4.0 * (double)k * k);
   const double tol = sqrt((double)std::max(k, 1)) * DBL_EPSILON;
-  // stop after a sweep whose largest pair cosine is <= 1e-8: its rotations
-  // leave cosines of O(1e-16) behind
-  const double stop_cos2 = 1e-16;
   launch_jacobi(s, pl, Y, (long)pl.rows, Y + k, (long)pl.rows, tol, stop_cos2, max_sweeps, sweeps_dev,
                 barrier_dev, err_dev, dbg);
   // polar correction of J (and of S J with it)

@@ -768,6 +768,91 @@ void mtimes_core(hipStream_t s, int64_t ra0, int64_t n, int64_t m, int64_t ra1, 
   TTG_HIP(hipGetLastError());
 }
 template void mtimes_core<double>(hipStream_t, int64_t, int64_t, int64_t, int64_t, const double *, int64_t, int64_t, const double *, double *);
+
+// C (M x N) = R (M x K) * P, where P is the (ra0 rx0) x (n ra1 rx1) unfolding of
+// the product core of TT-matrix core A (ra0 x n x m x ra1) and TT core X
+// (rx0 x m x rx1), in the layout of @tt_matrix/mtimes.m:49-64 (row a + ra0*al,
+// column j + n*(b + ra1*be)).  P is built one 32 x 64 tile at a time in LDS
+// (m FMAs per entry, A and X stay in cache) and never exists in HBM.  FP64
+// MFMA on the transposed product (R P)^T so that 16 lanes store 16
+// consecutive rows of C.
+constexpr int MP_TM = 32, MP_TN = 64, MP_TK = 32;
+constexpr int MP_RLD = MP_TM + 16, MP_PLD = MP_TN + 16;  // lanes 16-31 of a read land on the other 32 banks
+__global__ __launch_bounds__(256) void k_gemm_mtimes(int M, int N, int K, const double *__restrict__ R,
+                                                     long ldr, int ra0, int nn, int mm, int ra1,
+                                                     const double *__restrict__ A, int rx0,
+                                                     const double *__restrict__ X, double *__restrict__ C,
+                                                     long ldc) {
+  __shared__ __align__(16) double Rs[MP_TK * MP_RLD];  // Rs[k][m] = R(m0 + m, k0 + k)
+  __shared__ __align__(16) double Ps[MP_TK * MP_PLD];  // Ps[k][n] = P(k0 + k, n0 + n)
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = wv & 1, wc = wv >> 1;  // wave tile: rows wr*16 +16, columns wc*32 +32
+  const int m0 = blockIdx.x * MP_TM, n0 = blockIdx.y * MP_TN;
+  const long qa = (long)ra0 * nn, qx = rx0;  // strides of the contracted index in A and X
+  v4d acc[2] = {v4d{0, 0, 0, 0}, v4d{0, 0, 0, 0}};
+  for (int k0 = 0; k0 < K; k0 += MP_TK) {
+    for (int idx = tid; idx < MP_TM * MP_TK; idx += 256) {
+      const int mi = idx % MP_TM, ki = idx / MP_TM;
+      const int gm = m0 + mi, gk = k0 + ki;
+      Rs[ki * MP_RLD + mi] = (gm < M && gk < K) ? R[gm + (long)gk * ldr] : 0.0;
+    }
+    for (int idx = tid; idx < MP_TK * MP_TN; idx += 256) {
+      const int ni = idx % MP_TN, ki = idx / MP_TN;
+      const int gk = k0 + ki, gn = n0 + ni;
+      double v = 0.0;
+      if (gk < K && gn < N) {
+        const int a = gk % ra0, al = gk / ra0;
+        const int j = gn % nn, rest = gn / nn;
+        const int b = rest % ra1, be = rest / ra1;
+        const double *ap = A + a + (long)ra0 * (j + (long)nn * mm * b);
+        const double *xp = X + al + (long)rx0 * ((long)mm * be);
+        for (int q = 0; q < mm; ++q) v = __builtin_fma(ap[qa * q], xp[qx * q], v);
+      }
+      Ps[ki * MP_PLD + ni] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < MP_TK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      const double rv = Rs[kr * MP_RLD + wr * 16 + (lane & 15)];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const double pv = Ps[kr * MP_PLD + wc * 32 + t * 16 + (lane & 15)];
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv, rv, acc[t], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // acc[t][v] of lane l = (R P)(m0 + wr*16 + l%16, n0 + wc*32 + t*16 + l/16 + 4v)
+  const int gm = m0 + wr * 16 + (lane & 15);
+  if (gm < M) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gn = n0 + wc * 32 + t * 16 + (lane >> 4) + 4 * v;
+        if (gn < N) C[gm + (long)gn * ldc] = acc[t][v];
+      }
+  }
+}
+
+void gemm_mtimes(hipStream_t s, int64_t M, const double *R, int64_t ldr, int64_t ra0, int64_t n,
+                 int64_t m, int64_t ra1, const double *A, int64_t rx0, int64_t rx1, const double *X,
+                 double *C, int64_t ldc) {
+  const int64_t K = ra0 * rx0, N = n * ra1 * rx1;
+  if (M <= 0 || N <= 0) return;
+  TTG_CHECK(K < (1LL << 31) && N < (1LL << 31) && M < (1LL << 31), TTG_BAD_ARG, "product core too large");
+  TTG_CHECK(ldr >= M && ldc >= M, TTG_BAD_ARG, "bad leading dimension");
+  const int64_t gy = (N + MP_TN - 1) / MP_TN;
+  TTG_CHECK(gy <= 65535, TTG_BAD_ARG, "product core has too many columns");
+  // algorithmic flops of R * P; bytes: R, C and the two factor cores (P is never stored)
+  ProfScope prof(KC_GEMM, s, 2.0 * (double)M * N * K,
+                 8.0 * ((double)M * K + (double)M * N + (double)ra0 * n * m * ra1 + (double)rx0 * m * rx1));
+  hipLaunchKernelGGL(k_gemm_mtimes, dim3((unsigned)((M + MP_TM - 1) / MP_TM), (unsigned)gy), dim3(256), 0, s,
+                     (int)M, (int)N, (int)K, R, (long)ldr, (int)ra0, (int)n, (int)m, (int)ra1, A, (int)rx0, X,
+                     C, (long)ldc);
+  TTG_HIP(hipGetLastError());
+}
 template void mtimes_core<float>(hipStream_t, int64_t, int64_t, int64_t, int64_t, const double *, int64_t, int64_t, const double *, float *);
 
 }  // namespace ttg

@@ -1,0 +1,587 @@
+// Persistent batched rounding: one workgroup rounds one whole TT tensor.
+//
+// A batch of many small-to-medium TT tensors (uneven d, n, r) is rounded by a
+// single launch of k_round_persist: NT-thread workgroups, one per slot of
+// device workspace, take tensors from a device work counter in longest-first
+// order (the host sorts the batch by estimated QR work) until the batch is
+// exhausted.  Each workgroup runs @tt_tensor/round.m for its tensor entirely
+// on chip, with no host round trip:
+//   * sweep 1 (round.m:35-53): Householder QR of the (r_i n_i) x r_{i+1}
+//     unfolding (robust for the rank-deficient unfoldings of tensors built by
+//     addition), nrm(i+1) = ||R||_F, R /= nrm, next core <- R * core_{i+1}
+//     (FP64 MFMA), ranks capped at min(r_i n_i, r_{i+1});
+//   * sweep 2 (round.m:59-89): SVD of the r_i x (n_i r_{i+1}) unfolding as a
+//     Householder QR of its tall orientation and a one-sided cyclic Jacobi
+//     SVD of the k x k R factor (k = min of the two dimensions), singular
+//     values sorted on chip and truncated by my_chop2 (core/my_chop2.m:18-34)
+//     with ep = eps/sqrt(d-1); core_i <- V', core_{i-1} <- core_{i-1} U S;
+//   * normalisation and balancing (round.m:90-109), packing with the final
+//     ranks.
+// Working cores live in the caller's output region (its capacity is the
+// QR-capped size, an upper bound of every intermediate core); per-slot scratch
+// holds the QR operand, the Q factor, the Jacobi matrix [S; J] and small
+// factors.  Every scratch use is checked against the slot's capacities; a
+// tensor that would not fit is abandoned with an error status, never written
+// out of bounds.
+#include "ttg_internal.h"
+#include "ttg_device.h"
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace ttg {
+
+typedef double v4d_p __attribute__((ext_vector_type(4)));
+
+constexpr int PMAX_D = 64;        // tensors with more cores use the stream path
+constexpr int PMAX_K = 512;       // largest bond rank
+
+// Block-wide sum with a fixed combination order (deterministic).
+template <int NT>
+__device__ double p_block_sum(double v, double *red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+template <int NT>
+__device__ void p_copy(double *__restrict__ dst, const double *__restrict__ src, long count) {
+  for (long i = threadIdx.x; i < count; i += NT) dst[i] = src[i];
+  __syncthreads();
+}
+
+// dst (cols x rows) = src (rows x cols)^T, both column-major
+template <int NT>
+__device__ void p_transpose(double *__restrict__ dst, const double *__restrict__ src, long rows, long cols) {
+  const long total = rows * cols;
+  for (long idx = threadIdx.x; idx < total; idx += NT) {
+    const long c = idx / rows, i = idx - c * rows;
+    dst[c + i * cols] = src[idx];
+  }
+  __syncthreads();
+}
+
+// C (M x N, ldc) = op(A) (M x K) * op(B) (K x N) with FP64 MFMA; each wave
+// takes 16 x 16 output tiles round-robin and forms the transposed tile
+// (op(B)^T op(A)^T) so that 16 lanes store 16 consecutive rows of C.
+template <int NT>
+__device__ void p_gemm(bool ta, bool tb, long M, long N, long K, const double *A, long lda,
+                       const double *B, long ldb, double *C, long ldc) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long tm = (M + 15) >> 4, tn = (N + 15) >> 4;
+  const int li = lane & 15, lk = lane >> 4;
+  for (long t = wv; t < tm * tn; t += NT / 64) {
+    const long m0 = (t % tm) * 16, n0 = (t / tm) * 16;
+    const long mrow = m0 + li, ncol = n0 + li;
+    v4d_p acc = {0, 0, 0, 0};
+    for (long k0 = 0; k0 < K; k0 += 16) {
+      double a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long kk = k0 + 4 * q + lk;
+        const bool ok = kk < K;
+        // operand "a" of the MFMA: rows = n of op(B)^T
+        b[q] = (ok && ncol < N) ? (tb ? B[ncol + kk * ldb] : B[kk + ncol * ldb]) : 0.0;
+        a[q] = (ok && mrow < M) ? (ta ? A[kk + mrow * lda] : A[mrow + kk * lda]) : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(b[q], a[q], acc, 0, 0, 0);
+    }
+    // acc[v] of lane l: C(m0 + l%16, n0 + l/16 + 4v)
+    if (mrow < M) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long nc = n0 + lk + 4 * v;
+        if (nc < N) C[mrow + nc * ldc] = acc[v];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Householder QR of A (m x k, column-major, ld m), in place: R in the upper
+// trapezoid, the vectors v_j (v_j[j] = 1 implicit) below the diagonal, tau[j]
+// (dgeqr2).  Then Q (m x kk, ld m, kk = min(m, k)) explicitly (dorg2r:
+// Q = H_0 ... H_{kk-1} [I; 0], applied backwards).  One wave per column in
+// the updates.
+template <int NT>
+__device__ void p_householder(double *__restrict__ A, long m, long k, double *__restrict__ tau,
+                              double *__restrict__ Q, double *red, double *bcast) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long kk = m < k ? m : k;
+  for (long j = 0; j < kk; ++j) {
+    double *aj = A + j * m;
+    double s = 0.0;
+    for (long i = j + 1 + threadIdx.x; i < m; i += NT) s = __builtin_fma(aj[i], aj[i], s);
+    s = p_block_sum<NT>(s, red);
+    const double al = aj[j];
+    double tj = 0.0, scale = 0.0, beta = al;
+    if (s > 0.0) {
+      const double nrm = sqrt(__builtin_fma(al, al, s));
+      beta = al >= 0.0 ? -nrm : nrm;
+      tj = (beta - al) / beta;
+      scale = 1.0 / (al - beta);
+    }
+    if (s > 0.0)
+      for (long i = j + 1 + threadIdx.x; i < m; i += NT) aj[i] *= scale;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      aj[j] = beta;
+      tau[j] = tj;
+    }
+    // H_j on the trailing columns j+1 .. k-1
+    if (tj != 0.0) {
+      for (long c = j + 1 + wv; c < k; c += NT / 64) {
+        double *ac = A + c * m;
+        double w = (lane == 0) ? ac[j] : 0.0;
+        for (long i = j + 1 + lane; i < m; i += 64) w = __builtin_fma(aj[i], ac[i], w);
+        w = wave_sum(w) * tj;
+        if (lane == 0) ac[j] -= w;
+        for (long i = j + 1 + lane; i < m; i += 64) ac[i] = __builtin_fma(-w, aj[i], ac[i]);
+      }
+    }
+    __syncthreads();
+  }
+  (void)bcast;
+  // Q = [I; 0] then H_j, j = kk-1 .. 0, on rows j.., columns j..kk-1
+  for (long idx = threadIdx.x; idx < m * kk; idx += NT) {
+    const long c = idx / m, i = idx - c * m;
+    Q[idx] = (i == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (long j = kk - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    const double *vj = A + j * m;
+    if (tj != 0.0) {
+      for (long c = j + wv; c < kk; c += NT / 64) {
+        double *qc = Q + c * m;
+        double w = (lane == 0) ? qc[j] : 0.0;
+        for (long i = j + 1 + lane; i < m; i += 64) w = __builtin_fma(vj[i], qc[i], w);
+        w = wave_sum(w) * tj;
+        if (lane == 0) qc[j] -= w;
+        for (long i = j + 1 + lane; i < m; i += 64) qc[i] = __builtin_fma(-w, vj[i], qc[i]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Rotation of columns with squared norms al, be and inner product ga (Rutishauser).
+__device__ __forceinline__ void p_jrot(double al, double be, double ga, double &cs, double &sn) {
+  const double dd = be - al, g2 = 2.0 * ga;
+  const double t = (dd >= 0.0 ? g2 : -g2) / (fabs(dd) + sqrt(__builtin_fma(dd, dd, g2 * g2)));
+  cs = 1.0 / sqrt(__builtin_fma(t, t, 1.0));
+  sn = cs * t;
+}
+
+// One-sided cyclic Jacobi on Y = [S; J] (2k x k, ld 2k; S k x k, J = I on
+// entry): round-robin parallel ordering, k/2 disjoint pairs per round (one
+// wave per pair), dots over S's k rows, rotation if |g| > tol sqrt(a b) and
+// neither column is numerically zero (||s||^2 <= k (2u)^2 ||S||_F^2); a sweep
+// none of whose rotations had cos^2 > 1e-16 ends the iteration.  Returns the
+// number of sweeps (max_sweeps + 1 when not converged).
+template <int NT>
+__device__ int p_jacobi(double *Y, int k, int max_sweeps, double *red, int *cnt) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long ld = 2L * k;
+  double fro = 0.0;
+  for (long idx = threadIdx.x; idx < (long)k * k; idx += NT) {
+    const long c = idx / k, i = idx - c * k;
+    const double v = Y[i + c * ld];
+    fro = __builtin_fma(v, v, fro);
+  }
+  fro = p_block_sum<NT>(fro, red);
+  const double small2 = (double)k * 4.0 * DBL_EPSILON * DBL_EPSILON * fro;
+  const double tol = sqrt((double)k) * DBL_EPSILON, tol2 = tol * tol;
+  const int P = k + (k & 1);  // players (one bye per round when k is odd)
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    if (threadIdx.x == 0) cnt[0] = 0;
+    __syncthreads();
+    int mybig = 0;
+    for (int rnd = 0; rnd < P - 1; ++rnd) {
+      for (int q = wv; q < P / 2; q += NT / 64) {
+        int a, b;
+        if (q == 0) {
+          a = P - 1;
+          b = rnd;
+        } else {
+          a = (rnd + q) % (P - 1);
+          b = (rnd - q + P - 1) % (P - 1);
+        }
+        if (a >= k || b >= k) continue;
+        double *ya = Y + (long)a * ld, *yb = Y + (long)b * ld;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int i = lane; i < k; i += 64) {
+          const double x = ya[i], y = yb[i];
+          al = __builtin_fma(x, x, al);
+          be = __builtin_fma(y, y, be);
+          ga = __builtin_fma(x, y, ga);
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        ga = wave_sum(ga);
+        const double g2 = ga * ga, ab = al * be;
+        const bool rot = g2 > tol2 * ab && ab >= DBL_MIN && al > small2 && be > small2;
+        if (!rot) continue;
+        mybig += (g2 > 1e-16 * ab) ? 1 : 0;
+        double cs, sn;
+        p_jrot(al, be, ga, cs, sn);
+        for (long i = lane; i < ld; i += 64) {
+          const double x = ya[i], y = yb[i];
+          ya[i] = cs * x - sn * y;
+          yb[i] = sn * x + cs * y;
+        }
+      }
+      __syncthreads();
+    }
+    if (lane == 0 && mybig > 0) atomicAdd(cnt, mybig);
+    __syncthreads();
+    const int big = cnt[0];
+    __syncthreads();
+    if (big == 0) return sweep + 1;
+  }
+  return max_sweeps + 1;
+}
+
+// Column norms of Y's top k rows, sorted descending (ties by index), and
+// my_chop2(sv, ||sv|| ep) capped at rmax (core/my_chop2.m:18-34); perm maps
+// sorted position -> column.  Returns r1.
+template <int NT>
+__device__ int p_sort_chop(const double *Y, long ldy, int k, double ep, double *sig, double *sv,
+                           int *perm, int *bc) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j = wv; j < k; j += NT / 64) {
+    const double *col = Y + (long)j * ldy;
+    double s = 0.0;
+    for (int i = lane; i < k; i += 64) s = __builtin_fma(col[i], col[i], s);
+    s = wave_sum(s);
+    if (lane == 0) sig[j] = sqrt(s);
+  }
+  for (int j = threadIdx.x; j < k; j += NT) perm[j] = j;  // stays a valid index even for NaN norms
+  __syncthreads();
+  for (int j = threadIdx.x; j < k; j += NT) {
+    const double v = sig[j];
+    int pos = 0;
+    for (int i = 0; i < k; ++i) {
+      const double u = sig[i];
+      pos += (u > v) || (u == v && i < j);
+    }
+    sv[pos] = v;
+    perm[pos] = j;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ss = 0.0;
+    for (int j = 0; j < k; ++j) ss += sv[j] * sv[j];
+    const double nrm = sqrt(ss);
+    const double e = nrm * ep;
+    int r;
+    if (nrm == 0.0) {
+      r = 1;
+    } else if (!(e > 0.0)) {
+      r = k;
+    } else {
+      const double e2 = e * e;
+      double c = 0.0;
+      int K = 0;
+      for (int j = k - 1; j >= 0; --j) {
+        c += sv[j] * sv[j];
+        if (c < e2) K = k - j;
+      }
+      r = k - K;
+    }
+    bc[0] = r;
+  }
+  __syncthreads();
+  const int r1 = bc[0];
+  __syncthreads();
+  return r1;
+}
+
+// Per-slot scratch (doubles): Wa, Wq (core-sized), Y (2 kmax x kmax), Ra, Rb (kmax^2), tau (kmax).
+struct PSlot {
+  long core, kmax;
+  __host__ __device__ long elems() const { return 2 * core + 2 * kmax * kmax + 2 * kmax * kmax + kmax + 64; }
+};
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDesc *__restrict__ tens,
+                                                      const int *__restrict__ order,
+                                                      const long *__restrict__ nall,
+                                                      const long *__restrict__ rall,
+                                                      const double *__restrict__ in, double *__restrict__ out,
+                                                      long *__restrict__ r_out, double *__restrict__ ws,
+                                                      PSlot slot, int *ctr, int *tstat, double eps,
+                                                      int max_sweeps, long ycap) {
+  __shared__ double red[NT / 64 + 2];
+  __shared__ double sig[PMAX_K], sv[PMAX_K], nrm[PMAX_D + 2];
+  __shared__ int perm[PMAX_K];
+  __shared__ long rc[PMAX_D + 1], nn[PMAX_D], wofs[PMAX_D + 1];
+  __shared__ int sh_t, sh_bc[2], sh_fail;
+  extern __shared__ __align__(16) double ylds[];
+  double *Wa = ws + (long)blockIdx.x * slot.elems();
+  double *Wq = Wa + slot.core;
+  double *Yg = Wq + slot.core;
+  double *Ra = Yg + 2 * slot.kmax * slot.kmax;
+  double *Rb = Ra + slot.kmax * slot.kmax;
+  double *tau = Rb + slot.kmax * slot.kmax;
+  for (;;) {
+    if (threadIdx.x == 0) sh_t = atomicAdd(ctr, 1);
+    __syncthreads();
+    const int ti = sh_t;
+    __syncthreads();
+    if (ti >= count) break;
+    const int t = order[ti];
+    const PTensorDesc td = tens[t];
+    const int d = td.d;
+    const double *xin = in + td.in_ofs;
+    double *xout = out + td.out_ofs;
+    int sweeps_tot = 0, sweeps_max = 0, status = 0;
+    // ranks, modes, offsets of the input cores and of the (QR-capped) working slots
+    if (threadIdx.x == 0) {
+      sh_fail = 0;
+      long o = 0;
+      for (int i = 0; i < d; ++i) nn[i] = nall[td.nofs + i];
+      for (int i = 0; i <= d; ++i) rc[i] = rall[td.rofs + i];
+      // capped ranks (round.m:48) give the working layout
+      long rq = rc[0];
+      for (int i = 0; i < d; ++i) {
+        const long rn = (i + 1 < d) ? ((rq * nn[i] < rc[i + 1]) ? rq * nn[i] : rc[i + 1]) : rc[d];
+        wofs[i] = o;
+        o += rq * nn[i] * rn;
+        rq = rn;
+      }
+      wofs[d] = o;
+    }
+    __syncthreads();
+    // non-finite input is an error for the tensor (like round_dev)
+    {
+      long insize = 0;
+      for (int i = 0; i < d; ++i) insize += rc[i] * nn[i] * rc[i + 1];
+      int bad = 0;
+      for (long i = threadIdx.x; i < insize; i += NT) bad |= !isfinite(xin[i]);
+      if (bad) atomicOr(&sh_fail, 2);
+      __syncthreads();
+    }
+    if (sh_fail) {
+      status = sh_fail;
+    } else if (d == 1) {
+      const long sz = rc[0] * nn[0] * rc[1];
+      if (threadIdx.x == 0) nrm[0] = 0.0;
+      p_copy<NT>(xout, xin, sz);
+    } else {
+      // ---- sweep 1: L -> R Householder QR (round.m:35-53) ------------------
+      {
+        const long sz0 = rc[0] * nn[0] * rc[1];
+        if (sz0 > slot.core) atomicOr(&sh_fail, 4);
+        __syncthreads();
+        if (!sh_fail) p_copy<NT>(Wa, xin, sz0);
+      }
+      long in_ofs_next = rc[0] * nn[0] * rc[1];
+      for (int i = 0; i + 1 < d && !sh_fail; ++i) {
+        const long m = rc[i] * nn[i], k = rc[i + 1], kk = m < k ? m : k;
+        const long N1 = nn[i + 1] * rc[i + 2];
+        if (m * kk > slot.core || kk * N1 > slot.core || kk * k > slot.kmax * slot.kmax || kk > slot.kmax) {
+          if (threadIdx.x == 0) sh_fail = 4;
+          __syncthreads();
+          break;
+        }
+        // Wa holds core i as (m x k); Q -> working slot i (m x kk), R -> Ra (kk x k)
+        p_householder<NT>(Wa, m, k, tau, xout + wofs[i], red, nullptr);
+        for (long idx = threadIdx.x; idx < kk * k; idx += NT) {
+          const long c = idx / kk, r = idx - c * kk;
+          Ra[idx] = (r <= c) ? Wa[r + c * m] : 0.0;
+        }
+        __syncthreads();
+        double s = 0.0;
+        for (long idx = threadIdx.x; idx < kk * k; idx += NT) s = __builtin_fma(Ra[idx], Ra[idx], s);
+        s = p_block_sum<NT>(s, red);
+        const double nv = sqrt(s);
+        if (threadIdx.x == 0) nrm[i + 1] = nv;
+        if (nv != 0.0) {
+          const double inv = 1.0 / nv;
+          for (long idx = threadIdx.x; idx < kk * k; idx += NT) Ra[idx] *= inv;
+        }
+        __syncthreads();
+        // core_{i+1} <- R core_{i+1} (input core i+1 is k x N1), into Wa
+        p_gemm<NT>(false, false, kk, N1, k, Ra, kk, xin + in_ofs_next, k, Wa, kk);
+        in_ofs_next += k * N1;
+        if (threadIdx.x == 0) rc[i + 1] = kk;
+        __syncthreads();
+      }
+      if (!sh_fail) {
+        // the last core of sweep 1 -> its working slot
+        const long szl = rc[d - 1] * nn[d - 1] * rc[d];
+        p_copy<NT>(xout + wofs[d - 1], Wa, szl);
+      }
+      // ---- sweep 2: R -> L truncated SVDs (round.m:59-89) -------------------
+      const double ep = eps / sqrt((double)(d - 1));
+      for (int i = d - 1; i >= 1 && !sh_fail; --i) {
+        const long kw = rc[i], N = nn[i] * rc[i + 1];
+        const bool wide = N >= kw;
+        const long kj = wide ? kw : N, mt = wide ? N : kw;
+        double *Wc = xout + wofs[i];
+        if (mt * kj > slot.core || kj > slot.kmax || kj > PMAX_K) {
+          if (threadIdx.x == 0) sh_fail = 4;
+          __syncthreads();
+          break;
+        }
+        // tall orientation B (mt x kj) in Wa, QR: R in Wa's upper part, Q -> Wq
+        if (wide) p_transpose<NT>(Wa, Wc, kw, N);
+        else p_copy<NT>(Wa, Wc, kw * N);
+        p_householder<NT>(Wa, mt, kj, tau, Wq, red, nullptr);
+        // Y = [S; I] with S = R^T (wide) or R (tall), in LDS when it fits
+        const long ldy = 2 * kj;
+        double *Y = (ldy * kj <= ycap) ? ylds : Yg;
+        for (long idx = threadIdx.x; idx < ldy * kj; idx += NT) {
+          const long c = idx / ldy, r = idx - c * ldy;
+          double v;
+          if (r < kj) {
+            const long rr = wide ? c : r, cc = wide ? r : c;  // S(r, c) = R(c, r) when wide
+            v = (rr <= cc) ? Wa[rr + cc * mt] : 0.0;
+          } else {
+            v = (r - kj == c) ? 1.0 : 0.0;
+          }
+          Y[idx] = v;
+        }
+        __syncthreads();
+        const int sw = p_jacobi<NT>(Y, (int)kj, max_sweeps, red, sh_bc);
+        if (sw > max_sweeps) status |= 1;
+        sweeps_tot += sw > max_sweeps ? max_sweeps : sw;
+        sweeps_max = sw > sweeps_max ? sw : sweeps_max;
+        const int r1 = p_sort_chop<NT>(Y, ldy, (int)kj, ep, sig, sv, perm, sh_bc);
+        // U S and V from Y = [S J; J] (column perm[c] is the c-th singular triple)
+        if (wide) {
+          // W^T = Q R, R^T J = Y_top: U S = Y_top[:, perm] (kw x r1) -> Rb,
+          // V = Q J[:, perm]: core_i = J_r^T Q^T (r1 x N)
+          for (long idx = threadIdx.x; idx < kw * r1; idx += NT) {
+            const long c = idx / kw, r = idx - c * kw;
+            const long src = r + (long)perm[c] * ldy;
+            Rb[idx] = Y[src];
+            Ra[idx] = Y[src + kj];  // J_r (kw x r1)
+          }
+          __syncthreads();
+          p_gemm<NT>(true, true, r1, N, kw, Ra, kw, Wq, N, Wc, r1);
+        } else {
+          // W = Q R, R J = Y_top: U S = Q Y_top[:, perm] (kw x r1), V = J[:, perm]
+          for (long idx = threadIdx.x; idx < N * r1; idx += NT) {
+            const long c = idx / N, r = idx - c * N;
+            const long src = r + (long)perm[c] * ldy;
+            Ra[idx] = Y[src];            // Y_r (N x r1)
+            Wa[c + r * r1] = Y[src + kj];  // core_i = J_r^T (r1 x N)
+          }
+          __syncthreads();
+          p_gemm<NT>(false, false, kw, r1, N, Wq, kw, Ra, N, Rb, kw);
+          p_copy<NT>(Wc, Wa, r1 * N);
+        }
+        // core_{i-1} <- core_{i-1} (U S)   (round.m:81-83), via Wa
+        const long mp = rc[i - 1] * nn[i - 1];
+        p_gemm<NT>(false, false, mp, r1, kw, xout + wofs[i - 1], mp, Rb, kw, Wa, mp);
+        p_copy<NT>(xout + wofs[i - 1], Wa, mp * r1);
+        if (threadIdx.x == 0) rc[i] = r1;
+        __syncthreads();
+      }
+    }
+    if (sh_fail) status |= sh_fail;
+    if (!(status & ~1)) {
+      // ---- nrm(1), balance, pack (round.m:90-109) ---------------------------
+      const long s0 = rc[0] * nn[0] * rc[1];
+      double s = 0.0;
+      for (long idx = threadIdx.x; idx < s0; idx += NT) s = __builtin_fma(xout[idx], xout[idx], s);
+      s = p_block_sum<NT>(s, red);
+      double sc0 = 1.0, sc = 1.0;
+      if (d > 1) {
+        nrm[0] = sqrt(s);  // every thread holds the same value
+        double lsum = 0.0;
+        for (int i = 0; i < d; ++i) lsum += log(fabs(nrm[i]));
+        const double nrm0 = exp(lsum / d);
+        sc0 = (nrm[0] != 0.0) ? nrm0 / nrm[0] : nrm0;
+        sc = nrm0;
+      }
+      __syncthreads();
+      // packed offsets never exceed the working ones: copy forward in chunks
+      long o = 0;
+      for (int i = 0; i < d; ++i) {
+        const long sz = rc[i] * nn[i] * rc[i + 1];
+        const double f = (i == 0) ? sc0 : sc;
+        const double *src = xout + wofs[i];
+        double *dst = xout + o;
+        for (long base = 0; base < sz; base += NT) {
+          const long idx = base + threadIdx.x;
+          const double v = idx < sz ? src[idx] : 0.0;
+          __syncthreads();
+          if (idx < sz) dst[idx] = f * v;
+          __syncthreads();
+        }
+        o += sz;
+      }
+      for (int i = threadIdx.x; i <= d; i += NT) r_out[td.rofs + i] = rc[i];
+    }
+    if (threadIdx.x == 0) {
+      tstat[4 * t + 0] = status;
+      tstat[4 * t + 1] = sweeps_tot;
+      tstat[4 * t + 2] = sweeps_max;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_persist_reset(int *ctr) {
+  if (threadIdx.x == 0) ctr[0] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+bool persist_supported(int64_t d, const int64_t *n, const int64_t *r) {
+  if (d < 1 || d > PMAX_D) return false;
+  std::vector<int64_t> rc(r, r + d + 1);
+  for (int64_t i = 0; i + 1 < d; ++i) rc[i + 1] = std::min(rc[i] * n[i], rc[i + 1]);
+  for (int64_t i = 0; i <= d; ++i)
+    if (r[i] > PMAX_K) return false;
+  for (int64_t i = 0; i < d; ++i)
+    if (std::min(rc[i], n[i] * rc[i + 1]) > PMAX_K) return false;
+  return true;
+}
+
+void persist_prepare(int nt, int64_t ycap) {
+  const size_t lds = (size_t)ycap * sizeof(double);
+  if (nt == 512)
+    TTG_HIP(hipFuncSetAttribute((const void *)k_round_persist<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  else
+    TTG_HIP(hipFuncSetAttribute((const void *)k_round_persist<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
+void persist_launch(hipStream_t s, int nt, int slots, int count, const PTensorDesc *tens,
+                    const int *order, const int64_t *nall, const int64_t *rall, const double *in,
+                    double *out, int64_t *r_out, double *ws, int64_t slot_core, int64_t slot_kmax, int *ctr,
+                    int *tstat, double eps, int max_sweeps, int64_t ycap) {
+  PSlot sl{(long)slot_core, (long)slot_kmax};
+  hipLaunchKernelGGL(k_persist_reset, dim3(1), dim3(64), 0, s, ctr);
+  TTG_HIP(hipGetLastError());
+  const size_t lds = (size_t)ycap * sizeof(double);
+  if (nt == 512) {
+    hipLaunchKernelGGL((k_round_persist<512>), dim3(slots), dim3(512), lds, s, count, tens, order,
+                       (const long *)nall, (const long *)rall, in, out, (long *)r_out, ws, sl, ctr, tstat, eps,
+                       max_sweeps, (long)ycap);
+  } else {
+    hipLaunchKernelGGL((k_round_persist<256>), dim3(slots), dim3(256), lds, s, count, tens, order,
+                       (const long *)nall, (const long *)rall, in, out, (long *)r_out, ws, sl, ctr, tstat, eps,
+                       max_sweeps, (long)ycap);
+  }
+  TTG_HIP(hipGetLastError());
+}
+
+int64_t persist_slot_elems(int64_t slot_core, int64_t slot_kmax) {
+  return PSlot{(long)slot_core, (long)slot_kmax}.elems();
+}
+
+}  // namespace ttg

@@ -72,6 +72,9 @@ template <class T> struct Rounder {
   double *scal_d;
   int64_t max_core, kmax, rsq;
   bool mixed;
+  // optional: dst (kk x n_i r_{i+1}) = Rm (kk x r_i) * core_i for generated
+  // cores (fused A*x: the product core is built on chip, never stored)
+  const std::function<void(int64_t, const T *, int64_t, T *)> *fmul = nullptr;
   int64_t fallbacks = 0, sweeps_total = 0, sweeps_max = 0;
   bool noconv = false;
 
@@ -241,11 +244,15 @@ template <class T> struct Rounder {
     scale_by_inv<double>(st, R, kk * k, nrm + i + 1);
     // core_{i+1} <- R core_{i+1}   (round.m:45-47)
     const int64_t N1 = n[i + 1] * r[i + 2];
-    T *src = buf[0] + ofs[i + 1];
-    if (gen) gen(i + 1, src);
     to_T(RT, R, kk * k);
     T *dst = buf[1] + ofs[i + 1];
-    g(false, false, kk, N1, k, RT, kk, src, k, dst, kk);
+    if (fmul && *fmul) {
+      (*fmul)(i + 1, RT, kk, dst);  // core_{i+1} generated on chip inside the product
+    } else {
+      T *src = buf[0] + ofs[i + 1];
+      if (gen) gen(i + 1, src);
+      g(false, false, kk, N1, k, RT, kk, src, k, dst, kk);
+    }
     cp[i + 1] = dst;
     r[i + 1] = kk;  // round.m:48
     return pending;
@@ -273,7 +280,11 @@ template <class T> struct Rounder {
       qr_householder(tallA, mt, kj, Qs, R);
     }
     JacobiPlan pl = jacobi_plan(kj);
-    jacobi_svd(st, pl, R, /*transS=*/wide, Y, jsweeps, bar, jerr, kMaxSweeps, Yws);
+    // stopping cosine^2: plain form 1e-16 (the last sweep's rotations leave
+    // O(1e-16) behind); mixed form 1e-8 (O(1e-8) left, far below its 1e-6
+    // error budget, one sweep fewer)
+    jacobi_svd(st, pl, R, /*transS=*/wide, Y, jsweeps, bar, jerr, kMaxSweeps, Yws, nullptr,
+               mixed ? 1e-8 : 1e-16);
     sort_chop(st, Y, pl.rows, kj, pl.kpad, ep, rmax_i, sv, perm, scal_i, scal_d + 8);
     // one synchronisation: r1, Jacobi status, QR acceptance data
     int h_i[4] = {0, 0, 0, 0};
@@ -399,7 +410,8 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
                              const double *dcore, double eps, const int64_t *rmax, int64_t *r_out,
                              double *dcore_out, int64_t cap, int64_t *len_out, ttg_info *info,
                              const std::function<void(int64_t, T *)> &gen,
-                             const std::vector<int64_t> *gen_ranks, bool mixed) {
+                             const std::vector<int64_t> *gen_ranks, bool mixed,
+                             const std::function<void(int64_t, const T *, int64_t, T *)> *fmul = nullptr) {
   auto t0 = std::chrono::steady_clock::now();
   validate_tt(d, n, r);
   if (rmax)
@@ -414,6 +426,7 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
   R.n.assign(n, n + d);
   R.r.assign(r, r + d + 1);
   R.mixed = mixed;
+  R.fmul = fmul;
   std::vector<int64_t> sizes(d);
   R.ofs.assign(d + 1, 0);
   int64_t max_core = 1, kmax = 1;
@@ -557,12 +570,19 @@ ttg_status mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int
   hipStream_t st = ctx->stream;
   std::vector<int64_t> nn(n, n + d), mm(m, m + d), raa(ra, ra + d + 1), rxx(rx, rx + d + 1);
   TTG_CHECK(prec == TTG_PREC_FP64 || prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  // core 0 is materialised (it is the first QR's input); every later product
+  // core only ever feeds sweep 1's R * core product and is built inside it
   std::function<void(int64_t, double *)> gen = [&](int64_t i, double *dst) {
     mtimes_core<double>(st, raa[i], nn[i], mm[i], raa[i + 1], dacore + oa[i], rxx[i], rxx[i + 1],
                         dxcore + ox[i], dst);
   };
+  std::function<void(int64_t, const double *, int64_t, double *)> fmul =
+      [&](int64_t i, const double *Rm, int64_t kk, double *dst) {
+        gemm_mtimes(st, kk, Rm, kk, raa[i], nn[i], mm[i], raa[i + 1], dacore + oa[i], rxx[i], rxx[i + 1],
+                    dxcore + ox[i], dst, kk);
+      };
   return round_impl<double>(ctx, d, n, rp.data(), nullptr, eps, rmax, r_out, dcore_out, cap,
-                            len_out, info, gen, &rp, prec == TTG_PREC_FP32);
+                            len_out, info, gen, &rp, prec == TTG_PREC_FP32, &fmul);
 }
 
 }  // namespace ttg

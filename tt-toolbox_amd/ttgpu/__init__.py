@@ -6,12 +6,12 @@ libttgpu.so (hand-written HIP kernels behind a C ABI, include/ttgpu.h).
 """
 from ._lib import (LIB_PATH, EXPORTS, Context, TTGError, TTGInfo, default_context, load,
                    PREC_FP32, PREC_FP64, STATUS_NAMES)
-from .tt import (TTTensor, TTMatrix, capacity, mtimes, mtimes_round, mtimes_round_dev, round,
+from .tt import (TTTensor, TTMatrix, capacity, mtimes, mtimes_dev, mtimes_round, mtimes_round_dev, round,
                  round_batched_dev, round_dev, round_matrix, torch_context)
 
 __all__ = [
     "LIB_PATH", "EXPORTS", "Context", "TTGError", "TTGInfo", "default_context", "load",
     "PREC_FP32", "PREC_FP64", "STATUS_NAMES", "TTTensor", "TTMatrix", "capacity", "mtimes",
-    "mtimes_round", "mtimes_round_dev", "round", "round_batched_dev", "round_dev",
+    "mtimes_dev", "mtimes_round", "mtimes_round_dev", "round", "round_batched_dev", "round_dev",
     "round_matrix", "torch_context",
 ]

@@ -217,6 +217,22 @@ def round_dev(ctx, n, r, dcore, eps, rmax=None, precision="fp64", out=None, info
     return np.array(r_out[:], dtype=np.int64), out[:used.value]
 
 
+def mtimes_dev(ctx, n, m, ra, dacore, rx, dxcore, out=None):
+    """``A*x`` (``@tt_matrix/mtimes.m:38-65``) on CUDA float64 cores; returns (ry, core view)."""
+    import torch
+    d = len(n)
+    ry = np.asarray(ra, dtype=np.int64) * np.asarray(rx, dtype=np.int64)
+    cap = int(_ps(n, ry)[-1])
+    if out is None:
+        out = torch.empty(max(cap, 1), dtype=torch.float64, device=dacore.device)
+    r_out = (C.c_int64 * (d + 1))()
+    used = C.c_int64()
+    s = ctx.lib.ttg_mtimes_dev(ctx.h, d, i64(n), i64(m), i64(ra), _ptr(dacore), i64(rx), _ptr(dxcore),
+                               r_out, _ptr(out), int(out.numel()), C.byref(used))
+    ctx.check(s)
+    return np.array(r_out[:], dtype=np.int64), out[:used.value]
+
+
 def mtimes_round_dev(ctx, n, m, ra, dacore, rx, dxcore, eps, rmax=None, precision="fp64", out=None,
                      info=None):
     import torch

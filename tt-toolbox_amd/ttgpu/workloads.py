@@ -126,6 +126,18 @@ def batch4(count=512, seed=4, dmin=8, dmax=24, nset=(2, 4, 8, 16), rmin=8, rmax=
     return {"tensors": out, "eps": 1e-8}
 
 
+def bilinear1(d=20, n=4, rank=20, arank=5, seed=6):
+    """<y, A x> at the config-1 shapes: x, y rank 20, A rank 5, n = m = 4."""
+    rng = np.random.default_rng(seed)
+    ra = _ranks(d, arank)
+    acores = [rng.standard_normal((ra[k], n, n, ra[k + 1])) / math.sqrt(n * ra[k]) for k in range(d)]
+    x = rand_cores(rng, [n] * d, _ranks(d, rank))
+    y = rand_cores(rng, [n] * d, _ranks(d, rank))
+    am = np.asarray([n] * d, np.int64)
+    aflat = np.concatenate([c.ravel(order="F") for c in acores])
+    return {"a": (am, am.copy(), np.asarray(ra, np.int64), aflat), "x": flatten(x), "y": flatten(y)}
+
+
 def completion5(d=16, n=8, target_rank=8, model_rank=32, samples=65536, noise=1e-3, seed=5):
     rng = np.random.default_rng(seed)
     tgt = rand_cores(rng, [n] * d, _ranks(d, target_rank))
