@@ -116,6 +116,54 @@ def other_configs(ttgpu, W, ctx, dev):
     return out
 
 
+def batched_leg(ttgpu, W, ctx, dev, count):
+    """Config 4 recipe: one call for the whole batch (the first call on a context
+    also tunes the execution) against a per-tensor loop of round_dev on one stream
+    (32-tensor sample, scaled by QR work)."""
+    import numpy as np
+    import torch
+    log(f"generating {count} uneven tensors (config 4 recipe)")
+    bb = W.batch4(count=count)
+    tens = bb["tensors"]
+    cofs = np.concatenate([[0], np.cumsum([t[2].size for t in tens])])
+    dall = torch.empty(int(cofs[-1]), dtype=torch.float64, device=dev)
+    for i, t in enumerate(tens):
+        dall[int(cofs[i]):int(cofs[i + 1])].copy_(torch.from_numpy(t[2]))
+    ns, rs = [t[0] for t in tens], [t[1] for t in tens]
+    torch.cuda.synchronize()
+    binfo = {}
+    t0 = time.perf_counter()
+    ttgpu.round_batched_dev(ctx, ns, rs, dall, cofs[:-1], bb["eps"], info=binfo)  # first call tunes
+    t1 = time.perf_counter()
+    ttgpu.round_batched_dev(ctx, ns, rs, dall, cofs[:-1], bb["eps"], info=binfo)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    m = min(32, len(tens))
+    for i in range(m):
+        ttgpu.round_dev(ctx, ns[i], rs[i], dall[int(cofs[i]):int(cofs[i + 1])], bb["eps"])
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+
+    def qr_work(n_, r_):
+        c, rk = 0.0, float(r_[0])
+        for j in range(len(n_) - 1):
+            mm = rk * n_[j]
+            kk = min(mm, float(r_[j + 1]))
+            c += mm * kk * kk
+            rk = kk
+        return c
+    wall = sum(qr_work(n_, r_) for n_, r_ in zip(ns, rs))
+    wsamp = sum(qr_work(n_, r_) for n_, r_ in zip(ns[:m], rs[:m]))
+    loop_est = (t3 - t2) * wall / wsamp
+    return {
+        "workload": f"{len(tens)} TT tensors, d in [8,24], n in {{2,4,8,16}}, ranks in [8,128] doubled "
+                    f"by self-addition, eps=1e-8, FP64",
+        "tensors_per_s": len(tens) / (t2 - t1), "seconds": t2 - t1, "first_call_seconds_incl_tuning": t1 - t0,
+        "per_tensor_loop_tensors_per_s": len(tens) / loop_est,
+        "per_tensor_loop_sample": f"first {m} tensors timed, scaled by QR work to the batch",
+        "speedup_vs_loop": loop_est / (t2 - t1), "jacobi_sweeps": binfo.get("jacobi_sweeps")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +175,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-rounding leg (512 uneven tensors, config 4 recipe)")
     ap.add_argument("--batched-count", type=int, default=512)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched leg (profiling; prints its JSON object)")
     ap.add_argument("--no-other", action="store_true",
                     help="skip the configs 1, 2 and 5 throughput leg")
     ap.add_argument("--copy-probe", action="store_true",
@@ -152,6 +202,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": nb, "copy_probe_bytes_written": nb, "launches": 3}), flush=True)
+        return
+    if args.batched_only:
+        ctx = ttgpu.Context(0)
+        print(json.dumps({"batched": batched_leg(ttgpu, W, ctx, torch.device("cuda:0"), args.batched_count)}),
+              flush=True)
         return
     log("generating input (d=32, n=8, r=256)")
     c = W.mixed3()
@@ -276,54 +331,12 @@ def main():
                           "unit": unit, "frac": achieved / peak, "traffic": traffic,
                           "launches_per_round": prof["launches"] / args.steps, "avg_launch_ms": avg_ms}
 
-    # batched rounding of uneven tensors (config-4 recipe): one call for the whole
-    # batch against a per-tensor loop of round_dev on one stream (32-tensor sample)
+    # batched rounding of uneven tensors (config-4 recipe)
     if not args.no_batched:
         del dcore, out_mixed, out_plain
         torch.cuda.empty_cache()
-        log(f"generating {args.batched_count} uneven tensors (config 4 recipe)")
-        bb = W.batch4(count=args.batched_count)
-        tens = bb["tensors"]
-        cofs = np.concatenate([[0], np.cumsum([t[2].size for t in tens])])
-        dall = torch.empty(int(cofs[-1]), dtype=torch.float64, device=dev)
-        for i, t in enumerate(tens):
-            dall[int(cofs[i]):int(cofs[i + 1])].copy_(torch.from_numpy(t[2]))
-        ns, rs = [t[0] for t in tens], [t[1] for t in tens]
-        torch.cuda.synchronize()
-        binfo = {}
-        t0 = time.perf_counter()
-        ttgpu.round_batched_dev(ctx, ns, rs, dall, cofs[:-1], bb["eps"], info=binfo)  # first call tunes
-        t1 = time.perf_counter()
-        ttgpu.round_batched_dev(ctx, ns, rs, dall, cofs[:-1], bb["eps"], info=binfo)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        m = min(32, len(tens))
-        for i in range(m):
-            ttgpu.round_dev(ctx, ns[i], rs[i], dall[int(cofs[i]):int(cofs[i + 1])], bb["eps"])
-        torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        # the loop sample is the first m tensors; scale by their share of the batch's QR work
-        def qr_work(n_, r_):
-            c, rk = 0.0, float(r_[0])
-            for j in range(len(n_) - 1):
-                mm = rk * n_[j]
-                kk = min(mm, float(r_[j + 1]))
-                c += mm * kk * kk
-                rk = kk
-            return c
-        wall = sum(qr_work(n_, r_) for n_, r_ in zip(ns, rs))
-        wsamp = sum(qr_work(n_, r_) for n_, r_ in zip(ns[:m], rs[:m]))
-        loop_est = (t3 - t2) * wall / wsamp
-        result["batched"] = {
-            "workload": f"{len(tens)} TT tensors, d in [8,24], n in {{2,4,8,16}}, ranks in [8,128] doubled "
-                        f"by self-addition, eps=1e-8, FP64",
-            "tensors_per_s": len(tens) / (t2 - t1), "seconds": t2 - t1, "first_call_seconds_incl_tuning": t1 - t0,
-            "per_tensor_loop_tensors_per_s": len(tens) / loop_est,
-            "per_tensor_loop_sample": f"first {m} tensors timed, scaled by QR work to the batch",
-            "speedup_vs_loop": loop_est / (t2 - t1), "qr_fallbacks": binfo.get("qr_fallbacks"),
-            "jacobi_sweeps": binfo.get("jacobi_sweeps")}
+        result["batched"] = batched_leg(ttgpu, W, ctx, dev, args.batched_count)
         log("batched:", result["batched"])
-        del dall
 
     # the other configurations at one GPU: rounds/s, contractions/s, gradient evaluations/s
     if not args.no_other:

@@ -6,19 +6,23 @@
 //     counter (greedy LPT; the cost estimate is the QR work sum_i m_i k_i^2
 //     at the QR-capped ranks).  The launch (counter reset + kernel) is
 //     captured in a hipGraph and replayed while its arguments are unchanged.
-//     Launch shapes: 256 threads with 112 KB of LDS for the Jacobi matrix
-//     (one workgroup per CU), 256 threads with 48 KB (two per CU), or 512
-//     threads with 112 KB;
+//     Launch shapes: 256 threads with 128.5 KB of LDS for the Jacobi
+//     matrix (one workgroup per CU), 256 threads with 48 KB (two per CU), or
+//     512 threads with 128.5 KB;
 //   * W concurrent chains of the single-tensor rounder: W worker contexts,
 //     each with its own HIP stream and device workspace, driven by W host
 //     threads over the same LPT order (any precision, any size).
-// The first batch of at least 64 tensors on a context is split into four
-// interleaved quarters of the LPT order, each run with one candidate
-// (three persistent shapes, 16 stream workers); the candidate with the least
-// time per unit of estimated work is kept for later batches.
+// The first batch of at least 64 tensors on a context is tuned on
+// interleaved slices of the LPT order: each persistent shape rounds an eighth
+// of the batch, 16 stream workers a thirty-second, and the rest runs with the
+// candidate of least time per unit of estimated work, which is kept for
+// later batches (smaller untuned batches use the 512-thread shape).  The
+// environment variable TTG_BATCH_MODE (1-3: persistent shapes, 4: stream
+// workers) fixes the choice instead.
 #include "ttg_internal.h"
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <atomic>
 #include <chrono>
 #include <numeric>
@@ -102,7 +106,7 @@ struct PShape {
   int nt, per_cu;
   int64_t ycap;
 };
-static const PShape kPShapes[3] = {{256, 1, 14336}, {256, 2, 6144}, {512, 1, 14336}};
+static const PShape kPShapes[3] = {{256, 1, 16448}, {256, 2, 6144}, {512, 1, 16448}};
 
 static void run_persist(ttg_ctx *ctx, const BatchArgs &a, int64_t count, const std::vector<int64_t> &sel,
                         const PShape &sh, std::vector<ttg_status> &st, std::vector<ttg_info> &inf,
@@ -252,31 +256,40 @@ ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, cons
       for (int64_t t : sel) c += cost[t];
       return c;
     };
+    if (ctx->batch_mode == 0) {
+      // TTG_BATCH_MODE=1..4 fixes the execution (1-3: persistent launch
+      // shapes, 4: stream workers) instead of tuning
+      const char *env = std::getenv("TTG_BATCH_MODE");
+      if (env && env[0] >= '1' && env[0] <= '4' && env[1] == '\0') ctx->batch_mode = env[0] - '0';
+    }
     if (ctx->batch_mode == 0 && count >= 64) {
-      // tune on four interleaved quarters of the LPT order (every tensor is
-      // rounded exactly once): three persistent shapes and 16 stream workers
+      // tune on interleaved slices of the LPT order (every tensor is rounded
+      // exactly once): the three persistent shapes on an eighth each, the
+      // stream workers on a thirty-second, the rest with the fastest
+      std::vector<std::vector<int64_t>> sl(5);
+      for (int64_t j = 0; j < count; ++j) {
+        const int64_t c8 = j % 8;
+        sl[c8 < 3 ? c8 : (j % 32 == 3 ? 3 : 4)].push_back(order[j]);
+      }
       double best = 1e300;
-      int bestm = 1;
+      int bestm = 3;
       for (int c = 0; c < 4; ++c) {
-        std::vector<int64_t> sel;
-        for (int64_t j = c; j < count; j += 4) sel.push_back(order[j]);
+        if (sl[c].empty()) continue;
         auto s0 = std::chrono::steady_clock::now();
-        if (c < 3) {
-          run_persist(ctx, a, count, sel, kPShapes[c], st, inf, err);
-        } else {
-          std::vector<int64_t> sub(sel);
-          run_batch(ctx, a, sub, 0, (int64_t)sub.size(), 16, st, inf, err);
-        }
+        if (c < 3) run_persist(ctx, a, count, sl[c], kPShapes[c], st, inf, err);
+        else run_batch(ctx, a, sl[c], 0, (int64_t)sl[c].size(), (int)std::min<int64_t>(16, sl[c].size()), st, inf, err);
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count();
-        const double rate = sec / work(sel);
+        const double rate = sec / work(sl[c]);
         if (rate < best) {
           best = rate;
           bestm = c + 1;
         }
       }
       ctx->batch_mode = bestm;
+      if (bestm <= 3) run_persist(ctx, a, count, sl[4], kPShapes[bestm - 1], st, inf, err);
+      else run_batch(ctx, a, sl[4], 0, (int64_t)sl[4].size(), (int)std::min<int64_t>(16, sl[4].size()), st, inf, err);
     } else {
-      const int mode = ctx->batch_mode == 0 ? 1 : ctx->batch_mode;
+      const int mode = ctx->batch_mode == 0 ? 3 : ctx->batch_mode;
       if (mode <= 3) {
         run_persist(ctx, a, count, order, kPShapes[mode - 1], st, inf, err);
       } else {

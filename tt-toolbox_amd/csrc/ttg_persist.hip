@@ -69,39 +69,98 @@ __device__ void p_transpose(double *__restrict__ dst, const double *__restrict__
   __syncthreads();
 }
 
-// C (M x N, ldc) = op(A) (M x K) * op(B) (K x N) with FP64 MFMA; each wave
-// takes 16 x 16 output tiles round-robin and forms the transposed tile
-// (op(B)^T op(A)^T) so that 16 lanes store 16 consecutive rows of C.
+// C (M x N, ldc) = alpha op(A) (M x K) op(B) (K x N) + beta C with FP64 MFMA
+// (beta == 0: C is not read).  64 x 64 blocks of C; per 16-deep K step the
+// op(A) and op(B) tiles are loaded cooperatively (coalesced along the
+// contiguous index of each operand) into LDS (`lds`, 2560 doubles), the next
+// step's tiles are prefetched into registers while the waves multiply; the
+// waves form the transposed product (op(B)^T op(A)^T) so that 16 lanes store
+// 16 consecutive rows of C.
+constexpr int PG_BM = 64, PG_BN = 64, PG_BK = 16, PG_LD = 80;  // LD = 80: lanes 16-31 hit the other 32 banks
 template <int NT>
-__device__ void p_gemm(bool ta, bool tb, long M, long N, long K, const double *A, long lda,
-                       const double *B, long ldb, double *C, long ldc) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long tm = (M + 15) >> 4, tn = (N + 15) >> 4;
+__device__ __noinline__ void p_gemm(bool ta, bool tb, long M, long N, long K, const double *A, long lda,
+                       const double *B, long ldb, double *C, long ldc, double *lds, double alpha = 1.0,
+                       double beta = 0.0) {
+  constexpr int NW = NT / 64, WN = NW / 2, NTW = (PG_BN / WN) / 16;  // n-tiles per wave
+  constexpr int LA = PG_BM * PG_BK / NT, LB = PG_BK * PG_BN / NT;   // elements per thread
+  double *As = lds, *Bs = lds + PG_BK * PG_LD;  // As[k][m], Bs[k][n]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  for (long t = wv; t < tm * tn; t += NT / 64) {
-    const long m0 = (t % tm) * 16, n0 = (t / tm) * 16;
-    const long mrow = m0 + li, ncol = n0 + li;
-    v4d_p acc = {0, 0, 0, 0};
-    for (long k0 = 0; k0 < K; k0 += 16) {
-      double a[4], b[4];
+  const int wm = wv & 1, wn = wv >> 1;
+  const long bm = (M + PG_BM - 1) / PG_BM, bn = (N + PG_BN - 1) / PG_BN;
+  for (long blk = 0; blk < bm * bn; ++blk) {
+    const long m0 = (blk % bm) * PG_BM, n0 = (blk / bm) * PG_BN;
+    v4d_p acc[2][NTW];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const long kk = k0 + 4 * q + lk;
-        const bool ok = kk < K;
-        // operand "a" of the MFMA: rows = n of op(B)^T
-        b[q] = (ok && ncol < N) ? (tb ? B[ncol + kk * ldb] : B[kk + ncol * ldb]) : 0.0;
-        a[q] = (ok && mrow < M) ? (ta ? A[kk + mrow * lda] : A[mrow + kk * lda]) : 0.0;
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < NTW; ++j) acc[i][j] = v4d_p{0, 0, 0, 0};
+    double ra[LA], rb[LB];
+    auto fetch = [&](long k0) {
+#pragma unroll
+      for (int e = 0; e < LA; ++e) {
+        const int idx = tid + e * NT;
+        const int mi = ta ? idx / PG_BK : idx % PG_BM, ki = ta ? idx % PG_BK : idx / PG_BM;
+        const long gm = m0 + mi, gk = k0 + ki;
+        ra[e] = (gm < M && gk < K) ? (ta ? A[gk + gm * lda] : A[gm + gk * lda]) : 0.0;
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(b[q], a[q], acc, 0, 0, 0);
+      for (int e = 0; e < LB; ++e) {
+        const int idx = tid + e * NT;
+        const int ni = tb ? idx % PG_BN : idx / PG_BK, ki = tb ? idx / PG_BN : idx % PG_BK;
+        const long gn = n0 + ni, gk = k0 + ki;
+        rb[e] = (gn < N && gk < K) ? (tb ? B[gn + gk * ldb] : B[gk + gn * ldb]) : 0.0;
+      }
+    };
+    auto stash = [&]() {
+#pragma unroll
+      for (int e = 0; e < LA; ++e) {
+        const int idx = tid + e * NT;
+        const int mi = ta ? idx / PG_BK : idx % PG_BM, ki = ta ? idx % PG_BK : idx / PG_BM;
+        As[ki * PG_LD + mi] = ra[e];
+      }
+#pragma unroll
+      for (int e = 0; e < LB; ++e) {
+        const int idx = tid + e * NT;
+        const int ni = tb ? idx % PG_BN : idx / PG_BK, ki = tb ? idx / PG_BN : idx % PG_BK;
+        Bs[ki * PG_LD + ni] = rb[e];
+      }
+    };
+    fetch(0);
+    for (long k0 = 0; k0 < K; k0 += PG_BK) {
+      __syncthreads();
+      stash();
+      __syncthreads();
+      if (k0 + PG_BK < K) fetch(k0 + PG_BK);
+#pragma unroll
+      for (int kk = 0; kk < PG_BK; kk += 4) {
+        const int kr = kk + lk;
+        double av[2], bv[NTW];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) av[i] = As[kr * PG_LD + wm * 32 + i * 16 + li];
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) bv[j] = Bs[kr * PG_LD + wn * (16 * NTW) + j * 16 + li];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < NTW; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+      }
     }
-    // acc[v] of lane l: C(m0 + l%16, n0 + l/16 + 4v)
-    if (mrow < M) {
+    // acc[i][j][v] of lane l: C(m0 + wm*32 + i*16 + l%16, n0 + wn*16*NTW + j*16 + l/16 + 4v)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const long nc = n0 + lk + 4 * v;
-        if (nc < N) C[mrow + nc * ldc] = acc[v];
-      }
+    for (int i = 0; i < 2; ++i) {
+      const long gm = m0 + wm * 32 + i * 16 + li;
+      if (gm >= M) continue;
+#pragma unroll
+      for (int j = 0; j < NTW; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const long gn = n0 + wn * (16 * NTW) + j * 16 + lk + 4 * v;
+          if (gn < N) {
+            double *cp = C + gm + gn * ldc;
+            *cp = (beta == 0.0) ? alpha * acc[i][j][v] : __builtin_fma(alpha, acc[i][j][v], beta * *cp);
+          }
+        }
     }
   }
   __syncthreads();
@@ -113,7 +172,7 @@ __device__ void p_gemm(bool ta, bool tb, long M, long N, long K, const double *A
 // Q = H_0 ... H_{kk-1} [I; 0], applied backwards).  One wave per column in
 // the updates.
 template <int NT>
-__device__ void p_householder(double *__restrict__ A, long m, long k, double *__restrict__ tau,
+__device__ __noinline__ void p_householder(double *__restrict__ A, long m, long k, double *__restrict__ tau,
                               double *__restrict__ Q, double *red, double *bcast) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long kk = m < k ? m : k;
@@ -174,6 +233,123 @@ __device__ void p_householder(double *__restrict__ A, long m, long k, double *__
   }
 }
 
+// Blocked Householder QR (dgeqrf/dorgqr structure, compact WY), same output
+// as p_householder: 32-column panels factorised column by column (updates
+// confined to the panel), then with the panel's explicit reflector block V
+// (Vs, unit diagonal, zeros above; ld m - j0) and triangular factor T
+// (dlarft, forward columnwise, from V^T V) the trailing columns get
+// A2 <- A2 - V (T^T (V^T A2)) as three MFMA GEMMs; Q = H_0 ... H_{kk-1} [I; 0]
+// is accumulated backwards panel by panel, Q2 <- Q2 - V (T (V^T Q2)).
+// Scratch: Vs ((m - j0) x 32 <= m x k), W and W2 (32 x k each), Ts (32 x 32
+// per panel), lds (2560 + 2 * 32 * 33 doubles).
+constexpr int PH = 32;
+template <int NT>
+__device__ void p_hh_panel_vt(double *A, long m, long j0, int jb, const double *tau, double *Vs, double *Ts,
+                              double *lds) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long rows = m - j0;
+  for (long idx = threadIdx.x; idx < rows * jb; idx += NT) {
+    const long c = idx / rows, i = idx - c * rows;  // row j0 + i of column j0 + c
+    Vs[idx] = (i < c) ? 0.0 : (i == c ? 1.0 : A[j0 + i + (j0 + c) * m]);
+  }
+  __syncthreads();
+  double *Gv = lds + 2560, *Tl = Gv + PH * (PH + 1);
+  p_gemm<NT>(true, false, jb, jb, rows, Vs, rows, Vs, rows, Gv, PH + 1, lds);
+  // T(i,i) = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i)  (one wave, lane r owns row r)
+  if (wv == 0) {
+    const int r = lane;
+    for (int i = 0; i < jb; ++i) {
+      const double ti = tau[j0 + i];
+      double t = 0.0;
+      if (r < i) {
+        for (int l = r; l < i; ++l) t = __builtin_fma(Tl[r + l * (PH + 1)], Gv[l + i * (PH + 1)], t);
+        t *= -ti;
+      } else if (r == i) {
+        t = ti;
+      }
+      if (r < PH) Tl[r + i * (PH + 1)] = (r <= i) ? t : 0.0;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < PH * PH; idx += NT) {
+    const int r = idx % PH, c = idx / PH;
+    Ts[idx] = (r < jb && c < jb) ? Tl[r + c * (PH + 1)] : 0.0;
+  }
+  __syncthreads();
+}
+
+template <int NT>
+__device__ __noinline__ void p_householder_blocked(double *__restrict__ A, long m, long k, double *__restrict__ tau,
+                                      double *__restrict__ Q, double *red, double *Vs, double *W, double *W2,
+                                      double *Tall, double *lds) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long kk = m < k ? m : k;
+  for (long j0 = 0; j0 < kk; j0 += PH) {
+    const int jb = (int)((kk - j0) < PH ? (kk - j0) : PH);
+    for (long j = j0; j < j0 + jb; ++j) {
+      double *aj = A + j * m;
+      double s = 0.0;
+      for (long i = j + 1 + threadIdx.x; i < m; i += NT) s = __builtin_fma(aj[i], aj[i], s);
+      s = p_block_sum<NT>(s, red);
+      const double al = aj[j];
+      double tj = 0.0, scale = 0.0, beta = al;
+      if (s > 0.0) {
+        const double nrm = sqrt(__builtin_fma(al, al, s));
+        beta = al >= 0.0 ? -nrm : nrm;
+        tj = (beta - al) / beta;
+        scale = 1.0 / (al - beta);
+        for (long i = j + 1 + threadIdx.x; i < m; i += NT) aj[i] *= scale;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        aj[j] = beta;
+        tau[j] = tj;
+      }
+      if (tj != 0.0) {
+        for (long c = j + 1 + wv; c < j0 + jb; c += NT / 64) {
+          double *ac = A + c * m;
+          double w = (lane == 0) ? ac[j] : 0.0;
+          for (long i = j + 1 + lane; i < m; i += 64) w = __builtin_fma(aj[i], ac[i], w);
+          w = wave_sum(w) * tj;
+          if (lane == 0) ac[j] -= w;
+          for (long i = j + 1 + lane; i < m; i += 64) ac[i] = __builtin_fma(-w, aj[i], ac[i]);
+        }
+      }
+      __syncthreads();
+    }
+    double *Ts = Tall + (j0 / PH) * PH * PH;
+    p_hh_panel_vt<NT>(A, m, j0, jb, tau, Vs, Ts, lds);
+    const long n2 = k - j0 - jb, rows = m - j0;
+    if (n2 > 0) {
+      double *A2 = A + j0 + (j0 + jb) * m;
+      p_gemm<NT>(true, false, jb, n2, rows, Vs, rows, A2, m, W, jb, lds);
+      p_gemm<NT>(true, false, jb, n2, jb, Ts, PH, W, jb, W2, jb, lds);
+      p_gemm<NT>(false, false, rows, n2, jb, Vs, rows, W2, jb, A2, m, lds, -1.0, 1.0);
+    }
+  }
+  for (long idx = threadIdx.x; idx < m * kk; idx += NT) {
+    const long c = idx / m, i = idx - c * m;
+    Q[idx] = (i == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (long j0 = ((kk - 1) / PH) * PH; j0 >= 0; j0 -= PH) {
+    const int jb = (int)((kk - j0) < PH ? (kk - j0) : PH);
+    const long rows = m - j0, nc = kk - j0;
+    const double *Ts = Tall + (j0 / PH) * PH * PH;
+    for (long idx = threadIdx.x; idx < rows * jb; idx += NT) {
+      const long c = idx / rows, i = idx - c * rows;
+      Vs[idx] = (i < c) ? 0.0 : (i == c ? 1.0 : A[j0 + i + (j0 + c) * m]);
+    }
+    __syncthreads();
+    double *Q2 = Q + j0 + j0 * m;
+    p_gemm<NT>(true, false, jb, nc, rows, Vs, rows, Q2, m, W, jb, lds);
+    p_gemm<NT>(false, false, jb, nc, jb, Ts, PH, W, jb, W2, jb, lds);
+    p_gemm<NT>(false, false, rows, nc, jb, Vs, rows, W2, jb, Q2, m, lds, -1.0, 1.0);
+  }
+}
+
 // Rotation of columns with squared norms al, be and inner product ga (Rutishauser).
 __device__ __forceinline__ void p_jrot(double al, double be, double ga, double &cs, double &sn) {
   const double dd = be - al, g2 = 2.0 * ga;
@@ -189,7 +365,7 @@ __device__ __forceinline__ void p_jrot(double al, double be, double ga, double &
 // none of whose rotations had cos^2 > 1e-16 ends the iteration.  Returns the
 // number of sweeps (max_sweeps + 1 when not converged).
 template <int NT>
-__device__ int p_jacobi(double *Y, int k, int max_sweeps, double *red, int *cnt) {
+__device__ __noinline__ int p_jacobi(double *Y, int k, int max_sweeps, double *red, int *cnt) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long ld = 2L * k;
   double fro = 0.0;
@@ -241,6 +417,118 @@ __device__ int p_jacobi(double *Y, int k, int max_sweeps, double *red, int *cnt)
         }
       }
       __syncthreads();
+    }
+    if (lane == 0 && mybig > 0) atomicAdd(cnt, mybig);
+    __syncthreads();
+    const int big = cnt[0];
+    __syncthreads();
+    if (big == 0) return sweep + 1;
+  }
+  return max_sweeps + 1;
+}
+
+// The same iteration for a Y = [S; J] (global, ld 2k) too large for LDS:
+// block-cyclic ordering.  Columns form blocks of bw; the blocks play a
+// round-robin tournament, and each block pair is staged in LDS (2 bw
+// columns of 2k rows), rotated there and written back.  In the first block
+// round of a sweep every pair inside the two blocks is visited (2 bw - 1
+// rounds), later only the bw x bw cross pairs (bw rounds), so a sweep
+// visits every column pair once.  Same rotation rule and stopping test as
+// p_jacobi.
+template <int NT>
+__device__ __noinline__ int p_jacobi_blocks(double *Yg, int k, int bw, double *L, int max_sweeps, double *red, int *cnt) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long ld = 2L * k, ldl = 2L * k + 1;  // odd LDS column stride
+  double fro = 0.0;
+  for (long idx = threadIdx.x; idx < (long)k * k; idx += NT) {
+    const long c = idx / k, i = idx - c * k;
+    const double v = Yg[i + c * ld];
+    fro = __builtin_fma(v, v, fro);
+  }
+  fro = p_block_sum<NT>(fro, red);
+  const double small2 = (double)k * 4.0 * DBL_EPSILON * DBL_EPSILON * fro;
+  const double tol = sqrt((double)k) * DBL_EPSILON, tol2 = tol * tol;
+  const int nb = (k + bw - 1) / bw, PB = nb + (nb & 1), LC = 2 * bw;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    if (threadIdx.x == 0) cnt[0] = 0;
+    __syncthreads();
+    int mybig = 0;
+    for (int bt = 0; bt < PB - 1; ++bt) {
+      for (int bq = 0; bq < PB / 2; ++bq) {
+        int ba, bb;
+        if (bq == 0) {
+          ba = PB - 1;
+          bb = bt;
+        } else {
+          ba = (bt + bq) % (PB - 1);
+          bb = (bt - bq + PB - 1) % (PB - 1);
+        }
+        const bool va = ba < nb, vb = bb < nb;
+        if (!(va && vb) && !(bt == 0 && (va || vb))) continue;  // a lone block only needs its inner pairs
+        // local column l < bw: column ba*bw + l; l >= bw: column bb*bw + l - bw (-1: absent)
+        auto gcol = [&](int l) -> int {
+          const int blk = l < bw ? ba : bb, off = l < bw ? l : l - bw;
+          const bool v = l < bw ? va : vb;
+          const int c = blk * bw + off;
+          return (v && c < k) ? c : -1;
+        };
+        for (long idx = threadIdx.x; idx < (long)LC * ld; idx += NT) {
+          const int l = (int)(idx / ld);
+          const long i = idx - (long)l * ld;
+          const int c = gcol(l);
+          L[l * ldl + i] = c >= 0 ? Yg[i + (long)c * ld] : 0.0;
+        }
+        __syncthreads();
+        const int rounds = (bt == 0) ? LC - 1 : bw;
+        for (int t = 0; t < rounds; ++t) {
+          for (int q = wv; q < bw; q += NT / 64) {
+            int a, b;
+            if (bt == 0) {
+              if (q == 0) {
+                a = LC - 1;
+                b = t;
+              } else {
+                a = (t + q) % (LC - 1);
+                b = (t - q + LC - 1) % (LC - 1);
+              }
+            } else {
+              a = q;
+              b = bw + ((q + t) % bw);
+            }
+            if (gcol(a) < 0 || gcol(b) < 0) continue;
+            double *ya = L + (long)a * ldl, *yb = L + (long)b * ldl;
+            double al = 0.0, be = 0.0, ga = 0.0;
+            for (int i = lane; i < k; i += 64) {
+              const double x = ya[i], y = yb[i];
+              al = __builtin_fma(x, x, al);
+              be = __builtin_fma(y, y, be);
+              ga = __builtin_fma(x, y, ga);
+            }
+            al = wave_sum(al);
+            be = wave_sum(be);
+            ga = wave_sum(ga);
+            const double g2 = ga * ga, ab = al * be;
+            const bool rot = g2 > tol2 * ab && ab >= DBL_MIN && al > small2 && be > small2;
+            if (!rot) continue;
+            mybig += (g2 > 1e-16 * ab) ? 1 : 0;
+            double cs, sn;
+            p_jrot(al, be, ga, cs, sn);
+            for (long i = lane; i < ld; i += 64) {
+              const double x = ya[i], y = yb[i];
+              ya[i] = cs * x - sn * y;
+              yb[i] = sn * x + cs * y;
+            }
+          }
+          __syncthreads();
+        }
+        for (long idx = threadIdx.x; idx < (long)LC * ld; idx += NT) {
+          const int l = (int)(idx / ld);
+          const long i = idx - (long)l * ld;
+          const int c = gcol(l);
+          if (c >= 0) Yg[i + (long)c * ld] = L[l * ldl + i];
+        }
+        __syncthreads();
+      }
     }
     if (lane == 0 && mybig > 0) atomicAdd(cnt, mybig);
     __syncthreads();
@@ -306,10 +594,13 @@ __device__ int p_sort_chop(const double *Y, long ldy, int k, double ep, double *
   return r1;
 }
 
-// Per-slot scratch (doubles): Wa, Wq (core-sized), Y (2 kmax x kmax), Ra, Rb (kmax^2), tau (kmax).
+// Per-slot scratch (doubles): Wa, Wq, Vs (core-sized), Y (2 kmax x kmax),
+// Ra, Rb (kmax^2), tau (kmax), W, W2 (32 kmax), T factors (32 kmax + 1024).
 struct PSlot {
   long core, kmax;
-  __host__ __device__ long elems() const { return 2 * core + 2 * kmax * kmax + 2 * kmax * kmax + kmax + 64; }
+  __host__ __device__ long elems() const {
+    return 3 * core + 4 * kmax * kmax + kmax + 3 * 32 * kmax + 1024 + 64;
+  }
 };
 
 template <int NT>
@@ -333,6 +624,10 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
   double *Ra = Yg + 2 * slot.kmax * slot.kmax;
   double *Rb = Ra + slot.kmax * slot.kmax;
   double *tau = Rb + slot.kmax * slot.kmax;
+  double *Vs = tau + slot.kmax;
+  double *Wg = Vs + slot.core;
+  double *Wg2 = Wg + 32 * slot.kmax;
+  double *Tall = Wg2 + 32 * slot.kmax;
   for (;;) {
     if (threadIdx.x == 0) sh_t = atomicAdd(ctr, 1);
     __syncthreads();
@@ -395,7 +690,8 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
           break;
         }
         // Wa holds core i as (m x k); Q -> working slot i (m x kk), R -> Ra (kk x k)
-        p_householder<NT>(Wa, m, k, tau, xout + wofs[i], red, nullptr);
+        if (k <= PH) p_householder<NT>(Wa, m, k, tau, xout + wofs[i], red, nullptr);
+        else p_householder_blocked<NT>(Wa, m, k, tau, xout + wofs[i], red, Vs, Wg, Wg2, Tall, ylds);
         for (long idx = threadIdx.x; idx < kk * k; idx += NT) {
           const long c = idx / kk, r = idx - c * kk;
           Ra[idx] = (r <= c) ? Wa[r + c * m] : 0.0;
@@ -412,7 +708,7 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
         }
         __syncthreads();
         // core_{i+1} <- R core_{i+1} (input core i+1 is k x N1), into Wa
-        p_gemm<NT>(false, false, kk, N1, k, Ra, kk, xin + in_ofs_next, k, Wa, kk);
+        p_gemm<NT>(false, false, kk, N1, k, Ra, kk, xin + in_ofs_next, k, Wa, kk, ylds);
         in_ofs_next += k * N1;
         if (threadIdx.x == 0) rc[i + 1] = kk;
         __syncthreads();
@@ -437,10 +733,15 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
         // tall orientation B (mt x kj) in Wa, QR: R in Wa's upper part, Q -> Wq
         if (wide) p_transpose<NT>(Wa, Wc, kw, N);
         else p_copy<NT>(Wa, Wc, kw * N);
-        p_householder<NT>(Wa, mt, kj, tau, Wq, red, nullptr);
-        // Y = [S; I] with S = R^T (wide) or R (tall), in LDS when it fits
+        if (kj <= PH) p_householder<NT>(Wa, mt, kj, tau, Wq, red, nullptr);
+        else p_householder_blocked<NT>(Wa, mt, kj, tau, Wq, red, Vs, Wg, Wg2, Tall, ylds);
+        // Y = [S; I] with S = R^T (wide) or R (tall): in LDS when it fits,
+        // else in the slot, rotated block pair by block pair through LDS
         const long ldy = 2 * kj;
-        double *Y = (ldy * kj <= ycap) ? ylds : Yg;
+        const bool in_lds = ldy * kj <= ycap;
+        int bw = 0;
+        if (!in_lds) bw = (32 * (ldy + 1) <= ycap) ? 16 : ((16 * (ldy + 1) <= ycap) ? 8 : 0);
+        double *Y = in_lds ? ylds : Yg;
         for (long idx = threadIdx.x; idx < ldy * kj; idx += NT) {
           const long c = idx / ldy, r = idx - c * ldy;
           double v;
@@ -453,7 +754,8 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
           Y[idx] = v;
         }
         __syncthreads();
-        const int sw = p_jacobi<NT>(Y, (int)kj, max_sweeps, red, sh_bc);
+        const int sw = bw ? p_jacobi_blocks<NT>(Y, (int)kj, bw, ylds, max_sweeps, red, sh_bc)
+                          : p_jacobi<NT>(Y, (int)kj, max_sweeps, red, sh_bc);
         if (sw > max_sweeps) status |= 1;
         sweeps_tot += sw > max_sweeps ? max_sweeps : sw;
         sweeps_max = sw > sweeps_max ? sw : sweeps_max;
@@ -469,7 +771,7 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
             Ra[idx] = Y[src + kj];  // J_r (kw x r1)
           }
           __syncthreads();
-          p_gemm<NT>(true, true, r1, N, kw, Ra, kw, Wq, N, Wc, r1);
+          p_gemm<NT>(true, true, r1, N, kw, Ra, kw, Wq, N, Wc, r1, ylds);
         } else {
           // W = Q R, R J = Y_top: U S = Q Y_top[:, perm] (kw x r1), V = J[:, perm]
           for (long idx = threadIdx.x; idx < N * r1; idx += NT) {
@@ -479,12 +781,12 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
             Wa[c + r * r1] = Y[src + kj];  // core_i = J_r^T (r1 x N)
           }
           __syncthreads();
-          p_gemm<NT>(false, false, kw, r1, N, Wq, kw, Ra, N, Rb, kw);
+          p_gemm<NT>(false, false, kw, r1, N, Wq, kw, Ra, N, Rb, kw, ylds);
           p_copy<NT>(Wc, Wa, r1 * N);
         }
         // core_{i-1} <- core_{i-1} (U S)   (round.m:81-83), via Wa
         const long mp = rc[i - 1] * nn[i - 1];
-        p_gemm<NT>(false, false, mp, r1, kw, xout + wofs[i - 1], mp, Rb, kw, Wa, mp);
+        p_gemm<NT>(false, false, mp, r1, kw, xout + wofs[i - 1], mp, Rb, kw, Wa, mp, ylds);
         p_copy<NT>(xout + wofs[i - 1], Wa, mp * r1);
         if (threadIdx.x == 0) rc[i] = r1;
         __syncthreads();
