@@ -337,6 +337,7 @@ ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, cons
     acc.qr_fallbacks += inf[t].qr_fallbacks;
     acc.jacobi_sweeps += inf[t].jacobi_sweeps;
     acc.jacobi_maxsweeps = std::max(acc.jacobi_maxsweeps, inf[t].jacobi_maxsweeps);
+    acc.qr_retries += inf[t].qr_retries;
   }
   acc.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (info) *info = acc;

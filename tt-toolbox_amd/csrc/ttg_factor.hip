@@ -268,9 +268,12 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   typedef typename MFf<T>::V V;
   const int kp = (k + CB - 1) / CB * CB;
+  // k <= 256: the whole column panel R(0:J0, J0:J0+32) is staged at once
+  // (one batch of loads per block column); larger k: 32 x 32 chunks
+  const bool panel = kp <= 256;
   T *Xs = reinterpret_cast<T *>(smem);   // 32 x kp, column-major (Xs[col*32 + row])
-  T *Rs = Xs + (size_t)CB * kp;           // 32 x 32 chunk of R, row-major (Rs[t*32 + n])
-  double *Ds = reinterpret_cast<double *>(Rs + CB * CB);  // R_JJ, row-major (Ds[l*32 + j])
+  T *Rs = Xs + (size_t)CB * kp;           // R rows t of the panel, row-major (Rs[t*32 + n])
+  double *Ds = reinterpret_cast<double *>(Rs + (panel ? kp : CB) * CB);  // R_JJ, row-major (Ds[l*32 + j])
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int tr = wv & 1, tc = wv >> 1;
   const int r0 = blockIdx.x * CB;
@@ -283,19 +286,50 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
   for (int J0 = 0; J0 < k; J0 += CB) {
     const int jb = min(CB, k - J0);
     V acc = {0, 0, 0, 0};
-    for (int kc = 0; kc < J0; kc += CB) {
-      __syncthreads();
-      for (int idx = tid; idx < CB * CB; idx += blockDim.x) {
-        const int t = idx % CB, n = idx / CB;
-        Rs[t * CB + n] = (n < jb) ? (T)R[(kc + t) + (long)(J0 + n) * k] : T(0);
-      }
-      __syncthreads();
+    if (panel) {
+      if (J0 > 0) {
+        // R(t, J0 + n) for t < J0, n < 32: contiguous in t; all loads in flight at once
+        constexpr int MAXL = 256 * CB / 256;
+        T v[MAXL];
+        const int tot = J0 * CB;
 #pragma unroll
-      for (int kk = 0; kk < CB; kk += 4) {
-        const int t = kk + (lane >> 4);
-        const T a = Xs[(kc + t) * CB + tr * 16 + (lane & 15)];
-        const T b = Rs[t * CB + tc * 16 + (lane & 15)];
-        acc = MFf<T>::op(a, b, acc);
+        for (int e = 0; e < MAXL; ++e) {
+          const int idx = tid + e * 256;
+          const int t = idx % J0, n = idx / J0;
+          v[e] = (idx < tot && n < jb) ? (T)R[t + (long)(J0 + n) * k] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < MAXL; ++e) {
+          const int idx = tid + e * 256;
+          if (idx < tot) Rs[(idx % J0) * CB + idx / J0] = v[e];
+        }
+        __syncthreads();
+        for (int kc = 0; kc < J0; kc += CB) {
+#pragma unroll
+          for (int kk = 0; kk < CB; kk += 4) {
+            const int t = kk + (lane >> 4);
+            const T a = Xs[(kc + t) * CB + tr * 16 + (lane & 15)];
+            const T b = Rs[(kc + t) * CB + tc * 16 + (lane & 15)];
+            acc = MFf<T>::op(a, b, acc);
+          }
+        }
+      }
+    } else {
+      for (int kc = 0; kc < J0; kc += CB) {
+        __syncthreads();
+        for (int idx = tid; idx < CB * CB; idx += blockDim.x) {
+          const int t = idx % CB, n = idx / CB;
+          Rs[t * CB + n] = (n < jb) ? (T)R[(kc + t) + (long)(J0 + n) * k] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < CB; kk += 4) {
+          const int t = kk + (lane >> 4);
+          const T a = Xs[(kc + t) * CB + tr * 16 + (lane & 15)];
+          const T b = Rs[t * CB + tc * 16 + (lane & 15)];
+          acc = MFf<T>::op(a, b, acc);
+        }
       }
     }
     __syncthreads();
@@ -340,12 +374,62 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
   }
 }
 
+// ---------------------------------------------------------------------------
+// Near-identity second pass of CholeskyQR2: G = Q1^T Q1 = I + E with small E.
+// The upper factor R = I + U of G solves U + U^T + U^T U = E; U = Phi(E),
+// Phi(X) = triu(X, 1) + diag(X)/2, misses it by O(|E|^2).  R^{-1} is
+// I - U (+ U^2 with terms = 2), off by O(|U|^2) (O(|U|^3)), and Q = Q1 R^{-1}
+// is one GEMM; R is so close to I that the explicit inverse costs no
+// accuracy.  The host accepts the pass only if max|E| is below the bound for
+// which these truncations stay at rounding level (or at the form's error
+// budget).
+// ---------------------------------------------------------------------------
+__global__ void k_nearid_init(int k, double *__restrict__ G, double *__restrict__ U, double *maxabs) {
+  double mx = 0.0;
+  const long total = (long)k * k;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const long c = idx / k, r = idx - c * k;
+    const double e = G[idx] - (r == c ? 1.0 : 0.0);
+    U[idx] = (r < c) ? e : (r == c ? 0.5 * e : 0.0);
+    mx = fmax(mx, fabs(e));
+  }
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0 && mx > 0.0)  // non-negative doubles order like their bit patterns
+    atomicMax(reinterpret_cast<unsigned long long *>(maxabs), (unsigned long long)__double_as_longlong(mx));
+}
+// X = I + a U + b P (P may be null), R2 = I + U
+__global__ void k_nearid_fin(int k, double *__restrict__ X, const double *__restrict__ U,
+                             const double *__restrict__ P, double *__restrict__ R2) {
+  const long total = (long)k * k;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const long c = idx / k, r = idx - c * k;
+    const double id = (r == c ? 1.0 : 0.0), u = U[idx];
+    X[idx] = id - u + (P ? P[idx] : 0.0);
+    R2[idx] = id + u;
+  }
+}
+
+void nearid_pass(hipStream_t s, int64_t m, int64_t k, double *G, const double *Q1, double *Qout,
+                 double *R2, double *U, double *P, double *X, double *maxabs, int terms, double *gws,
+                 int64_t gws_elems, int num_cus) {
+  const long total = k * k;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 1024);
+  TTG_HIP(hipMemsetAsync(maxabs, 0, sizeof(double), s));
+  hipLaunchKernelGGL(k_nearid_init, dim3(blocks), dim3(256), 0, s, (int)k, G, U, maxabs);
+  TTG_HIP(hipGetLastError());
+  if (terms >= 2) gemm<double>(s, false, false, k, k, k, 1.0, U, k, U, k, 0.0, P, k, gws, gws_elems, num_cus);
+  hipLaunchKernelGGL(k_nearid_fin, dim3(blocks), dim3(256), 0, s, (int)k, X, (const double *)U,
+                     terms >= 2 ? (const double *)P : nullptr, R2);
+  TTG_HIP(hipGetLastError());
+  gemm<double>(s, false, false, m, k, k, 1.0, Q1, m, X, k, 0.0, Qout, m, gws, gws_elems, num_cus);
+}
+
 template <class T>
 void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t lda, const double *R,
                       T *X, int64_t ldx) {
   if (m <= 0 || k <= 0) return;
   const int64_t kp = (k + CB - 1) / CB * CB;
-  const size_t lds = ((size_t)CB * kp + CB * CB) * sizeof(T) + CB * CB * sizeof(double);
+  const size_t lds = ((size_t)CB * kp + (kp <= 256 ? kp : CB) * CB) * sizeof(T) + CB * CB * sizeof(double);
   TTG_CHECK(lds <= 160 * 1024, TTG_BAD_ARG, "rank too large for the triangular solve");
   ProfScope prof(KC_TRTRI, s, (double)m * k * k, (double)sizeof(T) * 2.0 * m * k + 8.0 * k * k);
   TTG_HIP(hipFuncSetAttribute((const void *)k_trsm<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

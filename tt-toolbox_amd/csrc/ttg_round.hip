@@ -75,7 +75,8 @@ template <class T> struct Rounder {
   // optional: dst (kk x n_i r_{i+1}) = Rm (kk x r_i) * core_i for generated
   // cores (fused A*x: the product core is built on chip, never stored)
   const std::function<void(int64_t, const T *, int64_t, T *)> *fmul = nullptr;
-  int64_t fallbacks = 0, sweeps_total = 0, sweeps_max = 0;
+  int64_t fallbacks = 0, sweeps_total = 0, sweeps_max = 0, retries = 0;
+  int next_level = 2;  // QR level after the last rejected svd_bond
   bool noconv = false;
 
   // a buffer region for core k that is not the current one
@@ -187,21 +188,43 @@ template <class T> struct Rounder {
 
   // Tall QR (m >= k): Qout (m x k, T), Rout (k x k, FP64), by CholeskyQR2.
   // flags[0..1] and stt[0..3] feed the acceptance test (qr_accept), checked
-  // by the caller.  g32: FP32 MFMA Gram products (the mixed form).  Their
-  // error only perturbs the orthogonality of Q (both Q = A R^{-1} solves are
-  // FP64, so A = Q R holds to FP64 accuracy); a bond whose conditioning is
-  // beyond FP32 Grams (Cholesky breakdown or the second pass far from I) is
-  // rejected and redone with FP64 Grams by the caller.
+  // by the caller.  QR level 0: the second pass by the near-identity series
+  // (nearid_pass).  FP64 Grams: R2 = I + Phi(E), Q = Q1 (I - Phi(E)),
+  // accepted if max|E| < 1e-8 (both truncations O(1e-16)).  The mixed form's
+  // sweep-1 bonds (g32): Gram products in FP32 MFMA (their error only
+  // perturbs the orthogonality of Q; every Q = A R^{-1} step is FP64, so
+  // A = Q R holds to FP64 accuracy), inverse series to U^2, accepted if
+  // max|E| < 1e-5 (Q orthogonal to 1e-10, Q R2 = Q1 to 1e-15).  Sweep-2
+  // unfoldings carry the bond's singular-value spread, beyond that bound with
+  // FP32 Grams, so they use FP64 Grams in both forms.  Level 1: FP64 Grams,
+  // Cholesky + substitution second pass (accepted if its R is within 0.1 of
+  // I).  Level 2 (caller): Householder.
+  bool nearid_level(int level) const { return level == 0; }
   void qr_tall_fast(const T *A, int64_t m, int64_t k, T *Qout, double *Rout, int *flags,
-                    double *stt, bool g32) {
+                    double *stt, int level, bool allow_g32) {
+    const bool g32 = mixed && level == 0 && allow_g32;
     cholqr_pass(A, m, k, 0.0, Q1, R1, flags + 0, stt + 0, g32);
-    cholqr_pass(Q1, m, k, 0.0, Qout, R2, flags + 1, stt + 2, g32);
+    if (nearid_level(level)) {
+      gram(Q1, m, k, G, g32);
+      nearid_pass(st, m, k, G, Q1, Qout, R2, R3, Rtmp, Rinv, stt + 3, g32 ? 2 : 1, gws, gws_elems, cus);
+    } else {
+      cholqr_pass(Q1, m, k, 0.0, Qout, R2, flags + 1, stt + 2, g32);
+    }
     gemm<double>(st, false, false, k, k, k, 1.0, R2, k, R1, k, 0.0, Rout, k, nullptr, 0, cus);
   }
-  bool qr_accept(const int *hflags, const double *hstats) const {
+  // level after a rejection: a failed FP64 first pass (Cholesky breakdown or
+  // diagonal spread) fails the same way at level 1, so it goes to Householder
+  int qr_next(const int *hflags, const double *hstats, int level, bool allow_g32) const {
+    const bool g32 = mixed && level == 0 && allow_g32;
+    if (level == 0 && !g32 && (hflags[0] || !(hstats[0] < 1e7))) return 2;
+    return level + 1;
+  }
+  bool qr_accept(const int *hflags, const double *hstats, int level, bool allow_g32) const {
     if (hflags[0] || hflags[1]) return false;
     if (!(hstats[0] < 1e7)) return false;   // diag(R1) spread: kappa estimate
-    if (!(hstats[3] < 0.1)) return false;   // R2 ~ I: Q1 was nearly orthonormal
+    // Q1 nearly orthonormal: max|Q1^T Q1 - I| (level 0) or max|R2 - I| (level 1)
+    const bool g32 = mixed && level == 0 && allow_g32;
+    if (!(hstats[3] < (nearid_level(level) ? (g32 ? 1e-5 : 1e-8) : 0.1))) return false;
     return true;
   }
   // Householder fallback (FP64) for the tall/wide QR of A (m x k).
@@ -233,7 +256,7 @@ template <class T> struct Rounder {
     double *R = Rprod;
     bool pending = false;
     if (m >= k && level < 2) {
-      qr_tall_fast(A, m, k, Qdst, R, iflag + 4 * i, bstats + 8 * i, mixed && level == 0);
+      qr_tall_fast(A, m, k, Qdst, R, iflag + 4 * i, bstats + 8 * i, level, true);
       pending = true;
     } else {
       qr_householder(A, m, k, Qdst, R);
@@ -275,7 +298,7 @@ template <class T> struct Rounder {
     }
     TTG_HIP(hipMemsetAsync(fl, 0, 4 * sizeof(int), st));
     if (level < 2) {
-      qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats, mixed && level == 0);
+      qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats, level, false);
     } else {
       qr_householder(tallA, mt, kj, Qs, R);
     }
@@ -296,7 +319,8 @@ template <class T> struct Rounder {
     TTG_HIP(hipMemcpyAsync(h_fl, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
     TTG_HIP(hipMemcpyAsync(h_st, stats, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
     TTG_HIP(hipStreamSynchronize(st));
-    qr_ok = level == 2 || qr_accept(h_fl, h_st);
+    qr_ok = level == 2 || qr_accept(h_fl, h_st, level, false);
+    if (!qr_ok) next_level = qr_next(h_fl, h_st, level, false);
     r1 = h_i[0];
     if (qr_ok) {
       if (h_i[1] == 2) throw Error(TTG_HIP_ERROR, "Jacobi grid barrier timed out");
@@ -310,11 +334,13 @@ template <class T> struct Rounder {
   void sweep2_bond(int64_t i, double ep, int64_t rmax_i) {
     int64_t r1, kj;
     bool ok, wide;
-    // escalate on rejection: (mixed) FP64 CholeskyQR2, then Householder
+    // escalate on rejection: FP64 CholeskyQR2 with a Cholesky second pass,
+    // then Householder
     int level = 0;
     svd_bond(i, ep, rmax_i, level, r1, ok, kj, wide);
     while (!ok) {
-      level = (level == 0 && mixed) ? 1 : 2;
+      level = next_level;
+      ++retries;
       svd_bond(i, ep, rmax_i, level, r1, ok, kj, wide);
     }
     const int64_t kw = r[i], N = n[i] * r[i + 1];
@@ -366,12 +392,13 @@ template <class T> struct Rounder {
       TTG_HIP(hipStreamSynchronize(st));
       int64_t bad = -1;
       for (int64_t i : pending)
-        if (!qr_accept(&hfl[4 * i], &hst[8 * i])) {
+        if (!qr_accept(&hfl[4 * i], &hst[8 * i], force[i], true)) {
           bad = i;
           break;
         }
       if (bad < 0) break;
-      force[bad] = (force[bad] == 0 && mixed) ? 1 : 2;
+      force[bad] = qr_next(&hfl[4 * bad], &hst[8 * bad], force[bad], true);
+      ++retries;
       for (int64_t j = bad + 1; j <= d; ++j) r[j] = r_in[j];
       start = bad;
       careful = true;
@@ -384,8 +411,9 @@ template <class T> struct Rounder {
           TTG_HIP(hipMemcpyAsync(hfl.data() + 4 * i, iflag + 4 * i, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
           TTG_HIP(hipMemcpyAsync(hst.data() + 8 * i, bstats + 8 * i, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
           TTG_HIP(hipStreamSynchronize(st));
-          if (qr_accept(&hfl[4 * i], &hst[8 * i])) break;
-          force[i] = (force[i] == 0 && mixed) ? 1 : 2;
+          if (qr_accept(&hfl[4 * i], &hst[8 * i], force[i], true)) break;
+          force[i] = qr_next(&hfl[4 * i], &hst[8 * i], force[i], true);
+          ++retries;
           r[i + 1] = r_in[i + 1];
         }
       }
@@ -503,6 +531,7 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
     info->qr_fallbacks = R.fallbacks;
     info->jacobi_sweeps = R.sweeps_total;
     info->jacobi_maxsweeps = R.sweeps_max;
+    info->qr_retries = R.retries;
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return R.noconv ? TTG_NO_CONVERGENCE : TTG_OK;

@@ -55,6 +55,7 @@ class TTGInfo(C.Structure):
         ("jacobi_sweeps", C.c_int64),
         ("jacobi_maxsweeps", C.c_int64),
         ("seconds", C.c_double),
+        ("qr_retries", C.c_int64),
     ]
 
     def as_dict(self):
