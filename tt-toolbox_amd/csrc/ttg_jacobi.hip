@@ -128,7 +128,27 @@ __device__ __forceinline__ void jrot(double al, double be, double ga, double &cs
 static __host__ __device__ inline int jacobi_mc_offset(int max_sweeps) { return (2 + max_sweeps + 1) & ~1; }
 // then one double: ||S||_F^2 (8-byte aligned: the offset is even)
 static __host__ __device__ inline int jacobi_fro_offset(int max_sweeps) { return jacobi_mc_offset(max_sweeps) + 2 * max_sweeps; }
-int jacobi_bar_words(int max_sweeps) { return jacobi_fro_offset(max_sweeps) + 2; }
+// then the J workers' barrier (2 words), the V publication flags (2 x 64, by
+// step parity) and their acknowledgements (64), see k_jacobi4<..., true>
+static __host__ __device__ inline int jacobi_jw_offset(int max_sweeps) { return jacobi_fro_offset(max_sweeps) + 2; }
+constexpr int JW_MAXP = 64;
+int jacobi_bar_words(int max_sweeps) { return jacobi_jw_offset(max_sweeps) + 2 + 3 * JW_MAXP; }
+
+// Thread 0: wait until *p >= target (acquire).  Bounded like grid_sync:
+// gives up when the error flag is raised or after ~4 s (raises it: 3).
+__device__ bool spin_ge(unsigned *p, unsigned target, int *err) {
+  const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+  while (__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(1);
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    if (__builtin_amdgcn_s_memrealtime() - t0 > 400000000ull) {
+      atomicExch(err, 3);
+      return false;
+    }
+  }
+  return true;
+}
+constexpr unsigned JW_DONE = 0xFFFFFFFFu;
 
 // ||S||_F^2 of the top k x k block of Y (one workgroup, fixed summation order)
 __global__ __launch_bounds__(1024) void k_fro2(int k, const double *__restrict__ Y, long ldy, double *out) {
@@ -233,11 +253,12 @@ __device__ __forceinline__ void jacobi_pair_mul(int k, int w, int nw, const doub
 //      only after the rounds.
 // Convergence: a sweep without rotations, or one none of whose rotations had
 // cos^2 > stop_cos2 (they leave cosines of O(cos^2) behind).
-template <int KL, int LC, int NT>
+template <int KL, int LC, int NT, bool SJ>
 __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict__ Sg, long lds,
                                                 double *__restrict__ Jg, long ldj, double tol,
                                                 double stop_cos2, int max_sweeps, int *sweeps_out,
-                                                unsigned *bar, int *err, unsigned long long *dbg) {
+                                                unsigned *bar, int *err, unsigned long long *dbg,
+                                                double *__restrict__ Vx) {
   constexpr int B = LC / 2;
   constexpr int ROWS = 16 * KL;            // padded row count of S
   constexpr int LDC = ROWS + 1;            // odd column stride: conflict-free MFMA operand reads
@@ -258,11 +279,65 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
   double *Vl = Gl + LC * LDG;                     // LC x LC
   double *Jl = Vl + LC * LC;                      // LC x LDC (J columns of the pair)
   double *Pt = Jl;                                // Gram partial tiles (NTILE * KS * 256), dead before Jl is written
-  __shared__ int sh_steprot, sh_rot, sh_big, sh_total, sh_bigtot;
+  __shared__ int sh_steprot, sh_rot, sh_big, sh_total, sh_bigtot, sh_rot_ok;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int grp = tid >> 4, li = tid & 15;
   const int cc = tid / TPC, i0 = tid - cc * TPC;
   const int w = blockIdx.x, nblk = 2 * p;
+  unsigned *jbar = bar + jacobi_jw_offset(max_sweeps);
+  unsigned *vflag = jbar + 2, *vack = vflag + 2 * JW_MAXP;  // vflag[parity * JW_MAXP + w]
+  if (SJ && w >= p) {
+    // J worker wj: applies S worker wj's step rotations V to the J columns of
+    // the same block pair, one step behind, on its own CU.  V of global step g
+    // arrives through Vx slot g mod 2 with the release flag 2g + (any
+    // rotation) in flag slot g mod 2 (a slot is rewritten only after the J
+    // worker acknowledged it); the J workers keep their own grid barrier so
+    // that a step's J columns are complete before the next owners read them.
+    const int wj = w - p;
+    unsigned jgen = 0;
+    __shared__ unsigned sh_flag;
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+      bool done = false;
+      for (int step = 0; step < nblk - 1; ++step) {
+        const unsigned g = (unsigned)(sweep * (nblk - 1) + step + 1);
+        if (tid == 0) {
+          unsigned *fl = &vflag[(g & 1) * JW_MAXP + wj];
+          const bool ok = spin_ge(fl, 2 * g, err);
+          sh_flag = ok ? __hip_atomic_load(fl, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : JW_DONE;
+        }
+        __syncthreads();
+        const unsigned f = sh_flag;
+        if (f == JW_DONE) {
+          done = true;
+          break;
+        }
+        const double *vsrc = Vx + ((long)(g & 1) * p + wj) * (LC * LC);
+        for (int idx = tid; idx < LC * LC; idx += NT) Vl[idx] = vsrc[idx];
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(&vack[wj], g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (f & 1u) {
+          int bA, bB;
+          circle_pair(nblk, step, wj, bA, bB);
+          const int gcol_cc = (cc < B) ? bA * B + cc : bB * B + (cc - B);
+          const double *jsrc = Jg + (long)gcol_cc * ldj;
+          double jv[NJ];
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) {
+            const int i = i0 + TPC * j;
+            jv[j] = (i < k) ? jsrc[i] : 0.0;
+          }
+#pragma unroll
+          for (int j = 0; j < NJ; ++j)
+            if (LDC % TPC == 0 || i0 + TPC * j < LDC) Jl[cc * LDC + i0 + TPC * j] = jv[j];
+          __syncthreads();
+          jacobi_pair_mul<LC, 2, true>(k, wv, NW, Jl, LDC, Jg, ldj, Vl, bA, bB, lane);
+        }
+        if (!grid_sync(jbar, p, jgen, err)) return;
+      }
+      if (done) break;
+    }
+    return;
+  }
   unsigned *rotcnt = bar + 2;
   unsigned *bigcnt = bar + jacobi_mc_offset(max_sweeps);
   const double tol2 = tol * tol;
@@ -271,7 +346,7 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
   // stalls convergence on rank-deficient input)
   const double small2 = (double)k * 4.0 * DBL_EPSILON * DBL_EPSILON *
                         *reinterpret_cast<const double *>(bar + jacobi_fro_offset(max_sweeps));
-  unsigned gen = 0;
+  unsigned gen = 0, last_g = 0;
   bool converged = false;
   int sweep;
   for (sweep = 0; sweep < max_sweeps && !converged; ++sweep) {
@@ -293,10 +368,12 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
           const int i = i0 + TPC * j;
           v[j] = (i < k) ? src[i] : 0.0;
         }
+        if (!SJ) {
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-          const int i = i0 + TPC * j;
-          jv[j] = (i < k) ? jsrc[i] : 0.0;
+          for (int j = 0; j < NJ; ++j) {
+            const int i = i0 + TPC * j;
+            jv[j] = (i < k) ? jsrc[i] : 0.0;
+          }
         }
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
@@ -420,16 +497,33 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
       }
       if (li == 0 && myrot > 0) atomicAdd(&sh_steprot, myrot);
       if (li == 0 && mybig > 0) atomicAdd(&sh_big, mybig);
+      if (!SJ) {
 #pragma unroll
-      for (int j = 0; j < NJ; ++j)
-        if (LDC % TPC == 0 || i0 + TPC * j < LDC) Jl[cc * LDC + i0 + TPC * j] = jv[j];
+        for (int j = 0; j < NJ; ++j)
+          if (LDC % TPC == 0 || i0 + TPC * j < LDC) Jl[cc * LDC + i0 + TPC * j] = jv[j];
+      }
       __syncthreads();
       const int rot = sh_steprot;
+      if (SJ) {
+        // publish V for the J worker: wait until it took the V of two steps
+        // ago from this parity's slot, copy, release the flag
+        const unsigned g = (unsigned)(sweep * (nblk - 1) + step + 1);
+        if (tid == 0) sh_rot_ok = (g <= 2) ? 1 : (spin_ge(&vack[w], g - 2, err) ? 1 : 0);
+        __syncthreads();
+        if (!sh_rot_ok) return;
+        double *vdst = Vx + ((long)(g & 1) * p + w) * (LC * LC);
+        for (int idx = tid; idx < LC * LC; idx += NT) vdst[idx] = Vl[idx];
+        __syncthreads();  // with thread 0's agent-scope release below, publishes the whole block
+        if (tid == 0)
+          __hip_atomic_store(&vflag[(g & 1) * JW_MAXP + w], 2 * g + (rot > 0 ? 1u : 0u), __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        last_g = g;
+      }
       if (rec) dbg[step * 4 + 2] = __builtin_amdgcn_s_memrealtime();
       // 3. S_p <- S_p V and J_p <- J_p V, straight to HBM
       if (rot > 0) {
         jacobi_pair_mul<LC, 2, true>(k, wv, NW, Sl, LDC, Sg, lds, Vl, bA, bB, lane);
-        jacobi_pair_mul<LC, 2, true>(k, wv, NW, Jl, LDC, Jg, ldj, Vl, bA, bB, lane);
+        if (!SJ) jacobi_pair_mul<LC, 2, true>(k, wv, NW, Jl, LDC, Jg, ldj, Vl, bA, bB, lane);
       }
       if (tid == 0) sh_rot += rot;
       if (rec) dbg[step * 4 + 3] = __builtin_amdgcn_s_memrealtime();
@@ -458,9 +552,22 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
     __syncthreads();
     converged = (sh_total == 0) || (sh_bigtot == 0);
   }
+  if (SJ && tid == 0) {
+    // the J worker must have taken the last V before its flag slots say done
+    if (last_g == 0 || spin_ge(&vack[w], last_g, err)) {
+      __hip_atomic_store(&vflag[w], JW_DONE, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&vflag[JW_MAXP + w], JW_DONE, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
   if (w == 0 && tid == 0) {
     sweeps_out[0] += sweep;
-    if (!converged) atomicExch(err, 1);
+    if (!converged) {
+      // the soft flag also ends the J workers' waits: raise it only once every
+      // J worker has taken the last V (its remaining update has no wait)
+      if (SJ && last_g > 0)
+        for (int q = 0; q < p; ++q) spin_ge(&vack[q], last_g, err);
+      atomicExch(err, 1);
+    }
   }
 }
 
@@ -506,34 +613,38 @@ size_t jacobi_ws_bytes(const JacobiPlan &pl) { return (size_t)pl.rows * pl.kpad 
 template <int KL, int LC, int NT>
 static void launch_jacobi_t(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg,
                             long ldj, double tol, double stop_cos2, int max_sweeps, int *sweeps_dev,
-                            unsigned *bar, int *err, unsigned long long *dbg) {
+                            unsigned *bar, int *err, unsigned long long *dbg, double *Vx) {
   constexpr int NT16 = LC / 16, NTILE = NT16 * (NT16 + 1) / 2, KS = (NT / 64) / NTILE;
   static_assert((size_t)NTILE * KS * 256 <= (size_t)LC * (16 * KL + 1), "Gram partials must fit in the J area");
   const size_t lsz = (2 * (size_t)LC * (16 * KL + 1) + (size_t)LC * (LC + 1) + (size_t)LC * LC) * sizeof(double);
   TTG_CHECK(lsz <= 160 * 1024 - 256, TTG_BAD_ARG, "Jacobi block does not fit in LDS");
   TTG_CHECK(pl.k <= 16 * KL && 2 * pl.b == LC, TTG_BAD_ARG, "Jacobi plan/kernel mismatch");
-  TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi4<KL, LC, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
   int k = (int)pl.k, p = pl.p, ms = max_sweeps;
   if (p > 1) {
-    void *args[] = {&k, &p, &Sg, &lds, &Jg, &ldj, &tol, &stop_cos2, &ms, &sweeps_dev, &bar, &err, &dbg};
-    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_jacobi4<KL, LC, NT>, dim3(p), dim3(NT), args, lsz, s));
+    // J columns updated by p separate workgroups (split J) when V can be exchanged
+    const bool sj = Vx != nullptr && p <= JW_MAXP;
+    const void *fn = sj ? (const void *)k_jacobi4<KL, LC, NT, true> : (const void *)k_jacobi4<KL, LC, NT, false>;
+    TTG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
+    void *args[] = {&k, &p, &Sg, &lds, &Jg, &ldj, &tol, &stop_cos2, &ms, &sweeps_dev, &bar, &err, &dbg, &Vx};
+    TTG_HIP(hipLaunchCooperativeKernel(fn, dim3(sj ? 2 * p : p), dim3(NT), args, lsz, s));
   } else {
-    hipLaunchKernelGGL((k_jacobi4<KL, LC, NT>), dim3(1), dim3(NT), lsz, s, k, p, Sg, lds, Jg, ldj, tol,
-                       stop_cos2, ms, sweeps_dev, bar, err, dbg);
+    TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi4<KL, LC, NT, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
+    hipLaunchKernelGGL((k_jacobi4<KL, LC, NT, false>), dim3(1), dim3(NT), lsz, s, k, p, Sg, lds, Jg, ldj, tol,
+                       stop_cos2, ms, sweeps_dev, bar, err, dbg, Vx);
     TTG_HIP(hipGetLastError());
   }
 }
 
 static void launch_jacobi(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg, long ldj,
                           double tol, double stop_cos2, int max_sweeps, int *sweeps_dev, unsigned *bar,
-                          int *err, unsigned long long *dbg) {
+                          int *err, unsigned long long *dbg, double *Vx) {
   TTG_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * jacobi_bar_words(max_sweeps), s));
   hipLaunchKernelGGL(k_fro2, dim3(1), dim3(1024), 0, s, (int)pl.k, Sg, lds,
                      reinterpret_cast<double *>(bar + jacobi_fro_offset(max_sweeps)));
   TTG_HIP(hipGetLastError());
   const int64_t k = pl.k;
   const int lc = 2 * pl.b;
-#define TTG_JAC(KL, LC, NT) launch_jacobi_t<KL, LC, NT>(s, pl, Sg, lds, Jg, ldj, tol, stop_cos2, max_sweeps, sweeps_dev, bar, err, dbg)
+#define TTG_JAC(KL, LC, NT) launch_jacobi_t<KL, LC, NT>(s, pl, Sg, lds, Jg, ldj, tol, stop_cos2, max_sweeps, sweeps_dev, bar, err, dbg, Vx)
   if (k <= 16) {
     TTG_CHECK(lc == 16, TTG_BAD_ARG, "unsupported Jacobi block size");
     TTG_JAC(4, 16, 256);
@@ -575,8 +686,11 @@ void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool trans
   // algorithmic flops of the small SVD (R-SVD of the k x k factor, GVL Fig. 8.6.1: 20k^3)
   ProfScope prof(KC_JACOBI, s, 20.0 * (double)k * k * k, 8.0 * 4.0 * (double)k * k);
   const double tol = sqrt((double)std::max(k, 1)) * DBL_EPSILON;
+  // ws (the polar-step scratch, idle during the sweeps) carries the split-J
+  // exchange of V: 2 parities x p workgroups x (2b)^2 doubles
+  double *Vx = (2 * (size_t)pl.p * (2 * pl.b) * (2 * pl.b) <= jacobi_polar_ws_elems(pl)) ? ws : nullptr;
   launch_jacobi(s, pl, Y, (long)pl.rows, Y + k, (long)pl.rows, tol, stop_cos2, max_sweeps, sweeps_dev,
-                barrier_dev, err_dev, dbg);
+                barrier_dev, err_dev, dbg, Vx);
   // polar correction of J (and of S J with it)
   double *M = ws, *Y2 = ws + (size_t)kpad * kpad;
   gemm<double>(s, true, false, kpad, kpad, k, 1.0, Y + k, pl.rows, Y + k, pl.rows, 0.0, M, kpad,
