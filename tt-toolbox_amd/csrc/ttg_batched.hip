@@ -211,6 +211,50 @@ static void run_persist(ttg_ctx *ctx, const BatchArgs &a, int64_t count, const s
   }
 }
 
+// A single small FP64 rounding (every QR-capped rank <= 64) as a one-tensor
+// persistent launch: one workgroup runs the whole round with no host round
+// trip, which beats the multi-CU bond-by-bond path where every bond is a few
+// tiny kernels and a host synchronisation.  Returns false if the tensor is
+// not eligible (the caller then uses the general path).
+bool round_small_persist(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
+                         double eps, const int64_t *rmax, int64_t *r_out, double *dcore_out, int64_t cap,
+                         int64_t *len_out, ttg_info *info, ttg_status *status) {
+  const char *env = std::getenv("TTG_SMALL_PERSIST");
+  if (env && env[0] == '0') return false;
+  if (!persist_supported(d, n, r)) return false;
+  std::vector<int64_t> rc(r, r + d + 1);
+  for (int64_t i = 0; i + 1 < d; ++i) rc[i + 1] = std::min(rc[i] * n[i], rc[i + 1]);
+  for (int64_t i = 0; i <= d; ++i)
+    if (rc[i] > 64) return false;
+  if (rmax)
+    for (int64_t i = 1; i < d; ++i)
+      if (rmax[i] < rc[i]) return false;  // a binding rmax cap: general path
+  auto t0 = std::chrono::steady_clock::now();
+  int64_t need = 0, o = 0;
+  round_capacity(d, n, r, &need);
+  if (cap < need) {
+    if (len_out) *len_out = need;
+    *status = TTG_NEED_SPACE;
+    return true;
+  }
+  const int64_t zero = 0;
+  for (int64_t i = 0; i <= d; ++i) r_out[i] = r[i];
+  const BatchArgs a{&d, &zero, n, &zero, r, &zero, &zero, dcore, dcore_out, r_out, eps, TTG_PREC_FP64};
+  std::vector<ttg_status> st(1, TTG_OK);
+  std::vector<ttg_info> inf(1, ttg_info{0, 0, 0, 0.0});
+  std::vector<std::string> err(1);
+  run_persist(ctx, a, 1, std::vector<int64_t>{0}, kPShapes[2], st, inf, err);
+  if (st[0] != TTG_OK && st[0] != TTG_NO_CONVERGENCE) throw Error(st[0], err[0]);
+  for (int64_t i = 0; i < d; ++i) o += r_out[i] * n[i] * r_out[i + 1];
+  if (len_out) *len_out = o;
+  if (info) {
+    *info = inf[0];
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  *status = st[0];
+  return true;
+}
+
 ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, const int64_t *nofs,
                              const int64_t *n, const int64_t *rofs, const int64_t *r,
                              const int64_t *cofs, const double *dcore, double eps, int prec,

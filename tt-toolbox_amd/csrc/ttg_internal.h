@@ -129,6 +129,10 @@ struct PTensorDesc {
   int64_t nofs, rofs;     // into the flat n and r arrays
   int64_t in_ofs, out_ofs;  // first input core, first output/working core
 };
+// round_dev's one-workgroup path for small FP64 tensors (false: not eligible)
+bool round_small_persist(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
+                         double eps, const int64_t *rmax, int64_t *r_out, double *dcore_out, int64_t cap,
+                         int64_t *len_out, ttg_info *info, ttg_status *status);
 // true if one workgroup can round the tensor (d <= 64, ranks <= 512)
 bool persist_supported(int64_t d, const int64_t *n, const int64_t *r);
 // doubles of per-slot workspace for the largest core / bond rank of a batch

@@ -542,6 +542,15 @@ ttg_status round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r
                      int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
                      ttg_info *info) {
   TTG_CHECK(prec == TTG_PREC_FP64 || prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  if (prec == TTG_PREC_FP64) {
+    validate_tt(d, n, r);
+    TTG_CHECK(eps >= 0.0 && std::isfinite(eps), TTG_BAD_ARG, "eps must be finite and >= 0");
+    if (rmax)
+      for (int64_t i = 1; i < d; ++i) TTG_CHECK(rmax[i] >= 1, TTG_BAD_ARG, "rmax entries must be >= 1");
+    ttg_status st = TTG_OK;
+    if (round_small_persist(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info, &st))
+      return st;
+  }
   return round_impl<double>(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info,
                             nullptr, nullptr, prec == TTG_PREC_FP32);
 }
