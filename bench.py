@@ -3,11 +3,14 @@
 with graded core spectra), eps=1e-6.
 
 One step = one ``round(X, 1e-6)`` of that tensor with its cores already resident
-in HBM, in the mixed form (FP32 MFMA Gram products; FP64 storage, solves,
-contractions, r x r Cholesky and Jacobi SVD).  The FP64 plain form is timed on the same input for the
-speedup, both results are compared with the FP64 CPU oracle (the
-restatement of @tt_tensor/round.m), and the oracle itself is timed on the host
-cores as the CPU baseline.  Prints one JSON line.
+in HBM, in the mixed form (FP32 MFMA Gram products on the first sweep; FP64
+storage, solves, contractions, r x r Cholesky and Jacobi SVD).  The FP64 plain
+form is timed on the same input for the speedup, both results are compared with
+the FP64 CPU oracle (the restatement of @tt_tensor/round.m), and the oracle
+itself is timed on the host cores as the CPU baseline.  Bonds whose smallest
+singular value provably clears the truncation threshold skip their SVD (same
+tensor, same ranks); the rounds are also timed with an SVD at every bond
+(``with_svd_at_every_bond``).  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-steps P] [--no-cpu]
 """
@@ -267,6 +270,26 @@ def main():
     ms_plain = e0.elapsed_time(e1) / args.plain_steps
     log(f"plain: {ms_plain:.3f} ms/round")
 
+    # the same rounds with the no-truncation shortcut disabled (an SVD at every
+    # bond), for comparison
+    full_svd = {}
+    os.environ["TTG_SVD_ALWAYS"] = "1"
+    try:
+        for prec, out in (("fp32", out_mixed), ("fp64", out_plain)):
+            one(prec, out)
+            torch.cuda.synchronize()
+            ea = torch.cuda.Event(enable_timing=True)
+            eb = torch.cuda.Event(enable_timing=True)
+            ea.record(stream)
+            for i in range(2):
+                one(prec, out)
+            eb.record(stream)
+            torch.cuda.synchronize()
+            full_svd["mixed_ms_per_round" if prec == "fp32" else "plain_ms_per_round"] = ea.elapsed_time(eb) / 2
+    finally:
+        del os.environ["TTG_SVD_ALWAYS"]
+    log("SVD at every bond:", full_svd)
+
     # algorithmic work per round at the actual ranks (QR + apply + SVD + apply)
     d = len(n)
     rq = list(map(int, r))
@@ -309,16 +332,17 @@ def main():
         "algorithmic_gflop_per_round": flops / 1e9,
         "achieved_round_tflops": flops / (ms_step * 1e-3) / 1e12,
         "ranks_out": ro,
+        "svd_skipped_bonds_per_round": info.get("svd_skipped"),
+        "with_svd_at_every_bond": full_svd,
         "diag": {"qr_fallbacks": info.get("qr_fallbacks"), "jacobi_sweeps": info.get("jacobi_sweeps"),
-                 "jacobi_maxsweeps": info.get("jacobi_maxsweeps"),
+                 "jacobi_maxsweeps": info.get("jacobi_maxsweeps"), "qr_retries": info.get("qr_retries"),
                  "per_class_ms_per_round": {k: v["ms"] for k, v in per_class.items()}},
     }
     # roofline of the dominant kernel class
     avg_ms = prof["ms"] / max(prof["launches"], 1)
-    if dominant == "gemm":
-        peak, unit, bound = PEAK["fp32"], "TFLOP/s", "mfma"
-    else:
-        peak, unit, bound = PEAK["fp64"], "TFLOP/s", "mfma"
+    # every kernel class computes in FP64 (the mixed form's FP32 Gram
+    # products are a small share of the GEMM launches)
+    peak, unit, bound = PEAK["fp64"], "TFLOP/s", "mfma"
     achieved = (prof["flops"] / max(prof["launches"], 1)) / (avg_ms * 1e-3)
     traffic = None
     tfile = ROOT / "profiles" / "traffic.json"

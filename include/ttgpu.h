@@ -51,6 +51,7 @@ typedef struct {
   int64_t jacobi_maxsweeps; /* largest sweep count of one SVD */
   double  seconds;          /* wall time of the call on the host clock */
   int64_t qr_retries;       /* bond QRs redone at a higher level (FP64 / Cholesky pass, Householder) */
+  int64_t svd_skipped;      /* sweep-2 bonds proven not to truncate: completed from the QR, no SVD */
 } ttg_info;
 
 /* Context: owns a HIP stream (or borrows `stream` when non-NULL), device workspace.

@@ -116,3 +116,34 @@ def test_cooperative_jacobi_path(gpu_ctx):
     rng = np.random.default_rng(9)
     x = O.rand_tt(rng, [16, 16, 16, 16], [1, 16, 128, 16, 1])
     check_plain(x, 1e-9)
+
+
+@pytest.mark.parametrize("eps", [1e-6, 0.0])
+def test_no_truncation_shortcut(gpu_ctx, monkeypatch, eps):
+    """Bonds whose smallest singular value provably clears my_chop2's threshold
+    are completed from their QR factors without an SVD; the result must equal
+    the full SVD path's and the oracle's (ranks exact, 1e-12 relative)."""
+    rng = np.random.default_rng(21)
+    x = O.rand_tt(rng, [4] * 8, [1, 4, 16, 64, 80, 80, 16, 4, 1])  # ranks > 64: multi-CU path
+    ref = O.tt_round(x, eps)
+    info = {}
+    y = ttgpu.round(to_gpu_tt(x), eps, info=info)
+    assert info["svd_skipped"] > 0
+    monkeypatch.setenv("TTG_SVD_ALWAYS", "1")
+    info2 = {}
+    y2 = ttgpu.round(to_gpu_tt(x), eps, info=info2)
+    assert info2["svd_skipped"] == 0
+    for yy in (y, y2):
+        assert yy.r.tolist() == ref.r.tolist()
+        assert O.tt_rel_diff(O.TT(yy.n, yy.r, yy.core), ref) <= 1e-12
+
+
+def test_truncating_bonds_keep_the_svd(gpu_ctx):
+    rng = np.random.default_rng(22)
+    x = O.rand_tt(rng, [3] * 6, [1, 3, 9, 12, 9, 3, 1])
+    z = O.tt_plus(x, x)  # every bond truncates to half
+    info = {}
+    y = ttgpu.round(to_gpu_tt(z), 1e-10, info=info)
+    ref = O.tt_round(z, 1e-10)
+    assert y.r.tolist() == ref.r.tolist()
+    assert O.tt_rel_diff(O.TT(y.n, y.r, y.core), ref) <= 1e-12

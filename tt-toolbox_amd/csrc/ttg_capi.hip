@@ -100,6 +100,12 @@ ttg_status ttg_ctx_destroy(ttg_ctx *ctx) {
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ttg::destroy_workers(ctx);
   if (ctx->pgraph) (void)hipGraphExecDestroy(ctx->pgraph);
+  if (ctx->side) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
+  }
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
   delete ctx;

@@ -409,6 +409,21 @@ __global__ void k_nearid_fin(int k, double *__restrict__ X, const double *__rest
   }
 }
 
+// X = I + sign * U (U == nullptr: X = I)
+__global__ void k_eye_plus(int k, double *__restrict__ X, const double *__restrict__ U, double sign) {
+  const long total = (long)k * k;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const long c = idx / k, r = idx - c * k;
+    X[idx] = (r == c ? 1.0 : 0.0) + (U ? sign * U[idx] : 0.0);
+  }
+}
+void eye_plus(hipStream_t s, int64_t k, double *X, const double *U, double sign) {
+  const long total = k * k;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_eye_plus, dim3(blocks), dim3(256), 0, s, (int)k, X, U, sign);
+  TTG_HIP(hipGetLastError());
+}
+
 void nearid_pass(hipStream_t s, int64_t m, int64_t k, double *G, const double *Q1, double *Qout,
                  double *R2, double *U, double *P, double *X, double *maxabs, int terms, double *gws,
                  int64_t gws_elems, int num_cus) {

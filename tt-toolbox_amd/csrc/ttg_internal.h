@@ -113,6 +113,9 @@ struct ttg_ctx {
   // launch (reset + persistent kernel) with the arguments it was captured for,
   // and the tuned launch shape (0: not tuned yet)
   ttg::Arena parena;
+  // side stream + events (default priority) for work that overlaps the main chain
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t pgraph = nullptr;
   std::vector<uint64_t> pgraph_key;
   int batch_mode = 0;
@@ -211,12 +214,19 @@ size_t jacobi_polar_ws_elems(const JacobiPlan &pl);
 int jacobi_bar_words(int max_sweeps);
 void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
                 int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws,
-                unsigned long long *dbg = nullptr, double stop_cos2 = 1e-16);
+                unsigned long long *dbg = nullptr, double stop_cos2 = 1e-16, const int *skip = nullptr);
 
 // Sort the column norms of Y's top k rows (descending) and apply my_chop2.
 // sv[k], perm[k] written; out_i[0] = r1 (after rmax cap), out_d[0] = ||s||.
 void sort_chop(hipStream_t s, const double *Y, int64_t ldy, int64_t k, int64_t kpad,
-               double ep, int64_t rmax, double *sv, int *perm, int *out_i, double *out_d);
+               double ep, int64_t rmax, double *sv, int *perm, int *out_i, double *out_d,
+               const int *skip = nullptr);
+
+// skip = 1 iff 1/nrmX >= 2 ep nrmR, k <= rmax and nrmR > 0 (no truncation at the bond)
+void notrunc_decide(hipStream_t s, const double *nrmR, const double *nrmX, double ep, int64_t k,
+                    int64_t rmax, int *skip);
+// X = I + sign * U (k x k; U may be null)
+void eye_plus(hipStream_t s, int64_t k, double *X, const double *U, double sign);
 
 // Gather columns perm[0..r1) of Y (rows x *) with row offset into dst (rows_out x r1).
 template <class T>

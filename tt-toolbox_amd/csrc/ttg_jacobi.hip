@@ -258,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
                                                 double *__restrict__ Jg, long ldj, double tol,
                                                 double stop_cos2, int max_sweeps, int *sweeps_out,
                                                 unsigned *bar, int *err, unsigned long long *dbg,
-                                                double *__restrict__ Vx) {
+                                                double *__restrict__ Vx, const int *skip) {
   constexpr int B = LC / 2;
   constexpr int ROWS = 16 * KL;            // padded row count of S
   constexpr int LDC = ROWS + 1;            // odd column stride: conflict-free MFMA operand reads
@@ -284,6 +284,7 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
   const int grp = tid >> 4, li = tid & 15;
   const int cc = tid / TPC, i0 = tid - cc * TPC;
   const int w = blockIdx.x, nblk = 2 * p;
+  if (skip && *skip) return;  // the caller proved the SVD unnecessary (no truncation)
   unsigned *jbar = bar + jacobi_jw_offset(max_sweeps);
   unsigned *vflag = jbar + 2, *vack = vflag + 2 * JW_MAXP;  // vflag[parity * JW_MAXP + w]
   if (SJ && w >= p) {
@@ -613,7 +614,7 @@ size_t jacobi_ws_bytes(const JacobiPlan &pl) { return (size_t)pl.rows * pl.kpad 
 template <int KL, int LC, int NT>
 static void launch_jacobi_t(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg,
                             long ldj, double tol, double stop_cos2, int max_sweeps, int *sweeps_dev,
-                            unsigned *bar, int *err, unsigned long long *dbg, double *Vx) {
+                            unsigned *bar, int *err, unsigned long long *dbg, double *Vx, const int *skip) {
   constexpr int NT16 = LC / 16, NTILE = NT16 * (NT16 + 1) / 2, KS = (NT / 64) / NTILE;
   static_assert((size_t)NTILE * KS * 256 <= (size_t)LC * (16 * KL + 1), "Gram partials must fit in the J area");
   const size_t lsz = (2 * (size_t)LC * (16 * KL + 1) + (size_t)LC * (LC + 1) + (size_t)LC * LC) * sizeof(double);
@@ -625,26 +626,26 @@ static void launch_jacobi_t(hipStream_t s, const JacobiPlan &pl, double *Sg, lon
     const bool sj = Vx != nullptr && p <= JW_MAXP;
     const void *fn = sj ? (const void *)k_jacobi4<KL, LC, NT, true> : (const void *)k_jacobi4<KL, LC, NT, false>;
     TTG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
-    void *args[] = {&k, &p, &Sg, &lds, &Jg, &ldj, &tol, &stop_cos2, &ms, &sweeps_dev, &bar, &err, &dbg, &Vx};
+    void *args[] = {&k, &p, &Sg, &lds, &Jg, &ldj, &tol, &stop_cos2, &ms, &sweeps_dev, &bar, &err, &dbg, &Vx, &skip};
     TTG_HIP(hipLaunchCooperativeKernel(fn, dim3(sj ? 2 * p : p), dim3(NT), args, lsz, s));
   } else {
     TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi4<KL, LC, NT, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
     hipLaunchKernelGGL((k_jacobi4<KL, LC, NT, false>), dim3(1), dim3(NT), lsz, s, k, p, Sg, lds, Jg, ldj, tol,
-                       stop_cos2, ms, sweeps_dev, bar, err, dbg, Vx);
+                       stop_cos2, ms, sweeps_dev, bar, err, dbg, Vx, skip);
     TTG_HIP(hipGetLastError());
   }
 }
 
 static void launch_jacobi(hipStream_t s, const JacobiPlan &pl, double *Sg, long lds, double *Jg, long ldj,
                           double tol, double stop_cos2, int max_sweeps, int *sweeps_dev, unsigned *bar,
-                          int *err, unsigned long long *dbg, double *Vx) {
+                          int *err, unsigned long long *dbg, double *Vx, const int *skip) {
   TTG_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * jacobi_bar_words(max_sweeps), s));
   hipLaunchKernelGGL(k_fro2, dim3(1), dim3(1024), 0, s, (int)pl.k, Sg, lds,
                      reinterpret_cast<double *>(bar + jacobi_fro_offset(max_sweeps)));
   TTG_HIP(hipGetLastError());
   const int64_t k = pl.k;
   const int lc = 2 * pl.b;
-#define TTG_JAC(KL, LC, NT) launch_jacobi_t<KL, LC, NT>(s, pl, Sg, lds, Jg, ldj, tol, stop_cos2, max_sweeps, sweeps_dev, bar, err, dbg, Vx)
+#define TTG_JAC(KL, LC, NT) launch_jacobi_t<KL, LC, NT>(s, pl, Sg, lds, Jg, ldj, tol, stop_cos2, max_sweeps, sweeps_dev, bar, err, dbg, Vx, skip)
   if (k <= 16) {
     TTG_CHECK(lc == 16, TTG_BAD_ARG, "unsupported Jacobi block size");
     TTG_JAC(4, 16, 256);
@@ -675,7 +676,7 @@ size_t jacobi_polar_ws_elems(const JacobiPlan &pl) {
 
 void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool transS, double *Y,
                 int *sweeps_dev, unsigned *barrier_dev, int *err_dev, int max_sweeps, double *ws,
-                unsigned long long *dbg, double stop_cos2) {
+                unsigned long long *dbg, double stop_cos2, const int *skip) {
   const int k = (int)pl.k, kpad = (int)pl.kpad;
   const long total = (long)pl.rows * kpad;
   const int blocks = (int)std::min<long>((total + 255) / 256, 2048);
@@ -690,7 +691,7 @@ void jacobi_svd(hipStream_t s, const JacobiPlan &pl, const double *S, bool trans
   // exchange of V: 2 parities x p workgroups x (2b)^2 doubles
   double *Vx = (2 * (size_t)pl.p * (2 * pl.b) * (2 * pl.b) <= jacobi_polar_ws_elems(pl)) ? ws : nullptr;
   launch_jacobi(s, pl, Y, (long)pl.rows, Y + k, (long)pl.rows, tol, stop_cos2, max_sweeps, sweeps_dev,
-                barrier_dev, err_dev, dbg, Vx);
+                barrier_dev, err_dev, dbg, Vx, skip);
   // polar correction of J (and of S J with it)
   double *M = ws, *Y2 = ws + (size_t)kpad * kpad;
   gemm<double>(s, true, false, kpad, kpad, k, 1.0, Y + k, pl.rows, Y + k, pl.rows, 0.0, M, kpad,

@@ -599,9 +599,20 @@ void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, d
 __global__ __launch_bounds__(1024) void k_sort_chop(const double *__restrict__ Y, long ldy, int k,
                                                     int kpad, double ep, long rmax,
                                                     double *__restrict__ sv, int *__restrict__ perm,
-                                                    int *out_i, double *out_d) {
+                                                    int *out_i, double *out_d, const int *skip) {
   extern __shared__ __align__(16) double sig[];  // kpad
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  if (skip && *skip) {  // no truncation proven: keep all k columns in order
+    for (int j = tid; j < k; j += blockDim.x) {
+      perm[j] = j;
+      sv[j] = 0.0;
+    }
+    if (tid == 0) {
+      out_i[0] = k;
+      out_d[0] = 0.0;
+    }
+    return;
+  }
   for (int j = wv; j < kpad; j += nw) {
     const double *col = Y + (long)j * ldy;
     double s = 0.0;
@@ -650,9 +661,27 @@ __global__ __launch_bounds__(1024) void k_sort_chop(const double *__restrict__ Y
   }
 }
 void sort_chop(hipStream_t s, const double *Y, int64_t ldy, int64_t k, int64_t kpad, double ep,
-               int64_t rmax, double *sv, int *perm, int *out_i, double *out_d) {
+               int64_t rmax, double *sv, int *perm, int *out_i, double *out_d, const int *skip) {
   hipLaunchKernelGGL(k_sort_chop, dim3(1), dim3(1024), (size_t)kpad * sizeof(double), s, Y,
-                     (long)ldy, (int)k, (int)kpad, ep, (long)rmax, sv, perm, out_i, out_d);
+                     (long)ldy, (int)k, (int)kpad, ep, (long)rmax, sv, perm, out_i, out_d, skip);
+  TTG_HIP(hipGetLastError());
+}
+
+// No-truncation test of a bond: with W^T = Q R (k x k R), sigma_min(W) >=
+// 1/||R^{-1}||_F, and my_chop2(s, ep ||s||) keeps all k values whenever
+// sigma_min^2 >= (ep ||s||)^2 (core/my_chop2.m:25-31: the first reversed
+// cumulative sum is sigma_min^2, compared with strict <).  skip = 1 if the
+// bound clears the threshold with a factor 2, rmax does not bind and R != 0.
+__global__ void k_notrunc_decide(const double *nrmR, const double *nrmX, double ep, long k, long rmax,
+                                 int *skip) {
+  if (threadIdx.x == 0) {
+    const double a = nrmR[0], x = nrmX[0];
+    skip[0] = (k <= rmax && a > 0.0 && x > 0.0 && isfinite(x) && 1.0 / x >= 2.0 * ep * a) ? 1 : 0;
+  }
+}
+void notrunc_decide(hipStream_t s, const double *nrmR, const double *nrmX, double ep, int64_t k,
+                    int64_t rmax, int *skip) {
+  hipLaunchKernelGGL(k_notrunc_decide, dim3(1), dim3(64), 0, s, nrmR, nrmX, ep, (long)k, (long)rmax, skip);
   TTG_HIP(hipGetLastError());
 }
 

@@ -37,12 +37,13 @@ typedef double v4d_p __attribute__((ext_vector_type(4)));
 constexpr int PMAX_D = 64;        // tensors with more cores use the stream path
 constexpr int PMAX_K = 512;       // largest bond rank
 
-// Block-wide sum with a fixed combination order (deterministic).
+// Block-wide sum with a fixed combination order (deterministic).  `red` is
+// used by this function only; its trailing barrier makes the next call's
+// writes safe, so no leading barrier is needed.
 template <int NT>
 __device__ double p_block_sum(double v, double *red) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   v = wave_sum(v);
-  __syncthreads();
   if (lane == 0) red[wv] = v;
   __syncthreads();
   double s = 0.0;
@@ -689,12 +690,23 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
           __syncthreads();
           break;
         }
-        // Wa holds core i as (m x k); Q -> working slot i (m x kk), R -> Ra (kk x k)
-        if (k <= PH) p_householder<NT>(Wa, m, k, tau, xout + wofs[i], red, nullptr);
-        else p_householder_blocked<NT>(Wa, m, k, tau, xout + wofs[i], red, Vs, Wg, Wg2, Tall, ylds);
+        // Wa holds core i as (m x k); Q -> working slot i (m x kk), R -> Ra (kk x k).
+        // Small bonds are factorised in LDS (each column step is a chain of
+        // dependent reductions, so memory latency dominates).
+        const double *Af = Wa;
+        if (k <= 64 && m * (k + kk) <= ycap) {
+          p_copy<NT>(ylds, Wa, m * k);
+          p_householder<NT>(ylds, m, k, tau, ylds + m * k, red, nullptr);
+          p_copy<NT>(xout + wofs[i], ylds + m * k, m * kk);
+          Af = ylds;
+        } else if (k <= 64) {
+          p_householder<NT>(Wa, m, k, tau, xout + wofs[i], red, nullptr);
+        } else {
+          p_householder_blocked<NT>(Wa, m, k, tau, xout + wofs[i], red, Vs, Wg, Wg2, Tall, ylds);
+        }
         for (long idx = threadIdx.x; idx < kk * k; idx += NT) {
           const long c = idx / kk, r = idx - c * kk;
-          Ra[idx] = (r <= c) ? Wa[r + c * m] : 0.0;
+          Ra[idx] = (r <= c) ? Af[r + c * m] : 0.0;
         }
         __syncthreads();
         double s = 0.0;
@@ -733,8 +745,24 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
         // tall orientation B (mt x kj) in Wa, QR: R in Wa's upper part, Q -> Wq
         if (wide) p_transpose<NT>(Wa, Wc, kw, N);
         else p_copy<NT>(Wa, Wc, kw * N);
-        if (kj <= PH) p_householder<NT>(Wa, mt, kj, tau, Wq, red, nullptr);
-        else p_householder_blocked<NT>(Wa, mt, kj, tau, Wq, red, Vs, Wg, Wg2, Tall, ylds);
+        // R (kj x kj) -> Ra, in LDS for small bonds as in sweep 1
+        if (kj <= 64 && mt * 2 * kj <= ycap) {
+          p_copy<NT>(ylds, Wa, mt * kj);
+          p_householder<NT>(ylds, mt, kj, tau, ylds + mt * kj, red, nullptr);
+          p_copy<NT>(Wq, ylds + mt * kj, mt * kj);
+          for (long idx = threadIdx.x; idx < kj * kj; idx += NT) {
+            const long c = idx / kj, r = idx - c * kj;
+            Ra[idx] = (r <= c) ? ylds[r + c * mt] : 0.0;
+          }
+        } else {
+          if (kj <= 64) p_householder<NT>(Wa, mt, kj, tau, Wq, red, nullptr);
+          else p_householder_blocked<NT>(Wa, mt, kj, tau, Wq, red, Vs, Wg, Wg2, Tall, ylds);
+          for (long idx = threadIdx.x; idx < kj * kj; idx += NT) {
+            const long c = idx / kj, r = idx - c * kj;
+            Ra[idx] = (r <= c) ? Wa[r + c * mt] : 0.0;
+          }
+        }
+        __syncthreads();
         // Y = [S; I] with S = R^T (wide) or R (tall): in LDS when it fits,
         // else in the slot, rotated block pair by block pair through LDS
         const long ldy = 2 * kj;
@@ -747,7 +775,7 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
           double v;
           if (r < kj) {
             const long rr = wide ? c : r, cc = wide ? r : c;  // S(r, c) = R(c, r) when wide
-            v = (rr <= cc) ? Wa[rr + cc * mt] : 0.0;
+            v = Ra[rr + cc * kj];
           } else {
             v = (r - kj == c) ? 1.0 : 0.0;
           }

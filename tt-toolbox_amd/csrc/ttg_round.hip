@@ -20,6 +20,7 @@
 // a rejected bond and everything after it is recomputed).  The second sweep
 // synchronises once per bond to learn r1, which sizes the next launches.
 #include "ttg_internal.h"
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <functional>
@@ -63,7 +64,7 @@ template <class T> struct Rounder {
   T *buf[3];
   std::vector<T *> cp;
   // scratch
-  double *G, *R1, *R2, *R3, *Rinv, *Rprod, *Rtmp, *Y, *sv, *nrm, *stats, *bstats, *Yws;
+  double *G, *R1, *R2, *R3, *R4, *Rinv, *Rprod, *Rtmp, *Y, *sv, *nrm, *stats, *bstats, *Yws;
   T *RT, *Q1, *Q2, *Tt, *Qs, *US, *gws;
   int64_t gws_elems;
   double *hA, *hQ, *htau, *hws;
@@ -75,7 +76,8 @@ template <class T> struct Rounder {
   // optional: dst (kk x n_i r_{i+1}) = Rm (kk x r_i) * core_i for generated
   // cores (fused A*x: the product core is built on chip, never stored)
   const std::function<void(int64_t, const T *, int64_t, T *)> *fmul = nullptr;
-  int64_t fallbacks = 0, sweeps_total = 0, sweeps_max = 0, retries = 0;
+  int64_t fallbacks = 0, sweeps_total = 0, sweeps_max = 0, retries = 0, svd_skipped = 0;
+  bool allow_notrunc = true;
   int next_level = 2;  // QR level after the last rejected svd_bond
   bool noconv = false;
 
@@ -94,7 +96,7 @@ template <class T> struct Rounder {
 
   template <class A> static void layout(A &a, int64_t d, int64_t total, int64_t max_core, int64_t kmax, int64_t rsq) {
     for (int b = 0; b < 3; ++b) a.template take<T>(total);
-    for (int q = 0; q < 7; ++q) a.template take<double>(rsq);
+    for (int q = 0; q < 8; ++q) a.template take<double>(rsq);
     a.template take<double>(jac_elems(kmax));
     a.template take<double>(jac_elems(kmax) + (kmax + 64) * (kmax + 64));  // polar correction
     a.template take<double>(kmax + 64);      // sv
@@ -126,6 +128,7 @@ template <class T> struct Rounder {
     Rinv = a.take<double>(rsq);
     Rprod = a.take<double>(rsq);
     Rtmp = a.take<double>(rsq);
+    R4 = a.take<double>(rsq);
     Y = a.take<double>(jac_elems(kmax));
     Yws = a.take<double>(jac_elems(kmax) + (kmax + 64) * (kmax + 64));
     sv = a.take<double>(kmax + 64);
@@ -189,24 +192,37 @@ template <class T> struct Rounder {
   // Tall QR (m >= k): Qout (m x k, T), Rout (k x k, FP64), by CholeskyQR2.
   // flags[0..1] and stt[0..3] feed the acceptance test (qr_accept), checked
   // by the caller.  QR level 0: the second pass by the near-identity series
-  // (nearid_pass).  FP64 Grams: R2 = I + Phi(E), Q = Q1 (I - Phi(E)),
+  // (nearid_pass): R2 = I + Phi(E), Q = Q1 (I - Phi(E)).  FP64 Grams:
   // accepted if max|E| < 1e-8 (both truncations O(1e-16)).  The mixed form's
   // sweep-1 bonds (g32): Gram products in FP32 MFMA (their error only
-  // perturbs the orthogonality of Q; every Q = A R^{-1} step is FP64, so
-  // A = Q R holds to FP64 accuracy), inverse series to U^2, accepted if
-  // max|E| < 1e-5 (Q orthogonal to 1e-10, Q R2 = Q1 to 1e-15).  Sweep-2
+  // perturbs the orthogonality of Q; every Q = A R^{-1} step is FP64),
+  // accepted if max|E| < 1e-5 (Q orthogonal and Q R2 = Q1 to 1e-10, inside
+  // the form's 1e-6 budget).  Sweep-2
   // unfoldings carry the bond's singular-value spread, beyond that bound with
   // FP32 Grams, so they use FP64 Grams in both forms.  Level 1: FP64 Grams,
   // Cholesky + substitution second pass (accepted if its R is within 0.1 of
   // I).  Level 2 (caller): Householder.
   bool nearid_level(int level) const { return level == 0; }
+  // bound (level 0 only): also start, on the side stream, ||R1||_F and
+  // ||R1^{-1}||_F for the no-truncation test (R = R2 R1 with ||R2 - I|| of
+  // order k max|E| <= 5e-6, so the bounds carry over with the test's factor
+  // 2 margin); joined by the caller through ctx->ev_join.
   void qr_tall_fast(const T *A, int64_t m, int64_t k, T *Qout, double *Rout, int *flags,
-                    double *stt, int level, bool allow_g32) {
+                    double *stt, int level, bool allow_g32, bool bound = false) {
     const bool g32 = mixed && level == 0 && allow_g32;
     cholqr_pass(A, m, k, 0.0, Q1, R1, flags + 0, stt + 0, g32);
+    if (bound && nearid_level(level)) {
+      TTG_HIP(hipEventRecord(ctx->ev_fork, st));
+      TTG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+      eye_plus(ctx->side, k, R4, nullptr, 0.0);
+      trsm_right_upper<double>(ctx->side, k, k, R4, k, R1, Rtmp, k);  // Rtmp = R1^{-1}
+      fro_norm<double>(ctx->side, R1, k * k, scal_d + 0);
+      fro_norm<double>(ctx->side, Rtmp, k * k, scal_d + 1);
+      TTG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+    }
     if (nearid_level(level)) {
       gram(Q1, m, k, G, g32);
-      nearid_pass(st, m, k, G, Q1, Qout, R2, R3, Rtmp, Rinv, stt + 3, g32 ? 2 : 1, gws, gws_elems, cus);
+      nearid_pass(st, m, k, G, Q1, Qout, R2, R3, nullptr, Rinv, stt + 3, 1, gws, gws_elems, cus);
     } else {
       cholqr_pass(Q1, m, k, 0.0, Qout, R2, flags + 1, stt + 2, g32);
     }
@@ -283,7 +299,7 @@ template <class T> struct Rounder {
 
   // ---- sweep 2 -----------------------------------------------------------
   void svd_bond(int64_t i, double ep, int64_t rmax_i, int level, int64_t &r1,
-                bool &qr_ok, int64_t &kj, bool &wide) {
+                bool &qr_ok, int64_t &kj, bool &wide, bool &skipped) {
     const int64_t kw = r[i], N = n[i] * r[i + 1];
     wide = (N >= kw);
     const T *W = cp[i];
@@ -297,12 +313,45 @@ template <class T> struct Rounder {
       tallA = Tt;
     }
     TTG_HIP(hipMemsetAsync(fl, 0, 4 * sizeof(int), st));
+    const bool bound = wide && allow_notrunc && level == 0;
     if (level < 2) {
-      qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats, level, false);
+      qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats, level, false, bound);
     } else {
       qr_householder(tallA, mt, kj, Qs, R);
     }
     JacobiPlan pl = jacobi_plan(kj);
+    int h_fl[4] = {0, 0, 0, 0};
+    double h_st[6] = {0, 0, 0, 0, 0, 0};
+    skipped = false;
+    // No-truncation shortcut (wide bonds): when 1/||R^{-1}||_F, a lower bound
+    // of the smallest singular value, clears my_chop2's threshold, every
+    // singular value is kept and the SVD would only change the gauge of the
+    // bond, so it is completed from its QR factors, W = R^T Q^T:
+    // core_i = Q^T (orthonormal rows, as V' would have), core_{i-1} <-
+    // core_{i-1} R^T.  Same tensor, same ranks.  The decision and the QR
+    // acceptance are read in one synchronisation; a rejected QR is redone
+    // before any SVD work.
+    if (bound) {
+      int *skip = scal_i + 4;
+      TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+      notrunc_decide(st, scal_d + 0, scal_d + 1, ep, kj, rmax_i, skip);
+      int h_skip = 0;
+      TTG_HIP(hipMemcpyAsync(&h_skip, skip, sizeof(int), hipMemcpyDeviceToHost, st));
+      TTG_HIP(hipMemcpyAsync(h_fl, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+      TTG_HIP(hipMemcpyAsync(h_st, stats, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+      TTG_HIP(hipStreamSynchronize(st));
+      qr_ok = level == 2 || qr_accept(h_fl, h_st, level, false);
+      if (!qr_ok) {
+        next_level = qr_next(h_fl, h_st, level, false);
+        return;
+      }
+      if (h_skip) {
+        skipped = true;
+        ++svd_skipped;
+        r1 = kj;
+        return;
+      }
+    }
     // stopping cosine^2: plain form 1e-16 (the last sweep's rotations leave
     // O(1e-16) behind); mixed form 1e-8 (O(1e-8) left, far below its 1e-6
     // error budget, one sweep fewer)
@@ -311,8 +360,6 @@ template <class T> struct Rounder {
     sort_chop(st, Y, pl.rows, kj, pl.kpad, ep, rmax_i, sv, perm, scal_i, scal_d + 8);
     // one synchronisation: r1, Jacobi status, QR acceptance data
     int h_i[4] = {0, 0, 0, 0};
-    int h_fl[4] = {0, 0, 0, 0};
-    double h_st[6] = {0, 0, 0, 0, 0, 0};
     TTG_HIP(hipMemcpyAsync(h_i, scal_i, sizeof(int), hipMemcpyDeviceToHost, st));
     TTG_HIP(hipMemcpyAsync(h_i + 1, jerr, sizeof(int), hipMemcpyDeviceToHost, st));
     TTG_HIP(hipMemcpyAsync(h_i + 2, jsweeps, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -333,20 +380,24 @@ template <class T> struct Rounder {
 
   void sweep2_bond(int64_t i, double ep, int64_t rmax_i) {
     int64_t r1, kj;
-    bool ok, wide;
+    bool ok, wide, skipped;
     // escalate on rejection: FP64 CholeskyQR2 with a Cholesky second pass,
     // then Householder
     int level = 0;
-    svd_bond(i, ep, rmax_i, level, r1, ok, kj, wide);
+    svd_bond(i, ep, rmax_i, level, r1, ok, kj, wide, skipped);
     while (!ok) {
       level = next_level;
       ++retries;
-      svd_bond(i, ep, rmax_i, level, r1, ok, kj, wide);
+      svd_bond(i, ep, rmax_i, level, r1, ok, kj, wide, skipped);
     }
     const int64_t kw = r[i], N = n[i] * r[i + 1];
     JacobiPlan pl = jacobi_plan(kj);
     T *core_i = other(i);
-    if (wide) {
+    if (wide && skipped) {
+      // no truncation: U S = R^T, core_i = Qs^T
+      transpose<T>(st, Rprod, kw, kw, US);
+      transpose<T>(st, Qs, N, kw, core_i);
+    } else if (wide) {
       // W^T = Qs R, R^T J = Y:  U S = Y[0:kw, perm],  V = Qs J[:, perm]
       gather_cols<T>(st, Y, pl.rows, 0, kw, perm, r1, US, kw);
       gather_cols<T>(st, Y, pl.rows, kw, kw, perm, r1, Q1, kw);  // J(:, perm(1:r1))
@@ -455,6 +506,15 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
   R.r.assign(r, r + d + 1);
   R.mixed = mixed;
   R.fmul = fmul;
+  if (!ctx->side) {
+    TTG_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    TTG_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    TTG_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+  }
+  {
+    const char *env = std::getenv("TTG_SVD_ALWAYS");
+    R.allow_notrunc = !(env && env[0] == '1');
+  }
   std::vector<int64_t> sizes(d);
   R.ofs.assign(d + 1, 0);
   int64_t max_core = 1, kmax = 1;
@@ -532,6 +592,7 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
     info->jacobi_sweeps = R.sweeps_total;
     info->jacobi_maxsweeps = R.sweeps_max;
     info->qr_retries = R.retries;
+    info->svd_skipped = R.svd_skipped;
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return R.noconv ? TTG_NO_CONVERGENCE : TTG_OK;

@@ -56,6 +56,7 @@ class TTGInfo(C.Structure):
         ("jacobi_maxsweeps", C.c_int64),
         ("seconds", C.c_double),
         ("qr_retries", C.c_int64),
+        ("svd_skipped", C.c_int64),
     ]
 
     def as_dict(self):
