@@ -67,7 +67,11 @@ constexpr int GBM = 64, GBN = 64, GBK = 16;
 // C(MxN) = alpha*op(A)*op(B) + beta*C.  Block 64x64, 4 waves (2x2), each wave
 // a 32x32 sub-tile made of 2x2 MFMA 16x16x4 tiles.  Operand layout of
 // v_mfma_*_16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the
-// accumulator element v of lane l is D[MF<T>::row(l, v)][l%16].
+// accumulator element v of lane l is D[MF<T>::row(l, v)][l%16].  The waves
+// form the transposed tiles (op(B)^T op(A)^T), so that lane l holds rows
+// l%16 of C and 16 lanes store 16 consecutive rows (coalesced).  The next
+// K tile is loaded into registers while the current one is multiplied from
+// LDS.
 // Split-K: blockIdx.z selects K range [z*kchunk, ...); with gridDim.z > 1 the
 // raw partial product goes to P + z*M*N (ld M) and k_splitk_reduce finishes.
 // TI is the storage type of A and B, T the MFMA type: TI = double, T = float
@@ -91,21 +95,21 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = V{0, 0, 0, 0};
-
-  for (int k0 = kbeg; k0 < kend; k0 += GBK) {
+  T ra[4], rb[4];
+  auto fetch = [&](int k0) {
     if (!TA) {
       const int mm = tid & 63, kk = tid >> 6;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int k = kk + 4 * i, gm = m0 + mm, gk = k0 + k;
-        As[k][mm] = (gm < M && gk < kend) ? T(A[gm + (long)gk * lda]) : T(0);
+        ra[i] = (gm < M && gk < kend) ? T(A[gm + (long)gk * lda]) : T(0);
       }
     } else {
       const int kk = tid & 15, mm = tid >> 4;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int m = mm + 16 * i, gm = m0 + m, gk = k0 + kk;
-        As[kk][m] = (gm < M && gk < kend) ? T(A[gk + (long)gm * lda]) : T(0);
+        ra[i] = (gm < M && gk < kend) ? T(A[gk + (long)gm * lda]) : T(0);
       }
     }
     if (!TB) {
@@ -113,17 +117,43 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int n = nn + 16 * i, gn = n0 + n, gk = k0 + kk;
-        Bs[kk][n] = (gn < N && gk < kend) ? T(B[gk + (long)gn * ldb]) : T(0);
+        rb[i] = (gn < N && gk < kend) ? T(B[gk + (long)gn * ldb]) : T(0);
       }
     } else {
       const int nn = tid & 63, kk = tid >> 6;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int k = kk + 4 * i, gn = n0 + nn, gk = k0 + k;
-        Bs[k][nn] = (gn < N && gk < kend) ? T(B[gn + (long)gk * ldb]) : T(0);
+        rb[i] = (gn < N && gk < kend) ? T(B[gn + (long)gk * ldb]) : T(0);
       }
     }
+  };
+  auto stash = [&]() {
+    if (!TA) {
+      const int mm = tid & 63, kk = tid >> 6;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) As[kk + 4 * i][mm] = ra[i];
+    } else {
+      const int kk = tid & 15, mm = tid >> 4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) As[kk][mm + 16 * i] = ra[i];
+    }
+    if (!TB) {
+      const int kk = tid & 15, nn = tid >> 4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Bs[kk][nn + 16 * i] = rb[i];
+    } else {
+      const int nn = tid & 63, kk = tid >> 6;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Bs[kk + 4 * i][nn] = rb[i];
+    }
+  };
+  if (kbeg < kend) fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += GBK) {
     __syncthreads();
+    stash();
+    __syncthreads();
+    if (k0 + GBK < kend) fetch(k0 + GBK);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
       T a[2], b[2];
@@ -134,19 +164,19 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = MF<T>::op(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < 2; ++j) acc[i][j] = MF<T>::op(b[j], a[i], acc[i][j]);
     }
-    __syncthreads();
   }
   const bool split = gridDim.z > 1;
+  // acc[i][j][v] of lane l: C(m0 + wm*32 + i*16 + l%16, n0 + wn*32 + j*16 + row(l, v))
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int gm = m0 + wm * 32 + i * 16 + MF<T>::row(lane, v);
-        const int gn = n0 + wn * 32 + j * 16 + (lane & 15);
+        const int gm = m0 + wm * 32 + i * 16 + (lane & 15);
+        const int gn = n0 + wn * 32 + j * 16 + MF<T>::row(lane, v);
         if (gm < M && gn < N) {
           if (split) {
             P[(long)blockIdx.z * M * N + gm + (long)gn * M] = acc[i][j][v];
