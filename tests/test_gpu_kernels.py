@@ -49,33 +49,59 @@ def test_householder(gpu_ctx, m, n, rank):
     np.testing.assert_allclose(np.abs(np.diag(R)[:rank]), np.abs(np.diag(r2)[:rank]), rtol=1e-10)
 
 
-def _chol(gpu_ctx, G, shift=0.0):
+def _chol(gpu_ctx, G, shift=0.0, multi=0):
     k = G.shape[0]
     G = np.asfortranarray(G)
     R = np.zeros((k, k), order="F")
     flag = np.zeros(1, dtype=np.int32)
     stats = np.zeros(2)
-    gpu_ctx.check(gpu_ctx.lib.ttg_test_chol(gpu_ctx.h, k, f64p(G), shift, f64p(R), f64p(flag), f64p(stats)))
+    gpu_ctx.check(gpu_ctx.lib.ttg_test_chol(gpu_ctx.h, k, f64p(G), shift, f64p(R), f64p(flag), f64p(stats), multi))
     return R, int(flag[0]), stats
 
 
-@pytest.mark.parametrize("k", [5, 40, 100, 256, 320])
-def test_cholesky(gpu_ctx, k):
+@pytest.mark.parametrize("multi", [0, 1])
+@pytest.mark.parametrize("k", [5, 40, 65, 100, 256, 320, 512])
+def test_cholesky(gpu_ctx, k, multi):
+    """Single- and multi-workgroup factorisations (the latter for k > 64)
+    against LAPACK, with the diagonal-spread and max|R - I| statistics."""
     rng = np.random.default_rng(k)
     A = rng.standard_normal((3 * k, k))
     G = A.T @ A
-    R, flag, stats = _chol(gpu_ctx, G)
+    R, flag, stats = _chol(gpu_ctx, G, multi=multi)
     assert flag == 0
     ref = np.linalg.cholesky(G).T
     np.testing.assert_allclose(R, ref, rtol=1e-10, atol=1e-10 * np.max(np.abs(ref)))
     assert np.max(np.abs(np.tril(R, -1))) == 0.0
     np.testing.assert_allclose(stats[0], np.max(np.abs(np.diag(ref))) / np.min(np.abs(np.diag(ref))), rtol=1e-8)
+    np.testing.assert_allclose(stats[1], np.max(np.abs(ref - np.eye(k))), rtol=1e-8)
 
 
-def test_cholesky_flags_indefinite(gpu_ctx):
+@pytest.mark.parametrize("multi", [0, 1])
+def test_cholesky_multi_matches_single_bitwise(gpu_ctx, multi):
+    """Same factor from one and from eight workgroups (same operations in the
+    same order), including a near-identity Gram as in the second QR pass."""
+    rng = np.random.default_rng(3)
+    k = 200
+    E = rng.standard_normal((k, k)) * 1e-9
+    G = np.eye(k) + E + E.T
+    R0, f0, s0 = _chol(gpu_ctx, G, multi=0)
+    R1, f1, s1 = _chol(gpu_ctx, G, multi=multi)
+    assert f0 == f1 == 0
+    np.testing.assert_array_equal(R0, R1)
+    np.testing.assert_array_equal(s0, s1)
+
+
+@pytest.mark.parametrize("multi", [0, 1])
+def test_cholesky_flags_indefinite(gpu_ctx, multi):
     G = np.diag([4.0, 1.0, -1.0, 2.0, 1.0, 1.0])
-    _, flag, _ = _chol(gpu_ctx, G)
+    _, flag, _ = _chol(gpu_ctx, G, multi=multi)
     assert flag == 3
+    # a failing pivot in the fourth block of a multi-workgroup factorisation
+    d = np.ones(150)
+    d[100] = -2.0
+    _, flag, stats = _chol(gpu_ctx, np.diag(d), multi=multi)
+    assert flag == 101
+    assert np.isinf(stats[0]) and np.isinf(stats[1])
 
 
 @pytest.mark.parametrize("prec,m,k,logcond", [(0, 2048, 256, 8), (0, 300, 40, 12), (1, 2048, 256, 3), (0, 1000, 320, 6), (0, 50, 7, 10)])

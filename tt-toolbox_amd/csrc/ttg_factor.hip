@@ -12,6 +12,7 @@
 #include "ttg_device.h"
 #include <cmath>
 #include <algorithm>
+#include <type_traits>
 
 namespace ttg {
 
@@ -244,17 +245,472 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__r
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_chol4: k_chol3 with the panel solve fused into the pivot loop.  Per
+// 32-column block the diagonal block and the panel rows are staged in LDS
+// together; wave 0 factorises the diagonal block one pivot at a time and,
+// after each workgroup barrier, waves 1..7 apply that pivot (row c of R and
+// 1 / r_cc) to their panel columns held in registers while wave 0 already
+// computes the next pivot.  The panel's 32-step substitution chain thus runs
+// beside the diagonal factorisation instead of after it.  Same operations in
+// the same order per column as k_chol3: identical factors.
+// One panel column per thread of waves 1..7: panels up to 448 columns (k <= 480).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__restrict__ G, long ldg,
+                                                        double shift_rel, double *__restrict__ R,
+                                                        int *flag, double *stats,
+                                                        unsigned long long *dbg) {
+  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
+  double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
+  double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
+  __shared__ double red[CHOL_THREADS / 64];
+  __shared__ double dinv[CB];
+  __shared__ int sh_fail;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int NW = CHOL_THREADS / 64;
+  double tr = 0.0;
+  for (int j = tid; j < k; j += CHOL_THREADS) tr += G[j + (long)j * ldg];
+  tr = block_sum(tr, red);
+  const double shift = shift_rel * tr / (k > 0 ? k : 1);
+  for (int j = wv; j < k; j += NW)
+    for (int i = lane; i < k; i += 64)
+      R[i + (long)j * k] = (i <= j) ? G[i + (long)j * ldg] + (i == j ? shift : 0.0) : 0.0;
+  if (tid == 0) sh_fail = 0;
+  double dmax = 0.0, dmin = INFINITY, dev = 0.0;
+  for (int J0 = 0; J0 < k; J0 += CB) {
+    const int jb = min(CB, k - J0);
+    const int w = k - J0 - jb;  // > 0 only when jb == CB
+    const int wpad = (w + 15) & ~15, wps = wpad + 1;
+    __syncthreads();
+    if (dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    // diagonal block column-major in Dl (identity-padded past jb); panel rows in Ps
+    for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
+      const int i = idx & (CB - 1), j = idx >> 5;
+      Dl[idx] = (i < jb && j < jb) ? (i <= j ? R[(J0 + i) + (long)(J0 + j) * k] : 0.0) : (i == j ? 1.0 : 0.0);
+    }
+    for (int idx = tid; idx < CB * wpad; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), c = idx >> 5;
+      Ps[t * wps + c] = (c < w) ? R[(J0 + t) + (long)(J0 + jb + c) * k] : 0.0;
+    }
+    __syncthreads();
+    const int j = lane & (CB - 1);
+    const int pt = tid - 64;
+    double v[CB];  // wave 0: column j of the diagonal block; waves 1..7: panel column pt
+    if (wv == 0) {
+#pragma unroll
+      for (int i = 0; i < CB; ++i) v[i] = Dl[j * CB + i];
+    } else if (w > 0) {
+      const int c1 = pt < wpad ? pt : 0;  // threads past the panel repeat column 0 and store nothing
+#pragma unroll
+      for (int t = 0; t < CB; ++t) v[t] = Ps[t * wps + c1];
+    }
+    unsigned failmask = 0;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      if (wv == 0) {
+        double p = readlane_d(v[c], c);
+        // straight-line code only (branches here push v[] into scratch): a
+        // failed pivot is recorded in a bit mask and replaced by a harmless 1
+        const bool bad = !(p > 0.0) || !isfinite(p);
+        failmask |= bad ? (1u << c) : 0u;
+        p = bad ? 1.0 : p;
+        // 1/sqrt(p) from the hardware estimate and two Newton steps
+        double inv = __builtin_amdgcn_rsq(p);
+        {
+          const double h = 0.5 * p;
+          double e = __builtin_fma(-h * inv, inv, 0.5);
+          inv = __builtin_fma(inv, e, inv);
+          e = __builtin_fma(-h * inv, inv, 0.5);
+          inv = __builtin_fma(inv, e, inv);
+        }
+        const double dc = p * inv;
+        const double rc = (j == c) ? dc : (j > c ? v[c] * inv : 0.0);
+        v[c] = rc;
+        // every lane holds the same pivot; lanes 32..63 mirror lanes 0..31
+        Dl[c * CB + j] = rc;
+        dinv[c] = inv;
+        wave_sync_lds();
+#pragma unroll
+        for (int i = c + 1; i < CB; ++i) v[i] -= Dl[c * CB + i] * rc;
+      }
+      __syncthreads();  // row c of R and 1 / r_cc visible to the panel waves
+      if (wv != 0 && w > 0) {
+        const double x = v[c] * dinv[c];
+        v[c] = x;
+#pragma unroll
+        for (int l = c + 1; l < CB; ++l) v[l] -= Dl[c * CB + l] * x;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (wv == 0 && lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
+    if (wv != 0 && w > 0) {
+      if (pt < wpad) {
+#pragma unroll
+        for (int t = 0; t < CB; ++t) Ps[t * wps + pt] = v[t];
+      }
+    }
+    __syncthreads();
+    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
+    if (sh_fail) break;
+    // diagonal block to R (+ statistics)
+    for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), l = idx >> 5;
+      if (t < jb && l < jb && t <= l) {
+        const double val = Dl[t * CB + l];
+        R[(J0 + t) + (long)(J0 + l) * k] = val;
+        if (t == l) {
+          dmax = fmax(dmax, fabs(val));
+          dmin = fmin(dmin, fabs(val));
+          dev = fmax(dev, fabs(val - 1.0));
+        } else {
+          dev = fmax(dev, fabs(val));
+        }
+      }
+    }
+    if (w <= 0) continue;
+    // solved panel to R (coalesced: 32 consecutive threads write one column)
+    for (int idx = tid; idx < CB * w; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), c = idx >> 5;
+      const double val = Ps[t * wps + c];
+      R[(J0 + t) + (long)(J0 + jb + c) * k] = val;
+      dev = fmax(dev, fabs(val));
+    }
+    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
+    // trailing SYRK on the upper 16x16 tiles (as in k_chol3)
+    const int nt = wpad >> 4;
+    const int ntiles = nt * (nt + 1) / 2;
+    for (int base = wv; base < ntiles; base += 4 * NW) {
+      int ti[4], tj[4];
+      bool ok[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int tile = base + q * NW;
+        ok[q] = tile < ntiles;
+        int cc = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+        while (cc * (cc + 1) / 2 > tile) --cc;
+        while ((cc + 1) * (cc + 2) / 2 <= tile) ++cc;
+        tj[q] = cc;
+        ti[q] = tile - cc * (cc + 1) / 2;
+      }
+      const int i_l = lane & 15, j_l = lane >> 4;
+      double old[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = ti[q] * 16 + i_l, jj = tj[q] * 16 + j_l + 4 * u;
+          old[q][u] = (ok[q] && i < w && jj < w && i <= jj) ? R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] : 0.0;
+        }
+      v4d_f acc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = v4d_f{0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < CB; kk += 4) {
+        const int t = kk + (lane >> 4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc[q] = MFf<double>::op(Ps[t * wps + tj[q] * 16 + i_l], Ps[t * wps + ti[q] * 16 + i_l], acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = ti[q] * 16 + i_l, jj = tj[q] * 16 + j_l + 4 * u;
+          if (ok[q] && i < w && jj < w && i <= jj) R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] = old[q][u] - acc[q][u];
+        }
+    }
+  }
+  __syncthreads();
+  if (dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
+  const int failed = sh_fail;
+  dmax = block_max(dmax, red);
+  dmin = -block_max(-dmin, red);
+  dev = block_max(dev, red);
+  if (tid == 0) {
+    flag[0] = failed;
+    stats[0] = failed ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
+    stats[1] = failed ? INFINITY : dev;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky on NB cooperating workgroups (same factor, flag and statistics as
+// k_chol3).  Per 32-column block: every workgroup factorises the diagonal
+// block itself (identical inputs and code, so identical results; no barrier
+// between the diagonal factor and the panel), solves its share of the panel
+// columns and writes them; grid barrier; every workgroup stages the whole
+// panel and updates its share of the trailing upper 16x16 tiles (FP64 MFMA);
+// grid barrier.  The diagonal factorisation stays sequential; the panel and
+// the SYRK, most of k_chol3's time, spread over NB CUs.
+// bar: 2 barrier words (zeroed by the caller) + 1 timeout flag.
+// ---------------------------------------------------------------------------
+__device__ bool chol_grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen) {
+  __shared__ int sh_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned target = gen + 1;
+    const unsigned prev = atomicAdd(&bar[0], 1u);
+    int ok = 1;
+    if (prev == nblocks - 1) {
+      atomicExch(&bar[0], 0u);
+      __threadfence();
+      atomicExch(&bar[1], target);
+    } else {
+      const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          ok = 0;
+          break;
+        }
+        if (__builtin_amdgcn_s_memrealtime() - t0 > 400000000ull) {  // 4 s
+          atomicExch(&bar[2], 1u);
+          ok = 0;
+          break;
+        }
+      }
+    }
+    __threadfence();
+    gen = target;
+    sh_ok = ok;
+  }
+  __syncthreads();
+  return sh_ok != 0;
+}
+
+template <int NB>
+__global__ __launch_bounds__(CHOL_THREADS) void k_chol_mc(int k, const double *__restrict__ G, long ldg,
+                                                          double shift_rel, double *__restrict__ R,
+                                                          int *flag, double *stats, unsigned *bar) {
+  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
+  double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
+  double *Ps = Dl + CB * CB;                      // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
+  __shared__ double red[CHOL_THREADS / 64];
+  __shared__ double dinv[CB];
+  __shared__ int sh_fail;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, b = blockIdx.x;
+  constexpr int NW = CHOL_THREADS / 64;
+  unsigned gen = 0;
+  double tr = 0.0;
+  for (int j = tid; j < k; j += CHOL_THREADS) tr += G[j + (long)j * ldg];
+  tr = block_sum(tr, red);
+  const double shift = shift_rel * tr / (k > 0 ? k : 1);
+  // R = upper(G) + shift I, columns split over the workgroups
+  for (int j = b * NW + wv; j < k; j += NB * NW)
+    for (int i = lane; i < k; i += 64)
+      R[i + (long)j * k] = (i <= j) ? G[i + (long)j * ldg] + (i == j ? shift : 0.0) : 0.0;
+  if (b == 0 && tid == 0) stats[1] = 0.0;
+  if (tid == 0) sh_fail = 0;
+  bool ok = chol_grid_sync(bar, NB, gen);
+  double dmax = 0.0, dmin = INFINITY, dev = 0.0;
+  for (int J0 = 0; J0 < k && ok; J0 += CB) {
+    const int jb = min(CB, k - J0);
+    // the last diagonal block has no panel: workgroup 0 alone finishes it
+    if (J0 + jb >= k && b != 0) break;
+    for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
+      const int i = idx & (CB - 1), j = idx >> 5;
+      Dl[idx] = (i < jb && j < jb) ? (i <= j ? R[(J0 + i) + (long)(J0 + j) * k] : 0.0) : (i == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (wv == 0) {
+      const int j = lane & (CB - 1);
+      double col[CB];
+#pragma unroll
+      for (int i = 0; i < CB; ++i) col[i] = Dl[j * CB + i];
+      wave_sync_lds();
+      unsigned failmask = 0;
+      double myinv = 1.0;
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        double p = readlane_d(col[c], c);
+        const bool bad = !(p > 0.0) || !isfinite(p);
+        failmask |= bad ? (1u << c) : 0u;
+        p = bad ? 1.0 : p;
+        double inv = __builtin_amdgcn_rsq(p);
+        {
+          const double h = 0.5 * p;
+          double e = __builtin_fma(-h * inv, inv, 0.5);
+          inv = __builtin_fma(inv, e, inv);
+          e = __builtin_fma(-h * inv, inv, 0.5);
+          inv = __builtin_fma(inv, e, inv);
+        }
+        const double dc = p * inv;
+        const double rc = (j == c) ? dc : (j > c ? col[c] * inv : 0.0);
+        col[c] = rc;
+        myinv = (j == c) ? inv : myinv;
+        Dl[c * CB + j] = rc;
+        wave_sync_lds();
+#pragma unroll
+        for (int i = c + 1; i < CB; ++i) col[i] -= Dl[c * CB + i] * rc;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      if (lane < CB) dinv[j] = myinv;
+      if (lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
+    }
+    __syncthreads();
+    if (sh_fail) break;  // every workgroup sees the same pivots
+    const int w = k - J0 - jb;
+    // workgroup 0 stores the factored diagonal block once no workgroup can
+    // still be reading its unfactored entries: after the panel barrier (the
+    // last block, which only workgroup 0 runs, at once)
+    auto store_diag = [&]() {
+      for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
+        const int t = idx & (CB - 1), l = idx >> 5;
+        if (t < jb && l < jb && t <= l) {
+          const double v = Dl[t * CB + l];
+          R[(J0 + t) + (long)(J0 + l) * k] = v;
+          if (t == l) {
+            dmax = fmax(dmax, fabs(v));
+            dmin = fmin(dmin, fabs(v));
+            dev = fmax(dev, fabs(v - 1.0));
+          } else {
+            dev = fmax(dev, fabs(v));
+          }
+        }
+      }
+    };
+    if (w <= 0) {
+      store_diag();
+      continue;
+    }
+    const int wpad = (w + 15) & ~15, wps = wpad + 1;
+    // this workgroup's panel columns [c0, c1), a multiple of 16 wide
+    const int chunk = ((wpad / 16 + NB - 1) / NB) * 16;
+    const int c0 = min(b * chunk, wpad), c1 = min(c0 + chunk, wpad);
+    for (int idx = tid; idx < CB * (c1 - c0); idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), c = c0 + (idx >> 5);
+      Ps[t * wps + c] = (t < jb && c < w) ? R[(J0 + t) + (long)(J0 + jb + c) * k] : 0.0;
+    }
+    __syncthreads();
+    for (int c = c0 + tid; c < c1; c += CHOL_THREADS) {
+      double bb[CB];
+#pragma unroll
+      for (int t = 0; t < CB; ++t) bb[t] = Ps[t * wps + c];
+#pragma unroll
+      for (int t = 0; t < CB; ++t) {
+        const double x = bb[t] * dinv[t];
+        bb[t] = x;
+#pragma unroll
+        for (int l = t + 1; l < CB; ++l) bb[l] -= Dl[t * CB + l] * x;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int t = 0; t < CB; ++t) {
+        const double v = (t < jb && c < w) ? bb[t] : 0.0;
+        Ps[t * wps + c] = v;
+        dev = fmax(dev, fabs(v));
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < CB * (c1 - c0); idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), c = c0 + (idx >> 5);
+      if (t < jb && c < w) R[(J0 + t) + (long)(J0 + jb + c) * k] = Ps[t * wps + c];
+    }
+    if (!chol_grid_sync(bar, NB, gen)) {
+      ok = false;
+      break;
+    }
+    if (b == 0) store_diag();
+    // the whole solved panel, then this workgroup's trailing tiles
+    for (int idx = tid; idx < CB * wpad; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), c = idx >> 5;
+      Ps[t * wps + c] = (t < jb && c < w) ? R[(J0 + t) + (long)(J0 + jb + c) * k] : 0.0;
+    }
+    __syncthreads();
+    const int nt = wpad >> 4;
+    const int ntiles = nt * (nt + 1) / 2;
+    for (int base = b * NW + wv; base < ntiles; base += 4 * NB * NW) {
+      int ti[4], tj[4];
+      bool okq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int tile = base + q * NB * NW;
+        okq[q] = tile < ntiles;
+        int c = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+        while (c * (c + 1) / 2 > tile) --c;
+        while ((c + 1) * (c + 2) / 2 <= tile) ++c;
+        tj[q] = c;
+        ti[q] = tile - c * (c + 1) / 2;
+      }
+      const int i_l = lane & 15, j_l = lane >> 4;
+      double old[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = ti[q] * 16 + i_l, jj = tj[q] * 16 + j_l + 4 * v;
+          old[q][v] = (okq[q] && i < w && jj < w && i <= jj) ? R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] : 0.0;
+        }
+      v4d_f acc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = v4d_f{0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < CB; kk += 4) {
+        const int t = kk + (lane >> 4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc[q] = MFf<double>::op(Ps[t * wps + tj[q] * 16 + i_l], Ps[t * wps + ti[q] * 16 + i_l], acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = ti[q] * 16 + i_l, jj = tj[q] * 16 + j_l + 4 * v;
+          if (okq[q] && i < w && jj < w && i <= jj) R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] = old[q][v] - acc[q][v];
+        }
+    }
+    if (!chol_grid_sync(bar, NB, gen)) {
+      ok = false;
+      break;
+    }
+  }
+  __syncthreads();
+  const int failed = ok ? sh_fail : -1;
+  dmax = block_max(dmax, red);
+  dmin = -block_max(-dmin, red);
+  dev = block_max(dev, red);
+  if (tid == 0 && dev > 0.0)  // non-negative doubles order like their bit patterns
+    atomicMax(reinterpret_cast<unsigned long long *>(&stats[1]), (unsigned long long)__double_as_longlong(dev));
+  if (b == 0 && tid == 0) {
+    flag[0] = failed;
+    stats[0] = failed ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
+    if (failed)
+      atomicMax(reinterpret_cast<unsigned long long *>(&stats[1]), (unsigned long long)__double_as_longlong(INFINITY));
+  }
+}
+
 constexpr int CHOL_MAXK = 512;
 
 void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats, unsigned long long *dbg) {
+                 double *R, int *flag, double *stats, unsigned long long *dbg, unsigned *bar) {
   TTG_CHECK(k >= 1 && k <= CHOL_MAXK, TTG_BAD_ARG, "Cholesky kernel supports 1 <= k <= 512");
   const int64_t wmax = std::max<int64_t>(k - CB, 0);
   const int64_t wps = ((wmax + 15) & ~(int64_t)15) + 1;
   const size_t lds = ((size_t)CB * CB + (size_t)CB * wps) * sizeof(double);
   ProfScope prof(KC_CHOL, s, (double)k * k * k / 3.0, 8.0 * 2.0 * (double)k * k);
-  TTG_HIP(hipFuncSetAttribute((const void *)k_chol3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_chol3, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
+  if (bar && !dbg && k > 2 * CB) {
+    // several workgroups (cooperative launch; a failed barrier reports flag -1)
+    constexpr int NB = 8;
+    TTG_HIP(hipMemsetAsync(bar, 0, 3 * sizeof(unsigned), s));
+    TTG_HIP(hipFuncSetAttribute((const void *)k_chol_mc<NB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int kk = (int)k;
+    long lg = (long)ldg;
+    void *args[] = {&kk, (void *)&G, &lg, &shift_rel, &R, &flag, &stats, &bar};
+    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_chol_mc<NB>, dim3(NB), dim3(CHOL_THREADS), args, lds, s));
+    return;
+  }
+  static const bool v3 = [] {
+    const char *e = std::getenv("TTG_CHOL_V3");
+    return e && e[0] == '1';
+  }();
+  if (v3 || ((wmax + 15) & ~(int64_t)15) > CHOL_THREADS - 64) {  // panels beyond 448 columns
+    TTG_HIP(hipFuncSetAttribute((const void *)k_chol3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_chol3, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
+  } else {
+    TTG_HIP(hipFuncSetAttribute((const void *)k_chol4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_chol4, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
+  }
   TTG_HIP(hipGetLastError());
 }
 
@@ -264,7 +720,7 @@ void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
 template <class T>
 __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict__ A, long lda,
                                               const double *__restrict__ R, T *__restrict__ X,
-                                              long ldx) {
+                                              long ldx, unsigned long long *dbg) {
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   typedef typename MFf<T>::V V;
   const int kp = (k + CB - 1) / CB * CB;
@@ -278,13 +734,17 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
   const int tr = wv & 1, tc = wv >> 1;
   const int r0 = blockIdx.x * CB;
   const int srow = tid >> 3, sq = tid & 7;  // substitution: 8 lanes per row
+  const bool rec = dbg && blockIdx.x == 0 && tid == 0;
+  if (rec) dbg[0] = __builtin_amdgcn_s_memrealtime();
   for (int idx = tid; idx < CB * kp; idx += blockDim.x) {
     const int row = idx % CB, col = idx / CB;
     const int gr = r0 + row;
     Xs[idx] = (gr < m && col < k) ? A[gr + (long)col * lda] : T(0);
   }
+  if (rec) dbg[1] = __builtin_amdgcn_s_memrealtime();
   for (int J0 = 0; J0 < k; J0 += CB) {
     const int jb = min(CB, k - J0);
+    if (rec) dbg[2 + 3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
     V acc = {0, 0, 0, 0};
     if (panel) {
       if (J0 > 0) {
@@ -305,6 +765,7 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
           if (idx < tot) Rs[(idx % J0) * CB + idx / J0] = v[e];
         }
         __syncthreads();
+        if (rec) dbg[3 + 3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
         for (int kc = 0; kc < J0; kc += CB) {
 #pragma unroll
           for (int kk = 0; kk < CB; kk += 4) {
@@ -344,6 +805,7 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
       Ds[l * CB + j] = (l < jb && j < jb) ? R[(J0 + l) + (long)(J0 + j) * k] : (l == j ? 1.0 : 0.0);
     }
     __syncthreads();
+    if (rec) dbg[4 + 3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
     // X_J R_JJ = B: row srow, lane sq owns columns sq + 8s
     double bb[4], xx[4];
 #pragma unroll
@@ -367,10 +829,164 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
       if (sq + 8 * s4 < jb) Xs[(J0 + sq + 8 * s4) * CB + srow] = (T)xx[s4];
     __syncthreads();
   }
+  if (rec) dbg[2 + 3 * (kp / CB)] = __builtin_amdgcn_s_memrealtime();
   for (int idx = tid; idx < CB * k; idx += blockDim.x) {
     const int row = idx % CB, col = idx / CB;
     const int gr = r0 + row;
     if (gr < m) X[gr + (long)col * ldx] = Xs[col * CB + row];
+  }
+  if (rec) {
+    __builtin_amdgcn_s_waitcnt(0);
+    dbg[3 + 3 * (kp / CB)] = __builtin_amdgcn_s_memrealtime();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_trsm2: X = A R^{-1} for k <= 256, 32 rows per workgroup, 4 waves.
+// Per 32-column block J:
+//   B_J = A_J - X_{<J} R_{<J,J}    (all waves: FP64 / FP32 MFMA, 2 x 2 tiles)
+//   X_J R_JJ = B_J                 (wave 0: one row per lane, right-looking
+//                                   substitution in FP64 registers, rows of
+//                                   R_JJ read from LDS, no cross-lane traffic;
+//                                   X_J is stored to X at once)
+// while waves 1..3 stage block J+1 (A_{J+1} into Xs, R_{J+1,J+1} into the
+// other Ds buffer, the panel R(0:J0+32, J+1) into Rs, which the MFMA of
+// block J no longer reads).  Staged blocks use an odd LDS row stride (33):
+// column and row accesses are conflict-free.
+// ---------------------------------------------------------------------------
+constexpr int TRS = CB + 1;
+
+__host__ __device__ constexpr size_t trsm2_lds(int kp, size_t tsz) {
+  return (size_t)TRS * kp * tsz + (size_t)(kp > CB ? kp - CB : 0) * TRS * tsz + 2 * (size_t)CB * TRS * 8;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_trsm2(int m, int k, const T *__restrict__ A, long lda,
+                                               const double *__restrict__ R, T *__restrict__ X,
+                                               long ldx, unsigned long long *dbg) {
+  extern __shared__ __align__(16) char smem[];
+  typedef typename MFf<T>::V V;
+  const int kp = (k + CB - 1) / CB * CB;
+  T *Xs = reinterpret_cast<T *>(smem);  // Xs[col * TRS + row]: A_J, then B_J, then X_J
+  T *Rs = Xs + (size_t)TRS * kp;         // Rs[t * TRS + n] = R(t, J0 + n), t < J0
+  // Ds[buf][a * TRS + b] = R(J0 + a, J0 + b) (identity past jb), buf = block parity
+  double *Ds0 = reinterpret_cast<double *>(smem + trsm2_lds(kp, sizeof(T)) - 2 * (size_t)CB * TRS * 8);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tr = wv & 1, tc = wv >> 1;
+  const int r0 = blockIdx.x * CB;
+  const bool rec = dbg && blockIdx.x == 0 && tid == 0;
+  if (rec) dbg[0] = __builtin_amdgcn_s_memrealtime();
+  // stage A_J and R_JJ of block J0 with NTHR threads from thread FIRST on
+  // (every load in flight before the first LDS store)
+  auto stage_diag = [&](int J0, auto first_c, auto nthr_c) {
+    constexpr int FIRST = decltype(first_c)::value, NTHR = decltype(nthr_c)::value;
+    constexpr int NPT = (CB * CB + NTHR - 1) / NTHR;
+    const int jb = min(CB, k - J0);
+    double *Ds = Ds0 + (size_t)((J0 / CB) & 1) * CB * TRS;
+    T va[NPT];
+    double vd[NPT];
+#pragma unroll
+    for (int e = 0; e < NPT; ++e) {
+      const int idx = tid - FIRST + NTHR * e;
+      const int lo = idx & (CB - 1), hi = idx >> 5;
+      const bool in = idx < CB * CB;
+      va[e] = (in && r0 + lo < m && hi < jb) ? A[(r0 + lo) + (long)(J0 + hi) * lda] : T(0);
+      vd[e] = (in && lo < jb && hi < jb) ? R[(J0 + lo) + (long)(J0 + hi) * k] : (lo == hi ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int e = 0; e < NPT; ++e) {
+      const int idx = tid - FIRST + NTHR * e;
+      const int lo = idx & (CB - 1), hi = idx >> 5;
+      if (idx < CB * CB) {
+        Xs[(J0 + hi) * TRS + lo] = va[e];
+        Ds[lo * TRS + hi] = vd[e];
+      }
+    }
+  };
+  stage_diag(0, std::integral_constant<int, 0>{}, std::integral_constant<int, 256>{});
+  if (rec) dbg[1] = __builtin_amdgcn_s_memrealtime();
+  __syncthreads();
+  for (int J0 = 0; J0 < k; J0 += CB) {
+    const int jb = min(CB, k - J0);
+    if (rec) dbg[2 + 3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    if (J0 > 0) {
+      V acc = {0, 0, 0, 0};
+      for (int kc = 0; kc < J0; kc += CB) {
+#pragma unroll
+        for (int kk = 0; kk < CB; kk += 4) {
+          const int t = kc + kk + (lane >> 4);
+          acc = MFf<T>::op(Xs[t * TRS + tr * 16 + (lane & 15)], Rs[t * TRS + tc * 16 + (lane & 15)], acc);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tr * 16 + MFf<T>::row(lane, v), col = tc * 16 + (lane & 15);
+        Xs[(J0 + col) * TRS + row] -= acc[v];
+      }
+    }
+    __syncthreads();  // B_J complete; Rs free
+    if (rec) dbg[3 + 3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    if (wv == 0) {
+      // lanes 32..63 repeat rows 0..31 (identical values; only lanes < 32 store to X)
+      const double *Ds = Ds0 + (size_t)((J0 / CB) & 1) * CB * TRS;
+      const int row = lane & (CB - 1);
+      const double rinv = 1.0 / Ds[row * TRS + row];
+      double b[CB];
+#pragma unroll
+      for (int l = 0; l < CB; ++l) b[l] = (double)Xs[(J0 + l) * TRS + row];
+      // row j + 1 of R_JJ is read while row j is applied
+      double dc[CB], dn[CB];
+#pragma unroll
+      for (int l = 1; l < CB; ++l) dc[l] = Ds[l];
+#pragma unroll
+      for (int j = 0; j < CB; ++j) {
+        if (j + 1 < CB) {
+#pragma unroll
+          for (int l = j + 2; l < CB; ++l) dn[l] = Ds[(j + 1) * TRS + l];
+        }
+        const double x = b[j] * readlane_d(rinv, j);
+        b[j] = x;
+#pragma unroll
+        for (int l = j + 1; l < CB; ++l) b[l] -= x * dc[l];
+#pragma unroll
+        for (int l = j + 2; l < CB; ++l) dc[l] = dn[l];
+      }
+#pragma unroll
+      for (int l = 0; l < CB; ++l) Xs[(J0 + l) * TRS + row] = (T)b[l];
+      if (lane < CB && r0 + row < m) {
+#pragma unroll
+        for (int l = 0; l < CB; ++l)
+          if (l < jb) X[(r0 + row) + (long)(J0 + l) * ldx] = (T)b[l];
+      }
+    } else if (J0 + CB < k) {
+      const int J1 = J0 + CB, jb1 = min(CB, k - J1);
+      stage_diag(J1, std::integral_constant<int, 64>{}, std::integral_constant<int, 192>{});
+      // panel R(t, J1 + n), t < J1: 32 consecutive threads read 32 consecutive t
+      // (k <= 256: at most 7168 elements, one batch of loads)
+      const int tot = J1 * CB;
+      constexpr int CH = (7 * CB * CB + 191) / 192;
+      for (int base = tid - 64; base < tot; base += 192 * CH) {
+        T v[CH];
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+          const int idx = base + 192 * e;
+          const int t = ((idx >> 10) << 5) + (idx & (CB - 1)), n = (idx >> 5) & (CB - 1);
+          v[e] = (idx < tot && n < jb1) ? (T)R[t + (long)(J1 + n) * k] : T(0);
+        }
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+          const int idx = base + 192 * e;
+          const int t = ((idx >> 10) << 5) + (idx & (CB - 1)), n = (idx >> 5) & (CB - 1);
+          if (idx < tot) Rs[t * TRS + n] = v[e];
+        }
+      }
+    }
+    __syncthreads();  // X_J in Xs; block J+1 staged
+    if (rec) dbg[4 + 3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+  }
+  if (rec) {
+    dbg[2 + 3 * (kp / CB)] = __builtin_amdgcn_s_memrealtime();
+    dbg[3 + 3 * (kp / CB)] = __builtin_amdgcn_s_memrealtime();
   }
 }
 
@@ -441,18 +1057,30 @@ void nearid_pass(hipStream_t s, int64_t m, int64_t k, double *G, const double *Q
 
 template <class T>
 void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t lda, const double *R,
-                      T *X, int64_t ldx) {
+                      T *X, int64_t ldx, unsigned long long *dbg) {
   if (m <= 0 || k <= 0) return;
   const int64_t kp = (k + CB - 1) / CB * CB;
+  ProfScope prof(KC_TRTRI, s, (double)m * k * k, (double)sizeof(T) * 2.0 * m * k + 8.0 * k * k);
+  static const bool v1 = [] {
+    const char *e = std::getenv("TTG_TRSM_V1");
+    return e && e[0] == '1';
+  }();
+  if (kp <= 256 && !v1) {
+    const size_t lds2 = trsm2_lds((int)kp, sizeof(T));
+    TTG_HIP(hipFuncSetAttribute((const void *)k_trsm2<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    hipLaunchKernelGGL((k_trsm2<T>), dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds2, s, (int)m, (int)k,
+                       A, (long)lda, R, X, (long)ldx, dbg);
+    TTG_HIP(hipGetLastError());
+    return;
+  }
   const size_t lds = ((size_t)CB * kp + (kp <= 256 ? kp : CB) * CB) * sizeof(T) + CB * CB * sizeof(double);
   TTG_CHECK(lds <= 160 * 1024, TTG_BAD_ARG, "rank too large for the triangular solve");
-  ProfScope prof(KC_TRTRI, s, (double)m * k * k, (double)sizeof(T) * 2.0 * m * k + 8.0 * k * k);
   TTG_HIP(hipFuncSetAttribute((const void *)k_trsm<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL((k_trsm<T>), dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds, s, (int)m, (int)k,
-                     A, (long)lda, R, X, (long)ldx);
+                     A, (long)lda, R, X, (long)ldx, dbg);
   TTG_HIP(hipGetLastError());
 }
-template void trsm_right_upper<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const double *, double *, int64_t);
-template void trsm_right_upper<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const double *, float *, int64_t);
+template void trsm_right_upper<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const double *, double *, int64_t, unsigned long long *);
+template void trsm_right_upper<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const double *, float *, int64_t, unsigned long long *);
 
 }  // namespace ttg

@@ -182,7 +182,8 @@ void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, d
 // G is read from its upper triangle; shift = shift_rel * trace(G) / k.
 // flag[0] = 0 ok, j+1 if pivot j failed; stats[0] = diag spread, stats[1] = max|R - I|.
 void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats, unsigned long long *dbg = nullptr);
+                 double *R, int *flag, double *stats, unsigned long long *dbg = nullptr,
+                 unsigned *bar = nullptr);
 // Second CholeskyQR pass for G = Q1^T Q1 = I + E, small E: R2 = I + Phi(E)
 // (chol(G) to O(|E|^2)), Qout = Q1 R2^{-1} with the inverse series truncated
 // after `terms` powers of U; maxabs = max|E| for the acceptance test.
@@ -192,7 +193,7 @@ void nearid_pass(hipStream_t s, int64_t m, int64_t k, double *G, const double *Q
 // X (m x k) = A R^{-1}, R upper triangular (FP64), blocked substitution.
 template <class T>
 void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t lda, const double *R,
-                      T *X, int64_t ldx);
+                      T *X, int64_t ldx, unsigned long long *dbg = nullptr);
 
 // One-sided block Jacobi SVD on the stacked matrix Y = [S; I] (FP64).
 // Setup from S (k x k, column-major, optionally transposed) then rotate

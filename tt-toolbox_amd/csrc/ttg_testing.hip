@@ -92,14 +92,15 @@ ttg_status ttg_test_householder(ttg_ctx *ctx, int64_t m, int64_t n, const double
 
 // R (k x k) upper Cholesky factor of G + shift
 ttg_status ttg_test_chol(ttg_ctx *ctx, int64_t k, const double *G, double shift_rel, double *R,
-                         int *flag, double *stats) {
+                         int *flag, double *stats, int multi) {
   GUARD(ctx, {
     hipStream_t s = ctx->stream;
     Frees f;
     double *dG = f.add(up(G, k * k, s)), *dR = f.add(up<double>(nullptr, k * k, s));
     double *dst = f.add(up<double>(nullptr, 8, s));
     int *dfl = f.add(up<int>(nullptr, 4, s));
-    chol_factor(s, k, dG, k, shift_rel, dR, dfl, dst);
+    unsigned *bar = multi ? f.add(up<unsigned>(nullptr, 64, s)) : nullptr;
+    chol_factor(s, k, dG, k, shift_rel, dR, dfl, dst, nullptr, bar);
     down(R, dR, k * k, s);
     down(flag, dfl, 1, s);
     down(stats, dst, 2, s);
@@ -179,6 +180,23 @@ ttg_status ttg_test_chol_timing(ttg_ctx *ctx, int64_t k, const double *G, int64_
     unsigned long long *dbg = f.add(up<unsigned long long>(nullptr, ntimes, s));
     TTG_HIP(hipMemsetAsync(dbg, 0, ntimes * 8, s));
     chol_factor(s, k, dG, k, 0.0, dR, dfl, dst, dbg);
+    down(reinterpret_cast<unsigned long long *>(times), dbg, ntimes, s);
+  })
+}
+
+// per-phase clock stamps of workgroup 0 of the FP64 triangular solve
+ttg_status ttg_test_trsm_timing(ttg_ctx *ctx, int64_t m, int64_t k, const double *A, const double *R,
+                                int64_t *times, int64_t ntimes) {
+  GUARD(ctx, {
+    hipStream_t s = ctx->stream;
+    Frees f;
+    const int64_t nb = (k + 31) / 32;
+    TTG_CHECK(ntimes >= 3 * nb + 4, TTG_BAD_ARG, "times too short");
+    double *dA = f.add(up(A, m * k, s)), *dR = f.add(up(R, k * k, s));
+    double *dX = f.add(up<double>(nullptr, m * k, s));
+    unsigned long long *dbg = f.add(up<unsigned long long>(nullptr, ntimes, s));
+    TTG_HIP(hipMemsetAsync(dbg, 0, ntimes * 8, s));
+    trsm_right_upper<double>(s, m, k, dA, m, dR, dX, m, dbg);
     down(reinterpret_cast<unsigned long long *>(times), dbg, ntimes, s);
   })
 }

@@ -39,7 +39,7 @@ EXPORTS = [
 ]
 # kernel-level test hooks (include/ttgpu_testing.h)
 TEST_EXPORTS = ["ttg_test_gemm", "ttg_test_householder", "ttg_test_chol", "ttg_test_trsm", "ttg_test_jacobi", "ttg_test_jacobi_timing",
-                "ttg_test_chol_timing"]
+                "ttg_test_chol_timing", "ttg_test_trsm_timing"]
 KERNEL_CLASSES = {"gemm": 1, "jacobi": 2, "cholesky": 3, "trtri": 4, "householder": 5}
 
 
@@ -104,11 +104,12 @@ def load(path: Path | str | None = None):
         "ttg_profile_end": (st, [P, I64P, DP, DP, DP]),
         "ttg_test_gemm": (st, [P, C.c_int, C.c_int, C.c_int, I64, I64, I64, P, I64, P, I64, P, I64]),
         "ttg_test_householder": (st, [P, I64, I64, P, P, P]),
-        "ttg_test_chol": (st, [P, I64, P, C.c_double, P, P, P]),
+        "ttg_test_chol": (st, [P, I64, P, C.c_double, P, P, P, C.c_int]),
         "ttg_test_jacobi": (st, [P, I64, P, C.c_int, P, I64, I64P, P, P]),
         "ttg_test_trsm": (st, [P, C.c_int, I64, I64, P, P, P]),
         "ttg_test_jacobi_timing": (st, [P, I64, P, P, I64]),
         "ttg_test_chol_timing": (st, [P, I64, P, P, I64]),
+        "ttg_test_trsm_timing": (st, [P, I64, I64, P, P, P, I64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
