@@ -182,6 +182,8 @@ def main():
                     help="run only the batched leg (profiling; prints its JSON object)")
     ap.add_argument("--no-other", action="store_true",
                     help="skip the configs 1, 2 and 5 throughput leg")
+    ap.add_argument("--no-svd-always", action="store_true",
+                    help="skip the comparison rounds with an SVD at every bond (shorter profiles)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -275,7 +277,7 @@ def main():
     full_svd = {}
     os.environ["TTG_SVD_ALWAYS"] = "1"
     try:
-        for prec, out in (("fp32", out_mixed), ("fp64", out_plain)):
+        for prec, out in (() if args.no_svd_always else (("fp32", out_mixed), ("fp64", out_plain))):
             one(prec, out)
             torch.cuda.synchronize()
             ea = torch.cuda.Event(enable_timing=True)

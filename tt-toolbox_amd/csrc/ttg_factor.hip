@@ -908,7 +908,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   __syncthreads();
   for (int J0 = 0; J0 < k; J0 += CB) {
     const int jb = min(CB, k - J0);
-    if (rec) dbg[2 + 3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    if (rec) dbg[2 + 8 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
     if (J0 > 0) {
       V acc = {0, 0, 0, 0};
       for (int kc = 0; kc < J0; kc += CB) {
@@ -925,7 +925,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       }
     }
     __syncthreads();  // B_J complete; Rs free
-    if (rec) dbg[3 + 3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    if (rec) dbg[3 + 8 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
     if (wv == 0) {
       // lanes 32..63 repeat rows 0..31 (identical values; only lanes < 32 store to X)
       const double *Ds = Ds0 + (size_t)((J0 / CB) & 1) * CB * TRS;
@@ -934,6 +934,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       double b[CB];
 #pragma unroll
       for (int l = 0; l < CB; ++l) b[l] = (double)Xs[(J0 + l) * TRS + row];
+      if (rec) {
+        __builtin_amdgcn_s_waitcnt(0);
+        dbg[4 + 8 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+      }
       // row j + 1 of R_JJ is read while row j is applied
       double dc[CB], dn[CB];
 #pragma unroll
@@ -951,6 +955,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
         for (int l = j + 2; l < CB; ++l) dc[l] = dn[l];
       }
+      if (rec) dbg[5 + 8 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime() + (unsigned long long)(b[CB - 1] == 12345.0);
 #pragma unroll
       for (int l = 0; l < CB; ++l) Xs[(J0 + l) * TRS + row] = (T)b[l];
       if (lane < CB && r0 + row < m) {
@@ -958,6 +963,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         for (int l = 0; l < CB; ++l)
           if (l < jb) X[(r0 + row) + (long)(J0 + l) * ldx] = (T)b[l];
       }
+      if (rec) dbg[6 + 8 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
     } else if (J0 + CB < k) {
       const int J1 = J0 + CB, jb1 = min(CB, k - J1);
       stage_diag(J1, std::integral_constant<int, 64>{}, std::integral_constant<int, 192>{});
@@ -982,12 +988,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       }
     }
     __syncthreads();  // X_J in Xs; block J+1 staged
-    if (rec) dbg[4 + 3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    if (rec) dbg[7 + 8 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
   }
-  if (rec) {
-    dbg[2 + 3 * (kp / CB)] = __builtin_amdgcn_s_memrealtime();
-    dbg[3 + 3 * (kp / CB)] = __builtin_amdgcn_s_memrealtime();
-  }
+  if (rec) dbg[2 + 8 * (kp / CB)] = __builtin_amdgcn_s_memrealtime();
 }
 
 // ---------------------------------------------------------------------------
@@ -1009,8 +1012,9 @@ __global__ void k_nearid_init(int k, double *__restrict__ G, double *__restrict_
     U[idx] = (r < c) ? e : (r == c ? 0.5 * e : 0.0);
     mx = fmax(mx, fabs(e));
   }
-  mx = wave_max(mx);
-  if ((threadIdx.x & 63) == 0 && mx > 0.0)  // non-negative doubles order like their bit patterns
+  __shared__ double red[4];
+  mx = block_max(mx, red);  // one atomic per workgroup
+  if (threadIdx.x == 0 && mx > 0.0)  // non-negative doubles order like their bit patterns
     atomicMax(reinterpret_cast<unsigned long long *>(maxabs), (unsigned long long)__double_as_longlong(mx));
 }
 // X = I + a U + b P (P may be null), R2 = I + U
@@ -1046,7 +1050,8 @@ void nearid_pass(hipStream_t s, int64_t m, int64_t k, double *G, const double *Q
   const long total = k * k;
   const int blocks = (int)std::min<long>((total + 255) / 256, 1024);
   TTG_HIP(hipMemsetAsync(maxabs, 0, sizeof(double), s));
-  hipLaunchKernelGGL(k_nearid_init, dim3(blocks), dim3(256), 0, s, (int)k, G, U, maxabs);
+  hipLaunchKernelGGL(k_nearid_init, dim3((int)std::min<long>((total + 1023) / 1024, 128)), dim3(256), 0, s, (int)k, G,
+                     U, maxabs);
   TTG_HIP(hipGetLastError());
   if (terms >= 2) gemm<double>(s, false, false, k, k, k, 1.0, U, k, U, k, 0.0, P, k, gws, gws_elems, num_cus);
   hipLaunchKernelGGL(k_nearid_fin, dim3(blocks), dim3(256), 0, s, (int)k, X, (const double *)U,

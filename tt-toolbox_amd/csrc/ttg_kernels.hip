@@ -62,7 +62,8 @@ template <> struct MF<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-constexpr int GBM = 64, GBN = 64, GBK = 16;
+constexpr int GBM = 64, GBN = 64, GBK = 64;
+constexpr int GLD = GBM * GBK / 256;  // operand elements each thread stages per K tile
 
 // C(MxN) = alpha*op(A)*op(B) + beta*C.  Block 64x64, 4 waves (2x2), each wave
 // a 32x32 sub-tile made of 2x2 MFMA 16x16x4 tiles.  Operand layout of
@@ -71,7 +72,8 @@ constexpr int GBM = 64, GBN = 64, GBK = 16;
 // form the transposed tiles (op(B)^T op(A)^T), so that lane l holds rows
 // l%16 of C and 16 lanes store 16 consecutive rows (coalesced).  The next
 // K tile is loaded into registers while the current one is multiplied from
-// LDS.
+// LDS; a 64-deep K tile gives each wave 64 MFMAs per tile, enough to cover
+// the load latency of the next one.
 // Split-K: blockIdx.z selects K range [z*kchunk, ...); with gridDim.z > 1 the
 // raw partial product goes to P + z*M*N (ld M) and k_splitk_reduce finishes.
 // TI is the storage type of A and B, T the MFMA type: TI = double, T = float
@@ -95,57 +97,37 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = V{0, 0, 0, 0};
-  T ra[4], rb[4];
+  // 64 consecutive threads read 64 consecutive elements of the operand's
+  // contiguous dimension (M or K for A, K or N for B)
+  T ra[GLD], rb[GLD];
   auto fetch = [&](int k0) {
-    if (!TA) {
-      const int mm = tid & 63, kk = tid >> 6;
+    const int c = tid & 63, q = tid >> 6;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int k = kk + 4 * i, gm = m0 + mm, gk = k0 + k;
+    for (int i = 0; i < GLD; ++i) {
+      if (!TA) {
+        const int gm = m0 + c, gk = k0 + q + 4 * i;
         ra[i] = (gm < M && gk < kend) ? T(A[gm + (long)gk * lda]) : T(0);
-      }
-    } else {
-      const int kk = tid & 15, mm = tid >> 4;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int m = mm + 16 * i, gm = m0 + m, gk = k0 + kk;
+      } else {
+        const int gk = k0 + c, gm = m0 + q + 4 * i;
         ra[i] = (gm < M && gk < kend) ? T(A[gk + (long)gm * lda]) : T(0);
       }
-    }
-    if (!TB) {
-      const int kk = tid & 15, nn = tid >> 4;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int n = nn + 16 * i, gn = n0 + n, gk = k0 + kk;
+      if (!TB) {
+        const int gk = k0 + c, gn = n0 + q + 4 * i;
         rb[i] = (gn < N && gk < kend) ? T(B[gk + (long)gn * ldb]) : T(0);
-      }
-    } else {
-      const int nn = tid & 63, kk = tid >> 6;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int k = kk + 4 * i, gn = n0 + nn, gk = k0 + k;
+      } else {
+        const int gn = n0 + c, gk = k0 + q + 4 * i;
         rb[i] = (gn < N && gk < kend) ? T(B[gn + (long)gk * ldb]) : T(0);
       }
     }
   };
   auto stash = [&]() {
-    if (!TA) {
-      const int mm = tid & 63, kk = tid >> 6;
+    const int c = tid & 63, q = tid >> 6;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) As[kk + 4 * i][mm] = ra[i];
-    } else {
-      const int kk = tid & 15, mm = tid >> 4;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) As[kk][mm + 16 * i] = ra[i];
-    }
-    if (!TB) {
-      const int kk = tid & 15, nn = tid >> 4;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) Bs[kk][nn + 16 * i] = rb[i];
-    } else {
-      const int nn = tid & 63, kk = tid >> 6;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) Bs[kk + 4 * i][nn] = rb[i];
+    for (int i = 0; i < GLD; ++i) {
+      if (!TA) As[q + 4 * i][c] = ra[i];
+      else As[c][q + 4 * i] = ra[i];
+      if (!TB) Bs[c][q + 4 * i] = rb[i];
+      else Bs[q + 4 * i][c] = rb[i];
     }
   };
   if (kbeg < kend) fetch(kbeg);
@@ -324,17 +306,31 @@ template <class T> void transpose(hipStream_t s, const T *A, int64_t rows, int64
 template void transpose<double>(hipStream_t, const double *, int64_t, int64_t, double *);
 template void transpose<float>(hipStream_t, const float *, int64_t, int64_t, float *);
 
-// Frobenius norm in one block (count is at most a few million here).
+// Frobenius norm in one block (count is at most a few million here); 16
+// loads in flight per thread, 16 partial sums combined in a fixed order.
 template <class T>
 __global__ __launch_bounds__(1024) void k_fro_norm(const T *__restrict__ x, long count, double *out) {
   __shared__ double red[16];
-  double s = 0.0;
-  for (long i = threadIdx.x; i < count; i += blockDim.x) {
-    const double v = (double)x[i];
-    s += v * v;
+  constexpr int U = 16;
+  double s[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) s[u] = 0.0;
+  for (long i = threadIdx.x; i < count; i += (long)U * blockDim.x) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long idx = i + (long)u * blockDim.x;
+      v[u] = idx < count ? (double)x[idx] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) s[u] += v[u] * v[u];
   }
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) out[0] = sqrt(s);
+#pragma unroll
+  for (int w = U / 2; w >= 1; w >>= 1)
+#pragma unroll
+    for (int u = 0; u < w; ++u) s[u] += s[u + w];
+  const double t = block_sum(s[0], red);
+  if (threadIdx.x == 0) out[0] = sqrt(t);
 }
 template <class T> void fro_norm(hipStream_t s, const T *x, int64_t count, double *out) {
   hipLaunchKernelGGL((k_fro_norm<T>), dim3(1), dim3(1024), 0, s, x, (long)count, out);

@@ -191,7 +191,7 @@ ttg_status ttg_test_trsm_timing(ttg_ctx *ctx, int64_t m, int64_t k, const double
     hipStream_t s = ctx->stream;
     Frees f;
     const int64_t nb = (k + 31) / 32;
-    TTG_CHECK(ntimes >= 3 * nb + 4, TTG_BAD_ARG, "times too short");
+    TTG_CHECK(ntimes >= 8 * nb + 3, TTG_BAD_ARG, "times too short");
     double *dA = f.add(up(A, m * k, s)), *dR = f.add(up(R, k * k, s));
     double *dX = f.add(up<double>(nullptr, m * k, s));
     unsigned long long *dbg = f.add(up<unsigned long long>(nullptr, ntimes, s));
