@@ -10,6 +10,7 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+// prec 0: FP64; 1: FP32 data and MFMA, FP64 result; 2: FP64 data rounded to FP32 for the MFMA
 ttg_status ttg_test_gemm(ttg_ctx *ctx, int prec, int ta, int tb, int64_t M, int64_t N, int64_t K,
                          const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
                          int64_t ldc);

@@ -191,6 +191,13 @@ void nearid_pass(hipStream_t s, int64_t m, int64_t k, double *G, const double *Q
                  double *R2, double *U, double *P, double *X, double *maxabs, int terms, double *gws,
                  int64_t gws_elems, int num_cus);
 // X (m x k) = A R^{-1}, R upper triangular (FP64), blocked substitution.
+// FP32 operands / FP64 operands rounded to FP32 while staged; FP32 MFMA; FP64 result
+void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
+                     const float *A, int64_t lda, const float *B, int64_t ldb, double *C,
+                     int64_t ldc, float *ws, int64_t wse, int cus);
+void gemm_f64_via_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
+                      const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                      int64_t ldc, float *ws, int64_t wse, int cus);
 template <class T>
 void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t lda, const double *R,
                       T *X, int64_t ldx, unsigned long long *dbg = nullptr);

@@ -99,24 +99,25 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
     for (int j = 0; j < 2; ++j) acc[i][j] = V{0, 0, 0, 0};
   // 64 consecutive threads read 64 consecutive elements of the operand's
   // contiguous dimension (M or K for A, K or N for B)
-  T ra[GLD], rb[GLD];
+  // prefetched in the storage type: a conversion here would stall on the load
+  TI ra[GLD], rb[GLD];
   auto fetch = [&](int k0) {
     const int c = tid & 63, q = tid >> 6;
 #pragma unroll
     for (int i = 0; i < GLD; ++i) {
       if (!TA) {
         const int gm = m0 + c, gk = k0 + q + 4 * i;
-        ra[i] = (gm < M && gk < kend) ? T(A[gm + (long)gk * lda]) : T(0);
+        ra[i] = (gm < M && gk < kend) ? A[gm + (long)gk * lda] : TI(0);
       } else {
         const int gk = k0 + c, gm = m0 + q + 4 * i;
-        ra[i] = (gm < M && gk < kend) ? T(A[gk + (long)gm * lda]) : T(0);
+        ra[i] = (gm < M && gk < kend) ? A[gk + (long)gm * lda] : TI(0);
       }
       if (!TB) {
         const int gk = k0 + c, gn = n0 + q + 4 * i;
-        rb[i] = (gn < N && gk < kend) ? T(B[gk + (long)gn * ldb]) : T(0);
+        rb[i] = (gn < N && gk < kend) ? B[gk + (long)gn * ldb] : TI(0);
       } else {
         const int gn = n0 + c, gk = k0 + q + 4 * i;
-        rb[i] = (gn < N && gk < kend) ? T(B[gn + (long)gk * ldb]) : T(0);
+        rb[i] = (gn < N && gk < kend) ? B[gn + (long)gk * ldb] : TI(0);
       }
     }
   };
@@ -124,10 +125,10 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
     const int c = tid & 63, q = tid >> 6;
 #pragma unroll
     for (int i = 0; i < GLD; ++i) {
-      if (!TA) As[q + 4 * i][c] = ra[i];
-      else As[c][q + 4 * i] = ra[i];
-      if (!TB) Bs[c][q + 4 * i] = rb[i];
-      else Bs[q + 4 * i][c] = rb[i];
+      if (!TA) As[q + 4 * i][c] = T(ra[i]);
+      else As[c][q + 4 * i] = T(ra[i]);
+      if (!TB) Bs[c][q + 4 * i] = T(rb[i]);
+      else Bs[q + 4 * i][c] = T(rb[i]);
     }
   };
   if (kbeg < kend) fetch(kbeg);

@@ -28,12 +28,6 @@
 
 namespace ttg {
 
-void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
-                     const float *A, int64_t lda, const float *B, int64_t ldb, double *C,
-                     int64_t ldc, float *ws, int64_t wse, int cus);
-void gemm_f64_via_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
-                      const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
-                      int64_t ldc, float *ws, int64_t wse, int cus);
 
 static constexpr int kMaxRank = 512;
 static constexpr int kMaxSweeps = 60;

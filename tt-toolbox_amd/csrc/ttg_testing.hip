@@ -62,6 +62,12 @@ ttg_status ttg_test_gemm(ttg_ctx *ctx, int prec, int ta, int tb, int64_t M, int6
       TTG_HIP(hipMemsetAsync(dC, 0, nc * sizeof(double), s));
       gemm<double>(s, ta, tb, M, N, K, 1.0, dA, lda, dB, ldb, 0.0, dC, ldc, ws, wse, ctx->num_cus);
       down(C, dC, nc, s);
+    } else if (prec == 2) {  // FP64 operands rounded to FP32 while staged, FP32 MFMA
+      double *dA = f.add(up(A, na, s)), *dB = f.add(up(B, nb, s)), *dC = f.add(up<double>(nullptr, nc, s));
+      float *ws = f.add(up<float>(nullptr, wse, s));
+      TTG_HIP(hipMemsetAsync(dC, 0, nc * sizeof(double), s));
+      gemm_f64_via_f32(s, ta, tb, M, N, K, dA, lda, dB, ldb, dC, ldc, ws, wse, ctx->num_cus);
+      down(C, dC, nc, s);
     } else {
       double *dA = f.add(up(A, na, s)), *dB = f.add(up(B, nb, s)), *dC = f.add(up<double>(nullptr, nc, s));
       float *fA = f.add(up<float>(nullptr, na, s)), *fB = f.add(up<float>(nullptr, nb, s));
