@@ -16,9 +16,8 @@ ttg_status ttg_test_gemm(ttg_ctx *ctx, int prec, int ta, int tb, int64_t M, int6
                          int64_t ldc);
 ttg_status ttg_test_householder(ttg_ctx *ctx, int64_t m, int64_t n, const double *A, double *Q,
                                 double *R);
-// multi != 0: the multi-workgroup factorisation the rounder uses for k > 64
 ttg_status ttg_test_chol(ttg_ctx *ctx, int64_t k, const double *G, double shift_rel, double *R,
-                         int *flag, double *stats, int multi);
+                         int *flag, double *stats);
 ttg_status ttg_test_trsm(ttg_ctx *ctx, int prec, int64_t m, int64_t k, const double *A,
                          const double *R, double *X);
 ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS, double *Y,

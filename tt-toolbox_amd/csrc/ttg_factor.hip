@@ -305,9 +305,21 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__r
       for (int t = 0; t < CB; ++t) v[t] = Ps[t * wps + c1];
     }
     unsigned failmask = 0;
+    double rc_prev = 0.0;
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
       if (wv == 0) {
+        // Pivot c.  Critical path: v[c] -> pivot -> row c -> v[c+1] (the next
+        // pivot column), using only this lane's values and one readlane.
+        // Row c-1's update of the later columns (LDS reads issued first,
+        // consumed after the pivot) runs beside it.
+        double rr[CB];
+        if (c >= 1) {
+#pragma unroll
+          for (int i = 0; i < CB; ++i)
+            if (i > c) rr[i] = Dl[(c - 1) * CB + i];
+        }
+        __builtin_amdgcn_sched_barrier(0);
         double p = readlane_d(v[c], c);
         // straight-line code only (branches here push v[] into scratch): a
         // failed pivot is recorded in a bit mask and replaced by a harmless 1
@@ -329,16 +341,27 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__r
         // every lane holds the same pivot; lanes 32..63 mirror lanes 0..31
         Dl[c * CB + j] = rc;
         dinv[c] = inv;
-        wave_sync_lds();
+        if (c + 1 < CB) v[c + 1] -= readlane_d(rc, c + 1) * rc;  // R(c, c+1) R(c, j)
+        if (c >= 1) {
 #pragma unroll
-        for (int i = c + 1; i < CB; ++i) v[i] -= Dl[c * CB + i] * rc;
+          for (int i = 0; i < CB; ++i)
+            if (i > c) v[i] -= rr[i] * rc_prev;  // R(c-1, i) R(c-1, j)
+        }
+        rc_prev = rc;
       }
-      __syncthreads();  // row c of R and 1 / r_cc visible to the panel waves
-      if (wv != 0 && w > 0) {
-        const double x = v[c] * dinv[c];
-        v[c] = x;
+      if ((c & 3) == 3) {
+        __syncthreads();  // rows c-3..c of R and their 1 / r_cc visible to the panel waves
+        if (wv != 0 && w > 0) {
 #pragma unroll
-        for (int l = c + 1; l < CB; ++l) v[l] -= Dl[c * CB + l] * x;
+          for (int u = 0; u < 4; ++u) {
+            const int t = c - 3 + u;
+            const double x = v[t] * dinv[t];
+            v[t] = x;
+#pragma unroll
+            for (int l = 0; l < CB; ++l)
+              if (l > t) v[l] -= Dl[t * CB + l] * x;
+          }
+        }
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -433,273 +456,15 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__r
   }
 }
 
-// ---------------------------------------------------------------------------
-// Cholesky on NB cooperating workgroups (same factor, flag and statistics as
-// k_chol3).  Per 32-column block: every workgroup factorises the diagonal
-// block itself (identical inputs and code, so identical results; no barrier
-// between the diagonal factor and the panel), solves its share of the panel
-// columns and writes them; grid barrier; every workgroup stages the whole
-// panel and updates its share of the trailing upper 16x16 tiles (FP64 MFMA);
-// grid barrier.  The diagonal factorisation stays sequential; the panel and
-// the SYRK, most of k_chol3's time, spread over NB CUs.
-// bar: 2 barrier words (zeroed by the caller) + 1 timeout flag.
-// ---------------------------------------------------------------------------
-__device__ bool chol_grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen) {
-  __shared__ int sh_ok;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned target = gen + 1;
-    const unsigned prev = atomicAdd(&bar[0], 1u);
-    int ok = 1;
-    if (prev == nblocks - 1) {
-      atomicExch(&bar[0], 0u);
-      __threadfence();
-      atomicExch(&bar[1], target);
-    } else {
-      const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != target) {
-        __builtin_amdgcn_s_sleep(1);
-        if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          ok = 0;
-          break;
-        }
-        if (__builtin_amdgcn_s_memrealtime() - t0 > 400000000ull) {  // 4 s
-          atomicExch(&bar[2], 1u);
-          ok = 0;
-          break;
-        }
-      }
-    }
-    __threadfence();
-    gen = target;
-    sh_ok = ok;
-  }
-  __syncthreads();
-  return sh_ok != 0;
-}
-
-template <int NB>
-__global__ __launch_bounds__(CHOL_THREADS) void k_chol_mc(int k, const double *__restrict__ G, long ldg,
-                                                          double shift_rel, double *__restrict__ R,
-                                                          int *flag, double *stats, unsigned *bar) {
-  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
-  double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
-  double *Ps = Dl + CB * CB;                      // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
-  __shared__ double red[CHOL_THREADS / 64];
-  __shared__ double dinv[CB];
-  __shared__ int sh_fail;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, b = blockIdx.x;
-  constexpr int NW = CHOL_THREADS / 64;
-  unsigned gen = 0;
-  double tr = 0.0;
-  for (int j = tid; j < k; j += CHOL_THREADS) tr += G[j + (long)j * ldg];
-  tr = block_sum(tr, red);
-  const double shift = shift_rel * tr / (k > 0 ? k : 1);
-  // R = upper(G) + shift I, columns split over the workgroups
-  for (int j = b * NW + wv; j < k; j += NB * NW)
-    for (int i = lane; i < k; i += 64)
-      R[i + (long)j * k] = (i <= j) ? G[i + (long)j * ldg] + (i == j ? shift : 0.0) : 0.0;
-  if (b == 0 && tid == 0) stats[1] = 0.0;
-  if (tid == 0) sh_fail = 0;
-  bool ok = chol_grid_sync(bar, NB, gen);
-  double dmax = 0.0, dmin = INFINITY, dev = 0.0;
-  for (int J0 = 0; J0 < k && ok; J0 += CB) {
-    const int jb = min(CB, k - J0);
-    // the last diagonal block has no panel: workgroup 0 alone finishes it
-    if (J0 + jb >= k && b != 0) break;
-    for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
-      const int i = idx & (CB - 1), j = idx >> 5;
-      Dl[idx] = (i < jb && j < jb) ? (i <= j ? R[(J0 + i) + (long)(J0 + j) * k] : 0.0) : (i == j ? 1.0 : 0.0);
-    }
-    __syncthreads();
-    if (wv == 0) {
-      const int j = lane & (CB - 1);
-      double col[CB];
-#pragma unroll
-      for (int i = 0; i < CB; ++i) col[i] = Dl[j * CB + i];
-      wave_sync_lds();
-      unsigned failmask = 0;
-      double myinv = 1.0;
-#pragma unroll
-      for (int c = 0; c < CB; ++c) {
-        double p = readlane_d(col[c], c);
-        const bool bad = !(p > 0.0) || !isfinite(p);
-        failmask |= bad ? (1u << c) : 0u;
-        p = bad ? 1.0 : p;
-        double inv = __builtin_amdgcn_rsq(p);
-        {
-          const double h = 0.5 * p;
-          double e = __builtin_fma(-h * inv, inv, 0.5);
-          inv = __builtin_fma(inv, e, inv);
-          e = __builtin_fma(-h * inv, inv, 0.5);
-          inv = __builtin_fma(inv, e, inv);
-        }
-        const double dc = p * inv;
-        const double rc = (j == c) ? dc : (j > c ? col[c] * inv : 0.0);
-        col[c] = rc;
-        myinv = (j == c) ? inv : myinv;
-        Dl[c * CB + j] = rc;
-        wave_sync_lds();
-#pragma unroll
-        for (int i = c + 1; i < CB; ++i) col[i] -= Dl[c * CB + i] * rc;
-        __builtin_amdgcn_sched_barrier(0);
-      }
-      if (lane < CB) dinv[j] = myinv;
-      if (lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
-    }
-    __syncthreads();
-    if (sh_fail) break;  // every workgroup sees the same pivots
-    const int w = k - J0 - jb;
-    // workgroup 0 stores the factored diagonal block once no workgroup can
-    // still be reading its unfactored entries: after the panel barrier (the
-    // last block, which only workgroup 0 runs, at once)
-    auto store_diag = [&]() {
-      for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
-        const int t = idx & (CB - 1), l = idx >> 5;
-        if (t < jb && l < jb && t <= l) {
-          const double v = Dl[t * CB + l];
-          R[(J0 + t) + (long)(J0 + l) * k] = v;
-          if (t == l) {
-            dmax = fmax(dmax, fabs(v));
-            dmin = fmin(dmin, fabs(v));
-            dev = fmax(dev, fabs(v - 1.0));
-          } else {
-            dev = fmax(dev, fabs(v));
-          }
-        }
-      }
-    };
-    if (w <= 0) {
-      store_diag();
-      continue;
-    }
-    const int wpad = (w + 15) & ~15, wps = wpad + 1;
-    // this workgroup's panel columns [c0, c1), a multiple of 16 wide
-    const int chunk = ((wpad / 16 + NB - 1) / NB) * 16;
-    const int c0 = min(b * chunk, wpad), c1 = min(c0 + chunk, wpad);
-    for (int idx = tid; idx < CB * (c1 - c0); idx += CHOL_THREADS) {
-      const int t = idx & (CB - 1), c = c0 + (idx >> 5);
-      Ps[t * wps + c] = (t < jb && c < w) ? R[(J0 + t) + (long)(J0 + jb + c) * k] : 0.0;
-    }
-    __syncthreads();
-    for (int c = c0 + tid; c < c1; c += CHOL_THREADS) {
-      double bb[CB];
-#pragma unroll
-      for (int t = 0; t < CB; ++t) bb[t] = Ps[t * wps + c];
-#pragma unroll
-      for (int t = 0; t < CB; ++t) {
-        const double x = bb[t] * dinv[t];
-        bb[t] = x;
-#pragma unroll
-        for (int l = t + 1; l < CB; ++l) bb[l] -= Dl[t * CB + l] * x;
-        __builtin_amdgcn_sched_barrier(0);
-      }
-#pragma unroll
-      for (int t = 0; t < CB; ++t) {
-        const double v = (t < jb && c < w) ? bb[t] : 0.0;
-        Ps[t * wps + c] = v;
-        dev = fmax(dev, fabs(v));
-      }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < CB * (c1 - c0); idx += CHOL_THREADS) {
-      const int t = idx & (CB - 1), c = c0 + (idx >> 5);
-      if (t < jb && c < w) R[(J0 + t) + (long)(J0 + jb + c) * k] = Ps[t * wps + c];
-    }
-    if (!chol_grid_sync(bar, NB, gen)) {
-      ok = false;
-      break;
-    }
-    if (b == 0) store_diag();
-    // the whole solved panel, then this workgroup's trailing tiles
-    for (int idx = tid; idx < CB * wpad; idx += CHOL_THREADS) {
-      const int t = idx & (CB - 1), c = idx >> 5;
-      Ps[t * wps + c] = (t < jb && c < w) ? R[(J0 + t) + (long)(J0 + jb + c) * k] : 0.0;
-    }
-    __syncthreads();
-    const int nt = wpad >> 4;
-    const int ntiles = nt * (nt + 1) / 2;
-    for (int base = b * NW + wv; base < ntiles; base += 4 * NB * NW) {
-      int ti[4], tj[4];
-      bool okq[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int tile = base + q * NB * NW;
-        okq[q] = tile < ntiles;
-        int c = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
-        while (c * (c + 1) / 2 > tile) --c;
-        while ((c + 1) * (c + 2) / 2 <= tile) ++c;
-        tj[q] = c;
-        ti[q] = tile - c * (c + 1) / 2;
-      }
-      const int i_l = lane & 15, j_l = lane >> 4;
-      double old[4][4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = ti[q] * 16 + i_l, jj = tj[q] * 16 + j_l + 4 * v;
-          old[q][v] = (okq[q] && i < w && jj < w && i <= jj) ? R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] : 0.0;
-        }
-      v4d_f acc[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = v4d_f{0, 0, 0, 0};
-#pragma unroll
-      for (int kk = 0; kk < CB; kk += 4) {
-        const int t = kk + (lane >> 4);
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          acc[q] = MFf<double>::op(Ps[t * wps + tj[q] * 16 + i_l], Ps[t * wps + ti[q] * 16 + i_l], acc[q]);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = ti[q] * 16 + i_l, jj = tj[q] * 16 + j_l + 4 * v;
-          if (okq[q] && i < w && jj < w && i <= jj) R[(J0 + jb + i) + (long)(J0 + jb + jj) * k] = old[q][v] - acc[q][v];
-        }
-    }
-    if (!chol_grid_sync(bar, NB, gen)) {
-      ok = false;
-      break;
-    }
-  }
-  __syncthreads();
-  const int failed = ok ? sh_fail : -1;
-  dmax = block_max(dmax, red);
-  dmin = -block_max(-dmin, red);
-  dev = block_max(dev, red);
-  if (tid == 0 && dev > 0.0)  // non-negative doubles order like their bit patterns
-    atomicMax(reinterpret_cast<unsigned long long *>(&stats[1]), (unsigned long long)__double_as_longlong(dev));
-  if (b == 0 && tid == 0) {
-    flag[0] = failed;
-    stats[0] = failed ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
-    if (failed)
-      atomicMax(reinterpret_cast<unsigned long long *>(&stats[1]), (unsigned long long)__double_as_longlong(INFINITY));
-  }
-}
-
 constexpr int CHOL_MAXK = 512;
 
 void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats, unsigned long long *dbg, unsigned *bar) {
+                 double *R, int *flag, double *stats, unsigned long long *dbg) {
   TTG_CHECK(k >= 1 && k <= CHOL_MAXK, TTG_BAD_ARG, "Cholesky kernel supports 1 <= k <= 512");
   const int64_t wmax = std::max<int64_t>(k - CB, 0);
   const int64_t wps = ((wmax + 15) & ~(int64_t)15) + 1;
   const size_t lds = ((size_t)CB * CB + (size_t)CB * wps) * sizeof(double);
   ProfScope prof(KC_CHOL, s, (double)k * k * k / 3.0, 8.0 * 2.0 * (double)k * k);
-  if (bar && !dbg && k > 2 * CB) {
-    // several workgroups (cooperative launch; a failed barrier reports flag -1)
-    constexpr int NB = 8;
-    TTG_HIP(hipMemsetAsync(bar, 0, 3 * sizeof(unsigned), s));
-    TTG_HIP(hipFuncSetAttribute((const void *)k_chol_mc<NB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int kk = (int)k;
-    long lg = (long)ldg;
-    void *args[] = {&kk, (void *)&G, &lg, &shift_rel, &R, &flag, &stats, &bar};
-    TTG_HIP(hipLaunchCooperativeKernel((const void *)k_chol_mc<NB>, dim3(NB), dim3(CHOL_THREADS), args, lds, s));
-    return;
-  }
   static const bool v3 = [] {
     const char *e = std::getenv("TTG_CHOL_V3");
     return e && e[0] == '1';

@@ -182,8 +182,7 @@ void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, d
 // G is read from its upper triangle; shift = shift_rel * trace(G) / k.
 // flag[0] = 0 ok, j+1 if pivot j failed; stats[0] = diag spread, stats[1] = max|R - I|.
 void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats, unsigned long long *dbg = nullptr,
-                 unsigned *bar = nullptr);
+                 double *R, int *flag, double *stats, unsigned long long *dbg = nullptr);
 // Second CholeskyQR pass for G = Q1^T Q1 = I + E, small E: R2 = I + Phi(E)
 // (chol(G) to O(|E|^2)), Qout = Q1 R2^{-1} with the inverse series truncated
 // after `terms` powers of U; maxabs = max|E| for the acceptance test.

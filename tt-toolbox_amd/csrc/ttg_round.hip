@@ -63,7 +63,7 @@ template <class T> struct Rounder {
   int64_t gws_elems;
   double *hA, *hQ, *htau, *hws;
   int *perm, *iflag, *scal_i, *jerr, *jsweeps;
-  unsigned *bar, *cbar;
+  unsigned *bar;
   double *scal_d;
   int64_t max_core, kmax, rsq;
   bool mixed;
@@ -72,7 +72,6 @@ template <class T> struct Rounder {
   const std::function<void(int64_t, const T *, int64_t, T *)> *fmul = nullptr;
   int64_t fallbacks = 0, sweeps_total = 0, sweeps_max = 0, retries = 0, svd_skipped = 0;
   bool allow_notrunc = true;
-  bool chol_multi = false;  // Cholesky on several workgroups for k > 64
   int next_level = 2;  // QR level after the last rejected svd_bond
   bool noconv = false;
 
@@ -111,7 +110,6 @@ template <class T> struct Rounder {
     a.template take<int>(16);                // jerr
     a.template take<int>(16);                // jsweeps
     a.template take<unsigned>(jacobi_bar_words(kMaxSweeps));
-    a.template take<unsigned>(64);           // Cholesky grid barrier
     a.template take<double>(16);             // scal_d
   }
 
@@ -149,7 +147,6 @@ template <class T> struct Rounder {
     jerr = a.take<int>(16);
     jsweeps = a.take<int>(16);
     bar = a.take<unsigned>(jacobi_bar_words(kMaxSweeps));
-    cbar = a.take<unsigned>(64);
     scal_d = a.take<double>(16);
   }
 
@@ -182,7 +179,7 @@ template <class T> struct Rounder {
   void cholqr_pass(const T *A, int64_t m, int64_t k, double shift_rel, T *Qout, double *Rout,
                    int *flag, double *st2, bool g32) {
     gram(A, m, k, G, g32);
-    chol_factor(st, k, G, k, shift_rel, Rout, flag, st2, nullptr, chol_multi ? cbar : nullptr);
+    chol_factor(st, k, G, k, shift_rel, Rout, flag, st2);
     trsm_right_upper<T>(st, m, k, A, m, Rout, Qout, m);
   }
 
@@ -511,8 +508,6 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
   {
     const char *env = std::getenv("TTG_SVD_ALWAYS");
     R.allow_notrunc = !(env && env[0] == '1');
-    const char *env2 = std::getenv("TTG_CHOL_MULTI");
-    R.chol_multi = env2 && env2[0] == '1';
   }
   std::vector<int64_t> sizes(d);
   R.ofs.assign(d + 1, 0);

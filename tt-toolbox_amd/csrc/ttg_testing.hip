@@ -98,15 +98,14 @@ ttg_status ttg_test_householder(ttg_ctx *ctx, int64_t m, int64_t n, const double
 
 // R (k x k) upper Cholesky factor of G + shift
 ttg_status ttg_test_chol(ttg_ctx *ctx, int64_t k, const double *G, double shift_rel, double *R,
-                         int *flag, double *stats, int multi) {
+                         int *flag, double *stats) {
   GUARD(ctx, {
     hipStream_t s = ctx->stream;
     Frees f;
     double *dG = f.add(up(G, k * k, s)), *dR = f.add(up<double>(nullptr, k * k, s));
     double *dst = f.add(up<double>(nullptr, 8, s));
     int *dfl = f.add(up<int>(nullptr, 4, s));
-    unsigned *bar = multi ? f.add(up<unsigned>(nullptr, 64, s)) : nullptr;
-    chol_factor(s, k, dG, k, shift_rel, dR, dfl, dst, nullptr, bar);
+    chol_factor(s, k, dG, k, shift_rel, dR, dfl, dst);
     down(R, dR, k * k, s);
     down(flag, dfl, 1, s);
     down(stats, dst, 2, s);

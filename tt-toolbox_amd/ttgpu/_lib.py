@@ -104,7 +104,7 @@ def load(path: Path | str | None = None):
         "ttg_profile_end": (st, [P, I64P, DP, DP, DP]),
         "ttg_test_gemm": (st, [P, C.c_int, C.c_int, C.c_int, I64, I64, I64, P, I64, P, I64, P, I64]),
         "ttg_test_householder": (st, [P, I64, I64, P, P, P]),
-        "ttg_test_chol": (st, [P, I64, P, C.c_double, P, P, P, C.c_int]),
+        "ttg_test_chol": (st, [P, I64, P, C.c_double, P, P, P]),
         "ttg_test_jacobi": (st, [P, I64, P, C.c_int, P, I64, I64P, P, P]),
         "ttg_test_trsm": (st, [P, C.c_int, I64, I64, P, P, P]),
         "ttg_test_jacobi_timing": (st, [P, I64, P, P, I64]),
