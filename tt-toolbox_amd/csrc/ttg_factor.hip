@@ -1,12 +1,16 @@
 // Cholesky factorisation and triangular solve for Cholesky-QR (gfx950).
 //
-//   k_chol3 : one workgroup, G + shift = R^T R (upper R), blocked by 32:
-//             register-resident diagonal block (wave 0), forward-substitution
-//             panel staged in LDS, MFMA SYRK trailing update.
-//   k_trsm  : X = A R^{-1} (R upper, right side), 32 rows per workgroup held
-//             in LDS; per 32-column block J: B = A_J - X_{<J} R_{<J,J}
-//             (MFMA), then row-wise substitution with R_JJ (8 lanes per
-//             row).  No inverse of R is formed, so the residual
+//   k_chol4 : one workgroup, G + shift = R^T R (upper R), blocked by 32:
+//             register-resident diagonal block (wave 0, next pivot column
+//             updated through one readlane, row updates beside it), panel
+//             columns in registers of waves 1..7 solved every 4 pivots,
+//             MFMA SYRK trailing update.  k_chol3 (panel solved after the
+//             pivots) covers panels wider than 448 columns.
+//   k_trsm2 : X = A R^{-1} (R upper, right side), 32 rows per workgroup;
+//             per 32-column block J: B = A_J - X_{<J} R_{<J,J} (MFMA), then
+//             a right-looking substitution with R_JJ, one row per lane of
+//             wave 0, while waves 1..3 stage block J+1.  k_trsm covers
+//             k > 256.  No inverse of R is formed, so the residual
 //             ||X R - A|| stays O(u ||X|| ||R||) for any conditioning.
 #include "ttg_internal.h"
 #include "ttg_device.h"
