@@ -460,6 +460,246 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__r
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_chol5: left-looking form of k_chol4.  Block row J is brought up to date
+// just before it is factorised, C = R(0:J0, J-block)^T R(0:J0, J0:k) with
+// FP64 MFMA from the factored rows above (streamed through LDS, accumulated
+// in registers), instead of a right-looking read-modify-write of the whole
+// trailing matrix after every block; the pivot loop and the fused panel are
+// k_chol4's.
+// MAXT: output tiles per wave (2 * (k - 32 + 15) / 16 / 8 rounded up);
+// MAXE: staging elements per thread (32 * 16 * ntj / 512); US: staging row
+// stride (odd, > the widest staged block row; US * 32 + 16 * 4 * MAXT fits
+// the Dl + Ps allocation, so tiles past the block read inside it).
+// ---------------------------------------------------------------------------
+template <int MAXT, int MAXE, int US>
+__global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__restrict__ G, long ldg,
+                                                        double shift_rel, double *__restrict__ R,
+                                                        int *flag, double *stats,
+                                                        unsigned long long *dbg) {
+  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
+  double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
+  double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
+  double *Us = Dl;            // update staging: Us[t * US + c] = R(kc + t, J0 + c)
+  __shared__ double red[CHOL_THREADS / 64];
+  __shared__ double dinv[CB];
+  __shared__ int sh_fail;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int NW = CHOL_THREADS / 64;
+  double tr = 0.0;
+  for (int j = tid; j < k; j += CHOL_THREADS) tr += G[j + (long)j * ldg];
+  tr = block_sum(tr, red);
+  const double shift = shift_rel * tr / (k > 0 ? k : 1);
+  // R's strict lower triangle is zero; its upper triangle is written block
+  // row by block row (the update reads G itself, so nothing is copied)
+  for (int j = wv; j < k; j += NW)
+    for (int i = j + 1 + lane; i < k; i += 64) R[i + (long)j * k] = 0.0;
+  if (tid == 0) sh_fail = 0;
+  double dmax = 0.0, dmin = INFINITY, dev = 0.0;
+  for (int J0 = 0; J0 < k; J0 += CB) {
+    const int jb = min(CB, k - J0);
+    const int w = k - J0 - jb;  // > 0 only when jb == CB
+    const int wpad = (w + 15) & ~15, wps = wpad + 1;
+    __syncthreads();
+    if (dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    // Left-looking update of block row J: C = R(0:J0, J0:J0+32)^T R(0:J0, J0:k)
+    // by FP64 MFMA on 16x16 tiles (row tile ti in {0,1}, column tile tj),
+    // the factored rows above streamed through LDS 32 at a time (Us, odd
+    // row stride) with the next chunk loaded into registers meanwhile.
+    const int ntj = 2 + (wpad >> 4), ncols = ntj * 16;
+    const int ntiles = 2 * ntj;
+    // this wave's tiles: tile = wv + 8q, ti = wv & 1, tj = wv / 2 + 4q (LDS
+    // offsets affine in q and kk: immediates on one per-lane base)
+    const int ti = wv & 1, tj0 = wv >> 1;
+    v4d_f acc[MAXT];
+#pragma unroll
+    for (int q = 0; q < MAXT; ++q) acc[q] = v4d_f{0, 0, 0, 0};
+    // this block row's entries of G (upper triangle), loaded before the update
+    // so that their latency hides behind it
+    double gv[MAXT][4];
+#pragma unroll
+    for (int q = 0; q < MAXT; ++q) {
+      const int tj = tj0 + 4 * q;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = ti * 16 + (lane & 15), c = tj * 16 + (lane >> 4) + 4 * u;
+        const bool need = wv + NW * q < ntiles && r < jb && (c < CB ? (c < jb && r <= c) : c - CB < w);
+        gv[q][u] = need ? G[(J0 + r) + (long)(J0 + c) * ldg] : 0.0;
+      }
+    }
+    double pre[MAXE];
+    auto fetch = [&](int kc) {
+#pragma unroll
+      for (int e = 0; e < MAXE; ++e) {
+        const int idx = tid + CHOL_THREADS * e;
+        const int t = idx & (CB - 1), c = idx >> 5;
+        pre[e] = (c < ncols && J0 + c < k) ? R[(kc + t) + (long)(J0 + c) * k] : 0.0;
+      }
+    };
+    if (J0 > 0) fetch(0);
+    for (int kc = 0; kc < J0; kc += CB) {
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < MAXE; ++e) {
+        const int idx = tid + CHOL_THREADS * e;
+        const int t = idx & (CB - 1), c = idx >> 5;
+        if (c < ncols) Us[t * US + c] = pre[e];
+      }
+      __syncthreads();
+      if (kc + CB < J0) fetch(kc + CB);
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) {
+        const int tj = tj0 + 4 * q;
+        if (wv + NW * q < ntiles) {  // uniform per wave
+#pragma unroll
+          for (int kk = 0; kk < CB; kk += 4) {
+            const int t = kk + (lane >> 4);
+            acc[q] = MFf<double>::op(Us[t * US + tj * 16 + (lane & 15)], Us[t * US + ti * 16 + (lane & 15)], acc[q]);
+          }
+        }
+      }
+    }
+    __syncthreads();  // Us is dead: Dl and Ps reuse its space
+    // B = G rows - C into Dl (diagonal block, column-major, identity past jb)
+    // and Ps (panel rows, zero past w); acc[q][v] = C(ti*16 + lane%16, tj*16 + lane/16 + 4v)
+#pragma unroll
+    for (int q = 0; q < MAXT; ++q) {
+      const int tile = wv + NW * q;
+      if (tile < ntiles) {
+        const int tj = tj0 + 4 * q;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int r = ti * 16 + (lane & 15), c = tj * 16 + (lane >> 4) + 4 * u;
+          if (c < CB) {
+            Dl[c * CB + r] = (r < jb && c < jb) ? (r <= c ? gv[q][u] + (r == c ? shift : 0.0) - acc[q][u] : 0.0)
+                                                : (r == c ? 1.0 : 0.0);
+          } else if (c - CB < wpad) {
+            Ps[r * wps + (c - CB)] = (c - CB < w) ? gv[q][u] - acc[q][u] : 0.0;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const int j = lane & (CB - 1);
+    const int pt = tid - 64;
+    double v[CB];  // wave 0: column j of the diagonal block; waves 1..7: panel column pt
+    if (wv == 0) {
+#pragma unroll
+      for (int i = 0; i < CB; ++i) v[i] = Dl[j * CB + i];
+    } else if (w > 0) {
+      const int c1 = pt < wpad ? pt : 0;  // threads past the panel repeat column 0 and store nothing
+#pragma unroll
+      for (int t = 0; t < CB; ++t) v[t] = Ps[t * wps + c1];
+    }
+    unsigned failmask = 0;
+    double rc_prev = 0.0;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      if (wv == 0) {
+        // Pivot c.  Critical path: v[c] -> pivot -> row c -> v[c+1] (the next
+        // pivot column), using only this lane's values and one readlane.
+        // Row c-1's update of the later columns (LDS reads issued first,
+        // consumed after the pivot) runs beside it.
+        double rr[CB];
+        if (c >= 1) {
+#pragma unroll
+          for (int i = 0; i < CB; ++i)
+            if (i > c) rr[i] = Dl[(c - 1) * CB + i];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        double p = readlane_d(v[c], c);
+        // straight-line code only (branches here push v[] into scratch): a
+        // failed pivot is recorded in a bit mask and replaced by a harmless 1
+        const bool bad = !(p > 0.0) || !isfinite(p);
+        failmask |= bad ? (1u << c) : 0u;
+        p = bad ? 1.0 : p;
+        // 1/sqrt(p) from the hardware estimate and two Newton steps
+        double inv = __builtin_amdgcn_rsq(p);
+        {
+          const double h = 0.5 * p;
+          double e = __builtin_fma(-h * inv, inv, 0.5);
+          inv = __builtin_fma(inv, e, inv);
+          e = __builtin_fma(-h * inv, inv, 0.5);
+          inv = __builtin_fma(inv, e, inv);
+        }
+        const double dc = p * inv;
+        const double rc = (j == c) ? dc : (j > c ? v[c] * inv : 0.0);
+        v[c] = rc;
+        // every lane holds the same pivot; lanes 32..63 mirror lanes 0..31
+        Dl[c * CB + j] = rc;
+        dinv[c] = inv;
+        if (c + 1 < CB) v[c + 1] -= readlane_d(rc, c + 1) * rc;  // R(c, c+1) R(c, j)
+        if (c >= 1) {
+#pragma unroll
+          for (int i = 0; i < CB; ++i)
+            if (i > c) v[i] -= rr[i] * rc_prev;  // R(c-1, i) R(c-1, j)
+        }
+        rc_prev = rc;
+      }
+      if ((c & 3) == 3) {
+        __syncthreads();  // rows c-3..c of R and their 1 / r_cc visible to the panel waves
+        if (wv != 0 && w > 0) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int t = c - 3 + u;
+            const double x = v[t] * dinv[t];
+            v[t] = x;
+#pragma unroll
+            for (int l = 0; l < CB; ++l)
+              if (l > t) v[l] -= Dl[t * CB + l] * x;
+          }
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (wv == 0 && lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
+    if (wv != 0 && w > 0) {
+      if (pt < wpad) {
+#pragma unroll
+        for (int t = 0; t < CB; ++t) Ps[t * wps + pt] = v[t];
+      }
+    }
+    __syncthreads();
+    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
+    if (sh_fail) break;
+    // diagonal block to R (+ statistics)
+    for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), l = idx >> 5;
+      if (t < jb && l < jb && t <= l) {
+        const double val = Dl[t * CB + l];
+        R[(J0 + t) + (long)(J0 + l) * k] = val;
+        if (t == l) {
+          dmax = fmax(dmax, fabs(val));
+          dmin = fmin(dmin, fabs(val));
+          dev = fmax(dev, fabs(val - 1.0));
+        } else {
+          dev = fmax(dev, fabs(val));
+        }
+      }
+    }
+    if (w <= 0) continue;
+    // solved panel to R (coalesced: 32 consecutive threads write one column)
+    for (int idx = tid; idx < CB * w; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), c = idx >> 5;
+      const double val = Ps[t * wps + c];
+      R[(J0 + t) + (long)(J0 + jb + c) * k] = val;
+      dev = fmax(dev, fabs(val));
+    }
+    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
+  }
+  __syncthreads();
+  if (dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
+  const int failed = sh_fail;
+  dmax = block_max(dmax, red);
+  dmin = -block_max(-dmin, red);
+  dev = block_max(dev, red);
+  if (tid == 0) {
+    flag[0] = failed;
+    stats[0] = failed ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
+    stats[1] = failed ? INFINITY : dev;
+  }
+}
+
 constexpr int CHOL_MAXK = 512;
 
 void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
@@ -477,8 +717,25 @@ void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
     TTG_HIP(hipFuncSetAttribute((const void *)k_chol3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_chol3, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
   } else {
-    TTG_HIP(hipFuncSetAttribute((const void *)k_chol4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_chol4, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
+    static const bool right = [] {
+      const char *e = std::getenv("TTG_CHOL_RIGHT");
+      return e && e[0] == '1';
+    }();
+    if (right || k > 256) {
+      TTG_HIP(hipFuncSetAttribute((const void *)k_chol4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(k_chol4, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
+    } else {
+      // k <= 256: at most 2 * 16 output tiles (4 per wave), 32 x 224 staged
+      // elements (14 per thread), staging stride 225; the staging buffer and
+      // the reads of the tiles past a block (up to column 255) stay inside
+      // the allocation
+      constexpr int MAXT = 4, MAXE = 14, US = 225;
+      const size_t lds5 = std::max(lds, (size_t)(CB * US + 16 * 4 * MAXT) * sizeof(double));
+      TTG_HIP(hipFuncSetAttribute((const void *)k_chol5<MAXT, MAXE, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds5));
+      hipLaunchKernelGGL((k_chol5<MAXT, MAXE, US>), dim3(1), dim3(CHOL_THREADS), lds5, s, (int)k, G, (long)ldg,
+                         shift_rel, R, flag, stats, dbg);
+    }
   }
   TTG_HIP(hipGetLastError());
 }
