@@ -46,6 +46,7 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 constexpr int CB = 32;  // block size
+constexpr int TRS = CB + 1;  // odd LDS row stride of staged 32-row blocks
 
 // ---------------------------------------------------------------------------
 // Cholesky
@@ -482,7 +483,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
   double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
   double *Us = Dl;            // update staging: Us[t * US + c] = R(kc + t, J0 + c)
   __shared__ double red[CHOL_THREADS / 64];
-  __shared__ double dinv[CB];
+  __shared__ double Wt[CB * TRS];  // Wt[x * TRS + y] = W(y, x), W = R_JJ^{-T}
   __shared__ int sh_fail;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   constexpr int NW = CHOL_THREADS / 64;
@@ -499,7 +500,8 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
   for (int J0 = 0; J0 < k; J0 += CB) {
     const int jb = min(CB, k - J0);
     const int w = k - J0 - jb;  // > 0 only when jb == CB
-    const int wpad = (w + 15) & ~15, wps = wpad + 1;
+    const int wpad = (w + 15) & ~15;
+    constexpr int wps = US;  // panel row stride (odd, > the widest panel)
     __syncthreads();
     if (dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
     // Left-looking update of block row J: C = R(0:J0, J0:J0+32)^T R(0:J0, J0:k)
@@ -581,15 +583,20 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
     }
     __syncthreads();
     const int j = lane & (CB - 1);
-    const int pt = tid - 64;
-    double v[CB];  // wave 0: column j of the diagonal block; waves 1..7: panel column pt
+    // Wave 0: lanes 0..31 factorise the diagonal block (lane j holds column
+    // j); lanes 32..63 run the same eliminations on the columns of the
+    // identity, so that they end with W = R_JJ^{-T}, which the panel below
+    // is multiplied by (instead of a 32-step substitution per panel column).
+    const bool idl = lane >= CB;
+    double *const wrow = idl ? Wt + j * TRS : Dl + j;
+    double v[CB];
     if (wv == 0) {
+      // (the identity columns are rebuilt in every block: as loop
+      // invariants they would stay live across the whole kernel)
+      int jj = j;
+      asm volatile("" : "+v"(jj));
 #pragma unroll
-      for (int i = 0; i < CB; ++i) v[i] = Dl[j * CB + i];
-    } else if (w > 0) {
-      const int c1 = pt < wpad ? pt : 0;  // threads past the panel repeat column 0 and store nothing
-#pragma unroll
-      for (int t = 0; t < CB; ++t) v[t] = Ps[t * wps + c1];
+      for (int i = 0; i < CB; ++i) v[i] = idl ? (i == jj ? 1.0 : 0.0) : Dl[j * CB + i];
     }
     unsigned failmask = 0;
     double rc_prev = 0.0;
@@ -623,40 +630,50 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
           inv = __builtin_fma(inv, e, inv);
         }
         const double dc = p * inv;
-        const double rc = (j == c) ? dc : (j > c ? v[c] * inv : 0.0);
+        const double rc = idl ? v[c] * inv : ((j == c) ? dc : (j > c ? v[c] * inv : 0.0));
         v[c] = rc;
-        // every lane holds the same pivot; lanes 32..63 mirror lanes 0..31
-        Dl[c * CB + j] = rc;
-        dinv[c] = inv;
-        if (c + 1 < CB) v[c + 1] -= readlane_d(rc, c + 1) * rc;  // R(c, c+1) R(c, j)
+        // row c of R_JJ (lanes 0..31) / entry (c, j) of W (lanes 32..63)
+        // (one LDS address per lane: identity lanes write W, the others R_JJ)
+        *(wrow + (idl ? c : c * CB)) = rc;
+        if (c + 1 < CB) v[c + 1] -= readlane_d(rc, c + 1) * rc;  // R(c, c+1) x_j
         if (c >= 1) {
 #pragma unroll
           for (int i = 0; i < CB; ++i)
-            if (i > c) v[i] -= rr[i] * rc_prev;  // R(c-1, i) R(c-1, j)
+            if (i > c) v[i] -= rr[i] * rc_prev;  // R(c-1, i) x_j
         }
         rc_prev = rc;
-      }
-      if ((c & 3) == 3) {
-        __syncthreads();  // rows c-3..c of R and their 1 / r_cc visible to the panel waves
-        if (wv != 0 && w > 0) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int t = c - 3 + u;
-            const double x = v[t] * dinv[t];
-            v[t] = x;
-#pragma unroll
-            for (int l = 0; l < CB; ++l)
-              if (l > t) v[l] -= Dl[t * CB + l] * x;
-          }
-        }
       }
       __builtin_amdgcn_sched_barrier(0);
     }
     if (wv == 0 && lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
-    if (wv != 0 && w > 0) {
-      if (pt < wpad) {
+    __syncthreads();  // R_JJ in Dl, W in Wt
+    if (!sh_fail && w > 0) {
+      // panel rows of R: P = W B (B: the updated panel rows in Ps), FP64 MFMA
+      // on 16x16 tiles: tile = wv + 8q, ti = wv & 1, tj = wv / 2 + 4q
+      const int ntp = 2 * (wpad >> 4);
+      const int ti = wv & 1, tj0 = wv >> 1;
+      v4d_f pacc[MAXT];
 #pragma unroll
-        for (int t = 0; t < CB; ++t) Ps[t * wps + pt] = v[t];
+      for (int q = 0; q < MAXT; ++q) {
+        pacc[q] = v4d_f{0, 0, 0, 0};
+        const int tj = tj0 + 4 * q;
+        if (wv + NW * q < ntp) {  // uniform per wave
+#pragma unroll
+          for (int kk = 0; kk < CB; kk += 4) {
+            const int t = kk + (lane >> 4);
+            pacc[q] = MFf<double>::op(Ps[t * wps + tj * 16 + (lane & 15)], Wt[t * TRS + ti * 16 + (lane & 15)], pacc[q]);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) {
+        const int tj = tj0 + 4 * q;
+        if (wv + NW * q < ntp) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            Ps[(ti * 16 + (lane & 15)) * wps + tj * 16 + (lane >> 4) + 4 * u] = pacc[q][u];
+        }
       }
     }
     __syncthreads();
@@ -726,11 +743,11 @@ void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
       hipLaunchKernelGGL(k_chol4, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
     } else {
       // k <= 256: at most 2 * 16 output tiles (4 per wave), 32 x 224 staged
-      // elements (14 per thread), staging stride 225; the staging buffer and
-      // the reads of the tiles past a block (up to column 255) stay inside
-      // the allocation
+      // elements (14 per thread), staging and panel row stride 225; Dl + Ps
+      // at that stride (8224 doubles) hold the staging buffer and the reads
+      // of the tiles past a block (up to index 7230)
       constexpr int MAXT = 4, MAXE = 14, US = 225;
-      const size_t lds5 = std::max(lds, (size_t)(CB * US + 16 * 4 * MAXT) * sizeof(double));
+      const size_t lds5 = std::max(lds, (size_t)(CB * CB + CB * US) * sizeof(double));
       TTG_HIP(hipFuncSetAttribute((const void *)k_chol5<MAXT, MAXE, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds5));
       hipLaunchKernelGGL((k_chol5<MAXT, MAXE, US>), dim3(1), dim3(CHOL_THREADS), lds5, s, (int)k, G, (long)ldg,
@@ -880,8 +897,6 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
 // block J no longer reads).  Staged blocks use an odd LDS row stride (33):
 // column and row accesses are conflict-free.
 // ---------------------------------------------------------------------------
-constexpr int TRS = CB + 1;
-
 __host__ __device__ constexpr size_t trsm2_lds(int kp, size_t tsz) {
   return (size_t)TRS * kp * tsz + (size_t)(kp > CB ? kp - CB : 0) * TRS * tsz + 2 * (size_t)CB * TRS * 8;
 }
