@@ -1034,42 +1034,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   if (rec) dbg[2 + 8 * (kp / CB)] = __builtin_amdgcn_s_memrealtime();
 }
 
-// ---------------------------------------------------------------------------
-// Near-identity second pass of CholeskyQR2: G = Q1^T Q1 = I + E with small E.
-// The upper factor R = I + U of G solves U + U^T + U^T U = E; U = Phi(E),
-// Phi(X) = triu(X, 1) + diag(X)/2, misses it by O(|E|^2).  R^{-1} is
-// I - U (+ U^2 with terms = 2), off by O(|U|^2) (O(|U|^3)), and Q = Q1 R^{-1}
-// is one GEMM; R is so close to I that the explicit inverse costs no
-// accuracy.  The host accepts the pass only if max|E| is below the bound for
-// which these truncations stay at rounding level (or at the form's error
-// budget).
-// ---------------------------------------------------------------------------
-__global__ void k_nearid_init(int k, double *__restrict__ G, double *__restrict__ U, double *maxabs) {
-  double mx = 0.0;
-  const long total = (long)k * k;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    const long c = idx / k, r = idx - c * k;
-    const double e = G[idx] - (r == c ? 1.0 : 0.0);
-    U[idx] = (r < c) ? e : (r == c ? 0.5 * e : 0.0);
-    mx = fmax(mx, fabs(e));
-  }
-  __shared__ double red[4];
-  mx = block_max(mx, red);  // one atomic per workgroup
-  if (threadIdx.x == 0 && mx > 0.0)  // non-negative doubles order like their bit patterns
-    atomicMax(reinterpret_cast<unsigned long long *>(maxabs), (unsigned long long)__double_as_longlong(mx));
-}
-// X = I + a U + b P (P may be null), R2 = I + U
-__global__ void k_nearid_fin(int k, double *__restrict__ X, const double *__restrict__ U,
-                             const double *__restrict__ P, double *__restrict__ R2) {
-  const long total = (long)k * k;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    const long c = idx / k, r = idx - c * k;
-    const double id = (r == c ? 1.0 : 0.0), u = U[idx];
-    X[idx] = id - u + (P ? P[idx] : 0.0);
-    R2[idx] = id + u;
-  }
-}
-
 // X = I + sign * U (U == nullptr: X = I)
 __global__ void k_eye_plus(int k, double *__restrict__ X, const double *__restrict__ U, double sign) {
   const long total = (long)k * k;
@@ -1083,22 +1047,6 @@ void eye_plus(hipStream_t s, int64_t k, double *X, const double *U, double sign)
   const int blocks = (int)std::min<long>((total + 255) / 256, 1024);
   hipLaunchKernelGGL(k_eye_plus, dim3(blocks), dim3(256), 0, s, (int)k, X, U, sign);
   TTG_HIP(hipGetLastError());
-}
-
-void nearid_pass(hipStream_t s, int64_t m, int64_t k, double *G, const double *Q1, double *Qout,
-                 double *R2, double *U, double *P, double *X, double *maxabs, int terms, double *gws,
-                 int64_t gws_elems, int num_cus) {
-  const long total = k * k;
-  const int blocks = (int)std::min<long>((total + 255) / 256, 1024);
-  TTG_HIP(hipMemsetAsync(maxabs, 0, sizeof(double), s));
-  hipLaunchKernelGGL(k_nearid_init, dim3((int)std::min<long>((total + 1023) / 1024, 128)), dim3(256), 0, s, (int)k, G,
-                     U, maxabs);
-  TTG_HIP(hipGetLastError());
-  if (terms >= 2) gemm<double>(s, false, false, k, k, k, 1.0, U, k, U, k, 0.0, P, k, gws, gws_elems, num_cus);
-  hipLaunchKernelGGL(k_nearid_fin, dim3(blocks), dim3(256), 0, s, (int)k, X, (const double *)U,
-                     terms >= 2 ? (const double *)P : nullptr, R2);
-  TTG_HIP(hipGetLastError());
-  gemm<double>(s, false, false, m, k, k, 1.0, Q1, m, X, k, 0.0, Qout, m, gws, gws_elems, num_cus);
 }
 
 template <class T>

@@ -183,13 +183,12 @@ void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, d
 // flag[0] = 0 ok, j+1 if pivot j failed; stats[0] = diag spread, stats[1] = max|R - I|.
 void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
                  double *R, int *flag, double *stats, unsigned long long *dbg = nullptr);
-// Second CholeskyQR pass for G = Q1^T Q1 = I + E, small E: R2 = I + Phi(E)
-// (chol(G) to O(|E|^2)), Qout = Q1 R2^{-1} with the inverse series truncated
-// after `terms` powers of U; maxabs = max|E| for the acceptance test.
-void nearid_pass(hipStream_t s, int64_t m, int64_t k, double *G, const double *Q1, double *Qout,
-                 double *R2, double *U, double *P, double *X, double *maxabs, int terms, double *gws,
-                 int64_t gws_elems, int num_cus);
 // X (m x k) = A R^{-1}, R upper triangular (FP64), blocked substitution.
+// Gram of A (m x k, FP32 MFMA when g32) fused with the near-identity second
+// CholeskyQR pass: X = I - Phi(E), R2 = I + Phi(E), *maxabs = max|E| (G is
+// scratch for the unsplit case; ws holds split-K partials)
+void gram_nearid(hipStream_t st, int64_t m, int64_t k, const double *A, bool g32, double *G, double *X,
+                 double *R2, double *maxabs, double *ws, int64_t ws_elems, int cus);
 // FP32 operands / FP64 operands rounded to FP32 while staged; FP32 MFMA; FP64 result
 void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
                      const float *A, int64_t lda, const float *B, int64_t ldb, double *C,

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cfloat>
 #include <algorithm>
+#include <type_traits>
 
 namespace ttg {
 
@@ -261,6 +262,76 @@ void gemm_f64_via_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int
                       const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
                       int64_t ldc, float *ws, int64_t wse, int cus) {
   gemm_impl<double, float, double>(s, ta, tb, M, N, K, 1.0f, A, lda, B, ldb, 0.0f, C, ldc, ws, wse, cus);
+}
+
+// ---------------------------------------------------------------------------
+// Near-identity second pass of CholeskyQR2, fused with the split-K sum of its
+// Gram product G = Q1^T Q1 = I + E (small E).  The upper factor R2 = I + U of
+// G solves U + U^T + U^T U = E; U = Phi(E), Phi(X) = triu(X, 1) + diag(X)/2,
+// misses it by O(|E|^2), and R2^{-1} = I - U to O(|U|^2), so Q = Q1 R2^{-1}
+// is one GEMM; R2 is so close to I that the explicit inverse costs no
+// accuracy.  The host accepts the pass only if max|E| is below the bound for
+// which these truncations stay at rounding level (or at the form's error
+// budget).  One kernel: G = sum_z P_z (fixed order, FP64), X = I - U,
+// R2 = I + U, maxabs = max|E| (one atomic per workgroup; zeroed first).
+// ---------------------------------------------------------------------------
+template <class TP>
+__global__ __launch_bounds__(256) void k_gram_nearid(int k, int S, const TP *__restrict__ P, double *__restrict__ X,
+                                                     double *__restrict__ R2, double *maxabs) {
+  __shared__ double red[4];
+  const long total = (long)k * k;
+  double mx = 0.0;
+  for (long idx = blockIdx.x * 256L + threadIdx.x; idx < total; idx += (long)gridDim.x * 256L) {
+    double g = 0.0;
+    for (int z = 0; z < S; ++z) g += (double)P[(long)z * total + idx];
+    const long c = idx / k, r = idx - c * k;
+    const double id = (r == c) ? 1.0 : 0.0;
+    const double e = g - id;
+    const double u = (r < c) ? e : (r == c ? 0.5 * e : 0.0);
+    X[idx] = id - u;
+    R2[idx] = id + u;
+    mx = fmax(mx, fabs(e));
+  }
+  mx = block_max(mx, red);
+  if (threadIdx.x == 0 && mx > 0.0)  // non-negative doubles order like their bit patterns
+    atomicMax(reinterpret_cast<unsigned long long *>(maxabs), (unsigned long long)__double_as_longlong(mx));
+}
+
+template <class T>
+static void gram_nearid_impl(hipStream_t st, int64_t m, int64_t k, const double *A, double *G, double *X,
+                             double *R2, double *maxabs, T *ws, int64_t ws_elems, int cus) {
+  TTG_HIP(hipMemsetAsync(maxabs, 0, sizeof(double), st));
+  int S = splitk_count(k, k, m, cus);
+  if (S > 1 && (ws == nullptr || (int64_t)S * k * k > ws_elems)) S = 1;
+  const long total = (long)k * k;
+  const int blocks = (int)std::min<long>((total + 1023) / 1024, 128);
+  if (S > 1) {
+    int kchunk = (int)(((m + S - 1) / S + GBK - 1) / GBK * GBK);
+    S = (int)((m + kchunk - 1) / kchunk);
+    dim3 grid((unsigned)((k + GBM - 1) / GBM), (unsigned)((k + GBN - 1) / GBN), (unsigned)S);
+    {
+      ProfScope prof(KC_GEMM, st, 2.0 * (double)k * k * m, 8.0 * (2.0 * m * k + k * k));
+      hipLaunchKernelGGL((k_gemm<double, T, double, true, false>), grid, dim3(256), 0, st, (int)k, (int)k, (int)m,
+                         kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws);
+      TTG_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL((k_gram_nearid<T>), dim3(blocks), dim3(256), 0, st, (int)k, S, (const T *)ws, X, R2, maxabs);
+  } else {
+    if constexpr (std::is_same<T, double>::value)
+      gemm<double>(st, true, false, k, k, m, 1.0, A, m, A, m, 0.0, G, k, nullptr, 0, cus);
+    else
+      gemm_f64_via_f32(st, true, false, k, k, m, A, m, A, m, G, k, nullptr, 0, cus);
+    hipLaunchKernelGGL((k_gram_nearid<double>), dim3(blocks), dim3(256), 0, st, (int)k, 1, (const double *)G, X, R2,
+                       maxabs);
+  }
+  TTG_HIP(hipGetLastError());
+}
+void gram_nearid(hipStream_t st, int64_t m, int64_t k, const double *A, bool g32, double *G, double *X,
+                 double *R2, double *maxabs, double *ws, int64_t ws_elems, int cus) {
+  if (g32)
+    gram_nearid_impl<float>(st, m, k, A, G, X, R2, maxabs, reinterpret_cast<float *>(ws), 2 * ws_elems, cus);
+  else
+    gram_nearid_impl<double>(st, m, k, A, G, X, R2, maxabs, ws, ws_elems, cus);
 }
 
 // ---------------------------------------------------------------------------

@@ -186,7 +186,7 @@ template <class T> struct Rounder {
   // Tall QR (m >= k): Qout (m x k, T), Rout (k x k, FP64), by CholeskyQR2.
   // flags[0..1] and stt[0..3] feed the acceptance test (qr_accept), checked
   // by the caller.  QR level 0: the second pass by the near-identity series
-  // (nearid_pass): R2 = I + Phi(E), Q = Q1 (I - Phi(E)).  FP64 Grams:
+  // (gram_nearid): R2 = I + Phi(E), Q = Q1 (I - Phi(E)).  FP64 Grams:
   // accepted if max|E| < 1e-8 (both truncations O(1e-16)).  The mixed form's
   // sweep-1 bonds (g32): Gram products in FP32 MFMA (their error only
   // perturbs the orthogonality of Q; every Q = A R^{-1} step is FP64),
@@ -215,8 +215,8 @@ template <class T> struct Rounder {
       TTG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
     }
     if (nearid_level(level)) {
-      gram(Q1, m, k, G, g32);
-      nearid_pass(st, m, k, G, Q1, Qout, R2, R3, nullptr, Rinv, stt + 3, 1, gws, gws_elems, cus);
+      gram_nearid(st, m, k, Q1, g32, G, Rinv, R2, stt + 3, gws, gws_elems, cus);
+      g(false, false, m, k, k, Q1, m, Rinv, k, Qout, m);  // Q = Q1 R2^{-1}
     } else {
       cholqr_pass(Q1, m, k, 0.0, Qout, R2, flags + 1, stt + 2, g32);
     }
