@@ -931,7 +931,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       const int idx = tid - FIRST + NTHR * e;
       const int lo = idx & (CB - 1), hi = idx >> 5;
       const bool in = idx < CB * CB;
-      va[e] = (in && r0 + lo < m && hi < jb) ? A[(r0 + lo) + (long)(J0 + hi) * lda] : T(0);
+      va[e] = (in && r0 + lo < m && hi < jb)
+                  ? (A ? A[(r0 + lo) + (long)(J0 + hi) * lda] : T(r0 + lo == J0 + hi ? 1 : 0))
+                  : T(0);
       vd[e] = (in && lo < jb && hi < jb) ? R[(J0 + lo) + (long)(J0 + hi) * k] : (lo == hi ? 1.0 : 0.0);
     }
 #pragma unroll
@@ -1055,11 +1057,8 @@ void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t l
   if (m <= 0 || k <= 0) return;
   const int64_t kp = (k + CB - 1) / CB * CB;
   ProfScope prof(KC_TRTRI, s, (double)m * k * k, (double)sizeof(T) * 2.0 * m * k + 8.0 * k * k);
-  static const bool v1 = [] {
-    const char *e = std::getenv("TTG_TRSM_V1");
-    return e && e[0] == '1';
-  }();
-  if (kp <= 256 && !v1) {
+  TTG_CHECK(A || kp <= 256, TTG_BAD_ARG, "identity right-hand side needs k <= 256");
+  if (kp <= 256) {
     const size_t lds2 = trsm2_lds((int)kp, sizeof(T));
     TTG_HIP(hipFuncSetAttribute((const void *)k_trsm2<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
     hipLaunchKernelGGL((k_trsm2<T>), dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds2, s, (int)m, (int)k,

@@ -208,11 +208,15 @@ template <class T> struct Rounder {
     if (bound && nearid_level(level)) {
       TTG_HIP(hipEventRecord(ctx->ev_fork, st));
       TTG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-      eye_plus(ctx->side, k, R4, nullptr, 0.0);
-      trsm_right_upper<double>(ctx->side, k, k, R4, k, R1, Rtmp, k);  // Rtmp = R1^{-1}
-      fro_norm<double>(ctx->side, R1, k * k, scal_d + 0);
+      if (k <= 256) {
+        trsm_right_upper<double>(ctx->side, k, k, nullptr, k, R1, Rtmp, k);  // Rtmp = I R1^{-1}
+      } else {
+        eye_plus(ctx->side, k, R4, nullptr, 0.0);
+        trsm_right_upper<double>(ctx->side, k, k, R4, k, R1, Rtmp, k);
+      }
       fro_norm<double>(ctx->side, Rtmp, k * k, scal_d + 1);
       TTG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+      fro_norm<double>(st, R1, k * k, scal_d + 0);  // the main stream has the shorter chain
     }
     if (nearid_level(level)) {
       gram_nearid(st, m, k, Q1, g32, G, Rinv, R2, stt + 3, gws, gws_elems, cus);
