@@ -18,6 +18,10 @@ ttg_status ttg_test_householder(ttg_ctx *ctx, int64_t m, int64_t n, const double
                                 double *R);
 ttg_status ttg_test_chol(ttg_ctx *ctx, int64_t k, const double *G, double shift_rel, double *R,
                          int *flag, double *stats);
+// R = chol(G), X = A R^{-1} through the block inverses the factorisation leaves
+// (*used_w = 1 when it left them: k <= 256)
+ttg_status ttg_test_chol_trsm(ttg_ctx *ctx, int64_t m, int64_t k, const double *G, const double *A,
+                              double *R, double *X, int *used_w);
 ttg_status ttg_test_trsm(ttg_ctx *ctx, int prec, int64_t m, int64_t k, const double *A,
                          const double *R, double *X);
 ttg_status ttg_test_jacobi(ttg_ctx *ctx, int64_t k, const double *S, int transS, double *Y,

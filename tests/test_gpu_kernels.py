@@ -101,6 +101,31 @@ def test_cholesky_flags_indefinite(gpu_ctx):
     assert np.isinf(stats[0]) and np.isinf(stats[1])
 
 
+@pytest.mark.parametrize("m,k,logcond", [(2048, 256, 3), (2048, 256, 7), (300, 40, 10), (256, 256, 5),
+                                          (1000, 100, 12), (77, 33, 4)])
+def test_chol_then_trsm_with_block_inverses(gpu_ctx, m, k, logcond):
+    """One CholeskyQR pass: R = chol(A^T A), X = A R^{-1} through the
+    inverses of R's diagonal blocks plus one refinement step; the residual
+    ||X R - A|| stays at rounding level however ill-conditioned A is, and
+    X matches LAPACK's solve to kappa u."""
+    rng = np.random.default_rng(m + 3 * k)
+    U, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    V, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    A = np.asfortranarray((U * np.logspace(0, -logcond / 2, k)) @ V.T)
+    G = np.asfortranarray(A.T @ A)
+    R = np.zeros((k, k), order="F")
+    X = np.zeros((m, k), order="F")
+    used = np.zeros(1, dtype=np.int32)
+    gpu_ctx.check(gpu_ctx.lib.ttg_test_chol_trsm(gpu_ctx.h, m, k, f64p(G), f64p(A), f64p(R), f64p(X), f64p(used)))
+    assert int(used[0]) == 1
+    u = np.finfo(float).eps
+    res = np.linalg.norm(X @ R - A) / (np.linalg.norm(X) * np.linalg.norm(R))
+    assert res <= 50 * u, res
+    ref = np.linalg.solve(R.T, A.T).T
+    cond = np.linalg.cond(R)
+    assert np.linalg.norm(X - ref) <= 50 * cond * u * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("prec,m,k,logcond", [(0, 2048, 256, 8), (0, 300, 40, 12), (1, 2048, 256, 3), (0, 1000, 320, 6), (0, 50, 7, 10)])
 def test_trsm_backward_stable(gpu_ctx, prec, m, k, logcond):
     """X = A R^{-1}: the residual ||X R - A|| stays O(u ||X|| ||R||) even for ill-conditioned R."""

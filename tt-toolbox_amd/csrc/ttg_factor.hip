@@ -1,11 +1,13 @@
 // Cholesky factorisation and triangular solve for Cholesky-QR (gfx950).
 //
-//   k_chol4 : one workgroup, G + shift = R^T R (upper R), blocked by 32:
-//             register-resident diagonal block (wave 0, next pivot column
-//             updated through one readlane, row updates beside it), panel
-//             columns in registers of waves 1..7 solved every 4 pivots,
-//             MFMA SYRK trailing update.  k_chol3 (panel solved after the
-//             pivots) covers panels wider than 448 columns.
+//   k_chol5 : k <= 256, one workgroup, G + shift = R^T R (upper R), blocked by
+//             32, left-looking: each block row updated from the factored rows
+//             above by FP64 MFMA just before its diagonal block is factorised
+//             in registers of wave 0 (which also forms R_JJ^{-T}); the panel
+//             is one MFMA product with R_JJ^{-T}.
+//   k_chol4 : k <= 480, right-looking: panel columns solved in registers of
+//             waves 1..7 beside the pivot loop, MFMA SYRK trailing update.
+//   k_chol3 : any k <= 512, right-looking, panel solved after the pivots.
 //   k_trsm2 : X = A R^{-1} (R upper, right side), 32 rows per workgroup;
 //             per 32-column block J: B = A_J - X_{<J} R_{<J,J} (MFMA), then
 //             a right-looking substitution with R_JJ, one row per lane of
@@ -477,7 +479,7 @@ template <int MAXT, int MAXE, int US>
 __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__restrict__ G, long ldg,
                                                         double shift_rel, double *__restrict__ R,
                                                         int *flag, double *stats,
-                                                        unsigned long long *dbg) {
+                                                        unsigned long long *dbg, double *Wout) {
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
   double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
@@ -647,6 +649,8 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
     }
     if (wv == 0 && lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
     __syncthreads();  // R_JJ in Dl, W in Wt
+    if (Wout)  // W_J for the triangular solves with this factor
+      for (int idx = tid; idx < CB * TRS; idx += CHOL_THREADS) Wout[(long)(J0 / CB) * (CB * TRS) + idx] = Wt[idx];
     if (!sh_fail && w > 0) {
       // panel rows of R: P = W B (B: the updated panel rows in Ps), FP64 MFMA
       // on 16x16 tiles: tile = wv + 8q, ti = wv & 1, tj = wv / 2 + 4q
@@ -719,8 +723,9 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
 
 constexpr int CHOL_MAXK = 512;
 
-void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats, unsigned long long *dbg) {
+bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
+                 double *R, int *flag, double *stats, unsigned long long *dbg, double *Wout) {
+  bool wrote_w = false;
   TTG_CHECK(k >= 1 && k <= CHOL_MAXK, TTG_BAD_ARG, "Cholesky kernel supports 1 <= k <= 512");
   const int64_t wmax = std::max<int64_t>(k - CB, 0);
   const int64_t wps = ((wmax + 15) & ~(int64_t)15) + 1;
@@ -751,10 +756,12 @@ void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
       TTG_HIP(hipFuncSetAttribute((const void *)k_chol5<MAXT, MAXE, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds5));
       hipLaunchKernelGGL((k_chol5<MAXT, MAXE, US>), dim3(1), dim3(CHOL_THREADS), lds5, s, (int)k, G, (long)ldg,
-                         shift_rel, R, flag, stats, dbg);
+                         shift_rel, R, flag, stats, dbg, Wout);
+      wrote_w = Wout != nullptr;
     }
   }
   TTG_HIP(hipGetLastError());
+  return wrote_w;
 }
 
 // ---------------------------------------------------------------------------
@@ -881,6 +888,154 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
   if (rec) {
     __builtin_amdgcn_s_waitcnt(0);
     dbg[3 + 3 * (kp / CB)] = __builtin_amdgcn_s_memrealtime();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_trsm3: X = A R^{-1} (FP64, k <= 256) when the Cholesky kernel has left the
+// inverses W_J = R_JJ^{-T} of the diagonal blocks (Winv, TRS-strided, W_J(c, s)
+// at [J][s * TRS + c]).  32 rows per workgroup, 4 waves, one 16 x 16 tile of
+// each 32 x 32 block per wave.  Per block J, all on FP64 MFMA:
+//   B   = A_J - X_{<J} R_{<J,J}
+//   X0  = B W^T                      (= B R_JJ^{-1} to the accuracy of W)
+//   X_J = X0 + (B - X0 R_JJ) W^T     (one refinement step: the error of the
+//                                     explicit block inverse, O(kappa(R_JJ) u),
+//                                     is squared away, so the residual
+//                                     ||X R - A|| stays at rounding level)
+// instead of a 32-step substitution chain.  The next block's A_J, R_JJ, W_J
+// and panel R(0:J0, J) are loaded into registers while the current block
+// multiplies.  Odd LDS row stride (33) throughout.  A == nullptr: A = I.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr size_t trsm3_lds(int kp) {
+  return ((size_t)TRS * kp + (size_t)(kp > CB ? kp - CB : 0) * TRS + 4 * (size_t)CB * TRS) * sizeof(double);
+}
+
+__global__ __launch_bounds__(256) void k_trsm3(int m, int k, const double *A, long lda,
+                                               const double *__restrict__ R, const double *__restrict__ Winv,
+                                               double *__restrict__ X, long ldx) {
+  extern __shared__ __align__(16) char smem[];
+  typedef v4d_f V;
+  const int kp = (k + CB - 1) / CB * CB;
+  double *Xs = reinterpret_cast<double *>(smem);  // Xs[col * TRS + row]: A_J, then B_J, then X_J
+  double *Rs = Xs + (size_t)TRS * kp;              // Rs[t * TRS + n] = R(t, J0 + n), t < J0
+  double *Ds = Rs + (size_t)(kp > CB ? kp - CB : 0) * TRS;  // Ds[a * TRS + b] = R(J0 + a, J0 + b)
+  double *Wb = Ds + CB * TRS;                      // Wb[s * TRS + c] = W_J(c, s)
+  double *X0s = Wb + CB * TRS;                     // X0s[c * TRS + r] = X0(r, c)
+  double *Rb = X0s + CB * TRS;                     // Rb[c * TRS + r] = (B - X0 R_JJ)(r, c)
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tr = wv & 1, tc = wv >> 1;
+  const int il = lane & 15;
+  const int r0 = blockIdx.x * CB;
+  constexpr int MAXP = 4 * (256 / CB - 1);  // panel elements per thread (J0 <= 224)
+  double pa[4], pd[4], pw[5], pr[MAXP];
+  auto fetch = [&](int J0) {
+    const int jb = min(CB, k - J0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int lo = idx & (CB - 1), hi = idx >> 5;
+      pa[e] = (r0 + lo < m && hi < jb) ? (A ? A[(r0 + lo) + (long)(J0 + hi) * lda] : (r0 + lo == J0 + hi ? 1.0 : 0.0))
+                                       : 0.0;
+      pd[e] = (lo < jb && hi < jb) ? R[(J0 + lo) + (long)(J0 + hi) * k] : (lo == hi ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const int idx = tid + 256 * e;
+      pw[e] = idx < CB * TRS ? Winv[(long)(J0 / CB) * (CB * TRS) + idx] : 0.0;
+    }
+    const int tot = J0 * CB;
+#pragma unroll
+    for (int e = 0; e < MAXP; ++e) {
+      const int idx = tid + 256 * e;
+      const int t = ((idx >> 10) << 5) + (idx & (CB - 1)), n = (idx >> 5) & (CB - 1);
+      pr[e] = (idx < tot && n < jb) ? R[t + (long)(J0 + n) * k] : 0.0;
+    }
+  };
+  fetch(0);
+  for (int J0 = 0; J0 < k; J0 += CB) {
+    const int jb = min(CB, k - J0);
+    __syncthreads();  // the previous block is done with Ds, Wb, Rs
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int lo = idx & (CB - 1), hi = idx >> 5;
+      Xs[(J0 + hi) * TRS + lo] = pa[e];
+      Ds[lo * TRS + hi] = pd[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const int idx = tid + 256 * e;
+      if (idx < CB * TRS) Wb[idx] = pw[e];
+    }
+    {
+      const int tot = J0 * CB;
+#pragma unroll
+      for (int e = 0; e < MAXP; ++e) {
+        const int idx = tid + 256 * e;
+        const int t = ((idx >> 10) << 5) + (idx & (CB - 1)), n = (idx >> 5) & (CB - 1);
+        if (idx < tot) Rs[t * TRS + n] = pr[e];
+      }
+    }
+    __syncthreads();
+    if (J0 + CB < k) fetch(J0 + CB);
+    // acc[v] = C(tr*16 + lane%16, tc*16 + lane/16 + 4v); Bv = this lane's entries of B
+    V acc = {0, 0, 0, 0};
+    if (J0 > 0) {
+      for (int kc = 0; kc < J0; kc += CB) {
+#pragma unroll
+        for (int kk = 0; kk < CB; kk += 4) {
+          const int t = kc + kk + (lane >> 4);
+          acc = MFf<double>::op(Rs[t * TRS + tc * 16 + il], Xs[t * TRS + tr * 16 + il], acc);
+        }
+      }
+    }
+    double Bv[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = tr * 16 + il, col = tc * 16 + (lane >> 4) + 4 * v;
+      Bv[v] = Xs[(J0 + col) * TRS + row] - acc[v];
+    }
+    // (each lane rewrites only the entries it read; the update read columns < J0)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = tr * 16 + il, col = tc * 16 + (lane >> 4) + 4 * v;
+      Xs[(J0 + col) * TRS + row] = Bv[v];
+    }
+    __syncthreads();
+    // X0 = B W^T
+    V x0 = {0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < CB; kk += 4) {
+      const int s2 = kk + (lane >> 4);
+      x0 = MFf<double>::op(Wb[s2 * TRS + tc * 16 + il], Xs[(J0 + s2) * TRS + tr * 16 + il], x0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) X0s[(tc * 16 + (lane >> 4) + 4 * v) * TRS + tr * 16 + il] = x0[v];
+    __syncthreads();
+    // residual B - X0 R_JJ
+    V pr2 = {0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < CB; kk += 4) {
+      const int s2 = kk + (lane >> 4);
+      pr2 = MFf<double>::op(Ds[s2 * TRS + tc * 16 + il], X0s[s2 * TRS + tr * 16 + il], pr2);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Rb[(tc * 16 + (lane >> 4) + 4 * v) * TRS + tr * 16 + il] = Bv[v] - pr2[v];
+    __syncthreads();
+    // X_J = X0 + (B - X0 R_JJ) W^T
+    V dx = {0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < CB; kk += 4) {
+      const int s2 = kk + (lane >> 4);
+      dx = MFf<double>::op(Wb[s2 * TRS + tc * 16 + il], Rb[s2 * TRS + tr * 16 + il], dx);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = tr * 16 + il, col = tc * 16 + (lane >> 4) + 4 * v;
+      const double xv = x0[v] + dx[v];
+      Xs[(J0 + col) * TRS + row] = xv;
+      if (r0 + row < m && col < jb) X[(r0 + row) + (long)(J0 + col) * ldx] = xv;
+    }
   }
 }
 
@@ -1053,10 +1208,20 @@ void eye_plus(hipStream_t s, int64_t k, double *X, const double *U, double sign)
 
 template <class T>
 void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t lda, const double *R,
-                      T *X, int64_t ldx, unsigned long long *dbg) {
+                      T *X, int64_t ldx, unsigned long long *dbg, const double *Winv) {
   if (m <= 0 || k <= 0) return;
   const int64_t kp = (k + CB - 1) / CB * CB;
   ProfScope prof(KC_TRTRI, s, (double)m * k * k, (double)sizeof(T) * 2.0 * m * k + 8.0 * k * k);
+  if constexpr (std::is_same<T, double>::value) {
+    if (Winv && kp <= 256 && !dbg) {
+      const size_t lds3 = trsm3_lds((int)kp);
+      TTG_HIP(hipFuncSetAttribute((const void *)k_trsm3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3));
+      hipLaunchKernelGGL(k_trsm3, dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds3, s, (int)m, (int)k, A, (long)lda,
+                         R, Winv, X, (long)ldx);
+      TTG_HIP(hipGetLastError());
+      return;
+    }
+  }
   TTG_CHECK(A || kp <= 256, TTG_BAD_ARG, "identity right-hand side needs k <= 256");
   if (kp <= 256) {
     const size_t lds2 = trsm2_lds((int)kp, sizeof(T));
@@ -1073,7 +1238,7 @@ void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t l
                      A, (long)lda, R, X, (long)ldx, dbg);
   TTG_HIP(hipGetLastError());
 }
-template void trsm_right_upper<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const double *, double *, int64_t, unsigned long long *);
-template void trsm_right_upper<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const double *, float *, int64_t, unsigned long long *);
+template void trsm_right_upper<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const double *, double *, int64_t, unsigned long long *, const double *);
+template void trsm_right_upper<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const double *, float *, int64_t, unsigned long long *, const double *);
 
 }  // namespace ttg

@@ -181,8 +181,11 @@ void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, d
 // One-workgroup Cholesky G + shift = R^T R (upper R), FP64, k x k (k <= 512).
 // G is read from its upper triangle; shift = shift_rel * trace(G) / k.
 // flag[0] = 0 ok, j+1 if pivot j failed; stats[0] = diag spread, stats[1] = max|R - I|.
-void chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats, unsigned long long *dbg = nullptr);
+// Wout (optional, (k/32 + 1) * 33 * 32 doubles): the inverses R_JJ^{-T} of the
+// diagonal blocks for trsm_right_upper; returns whether they were written.
+bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
+                 double *R, int *flag, double *stats, unsigned long long *dbg = nullptr,
+                 double *Wout = nullptr);
 // X (m x k) = A R^{-1}, R upper triangular (FP64), blocked substitution.
 // Gram of A (m x k, FP32 MFMA when g32) fused with the near-identity second
 // CholeskyQR pass: X = I - Phi(E), R2 = I + Phi(E), *maxabs = max|E| (G is
@@ -198,7 +201,8 @@ void gemm_f64_via_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int
                       int64_t ldc, float *ws, int64_t wse, int cus);
 template <class T>
 void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t lda, const double *R,
-                      T *X, int64_t ldx, unsigned long long *dbg = nullptr);
+                      T *X, int64_t ldx, unsigned long long *dbg = nullptr,
+                      const double *Winv = nullptr);
 
 // One-sided block Jacobi SVD on the stacked matrix Y = [S; I] (FP64).
 // Setup from S (k x k, column-major, optionally transposed) then rotate

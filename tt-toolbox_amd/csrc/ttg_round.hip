@@ -58,7 +58,8 @@ template <class T> struct Rounder {
   T *buf[3];
   std::vector<T *> cp;
   // scratch
-  double *G, *R1, *R2, *R3, *R4, *Rinv, *Rprod, *Rtmp, *Y, *sv, *nrm, *stats, *bstats, *Yws;
+  double *Winv1, *Winv2;  // inverses of the diagonal blocks of R1 / R2
+  double *G, *R1, *R2, *R4, *Rinv, *Rprod, *Rtmp, *Y, *sv, *nrm, *stats, *bstats, *Yws;
   T *RT, *Q1, *Q2, *Tt, *Qs, *US, *gws;
   int64_t gws_elems;
   double *hA, *hQ, *htau, *hws;
@@ -90,7 +91,8 @@ template <class T> struct Rounder {
 
   template <class A> static void layout(A &a, int64_t d, int64_t total, int64_t max_core, int64_t kmax, int64_t rsq) {
     for (int b = 0; b < 3; ++b) a.template take<T>(total);
-    for (int q = 0; q < 8; ++q) a.template take<double>(rsq);
+    for (int q = 0; q < 7; ++q) a.template take<double>(rsq);
+    for (int q = 0; q < 2; ++q) a.template take<double>((kmax / 32 + 1) * 33 * 32);  // Winv1, Winv2
     a.template take<double>(jac_elems(kmax));
     a.template take<double>(jac_elems(kmax) + (kmax + 64) * (kmax + 64));  // polar correction
     a.template take<double>(kmax + 64);      // sv
@@ -118,11 +120,12 @@ template <class T> struct Rounder {
     G = a.take<double>(rsq);
     R1 = a.take<double>(rsq);
     R2 = a.take<double>(rsq);
-    R3 = a.take<double>(rsq);
     Rinv = a.take<double>(rsq);
     Rprod = a.take<double>(rsq);
     Rtmp = a.take<double>(rsq);
     R4 = a.take<double>(rsq);
+    Winv1 = a.take<double>((kmax / 32 + 1) * 33 * 32);
+    Winv2 = a.take<double>((kmax / 32 + 1) * 33 * 32);
     Y = a.take<double>(jac_elems(kmax));
     Yws = a.take<double>(jac_elems(kmax) + (kmax + 64) * (kmax + 64));
     sv = a.take<double>(kmax + 64);
@@ -176,11 +179,14 @@ template <class T> struct Rounder {
   }
 
   // One Cholesky-QR pass: R = chol(A^T A (+ shift)), Qout = A R^{-1} by triangular solve.
-  void cholqr_pass(const T *A, int64_t m, int64_t k, double shift_rel, T *Qout, double *Rout,
-                   int *flag, double *st2, bool g32) {
+  // W (optional): receives the inverses of R's diagonal blocks, which the
+  // solve then uses (returns whether it did)
+  bool cholqr_pass(const T *A, int64_t m, int64_t k, double shift_rel, T *Qout, double *Rout,
+                   int *flag, double *st2, bool g32, double *W) {
     gram(A, m, k, G, g32);
-    chol_factor(st, k, G, k, shift_rel, Rout, flag, st2);
-    trsm_right_upper<T>(st, m, k, A, m, Rout, Qout, m);
+    const bool w = chol_factor(st, k, G, k, shift_rel, Rout, flag, st2, nullptr, W);
+    trsm_right_upper<T>(st, m, k, A, m, Rout, Qout, m, nullptr, w ? W : nullptr);
+    return w;
   }
 
   // Tall QR (m >= k): Qout (m x k, T), Rout (k x k, FP64), by CholeskyQR2.
@@ -204,12 +210,13 @@ template <class T> struct Rounder {
   void qr_tall_fast(const T *A, int64_t m, int64_t k, T *Qout, double *Rout, int *flags,
                     double *stt, int level, bool allow_g32, bool bound = false) {
     const bool g32 = mixed && level == 0 && allow_g32;
-    cholqr_pass(A, m, k, 0.0, Q1, R1, flags + 0, stt + 0, g32);
+    const bool w1 = cholqr_pass(A, m, k, 0.0, Q1, R1, flags + 0, stt + 0, g32, Winv1);
     if (bound && nearid_level(level)) {
       TTG_HIP(hipEventRecord(ctx->ev_fork, st));
       TTG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
       if (k <= 256) {
-        trsm_right_upper<double>(ctx->side, k, k, nullptr, k, R1, Rtmp, k);  // Rtmp = I R1^{-1}
+        trsm_right_upper<double>(ctx->side, k, k, nullptr, k, R1, Rtmp, k, nullptr,
+                                 w1 ? Winv1 : nullptr);  // Rtmp = I R1^{-1}
       } else {
         eye_plus(ctx->side, k, R4, nullptr, 0.0);
         trsm_right_upper<double>(ctx->side, k, k, R4, k, R1, Rtmp, k);
@@ -222,7 +229,7 @@ template <class T> struct Rounder {
       gram_nearid(st, m, k, Q1, g32, G, Rinv, R2, stt + 3, gws, gws_elems, cus);
       g(false, false, m, k, k, Q1, m, Rinv, k, Qout, m);  // Q = Q1 R2^{-1}
     } else {
-      cholqr_pass(Q1, m, k, 0.0, Qout, R2, flags + 1, stt + 2, g32);
+      cholqr_pass(Q1, m, k, 0.0, Qout, R2, flags + 1, stt + 2, g32, Winv2);
     }
     gemm<double>(st, false, false, k, k, k, 1.0, R2, k, R1, k, 0.0, Rout, k, nullptr, 0, cus);
   }

@@ -189,6 +189,26 @@ ttg_status ttg_test_chol_timing(ttg_ctx *ctx, int64_t k, const double *G, int64_
   })
 }
 
+// R = chol(G) (upper) and X = A R^{-1} with the inverses of R's diagonal
+// blocks that the factorisation leaves (k <= 256), as in one CholeskyQR pass
+ttg_status ttg_test_chol_trsm(ttg_ctx *ctx, int64_t m, int64_t k, const double *G, const double *A,
+                              double *R, double *X, int *used_w) {
+  GUARD(ctx, {
+    hipStream_t s = ctx->stream;
+    Frees f;
+    double *dG = f.add(up(G, k * k, s)), *dR = f.add(up<double>(nullptr, k * k, s));
+    double *dA = f.add(up(A, m * k, s)), *dX = f.add(up<double>(nullptr, m * k, s));
+    double *dst = f.add(up<double>(nullptr, 8, s));
+    int *dfl = f.add(up<int>(nullptr, 4, s));
+    double *dW = f.add(up<double>(nullptr, (k / 32 + 1) * 33 * 32, s));
+    const bool w = chol_factor(s, k, dG, k, 0.0, dR, dfl, dst, nullptr, dW);
+    trsm_right_upper<double>(s, m, k, dA, m, dR, dX, m, nullptr, w ? dW : nullptr);
+    *used_w = w ? 1 : 0;
+    down(R, dR, k * k, s);
+    down(X, dX, m * k, s);
+  })
+}
+
 // per-phase clock stamps of workgroup 0 of the FP64 triangular solve
 ttg_status ttg_test_trsm_timing(ttg_ctx *ctx, int64_t m, int64_t k, const double *A, const double *R,
                                 int64_t *times, int64_t ntimes) {

@@ -39,7 +39,7 @@ EXPORTS = [
 ]
 # kernel-level test hooks (include/ttgpu_testing.h)
 TEST_EXPORTS = ["ttg_test_gemm", "ttg_test_householder", "ttg_test_chol", "ttg_test_trsm", "ttg_test_jacobi", "ttg_test_jacobi_timing",
-                "ttg_test_chol_timing", "ttg_test_trsm_timing"]
+                "ttg_test_chol_timing", "ttg_test_trsm_timing", "ttg_test_chol_trsm"]
 KERNEL_CLASSES = {"gemm": 1, "jacobi": 2, "cholesky": 3, "trtri": 4, "householder": 5}
 
 
@@ -110,6 +110,7 @@ def load(path: Path | str | None = None):
         "ttg_test_jacobi_timing": (st, [P, I64, P, P, I64]),
         "ttg_test_chol_timing": (st, [P, I64, P, P, I64]),
         "ttg_test_trsm_timing": (st, [P, I64, I64, P, P, P, I64]),
+        "ttg_test_chol_trsm": (st, [P, I64, I64, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
