@@ -479,7 +479,7 @@ template <int MAXT, int MAXE, int US>
 __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__restrict__ G, long ldg,
                                                         double shift_rel, double *__restrict__ R,
                                                         int *flag, double *stats,
-                                                        unsigned long long *dbg, double *Wout) {
+                                                        unsigned long long *dbg, double *Wout, double *fro) {
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
   double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
   for (int j = wv; j < k; j += NW)
     for (int i = j + 1 + lane; i < k; i += 64) R[i + (long)j * k] = 0.0;
   if (tid == 0) sh_fail = 0;
-  double dmax = 0.0, dmin = INFINITY, dev = 0.0;
+  double dmax = 0.0, dmin = INFINITY, dev = 0.0, sq = 0.0;  // sq: ||R||_F^2 (this thread's share)
   for (int J0 = 0; J0 < k; J0 += CB) {
     const int jb = min(CB, k - J0);
     const int w = k - J0 - jb;  // > 0 only when jb == CB
@@ -689,6 +689,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
       if (t < jb && l < jb && t <= l) {
         const double val = Dl[t * CB + l];
         R[(J0 + t) + (long)(J0 + l) * k] = val;
+        sq += val * val;
         if (t == l) {
           dmax = fmax(dmax, fabs(val));
           dmin = fmin(dmin, fabs(val));
@@ -704,6 +705,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
       const int t = idx & (CB - 1), c = idx >> 5;
       const double val = Ps[t * wps + c];
       R[(J0 + t) + (long)(J0 + jb + c) * k] = val;
+      sq += val * val;
       dev = fmax(dev, fabs(val));
     }
     if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
@@ -714,17 +716,19 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
   dmax = block_max(dmax, red);
   dmin = -block_max(-dmin, red);
   dev = block_max(dev, red);
+  if (fro) sq = block_sum(sq, red);
   if (tid == 0) {
     flag[0] = failed;
     stats[0] = failed ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
     stats[1] = failed ? INFINITY : dev;
+    if (fro) fro[0] = sqrt(sq);
   }
 }
 
 constexpr int CHOL_MAXK = 512;
 
 bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
-                 double *R, int *flag, double *stats, unsigned long long *dbg, double *Wout) {
+                 double *R, int *flag, double *stats, unsigned long long *dbg, double *Wout, double *fro) {
   bool wrote_w = false;
   TTG_CHECK(k >= 1 && k <= CHOL_MAXK, TTG_BAD_ARG, "Cholesky kernel supports 1 <= k <= 512");
   const int64_t wmax = std::max<int64_t>(k - CB, 0);
@@ -756,8 +760,8 @@ bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
       TTG_HIP(hipFuncSetAttribute((const void *)k_chol5<MAXT, MAXE, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds5));
       hipLaunchKernelGGL((k_chol5<MAXT, MAXE, US>), dim3(1), dim3(CHOL_THREADS), lds5, s, (int)k, G, (long)ldg,
-                         shift_rel, R, flag, stats, dbg, Wout);
-      wrote_w = Wout != nullptr;
+                         shift_rel, R, flag, stats, dbg, Wout, fro);
+      wrote_w = true;
     }
   }
   TTG_HIP(hipGetLastError());

@@ -182,10 +182,11 @@ void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, d
 // G is read from its upper triangle; shift = shift_rel * trace(G) / k.
 // flag[0] = 0 ok, j+1 if pivot j failed; stats[0] = diag spread, stats[1] = max|R - I|.
 // Wout (optional, (k/32 + 1) * 33 * 32 doubles): the inverses R_JJ^{-T} of the
-// diagonal blocks for trsm_right_upper; returns whether they were written.
+// diagonal blocks for trsm_right_upper; fro (optional): ||R||_F.  Returns
+// whether the factorisation wrote them (k <= 256); otherwise they are untouched.
 bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
                  double *R, int *flag, double *stats, unsigned long long *dbg = nullptr,
-                 double *Wout = nullptr);
+                 double *Wout = nullptr, double *fro = nullptr);
 // X (m x k) = A R^{-1}, R upper triangular (FP64), blocked substitution.
 // Gram of A (m x k, FP32 MFMA when g32) fused with the near-identity second
 // CholeskyQR pass: X = I - Phi(E), R2 = I + Phi(E), *maxabs = max|E| (G is

@@ -182,9 +182,9 @@ template <class T> struct Rounder {
   // W (optional): receives the inverses of R's diagonal blocks, which the
   // solve then uses (returns whether it did)
   bool cholqr_pass(const T *A, int64_t m, int64_t k, double shift_rel, T *Qout, double *Rout,
-                   int *flag, double *st2, bool g32, double *W) {
+                   int *flag, double *st2, bool g32, double *W, double *fro = nullptr) {
     gram(A, m, k, G, g32);
-    const bool w = chol_factor(st, k, G, k, shift_rel, Rout, flag, st2, nullptr, W);
+    const bool w = chol_factor(st, k, G, k, shift_rel, Rout, flag, st2, nullptr, W, fro);
     trsm_right_upper<T>(st, m, k, A, m, Rout, Qout, m, nullptr, w ? W : nullptr);
     return w;
   }
@@ -210,7 +210,8 @@ template <class T> struct Rounder {
   void qr_tall_fast(const T *A, int64_t m, int64_t k, T *Qout, double *Rout, int *flags,
                     double *stt, int level, bool allow_g32, bool bound = false) {
     const bool g32 = mixed && level == 0 && allow_g32;
-    const bool w1 = cholqr_pass(A, m, k, 0.0, Q1, R1, flags + 0, stt + 0, g32, Winv1);
+    // (with the bound: ||R1||_F from the factorisation itself when it can)
+    const bool w1 = cholqr_pass(A, m, k, 0.0, Q1, R1, flags + 0, stt + 0, g32, Winv1, bound ? scal_d + 0 : nullptr);
     if (bound && nearid_level(level)) {
       TTG_HIP(hipEventRecord(ctx->ev_fork, st));
       TTG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
@@ -223,7 +224,7 @@ template <class T> struct Rounder {
       }
       fro_norm<double>(ctx->side, Rtmp, k * k, scal_d + 1);
       TTG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
-      fro_norm<double>(st, R1, k * k, scal_d + 0);  // the main stream has the shorter chain
+      if (!w1) fro_norm<double>(st, R1, k * k, scal_d + 0);
     }
     if (nearid_level(level)) {
       gram_nearid(st, m, k, Q1, g32, G, Rinv, R2, stt + 3, gws, gws_elems, cus);
