@@ -735,34 +735,24 @@ bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
   const int64_t wps = ((wmax + 15) & ~(int64_t)15) + 1;
   const size_t lds = ((size_t)CB * CB + (size_t)CB * wps) * sizeof(double);
   ProfScope prof(KC_CHOL, s, (double)k * k * k / 3.0, 8.0 * 2.0 * (double)k * k);
-  static const bool v3 = [] {
-    const char *e = std::getenv("TTG_CHOL_V3");
-    return e && e[0] == '1';
-  }();
-  if (v3 || ((wmax + 15) & ~(int64_t)15) > CHOL_THREADS - 64) {  // panels beyond 448 columns
+  if (((wmax + 15) & ~(int64_t)15) > CHOL_THREADS - 64) {  // panels beyond 448 columns
     TTG_HIP(hipFuncSetAttribute((const void *)k_chol3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_chol3, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
+  } else if (k > 256) {
+    TTG_HIP(hipFuncSetAttribute((const void *)k_chol4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_chol4, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
   } else {
-    static const bool right = [] {
-      const char *e = std::getenv("TTG_CHOL_RIGHT");
-      return e && e[0] == '1';
-    }();
-    if (right || k > 256) {
-      TTG_HIP(hipFuncSetAttribute((const void *)k_chol4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(k_chol4, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
-    } else {
-      // k <= 256: at most 2 * 16 output tiles (4 per wave), 32 x 224 staged
-      // elements (14 per thread), staging and panel row stride 225; Dl + Ps
-      // at that stride (8224 doubles) hold the staging buffer and the reads
-      // of the tiles past a block (up to index 7230)
-      constexpr int MAXT = 4, MAXE = 14, US = 225;
-      const size_t lds5 = std::max(lds, (size_t)(CB * CB + CB * US) * sizeof(double));
-      TTG_HIP(hipFuncSetAttribute((const void *)k_chol5<MAXT, MAXE, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds5));
-      hipLaunchKernelGGL((k_chol5<MAXT, MAXE, US>), dim3(1), dim3(CHOL_THREADS), lds5, s, (int)k, G, (long)ldg,
-                         shift_rel, R, flag, stats, dbg, Wout, fro);
-      wrote_w = true;
-    }
+    // k <= 256: at most 2 * 16 output tiles (4 per wave), 32 x 224 staged
+    // elements (14 per thread), staging and panel row stride 225; Dl + Ps
+    // at that stride (8224 doubles) hold the staging buffer and the reads
+    // of the tiles past a block (up to index 7230)
+    constexpr int MAXT = 4, MAXE = 14, US = 225;
+    const size_t lds5 = std::max(lds, (size_t)(CB * CB + CB * US) * sizeof(double));
+    TTG_HIP(hipFuncSetAttribute((const void *)k_chol5<MAXT, MAXE, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds5));
+    hipLaunchKernelGGL((k_chol5<MAXT, MAXE, US>), dim3(1), dim3(CHOL_THREADS), lds5, s, (int)k, G, (long)ldg,
+                       shift_rel, R, flag, stats, dbg, Wout, fro);
+    wrote_w = true;
   }
   TTG_HIP(hipGetLastError());
   return wrote_w;
