@@ -21,6 +21,7 @@
 // synchronises once per bond to learn r1, which sizes the next launches.
 #include "ttg_internal.h"
 #include <cstdlib>
+#include <cstring>
 #include <chrono>
 #include <cmath>
 #include <functional>
@@ -309,7 +310,10 @@ template <class T> struct Rounder {
     const int64_t kw = r[i], N = n[i] * r[i + 1];
     wide = (N >= kw);
     const T *W = cp[i];
-    int *fl = iflag + 4 * (d + 4);
+    // stats[0..5]: QR statistics; the QR flags and the shortcut's decision sit
+    // right behind them, so that one copy brings all three to the host
+    int *fl = reinterpret_cast<int *>(stats + 8);
+    int *skip = reinterpret_cast<int *>(stats + 10);
     double *R = Rprod;
     kj = wide ? kw : N;
     const int64_t mt = wide ? N : kw;  // rows of the tall orientation
@@ -338,14 +342,15 @@ template <class T> struct Rounder {
     // acceptance are read in one synchronisation; a rejected QR is redone
     // before any SVD work.
     if (bound) {
-      int *skip = scal_i + 4;
       TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
       notrunc_decide(st, scal_d + 0, scal_d + 1, ep, kj, rmax_i, skip);
-      int h_skip = 0;
-      TTG_HIP(hipMemcpyAsync(&h_skip, skip, sizeof(int), hipMemcpyDeviceToHost, st));
-      TTG_HIP(hipMemcpyAsync(h_fl, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
-      TTG_HIP(hipMemcpyAsync(h_st, stats, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+      double h_io[11];
+      TTG_HIP(hipMemcpyAsync(h_io, stats, sizeof(h_io), hipMemcpyDeviceToHost, st));
       TTG_HIP(hipStreamSynchronize(st));
+      std::memcpy(h_st, h_io, sizeof(h_st));
+      std::memcpy(h_fl, h_io + 8, sizeof(h_fl));
+      int h_skip = 0;
+      std::memcpy(&h_skip, h_io + 10, sizeof(int));
       qr_ok = level == 2 || qr_accept(h_fl, h_st, level, false);
       if (!qr_ok) {
         next_level = qr_next(h_fl, h_st, level, false);
