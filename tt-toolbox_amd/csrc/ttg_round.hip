@@ -155,6 +155,15 @@ template <class T> struct Rounder {
   }
 
   // ---- small helpers ------------------------------------------------------
+  // 4 KB of pinned host memory for the per-bond read-backs (a plain DMA; the
+  // synchronisation then waits only for the GPU)
+  double *pinned() {
+    if (!ctx->host_pinned) {
+      TTG_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
+      ctx->host_pinned_cap = 4096;
+    }
+    return static_cast<double *>(ctx->host_pinned);
+  }
   void g(bool ta, bool tb, int64_t M, int64_t N, int64_t K, const T *A, int64_t lda, const T *B,
          int64_t ldb, T *C, int64_t ldc) {
     gemm<T>(st, ta, tb, M, N, K, T(1), A, lda, B, ldb, T(0), C, ldc, gws, gws_elems, cus);
@@ -344,8 +353,8 @@ template <class T> struct Rounder {
     if (bound) {
       TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
       notrunc_decide(st, scal_d + 0, scal_d + 1, ep, kj, rmax_i, skip);
-      double h_io[11];
-      TTG_HIP(hipMemcpyAsync(h_io, stats, sizeof(h_io), hipMemcpyDeviceToHost, st));
+      double *h_io = pinned();
+      TTG_HIP(hipMemcpyAsync(h_io, stats, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
       TTG_HIP(hipStreamSynchronize(st));
       std::memcpy(h_st, h_io, sizeof(h_st));
       std::memcpy(h_fl, h_io + 8, sizeof(h_fl));
@@ -371,12 +380,18 @@ template <class T> struct Rounder {
     sort_chop(st, Y, pl.rows, kj, pl.kpad, ep, rmax_i, sv, perm, scal_i, scal_d + 8);
     // one synchronisation: r1, Jacobi status, QR acceptance data
     int h_i[4] = {0, 0, 0, 0};
-    TTG_HIP(hipMemcpyAsync(h_i, scal_i, sizeof(int), hipMemcpyDeviceToHost, st));
-    TTG_HIP(hipMemcpyAsync(h_i + 1, jerr, sizeof(int), hipMemcpyDeviceToHost, st));
-    TTG_HIP(hipMemcpyAsync(h_i + 2, jsweeps, sizeof(int), hipMemcpyDeviceToHost, st));
-    TTG_HIP(hipMemcpyAsync(h_fl, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
-    TTG_HIP(hipMemcpyAsync(h_st, stats, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
-    TTG_HIP(hipStreamSynchronize(st));
+    {
+      double *hp = pinned();  // [0..10] stats/flags (as above), then three ints
+      int *hpi = reinterpret_cast<int *>(hp + 12);
+      TTG_HIP(hipMemcpyAsync(hp, stats, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+      TTG_HIP(hipMemcpyAsync(hpi, scal_i, sizeof(int), hipMemcpyDeviceToHost, st));
+      TTG_HIP(hipMemcpyAsync(hpi + 1, jerr, sizeof(int), hipMemcpyDeviceToHost, st));
+      TTG_HIP(hipMemcpyAsync(hpi + 2, jsweeps, sizeof(int), hipMemcpyDeviceToHost, st));
+      TTG_HIP(hipStreamSynchronize(st));
+      std::memcpy(h_st, hp, sizeof(h_st));
+      std::memcpy(h_fl, hp + 8, sizeof(h_fl));
+      std::memcpy(h_i, hpi, 3 * sizeof(int));
+    }
     qr_ok = level == 2 || qr_accept(h_fl, h_st, level, false);
     if (!qr_ok) next_level = qr_next(h_fl, h_st, level, false);
     r1 = h_i[0];
@@ -444,10 +459,9 @@ template <class T> struct Rounder {
     std::vector<double> hst(8 * (d + 8));
     while (d > 1 && !careful) {
       std::vector<int64_t> pending;
-      for (int64_t i = start; i + 1 < d; ++i) {
-        TTG_HIP(hipMemsetAsync(iflag + 4 * i, 0, 4 * sizeof(int), st));
+      TTG_HIP(hipMemsetAsync(iflag + 4 * start, 0, 4 * (d - 1 - start) * sizeof(int), st));
+      for (int64_t i = start; i + 1 < d; ++i)
         if (sweep1_bond(i, force[i], gen)) pending.push_back(i);
-      }
       if (pending.empty()) break;
       TTG_HIP(hipMemcpyAsync(hfl.data(), iflag, hfl.size() * sizeof(int), hipMemcpyDeviceToHost, st));
       TTG_HIP(hipMemcpyAsync(hst.data(), bstats, hst.size() * sizeof(double), hipMemcpyDeviceToHost, st));
