@@ -267,5 +267,6 @@ void gemm_mtimes(hipStream_t s, int64_t M, const double *R, int64_t ldr, int64_t
 
 // check for non-finite values; sets *flag nonzero if any
 void check_finite(hipStream_t s, const double *x, int64_t count, int *flag);
+void copy_check_finite(hipStream_t s, double *y, const double *x, int64_t count, int *flag);
 
 }  // namespace ttg

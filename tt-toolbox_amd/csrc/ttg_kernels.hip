@@ -470,6 +470,22 @@ void check_finite(hipStream_t s, const double *x, int64_t count, int *flag) {
   hipLaunchKernelGGL(k_check_finite, dim3(blocks), dim3(256), 0, s, x, (long)count, flag);
   TTG_HIP(hipGetLastError());
 }
+// y = x with a NaN / Inf check on the way (one read of x instead of two)
+__global__ void k_copy_check_finite(double *__restrict__ y, const double *__restrict__ x, long count, int *flag) {
+  int bad = 0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+    const double v = x[i];
+    y[i] = v;
+    if (!isfinite(v)) bad = 1;
+  }
+  if (bad) atomicOr(flag, 1);
+}
+void copy_check_finite(hipStream_t s, double *y, const double *x, int64_t count, int *flag) {
+  if (count <= 0) return;
+  const int blocks = (int)std::min<int64_t>((count + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_copy_check_finite, dim3(blocks), dim3(256), 0, s, y, x, (long)count, flag);
+  TTG_HIP(hipGetLastError());
+}
 
 // ---------------------------------------------------------------------------
 // Householder QR, one workgroup (robust path: rank-deficient / wide / small)

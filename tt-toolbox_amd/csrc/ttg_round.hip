@@ -157,10 +157,15 @@ template <class T> struct Rounder {
   // ---- small helpers ------------------------------------------------------
   // 4 KB of pinned host memory for the per-bond read-backs (a plain DMA; the
   // synchronisation then waits only for the GPU)
-  double *pinned() {
-    if (!ctx->host_pinned) {
-      TTG_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
-      ctx->host_pinned_cap = 4096;
+  double *pinned(size_t bytes = 4096) {
+    if (!ctx->host_pinned || ctx->host_pinned_cap < bytes) {
+      // (every earlier use of the buffer has been waited for: each read-back
+      // synchronises, and the finalisation reuses it only after the sweeps)
+      if (ctx->host_pinned) TTG_HIP(hipHostFree(ctx->host_pinned));
+      ctx->host_pinned = nullptr;
+      const size_t cap = std::max<size_t>(bytes, 4096);
+      TTG_HIP(hipHostMalloc(&ctx->host_pinned, cap, hipHostMallocDefault));
+      ctx->host_pinned_cap = cap;
     }
     return static_cast<double *>(ctx->host_pinned);
   }
@@ -576,16 +581,16 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
   // input -> working buffer 0 (converted for the mixed form)
   if (!gen) {
     TTG_HIP(hipMemsetAsync(nf, 0, sizeof(int), st));
-    check_finite(st, dcore, R.ofs[d], nf);
-    int hnf = 0;
-    TTG_HIP(hipMemcpyAsync(&hnf, nf, sizeof(int), hipMemcpyDeviceToHost, st));
     if constexpr (std::is_same<T, double>::value) {
-      TTG_HIP(hipMemcpyAsync(R.buf[0], dcore, R.ofs[d] * sizeof(double), hipMemcpyDeviceToDevice, st));
+      copy_check_finite(st, R.buf[0], dcore, R.ofs[d], nf);
     } else {
+      check_finite(st, dcore, R.ofs[d], nf);
       convert<float, double>(st, R.buf[0], dcore, R.ofs[d]);
     }
+    int *hnf = reinterpret_cast<int *>(R.pinned());
+    TTG_HIP(hipMemcpyAsync(hnf, nf, sizeof(int), hipMemcpyDeviceToHost, st));
     TTG_HIP(hipStreamSynchronize(st));
-    if (hnf) throw Error(TTG_NONFINITE, "input core contains NaN or Inf");
+    if (*hnf) throw Error(TTG_NONFINITE, "input core contains NaN or Inf");
   } else {
     (void)gen_ranks;
   }
@@ -605,8 +610,15 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
   }
   std::vector<void *> hsrc(d);
   for (int64_t i = 0; i < d; ++i) hsrc[i] = R.cp[i];
-  TTG_HIP(hipMemcpyAsync(dtab, htab.data(), htab.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  TTG_HIP(hipMemcpyAsync(dsrc, hsrc.data(), d * sizeof(void *), hipMemcpyHostToDevice, st));
+  {
+    // the tables go through pinned host memory (no pageable staging)
+    const size_t tb = htab.size() * sizeof(int64_t), sb = d * sizeof(void *);
+    char *hp = reinterpret_cast<char *>(R.pinned(tb + sb));
+    std::memcpy(hp, htab.data(), tb);
+    std::memcpy(hp + tb, hsrc.data(), sb);
+    TTG_HIP(hipMemcpyAsync(dtab, hp, tb, hipMemcpyHostToDevice, st));
+    TTG_HIP(hipMemcpyAsync(dsrc, hp + tb, sb, hipMemcpyHostToDevice, st));
+  }
   finalize_balance<T>(st, d, reinterpret_cast<T *const *>(dsrc), dtab, dtab + (d + 1), s0, R.nrm,
                       dcore_out, o);
   TTG_HIP(hipStreamSynchronize(st));
