@@ -487,7 +487,8 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
   __shared__ double red[CHOL_THREADS / 64];
   __shared__ double Wt[CB * TRS];  // Wt[x * TRS + y] = W(y, x), W = R_JJ^{-T}
   __shared__ int sh_fail;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave: scalar branches
   constexpr int NW = CHOL_THREADS / 64;
   double tr = 0.0;
   for (int j = tid; j < k; j += CHOL_THREADS) tr += G[j + (long)j * ldg];
@@ -611,9 +612,15 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
         // consumed after the pivot) runs beside it.
         double rr[CB];
         if (c >= 1) {
+          // the row's address in one VGPR, the entries at immediate offsets
+          // from it (as a compile-time constant the compiler would keep one
+          // scalar base per pair of entries, spilled and reloaded each step)
+          int rb = (c - 1) * CB;
+          asm volatile("" : "+v"(rb));
+          const double *rowp = Dl + rb;
 #pragma unroll
           for (int i = 0; i < CB; ++i)
-            if (i > c) rr[i] = Dl[(c - 1) * CB + i];
+            if (i > c) rr[i] = rowp[i];
         }
         __builtin_amdgcn_sched_barrier(0);
         double p = readlane_d(v[c], c);
