@@ -14,6 +14,12 @@
 //             wave 0, while waves 1..3 stage block J+1.  k_trsm covers
 //             k > 256.  No inverse of R is formed, so the residual
 //             ||X R - A|| stays O(u ||X|| ||R||) for any conditioning.
+//   k_trsm3 : X = A R^{-1} for FP64 right after k_chol5, which left the
+//             inverses W_J = R_JJ^{-1} of the 32x32 diagonal blocks: per
+//             block B = A_J - X_{<J} R_{<J,J}, X0 = B W_J, then one
+//             refinement X_J = X0 + (B - X0 R_JJ) W_J, all by MFMA, so the
+//             serial substitution of k_trsm2 disappears and the residual
+//             stays at rounding level.
 #include "ttg_internal.h"
 #include "ttg_device.h"
 #include <cmath>
