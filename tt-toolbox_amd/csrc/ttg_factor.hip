@@ -1,10 +1,13 @@
 // Cholesky factorisation and triangular solve for Cholesky-QR (gfx950).
 //
-//   k_chol5 : k <= 256, one workgroup, G + shift = R^T R (upper R), blocked by
-//             32, left-looking: each block row updated from the factored rows
-//             above by FP64 MFMA just before its diagonal block is factorised
-//             in registers of wave 0 (which also forms R_JJ^{-T}); the panel
-//             is one MFMA product with R_JJ^{-T}.
+//   k_chol6 : k <= 256, one workgroup, G + shift = R^T R (upper R), blocked by
+//             32, left-looking: the diagonal block is factorised in registers
+//             of wave 0 (which also forms R_JJ^{-T}) while waves 1..7 update
+//             the next block row from the rows above by FP64 MFMA (R's rows
+//             kept transposed in its lower triangle meanwhile); the panel is
+//             one MFMA product with R_JJ^{-T}.
+//   k_chol5 : the same with the block-row update before the pivots (kept
+//             for reference timing; not dispatched).
 //   k_chol4 : k <= 480, right-looking: panel columns solved in registers of
 //             waves 1..7 beside the pivot loop, MFMA SYRK trailing update.
 //   k_chol3 : any k <= 512, right-looking, panel solved after the pivots.
@@ -470,6 +473,89 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__r
 }
 
 // ---------------------------------------------------------------------------
+// Wave 0's factorisation of the 32 x 32 diagonal block in Dl (column-major,
+// the updated block B): lanes 0..31 hold one column each; lanes 32..63 run
+// the same eliminations on the columns of the identity, so that they end
+// with W = R_JJ^{-T} in Wt (Wt[x * TRS + y] = W(y, x)), which the panel is
+// multiplied by instead of a 32-step substitution per panel column.  R_JJ
+// overwrites Dl row by row.  Returns the mask of failed pivots.
+__device__ __forceinline__ unsigned chol_pivot_chain(double *Dl, double *Wt, int lane) {
+  const int j = lane & (CB - 1);
+  const bool idl = lane >= CB;
+  double *const wrow = idl ? Wt + j * TRS : Dl + j;
+  double v[CB];
+  {
+    // (the identity columns are rebuilt in every block: as loop
+    // invariants they would stay live across the whole kernel)
+    int jj = j;
+    asm volatile("" : "+v"(jj));
+#pragma unroll
+    for (int i = 0; i < CB; ++i) v[i] = idl ? (i == jj ? 1.0 : 0.0) : Dl[j * CB + i];
+  }
+  unsigned failmask = 0;
+  double rc_prev = 0.0;
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    // Pivot c.  Critical path: v[c] -> pivot -> row c -> v[c+1] (the next
+    // pivot column), using only this lane's values and one readlane.
+    // Row c-1's update of the later columns (LDS reads issued first,
+    // consumed after the pivot) runs beside it.
+    double rr[CB];
+    if (c >= 1) {
+      // the row's address in one VGPR, the entries at immediate offsets
+      // from it (as a compile-time constant the compiler would keep one
+      // scalar base per pair of entries, spilled and reloaded each step)
+      int rb = (c - 1) * CB;
+      asm volatile("" : "+v"(rb));
+      const double *rowp = Dl + rb;
+#pragma unroll
+      for (int i = 0; i < CB; ++i)
+        if (i > c) rr[i] = rowp[i];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double p = readlane_d(v[c], c);
+    // straight-line code only (branches here push v[] into scratch): a
+    // failed pivot is recorded in a bit mask and replaced by a harmless 1
+    const bool bad = !(p > 0.0) || !isfinite(p);
+    failmask |= bad ? (1u << c) : 0u;
+    p = bad ? 1.0 : p;
+    // 1/sqrt(p) from the hardware estimate and two Newton steps
+    double inv = __builtin_amdgcn_rsq(p);
+    {
+      const double h = 0.5 * p;
+      double e = __builtin_fma(-h * inv, inv, 0.5);
+      inv = __builtin_fma(inv, e, inv);
+      e = __builtin_fma(-h * inv, inv, 0.5);
+      inv = __builtin_fma(inv, e, inv);
+    }
+    // the entry by per-lane selects, no branches: with exec-mask branches
+    // here the row update below could not be scheduled into the latency
+    // of the rsq/Newton chain (it would sit in a later basic block)
+    const double dc = p * inv, sc = v[c] * inv;
+    const bool keep = idl || j > c;
+    double rc = keep ? sc : 0.0;
+    rc = (!idl && j == c) ? dc : rc;
+    v[c] = rc;
+    // row c of R_JJ (lanes 0..31) / entry (c, j) of W (lanes 32..63)
+    // (one LDS address per lane: identity lanes write W, the others R_JJ;
+    // the stride made opaque in each step: hoisted out of the block loop,
+    // the 32 per-step addresses would be spilled to scratch)
+    int wst = idl ? 1 : CB;
+    asm volatile("" : "+v"(wst));
+    *(wrow + c * wst) = rc;
+    if (c + 1 < CB) v[c + 1] -= readlane_d(rc, c + 1) * rc;  // R(c, c+1) x_j
+    if (c >= 1) {
+#pragma unroll
+      for (int i = 0; i < CB; ++i)
+        if (i > c) v[i] -= rr[i] * rc_prev;  // R(c-1, i) x_j
+    }
+    rc_prev = rc;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  return failmask;
+}
+
+// ---------------------------------------------------------------------------
 // k_chol5: left-looking form of k_chol4.  Block row J is brought up to date
 // just before it is factorised, C = R(0:J0, J-block)^T R(0:J0, J0:k) with
 // FP64 MFMA from the factored rows above (streamed through LDS, accumulated
@@ -591,76 +677,10 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
       }
     }
     __syncthreads();
-    const int j = lane & (CB - 1);
-    // Wave 0: lanes 0..31 factorise the diagonal block (lane j holds column
-    // j); lanes 32..63 run the same eliminations on the columns of the
-    // identity, so that they end with W = R_JJ^{-T}, which the panel below
-    // is multiplied by (instead of a 32-step substitution per panel column).
-    const bool idl = lane >= CB;
-    double *const wrow = idl ? Wt + j * TRS : Dl + j;
-    double v[CB];
     if (wv == 0) {
-      // (the identity columns are rebuilt in every block: as loop
-      // invariants they would stay live across the whole kernel)
-      int jj = j;
-      asm volatile("" : "+v"(jj));
-#pragma unroll
-      for (int i = 0; i < CB; ++i) v[i] = idl ? (i == jj ? 1.0 : 0.0) : Dl[j * CB + i];
+      const unsigned failmask = chol_pivot_chain(Dl, Wt, lane);
+      if (lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
     }
-    unsigned failmask = 0;
-    double rc_prev = 0.0;
-#pragma unroll
-    for (int c = 0; c < CB; ++c) {
-      if (wv == 0) {
-        // Pivot c.  Critical path: v[c] -> pivot -> row c -> v[c+1] (the next
-        // pivot column), using only this lane's values and one readlane.
-        // Row c-1's update of the later columns (LDS reads issued first,
-        // consumed after the pivot) runs beside it.
-        double rr[CB];
-        if (c >= 1) {
-          // the row's address in one VGPR, the entries at immediate offsets
-          // from it (as a compile-time constant the compiler would keep one
-          // scalar base per pair of entries, spilled and reloaded each step)
-          int rb = (c - 1) * CB;
-          asm volatile("" : "+v"(rb));
-          const double *rowp = Dl + rb;
-#pragma unroll
-          for (int i = 0; i < CB; ++i)
-            if (i > c) rr[i] = rowp[i];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        double p = readlane_d(v[c], c);
-        // straight-line code only (branches here push v[] into scratch): a
-        // failed pivot is recorded in a bit mask and replaced by a harmless 1
-        const bool bad = !(p > 0.0) || !isfinite(p);
-        failmask |= bad ? (1u << c) : 0u;
-        p = bad ? 1.0 : p;
-        // 1/sqrt(p) from the hardware estimate and two Newton steps
-        double inv = __builtin_amdgcn_rsq(p);
-        {
-          const double h = 0.5 * p;
-          double e = __builtin_fma(-h * inv, inv, 0.5);
-          inv = __builtin_fma(inv, e, inv);
-          e = __builtin_fma(-h * inv, inv, 0.5);
-          inv = __builtin_fma(inv, e, inv);
-        }
-        const double dc = p * inv;
-        const double rc = idl ? v[c] * inv : ((j == c) ? dc : (j > c ? v[c] * inv : 0.0));
-        v[c] = rc;
-        // row c of R_JJ (lanes 0..31) / entry (c, j) of W (lanes 32..63)
-        // (one LDS address per lane: identity lanes write W, the others R_JJ)
-        *(wrow + (idl ? c : c * CB)) = rc;
-        if (c + 1 < CB) v[c + 1] -= readlane_d(rc, c + 1) * rc;  // R(c, c+1) x_j
-        if (c >= 1) {
-#pragma unroll
-          for (int i = 0; i < CB; ++i)
-            if (i > c) v[i] -= rr[i] * rc_prev;  // R(c-1, i) x_j
-        }
-        rc_prev = rc;
-      }
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    if (wv == 0 && lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
     __syncthreads();  // R_JJ in Dl, W in Wt
     if (Wout)  // W_J for the triangular solves with this factor
       for (int idx = tid; idx < CB * TRS; idx += CHOL_THREADS) Wout[(long)(J0 / CB) * (CB * TRS) + idx] = Wt[idx];
@@ -738,6 +758,279 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_chol6: k_chol5 with the block-row update moved off the critical path.
+// While wave 0 runs the pivot chain of block J, waves 1..7 compute the
+// update of block row J+1 from the rows above block J,
+// C = R(0:J0, J1:J1+32)^T R(0:J0, J1:k) (J1 = J0 + 32), by FP64 MFMA with
+// the operands read straight from memory: R's strict lower triangle holds
+// R's rows transposed while the kernel runs (R(t, c) also at (c, t) for
+// every panel entry), so a 16-column row segment is 128 contiguous bytes.
+// Block J's own rows, the last chunk of that update, are added from the
+// panel still in LDS once it is solved.  The strict lower triangle is
+// zeroed at the end.  The same waves prefetch block row J+1's entries of G.
+// Tiles of the update and of the B-write: tile = (wv - 1) + 7q, row tile
+// tile & 1, column tile tile >> 1 (at most 28 tiles for k <= 256).
+// ---------------------------------------------------------------------------
+template <int MAXT, int US>
+__global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__restrict__ G, long ldg,
+                                                        double shift_rel, double *__restrict__ R,
+                                                        int *flag, double *stats,
+                                                        unsigned long long *dbg, double *Wout, double *fro) {
+  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
+  double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
+  double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
+  __shared__ double red[CHOL_THREADS / 64];
+  __shared__ double Wt[CB * TRS];  // Wt[x * TRS + y] = W(y, x), W = R_JJ^{-T}
+  __shared__ int sh_fail;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave: scalar branches
+  constexpr int NW = CHOL_THREADS / 64, NU = NW - 1;       // NU update waves (1..7)
+  constexpr int wps = US;  // panel row stride (odd, > the widest panel)
+  const int u0 = wv - 1;
+  double tr = 0.0;
+  for (int j = tid; j < k; j += CHOL_THREADS) tr += G[j + (long)j * ldg];
+  tr = block_sum(tr, red);
+  const double shift = shift_rel * tr / (k > 0 ? k : 1);
+  if (tid == 0) sh_fail = 0;
+  double dmax = 0.0, dmin = INFINITY, dev = 0.0, sq = 0.0;  // sq: ||R||_F^2 (this thread's share)
+  // geometry of block row Jx: its rows jbx, the panel width wx, the tiles
+  auto geom = [&](int Jx, int &jbx, int &wx, int &wpadx, int &ntx) {
+    jbx = min(CB, k - Jx);
+    wx = k - Jx - jbx;  // > 0 only when jbx == CB
+    wpadx = (wx + 15) & ~15;
+    ntx = 2 * (2 + (wpadx >> 4));
+  };
+  // block row Jx's entries of G (upper triangle) for this wave's tiles
+  double gv[MAXT][4];
+  auto load_g = [&](int Jx, int ln) {
+    int jbx, wx, wpadx, ntx;
+    geom(Jx, jbx, wx, wpadx, ntx);
+#pragma unroll
+    for (int q = 0; q < MAXT; ++q) {
+      const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = ti * 16 + (ln & 15), c = tj * 16 + (ln >> 4) + 4 * u;
+        const bool need = tile < ntx && r < jbx && (c < CB ? (c < jbx && r <= c) : c - CB < wx);
+        gv[q][u] = need ? G[(Jx + r) + (long)(Jx + c) * ldg] : 0.0;
+      }
+    }
+  };
+  v4d_f acc[MAXT];  // C of the next block row to be factorised (this wave's tiles)
+  {
+    // block row 0 has no update: B = G rows + shift, copied by all threads
+    // (k <= 256: at most 32 x 256 entries, 16 per thread; all loads issue
+    // before the first store, from a clamped and so valid address)
+    int jb, w, wpad, nt;
+    geom(0, jb, w, wpad, nt);
+    constexpr int NCP = CB * 256 / CHOL_THREADS;
+    double cv[NCP];
+#pragma unroll
+    for (int e = 0; e < NCP; ++e) {
+      const int idx = tid + CHOL_THREADS * e, r = idx & (CB - 1), c = idx >> 5;
+      cv[e] = G[min(r, k - 1) + (long)min(c, k - 1) * ldg];
+    }
+#pragma unroll
+    for (int e = 0; e < NCP; ++e) {
+      const int idx = tid + CHOL_THREADS * e, r = idx & (CB - 1), c = idx >> 5;
+      if (c < CB)
+        Dl[c * CB + r] = (r < jb && c < jb) ? (r <= c ? cv[e] + (r == c ? shift : 0.0) : 0.0) : (r == c ? 1.0 : 0.0);
+      else if (c - CB < wpad)
+        Ps[r * wps + (c - CB)] = (c - CB < w) ? cv[e] : 0.0;
+    }
+  }
+  for (int J0 = 0; J0 < k; J0 += CB) {
+    int jb, w, wpad, ntiles;
+    geom(J0, jb, w, wpad, ntiles);
+    __syncthreads();  // the previous block's reads of Dl, Ps are done
+    if (dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    // per-lane index math of this block from an opaque copy of the lane
+    // index: left to LICM, the addresses of every tile would be hoisted out
+    // of the block loop and spilled
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    if (J0 > 0 && wv >= 1) {
+      // B = G rows + shift - C into Dl (diagonal block, column-major,
+      // identity past jb) and Ps (panel rows, zero past w);
+      // acc[q][v] = C(ti*16 + lane%16, tj*16 + lane/16 + 4v) (ntiles <= 28)
+      double sh = shift;  // opaque: its selects would be hoisted and spilled
+      asm volatile("" : "+v"(sh));
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) {
+        const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
+        if (tile < ntiles) {  // uniform
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int r = ti * 16 + (ln & 15), c = tj * 16 + (ln >> 4) + 4 * u;
+            if (c < CB) {
+              Dl[c * CB + r] = (r < jb && c < jb) ? (r <= c ? gv[q][u] + (r == c ? sh : 0.0) - acc[q][u] : 0.0)
+                                                  : (r == c ? 1.0 : 0.0);
+            } else if (c - CB < wpad) {
+              Ps[r * wps + (c - CB)] = (c - CB < w) ? gv[q][u] - acc[q][u] : 0.0;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const int J1 = J0 + CB;
+    int jb1, w1, wpad1, nt1;
+    geom(J1, jb1, w1, wpad1, nt1);
+    if (wv == 0) {
+      const unsigned failmask = chol_pivot_chain(Dl, Wt, lane);
+      if (lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) acc[q] = v4d_f{0, 0, 0, 0};
+    } else {
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) acc[q] = v4d_f{0, 0, 0, 0};
+      if (J1 < k) {
+        // update of block row J1 from rows 0..J0-1 (final since earlier
+        // blocks), from the transposed rows in R's lower triangle; columns
+        // past k (tile padding) read column k-1 instead, a finite value the
+        // B-write discards
+        if (J0 > 0) {
+#pragma unroll
+          for (int q = 0; q < MAXT; ++q) {
+            const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
+            if (tile < nt1) {  // uniform per wave: one MFMA chain per tile
+              const int ca = min(J1 + ti * 16 + (ln & 15), k - 1), cb = min(J1 + tj * 16 + (ln & 15), k - 1);
+              const double *pa = R + ca + (long)(ln >> 4) * k;
+              const double *pb = R + cb + (long)(ln >> 4) * k;
+              // operands of one 32-row chunk (8 MFMA steps) in registers, two
+              // buffers: the next chunk's loads are in flight while this
+              // chunk's MFMAs issue (the chunk offset opaque, so that the
+              // addresses are formed at each load instead of hoisted)
+              double xa[8], xb[8], ya[8], yb[8];
+              auto ld = [&](double *oa, double *ob, int kc) {
+                long off = (long)kc * k;
+                asm volatile("" : "+v"(off));
+#pragma unroll
+                for (int s8 = 0; s8 < 8; ++s8) {
+                  oa[s8] = pa[off + (long)(4 * s8) * k];
+                  ob[s8] = pb[off + (long)(4 * s8) * k];
+                }
+              };
+              auto mm = [&](const double *oa, const double *ob) {
+#pragma unroll
+                for (int s8 = 0; s8 < 8; ++s8) acc[q] = MFf<double>::op(ob[s8], oa[s8], acc[q]);
+              };
+              ld(xa, xb, 0);
+              for (int kc = 0; kc < J0; kc += 2 * CB) {
+                if (kc + CB < J0) ld(ya, yb, kc + CB);
+                mm(xa, xb);
+                if (kc + 2 * CB < J0) ld(xa, xb, kc + 2 * CB);
+                if (kc + CB < J0) mm(ya, yb);
+              }
+            }
+          }
+        }
+        load_g(J1, ln);
+      }
+    }
+    __syncthreads();  // R_JJ in Dl, W in Wt
+    if (Wout)  // W_J for the triangular solves with this factor
+      for (int idx = tid; idx < CB * TRS; idx += CHOL_THREADS) Wout[(long)(J0 / CB) * (CB * TRS) + idx] = Wt[idx];
+    if (!sh_fail && w > 0) {
+      // panel rows of R: P = W B (B: the updated panel rows in Ps), FP64 MFMA
+      // on 16x16 tiles: tile = wv + 8q, ti = wv & 1, tj = wv / 2 + 4q
+      const int ntp = 2 * (wpad >> 4);
+      const int ti = wv & 1, tj0 = wv >> 1;
+      int lp = lane;  // opaque: the tile bases are formed here, not hoisted and spilled
+      asm volatile("" : "+v"(lp));
+      v4d_f pacc[MAXT];
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) {
+        pacc[q] = v4d_f{0, 0, 0, 0};
+        const int tj = tj0 + 4 * q;
+        if (wv + NW * q < ntp) {  // uniform per wave
+#pragma unroll
+          for (int kk = 0; kk < CB; kk += 4) {
+            const int t = kk + (lp >> 4);
+            pacc[q] = MFf<double>::op(Ps[t * wps + tj * 16 + (lp & 15)], Wt[t * TRS + ti * 16 + (lp & 15)], pacc[q]);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) {
+        const int tj = tj0 + 4 * q;
+        if (wv + NW * q < ntp) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            Ps[(ti * 16 + (lp & 15)) * wps + tj * 16 + (lp >> 4) + 4 * u] = pacc[q][u];
+        }
+      }
+    }
+    __syncthreads();
+    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
+    if (sh_fail) break;
+    // diagonal block to R (+ statistics)
+    for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
+      const int t = idx & (CB - 1), l = idx >> 5;
+      if (t < jb && l < jb && t <= l) {
+        const double val = Dl[t * CB + l];
+        R[(J0 + t) + (long)(J0 + l) * k] = val;
+        sq += val * val;
+        if (t == l) {
+          dmax = fmax(dmax, fabs(val));
+          dmin = fmin(dmin, fabs(val));
+          dev = fmax(dev, fabs(val - 1.0));
+        } else {
+          dev = fmax(dev, fabs(val));
+        }
+      }
+    }
+    if (w > 0) {
+      // solved panel to R (coalesced: 32 consecutive threads write one column)
+      for (int idx = tid; idx < CB * w; idx += CHOL_THREADS) {
+        const int t = idx & (CB - 1), c = idx >> 5;
+        const double val = Ps[t * wps + c];
+        R[(J0 + t) + (long)(J0 + jb + c) * k] = val;
+        sq += val * val;
+        dev = fmax(dev, fabs(val));
+      }
+      // and transposed into the strict lower triangle (row J0 + t of R as
+      // part of column J0 + t), which the later updates read
+      for (int t = wv; t < CB; t += NW)
+        for (int c = lane; c < w; c += 64) R[(J0 + jb + c) + (long)(J0 + t) * k] = Ps[t * wps + c];
+      // block J's rows: the last chunk of block row J1's update, from the
+      // panel in LDS (Ps columns 0.. are block row J1's columns)
+      if (wv >= 1) {
+#pragma unroll
+        for (int q = 0; q < MAXT; ++q) {
+          const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
+          if (tile < nt1) {
+#pragma unroll
+            for (int kk = 0; kk < CB; kk += 4) {
+              const int t = kk + (ln >> 4);
+              acc[q] = MFf<double>::op(Ps[t * wps + tj * 16 + (ln & 15)], Ps[t * wps + ti * 16 + (ln & 15)], acc[q]);
+            }
+          }
+        }
+      }
+    }
+    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
+  }
+  __syncthreads();
+  if (dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
+  // R's strict lower triangle is zero on return
+  for (int j = wv; j < k; j += NW)
+    for (int i = j + 1 + lane; i < k; i += 64) R[i + (long)j * k] = 0.0;
+  const int failed = sh_fail;
+  dmax = block_max(dmax, red);
+  dmin = -block_max(-dmin, red);
+  dev = block_max(dev, red);
+  if (fro) sq = block_sum(sq, red);
+  if (tid == 0) {
+    flag[0] = failed;
+    stats[0] = failed ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
+    stats[1] = failed ? INFINITY : dev;
+    if (fro) fro[0] = sqrt(sq);
+  }
+}
+
 constexpr int CHOL_MAXK = 512;
 
 bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
@@ -759,11 +1052,11 @@ bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
     // elements (14 per thread), staging and panel row stride 225; Dl + Ps
     // at that stride (8224 doubles) hold the staging buffer and the reads
     // of the tiles past a block (up to index 7230)
-    constexpr int MAXT = 4, MAXE = 14, US = 225;
-    const size_t lds5 = std::max(lds, (size_t)(CB * CB + CB * US) * sizeof(double));
-    TTG_HIP(hipFuncSetAttribute((const void *)k_chol5<MAXT, MAXE, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds5));
-    hipLaunchKernelGGL((k_chol5<MAXT, MAXE, US>), dim3(1), dim3(CHOL_THREADS), lds5, s, (int)k, G, (long)ldg,
+    constexpr int MAXT = 4, US = 225;
+    const size_t lds6 = std::max(lds, (size_t)(CB * CB + CB * US) * sizeof(double));
+    TTG_HIP(hipFuncSetAttribute((const void *)k_chol6<MAXT, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds6));
+    hipLaunchKernelGGL((k_chol6<MAXT, US>), dim3(1), dim3(CHOL_THREADS), lds6, s, (int)k, G, (long)ldg,
                        shift_rel, R, flag, stats, dbg, Wout, fro);
     wrote_w = true;
   }
