@@ -769,10 +769,13 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__r
 // Block J's own rows, the last chunk of that update, are added from the
 // panel still in LDS once it is solved.  The strict lower triangle is
 // zeroed at the end.  The same waves prefetch block row J+1's entries of G.
-// Tiles of the update and of the B-write: tile = (wv - 1) + 7q, row tile
-// tile & 1, column tile tile >> 1 (at most 28 tiles for k <= 256).
+// Wave 4, which shares a SIMD with wave 0, takes no part in the update: its
+// FP64 MFMAs would compete with the pivot chain's FP64 arithmetic.  Tiles
+// of the update and of the B-write: update wave u = 0..5 (waves 1-3, 5-7),
+// tile = u + 6q, row tile u & 1, column tile u / 2 + 3q (at most 28 tiles
+// for k <= 256; MAXU = 5 per wave).
 // ---------------------------------------------------------------------------
-template <int MAXT, int US>
+template <int MAXT, int MAXU, int US>
 __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__restrict__ G, long ldg,
                                                         double shift_rel, double *__restrict__ R,
                                                         int *flag, double *stats,
@@ -785,9 +788,10 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__r
   __shared__ int sh_fail;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave: scalar branches
-  constexpr int NW = CHOL_THREADS / 64, NU = NW - 1;       // NU update waves (1..7)
+  constexpr int NW = CHOL_THREADS / 64, NU = NW - 2;       // NU update waves (1-3, 5-7)
   constexpr int wps = US;  // panel row stride (odd, > the widest panel)
-  const int u0 = wv - 1;
+  const bool upd = wv != 0 && wv != 4;
+  const int u0 = wv - 1 - (wv > 4 ? 1 : 0);
   double tr = 0.0;
   for (int j = tid; j < k; j += CHOL_THREADS) tr += G[j + (long)j * ldg];
   tr = block_sum(tr, red);
@@ -802,12 +806,12 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__r
     ntx = 2 * (2 + (wpadx >> 4));
   };
   // block row Jx's entries of G (upper triangle) for this wave's tiles
-  double gv[MAXT][4];
+  double gv[MAXU][4];
   auto load_g = [&](int Jx, int ln) {
     int jbx, wx, wpadx, ntx;
     geom(Jx, jbx, wx, wpadx, ntx);
 #pragma unroll
-    for (int q = 0; q < MAXT; ++q) {
+    for (int q = 0; q < MAXU; ++q) {
       const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -817,7 +821,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__r
       }
     }
   };
-  v4d_f acc[MAXT];  // C of the next block row to be factorised (this wave's tiles)
+  v4d_f acc[MAXU];  // C of the next block row to be factorised (this wave's tiles)
   {
     // block row 0 has no update: B = G rows + shift, copied by all threads
     // (k <= 256: at most 32 x 256 entries, 16 per thread; all loads issue
@@ -850,14 +854,14 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__r
     // of the block loop and spilled
     int ln = lane;
     asm volatile("" : "+v"(ln));
-    if (J0 > 0 && wv >= 1) {
+    if (J0 > 0 && upd) {
       // B = G rows + shift - C into Dl (diagonal block, column-major,
       // identity past jb) and Ps (panel rows, zero past w);
       // acc[q][v] = C(ti*16 + lane%16, tj*16 + lane/16 + 4v) (ntiles <= 28)
       double sh = shift;  // opaque: its selects would be hoisted and spilled
       asm volatile("" : "+v"(sh));
 #pragma unroll
-      for (int q = 0; q < MAXT; ++q) {
+      for (int q = 0; q < MAXU; ++q) {
         const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
         if (tile < ntiles) {  // uniform
 #pragma unroll
@@ -881,10 +885,10 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__r
       const unsigned failmask = chol_pivot_chain(Dl, Wt, lane);
       if (lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
 #pragma unroll
-      for (int q = 0; q < MAXT; ++q) acc[q] = v4d_f{0, 0, 0, 0};
-    } else {
+      for (int q = 0; q < MAXU; ++q) acc[q] = v4d_f{0, 0, 0, 0};
+    } else if (upd) {
 #pragma unroll
-      for (int q = 0; q < MAXT; ++q) acc[q] = v4d_f{0, 0, 0, 0};
+      for (int q = 0; q < MAXU; ++q) acc[q] = v4d_f{0, 0, 0, 0};
       if (J1 < k) {
         // update of block row J1 from rows 0..J0-1 (final since earlier
         // blocks), from the transposed rows in R's lower triangle; columns
@@ -892,7 +896,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__r
         // B-write discards
         if (J0 > 0) {
 #pragma unroll
-          for (int q = 0; q < MAXT; ++q) {
+          for (int q = 0; q < MAXU; ++q) {
             const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
             if (tile < nt1) {  // uniform per wave: one MFMA chain per tile
               const int ca = min(J1 + ti * 16 + (ln & 15), k - 1), cb = min(J1 + tj * 16 + (ln & 15), k - 1);
@@ -997,9 +1001,9 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__r
         for (int c = lane; c < w; c += 64) R[(J0 + jb + c) + (long)(J0 + t) * k] = Ps[t * wps + c];
       // block J's rows: the last chunk of block row J1's update, from the
       // panel in LDS (Ps columns 0.. are block row J1's columns)
-      if (wv >= 1) {
+      if (upd) {
 #pragma unroll
-        for (int q = 0; q < MAXT; ++q) {
+        for (int q = 0; q < MAXU; ++q) {
           const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
           if (tile < nt1) {
 #pragma unroll
@@ -1052,11 +1056,11 @@ bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
     // elements (14 per thread), staging and panel row stride 225; Dl + Ps
     // at that stride (8224 doubles) hold the staging buffer and the reads
     // of the tiles past a block (up to index 7230)
-    constexpr int MAXT = 4, US = 225;
+    constexpr int MAXT = 4, MAXU = 5, US = 225;
     const size_t lds6 = std::max(lds, (size_t)(CB * CB + CB * US) * sizeof(double));
-    TTG_HIP(hipFuncSetAttribute((const void *)k_chol6<MAXT, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    TTG_HIP(hipFuncSetAttribute((const void *)k_chol6<MAXT, MAXU, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds6));
-    hipLaunchKernelGGL((k_chol6<MAXT, US>), dim3(1), dim3(CHOL_THREADS), lds6, s, (int)k, G, (long)ldg,
+    hipLaunchKernelGGL((k_chol6<MAXT, MAXU, US>), dim3(1), dim3(CHOL_THREADS), lds6, s, (int)k, G, (long)ldg,
                        shift_rel, R, flag, stats, dbg, Wout, fro);
     wrote_w = true;
   }
