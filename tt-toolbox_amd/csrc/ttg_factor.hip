@@ -6,8 +6,6 @@
 //             the next block row from the rows above by FP64 MFMA (R's rows
 //             kept transposed in its lower triangle meanwhile); the panel is
 //             one MFMA product with R_JJ^{-T}.
-//   k_chol5 : the same with the block-row update before the pivots (kept
-//             for reference timing; not dispatched).
 //   k_chol4 : k <= 480, right-looking: panel columns solved in registers of
 //             waves 1..7 beside the pivot loop, MFMA SYRK trailing update.
 //   k_chol3 : any k <= 512, right-looking, panel solved after the pivots.
@@ -17,7 +15,7 @@
 //             wave 0, while waves 1..3 stage block J+1.  k_trsm covers
 //             k > 256.  No inverse of R is formed, so the residual
 //             ||X R - A|| stays O(u ||X|| ||R||) for any conditioning.
-//   k_trsm3 : X = A R^{-1} for FP64 right after k_chol5, which left the
+//   k_trsm3 : X = A R^{-1} for FP64 right after k_chol6, which left the
 //             inverses W_J = R_JJ^{-1} of the 32x32 diagonal blocks: per
 //             block B = A_J - X_{<J} R_{<J,J}, X0 = B W_J, then one
 //             refinement X_J = X0 + (B - X0 R_JJ) W_J, all by MFMA, so the
@@ -556,212 +554,12 @@ __device__ __forceinline__ unsigned chol_pivot_chain(double *Dl, double *Wt, int
 }
 
 // ---------------------------------------------------------------------------
-// k_chol5: left-looking form of k_chol4.  Block row J is brought up to date
-// just before it is factorised, C = R(0:J0, J-block)^T R(0:J0, J0:k) with
-// FP64 MFMA from the factored rows above (streamed through LDS, accumulated
-// in registers), instead of a right-looking read-modify-write of the whole
-// trailing matrix after every block; the pivot loop and the fused panel are
-// k_chol4's.
-// MAXT: output tiles per wave (2 * (k - 32 + 15) / 16 / 8 rounded up);
-// MAXE: staging elements per thread (32 * 16 * ntj / 512); US: staging row
-// stride (odd, > the widest staged block row; US * 32 + 16 * 4 * MAXT fits
-// the Dl + Ps allocation, so tiles past the block read inside it).
-// ---------------------------------------------------------------------------
-template <int MAXT, int MAXE, int US>
-__global__ __launch_bounds__(CHOL_THREADS) void k_chol5(int k, const double *__restrict__ G, long ldg,
-                                                        double shift_rel, double *__restrict__ R,
-                                                        int *flag, double *stats,
-                                                        unsigned long long *dbg, double *Wout, double *fro) {
-  extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
-  double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
-  double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
-  double *Us = Dl;            // update staging: Us[t * US + c] = R(kc + t, J0 + c)
-  __shared__ double red[CHOL_THREADS / 64];
-  __shared__ double Wt[CB * TRS];  // Wt[x * TRS + y] = W(y, x), W = R_JJ^{-T}
-  __shared__ int sh_fail;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave: scalar branches
-  constexpr int NW = CHOL_THREADS / 64;
-  double tr = 0.0;
-  for (int j = tid; j < k; j += CHOL_THREADS) tr += G[j + (long)j * ldg];
-  tr = block_sum(tr, red);
-  const double shift = shift_rel * tr / (k > 0 ? k : 1);
-  // R's strict lower triangle is zero; its upper triangle is written block
-  // row by block row (the update reads G itself, so nothing is copied)
-  for (int j = wv; j < k; j += NW)
-    for (int i = j + 1 + lane; i < k; i += 64) R[i + (long)j * k] = 0.0;
-  if (tid == 0) sh_fail = 0;
-  double dmax = 0.0, dmin = INFINITY, dev = 0.0, sq = 0.0;  // sq: ||R||_F^2 (this thread's share)
-  for (int J0 = 0; J0 < k; J0 += CB) {
-    const int jb = min(CB, k - J0);
-    const int w = k - J0 - jb;  // > 0 only when jb == CB
-    const int wpad = (w + 15) & ~15;
-    constexpr int wps = US;  // panel row stride (odd, > the widest panel)
-    __syncthreads();
-    if (dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
-    // Left-looking update of block row J: C = R(0:J0, J0:J0+32)^T R(0:J0, J0:k)
-    // by FP64 MFMA on 16x16 tiles (row tile ti in {0,1}, column tile tj),
-    // the factored rows above streamed through LDS 32 at a time (Us, odd
-    // row stride) with the next chunk loaded into registers meanwhile.
-    const int ntj = 2 + (wpad >> 4), ncols = ntj * 16;
-    const int ntiles = 2 * ntj;
-    // this wave's tiles: tile = wv + 8q, ti = wv & 1, tj = wv / 2 + 4q (LDS
-    // offsets affine in q and kk: immediates on one per-lane base)
-    const int ti = wv & 1, tj0 = wv >> 1;
-    v4d_f acc[MAXT];
-#pragma unroll
-    for (int q = 0; q < MAXT; ++q) acc[q] = v4d_f{0, 0, 0, 0};
-    // this block row's entries of G (upper triangle), loaded before the update
-    // so that their latency hides behind it
-    double gv[MAXT][4];
-#pragma unroll
-    for (int q = 0; q < MAXT; ++q) {
-      const int tj = tj0 + 4 * q;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int r = ti * 16 + (lane & 15), c = tj * 16 + (lane >> 4) + 4 * u;
-        const bool need = wv + NW * q < ntiles && r < jb && (c < CB ? (c < jb && r <= c) : c - CB < w);
-        gv[q][u] = need ? G[(J0 + r) + (long)(J0 + c) * ldg] : 0.0;
-      }
-    }
-    double pre[MAXE];
-    auto fetch = [&](int kc) {
-#pragma unroll
-      for (int e = 0; e < MAXE; ++e) {
-        const int idx = tid + CHOL_THREADS * e;
-        const int t = idx & (CB - 1), c = idx >> 5;
-        pre[e] = (c < ncols && J0 + c < k) ? R[(kc + t) + (long)(J0 + c) * k] : 0.0;
-      }
-    };
-    if (J0 > 0) fetch(0);
-    for (int kc = 0; kc < J0; kc += CB) {
-      __syncthreads();
-#pragma unroll
-      for (int e = 0; e < MAXE; ++e) {
-        const int idx = tid + CHOL_THREADS * e;
-        const int t = idx & (CB - 1), c = idx >> 5;
-        if (c < ncols) Us[t * US + c] = pre[e];
-      }
-      __syncthreads();
-      if (kc + CB < J0) fetch(kc + CB);
-#pragma unroll
-      for (int q = 0; q < MAXT; ++q) {
-        const int tj = tj0 + 4 * q;
-        if (wv + NW * q < ntiles) {  // uniform per wave
-#pragma unroll
-          for (int kk = 0; kk < CB; kk += 4) {
-            const int t = kk + (lane >> 4);
-            acc[q] = MFf<double>::op(Us[t * US + tj * 16 + (lane & 15)], Us[t * US + ti * 16 + (lane & 15)], acc[q]);
-          }
-        }
-      }
-    }
-    __syncthreads();  // Us is dead: Dl and Ps reuse its space
-    // B = G rows - C into Dl (diagonal block, column-major, identity past jb)
-    // and Ps (panel rows, zero past w); acc[q][v] = C(ti*16 + lane%16, tj*16 + lane/16 + 4v)
-#pragma unroll
-    for (int q = 0; q < MAXT; ++q) {
-      const int tile = wv + NW * q;
-      if (tile < ntiles) {
-        const int tj = tj0 + 4 * q;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int r = ti * 16 + (lane & 15), c = tj * 16 + (lane >> 4) + 4 * u;
-          if (c < CB) {
-            Dl[c * CB + r] = (r < jb && c < jb) ? (r <= c ? gv[q][u] + (r == c ? shift : 0.0) - acc[q][u] : 0.0)
-                                                : (r == c ? 1.0 : 0.0);
-          } else if (c - CB < wpad) {
-            Ps[r * wps + (c - CB)] = (c - CB < w) ? gv[q][u] - acc[q][u] : 0.0;
-          }
-        }
-      }
-    }
-    __syncthreads();
-    if (wv == 0) {
-      const unsigned failmask = chol_pivot_chain(Dl, Wt, lane);
-      if (lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
-    }
-    __syncthreads();  // R_JJ in Dl, W in Wt
-    if (Wout)  // W_J for the triangular solves with this factor
-      for (int idx = tid; idx < CB * TRS; idx += CHOL_THREADS) Wout[(long)(J0 / CB) * (CB * TRS) + idx] = Wt[idx];
-    if (!sh_fail && w > 0) {
-      // panel rows of R: P = W B (B: the updated panel rows in Ps), FP64 MFMA
-      // on 16x16 tiles: tile = wv + 8q, ti = wv & 1, tj = wv / 2 + 4q
-      const int ntp = 2 * (wpad >> 4);
-      const int ti = wv & 1, tj0 = wv >> 1;
-      v4d_f pacc[MAXT];
-#pragma unroll
-      for (int q = 0; q < MAXT; ++q) {
-        pacc[q] = v4d_f{0, 0, 0, 0};
-        const int tj = tj0 + 4 * q;
-        if (wv + NW * q < ntp) {  // uniform per wave
-#pragma unroll
-          for (int kk = 0; kk < CB; kk += 4) {
-            const int t = kk + (lane >> 4);
-            pacc[q] = MFf<double>::op(Ps[t * wps + tj * 16 + (lane & 15)], Wt[t * TRS + ti * 16 + (lane & 15)], pacc[q]);
-          }
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < MAXT; ++q) {
-        const int tj = tj0 + 4 * q;
-        if (wv + NW * q < ntp) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            Ps[(ti * 16 + (lane & 15)) * wps + tj * 16 + (lane >> 4) + 4 * u] = pacc[q][u];
-        }
-      }
-    }
-    __syncthreads();
-    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
-    if (sh_fail) break;
-    // diagonal block to R (+ statistics)
-    for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
-      const int t = idx & (CB - 1), l = idx >> 5;
-      if (t < jb && l < jb && t <= l) {
-        const double val = Dl[t * CB + l];
-        R[(J0 + t) + (long)(J0 + l) * k] = val;
-        sq += val * val;
-        if (t == l) {
-          dmax = fmax(dmax, fabs(val));
-          dmin = fmin(dmin, fabs(val));
-          dev = fmax(dev, fabs(val - 1.0));
-        } else {
-          dev = fmax(dev, fabs(val));
-        }
-      }
-    }
-    if (w <= 0) continue;
-    // solved panel to R (coalesced: 32 consecutive threads write one column)
-    for (int idx = tid; idx < CB * w; idx += CHOL_THREADS) {
-      const int t = idx & (CB - 1), c = idx >> 5;
-      const double val = Ps[t * wps + c];
-      R[(J0 + t) + (long)(J0 + jb + c) * k] = val;
-      sq += val * val;
-      dev = fmax(dev, fabs(val));
-    }
-    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
-  }
-  __syncthreads();
-  if (dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
-  const int failed = sh_fail;
-  dmax = block_max(dmax, red);
-  dmin = -block_max(-dmin, red);
-  dev = block_max(dev, red);
-  if (fro) sq = block_sum(sq, red);
-  if (tid == 0) {
-    flag[0] = failed;
-    stats[0] = failed ? INFINITY : (dmin > 0.0 ? dmax / dmin : INFINITY);
-    stats[1] = failed ? INFINITY : dev;
-    if (fro) fro[0] = sqrt(sq);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// k_chol6: k_chol5 with the block-row update moved off the critical path.
-// While wave 0 runs the pivot chain of block J, waves 1..7 compute the
-// update of block row J+1 from the rows above block J,
+// k_chol6: left-looking Cholesky with the block-row update off the critical path.
+// Block row J is brought up to date from the factored rows above it, then
+// its diagonal block is factorised in registers of wave 0 and its panel
+// solved as one MFMA product with R_JJ^{-T} (k_chol4's pivot loop and fused
+// panel).  While wave 0 runs the pivot chain of block J, the update waves
+// compute the update of block row J+1 from the rows above block J,
 // C = R(0:J0, J1:J1+32)^T R(0:J0, J1:k) (J1 = J0 + 32), by FP64 MFMA with
 // the operands read straight from memory: R's strict lower triangle holds
 // R's rows transposed while the kernel runs (R(t, c) also at (c, t) for
