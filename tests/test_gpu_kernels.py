@@ -16,6 +16,7 @@ def f64p(a):
 @pytest.mark.parametrize("prec,ta,tb,M,N,K", [
     (0, 0, 0, 37, 51, 29), (0, 1, 0, 64, 64, 2048), (0, 0, 1, 70, 33, 100), (0, 1, 1, 17, 90, 45),
     (1, 0, 0, 65, 130, 40), (1, 1, 0, 256, 256, 2048), (2, 1, 0, 256, 256, 2048), (2, 0, 0, 100, 70, 300),
+    (0, 0, 0, 2048, 256, 256), (0, 0, 1, 2048, 256, 256), (0, 1, 0, 256, 2048, 256),
 ])
 def test_gemm(gpu_ctx, prec, ta, tb, M, N, K):
     rng = np.random.default_rng(M * 7 + N)

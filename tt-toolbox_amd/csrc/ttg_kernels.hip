@@ -64,11 +64,12 @@ template <> struct MF<float> {
 };
 
 constexpr int GBM = 64, GBN = 64, GBK = 64;
-constexpr int GLD = GBM * GBK / 256;  // operand elements each thread stages per K tile
 
-// C(MxN) = alpha*op(A)*op(B) + beta*C.  Block BM x 64 (BM = 64, or 32 when
-// 64 x 64 blocks would not fill the CUs), 4 waves (2x2), each wave a
-// (BM/2) x 32 sub-tile made of (BM/32) x 2 MFMA 16x16x4 tiles.  Operand layout of
+// C(MxN) = alpha*op(A)*op(B) + beta*C.  Block BM x BN (64 x 64; 32 x 64 when
+// 64 x 64 blocks would not fill the CUs; 32 x 32 when even 32 x 64 blocks
+// would leave one workgroup per CU, so that two share each CU and cover each
+// other's load latency), 4 waves (2x2), each wave a (BM/2) x (BN/2)
+// sub-tile made of (BM/32) x (BN/32) MFMA 16x16x4 tiles.  Operand layout of
 // v_mfma_*_16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the
 // accumulator element v of lane l is D[MF<T>::row(l, v)][l%16].  The waves
 // form the transposed tiles (op(B)^T op(A)^T), so that lane l holds rows
@@ -81,30 +82,33 @@ constexpr int GLD = GBM * GBK / 256;  // operand elements each thread stages per
 // TI is the storage type of A and B, T the MFMA type: TI = double, T = float
 // rounds the operands to FP32 while staging them (FP32 Gram products of FP64
 // data).
-template <class TI, class T, class TO, bool TA, bool TB, int BM = GBM>
+template <class TI, class T, class TO, bool TA, bool TB, int BM = GBM, int BN = GBN>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T alpha,
                                               const TI *__restrict__ A, long lda,
                                               const TI *__restrict__ B, long ldb, T beta,
                                               TO *__restrict__ C, long ldc, T *__restrict__ P) {
+  static_assert(!TB || BN == GBN, "op(B) = B^T stages 64 columns per K tile");
   __shared__ T As[GBK][BM + 4];
   constexpr int MI = BM / 32;            // 16-row MFMA tiles per wave
+  constexpr int NJ = BN / 32;            // 16-column MFMA tiles per wave
   constexpr int LA = BM * GBK / 256;     // A elements each thread stages per K tile
-  __shared__ T Bs[GBK][GBN + 4];
+  constexpr int LB = BN * GBK / 256;     // B elements each thread stages per K tile
+  __shared__ T Bs[GBK][BN + 4];
   typedef typename MF<T>::V V;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * GBN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int kbeg = blockIdx.z * kchunk;
   const int kend = min(K, kbeg + kchunk);
-  V acc[MI][2];
+  V acc[MI][NJ];
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = V{0, 0, 0, 0};
+    for (int j = 0; j < NJ; ++j) acc[i][j] = V{0, 0, 0, 0};
   // 64 consecutive threads read 64 consecutive elements of the operand's
   // contiguous dimension (M or K for A, K or N for B)
   // prefetched in the storage type: a conversion here would stall on the load
-  TI ra[LA], rb[GLD];
+  TI ra[LA], rb[LB];
   // A, M contiguous: BM consecutive threads read BM consecutive rows
   constexpr int AQ = 256 / BM;
   auto fetch = [&](int k0) {
@@ -120,7 +124,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
       }
     }
 #pragma unroll
-    for (int i = 0; i < GLD; ++i) {
+    for (int i = 0; i < LB; ++i) {
       if (!TB) {
         const int gk = k0 + c, gn = n0 + q + 4 * i;
         rb[i] = (gn < N && gk < kend) ? B[gk + (long)gn * ldb] : TI(0);
@@ -138,7 +142,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
       else As[c][q + 4 * i] = T(ra[i]);
     }
 #pragma unroll
-    for (int i = 0; i < GLD; ++i) {
+    for (int i = 0; i < LB; ++i) {
       if (!TB) Bs[c][q + 4 * i] = T(rb[i]);
       else Bs[q + 4 * i][c] = T(rb[i]);
     }
@@ -151,27 +155,27 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
     if (k0 + GBK < kend) fetch(k0 + GBK);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
-      T a[MI], b[2];
+      T a[MI], b[NJ];
 #pragma unroll
       for (int i = 0; i < MI; ++i) a[i] = As[kk + (lane >> 4)][wm * (BM / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < NJ; ++j) b[j] = Bs[kk + (lane >> 4)][wn * (BN / 2) + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = MF<T>::op(b[j], a[i], acc[i][j]);
+        for (int j = 0; j < NJ; ++j) acc[i][j] = MF<T>::op(b[j], a[i], acc[i][j]);
     }
   }
   const bool split = gridDim.z > 1;
-  // acc[i][j][v] of lane l: C(m0 + wm*BM/2 + i*16 + l%16, n0 + wn*32 + j*16 + row(l, v))
+  // acc[i][j][v] of lane l: C(m0 + wm*BM/2 + i*16 + l%16, n0 + wn*BN/2 + j*16 + row(l, v))
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int gm = m0 + wm * (BM / 2) + i * 16 + (lane & 15);
-        const int gn = n0 + wn * 32 + j * 16 + MF<T>::row(lane, v);
+        const int gn = n0 + wn * (BN / 2) + j * 16 + MF<T>::row(lane, v);
         if (gm < M && gn < N) {
           if (split) {
             P[(long)blockIdx.z * M * N + gm + (long)gn * M] = acc[i][j][v];
@@ -215,6 +219,13 @@ static bool gemm_half_tiles(int64_t M, int64_t N, int S, int num_cus) {
   return tiles * S < num_cus;
 }
 
+// 32 x 32 blocks: 32 x 64 blocks would give at most one workgroup per CU
+// (op(B) = B only)
+static bool gemm_quarter_tiles(int64_t M, int64_t N, int S, int num_cus, bool tb) {
+  const int64_t tiles = ((M + 31) / 32) * ((N + GBN - 1) / GBN);
+  return !tb && tiles * S <= num_cus;
+}
+
 int64_t gemm_ws_elems(int64_t M, int64_t N, int64_t K, int num_cus) {
   const int s = splitk_count(M, N, K, num_cus);
   return s > 1 ? (int64_t)s * M * N : 0;
@@ -235,13 +246,17 @@ static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, in
   if (K <= 0) kchunk = 0;
   // 32-row tiles when 64 x 64 tiles would leave CUs idle (bond-sized products)
   const bool half = gemm_half_tiles(M, N, S, num_cus);
-  const int bm = half ? 32 : GBM;
-  dim3 grid((unsigned)((M + bm - 1) / bm), (unsigned)((N + GBN - 1) / GBN), (unsigned)S);
+  const bool quarter = half && gemm_quarter_tiles(M, N, S, num_cus, tb);
+  const int bm = half ? 32 : GBM, bn = quarter ? 32 : GBN;
+  dim3 grid((unsigned)((M + bm - 1) / bm), (unsigned)((N + bn - 1) / bn), (unsigned)S);
   ProfScope prof(KC_GEMM, st, 2.0 * (double)M * (double)N * (double)K,
                  (double)sizeof(TI) * ((double)M * K + (double)K * N) + (double)sizeof(TO) * M * N);
 #define TTG_GEMM_LAUNCH(TA_, TB_)                                                                             \
   do {                                                                                                        \
-    if (half)                                                                                                 \
+    if (quarter)                                                                                              \
+      hipLaunchKernelGGL((k_gemm<TI, T, TO, TA_, false, 32, 32>), grid, dim3(256), 0, st, (int)M, (int)N,     \
+                         (int)K, kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws);          \
+    else if (half)                                                                                            \
       hipLaunchKernelGGL((k_gemm<TI, T, TO, TA_, TB_, 32>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K,   \
                          kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws);                  \
     else                                                                                                      \
@@ -334,11 +349,15 @@ static void gram_nearid_impl(hipStream_t st, int64_t m, int64_t k, const double 
     int kchunk = (int)(((m + S - 1) / S + GBK - 1) / GBK * GBK);
     S = (int)((m + kchunk - 1) / kchunk);
     const bool half = gemm_half_tiles(k, k, S, cus);
-    const int bm = half ? 32 : GBM;
-    dim3 grid((unsigned)((k + bm - 1) / bm), (unsigned)((k + GBN - 1) / GBN), (unsigned)S);
+    const bool quarter = half && gemm_quarter_tiles(k, k, S, cus, false);
+    const int bm = half ? 32 : GBM, bn = quarter ? 32 : GBN;
+    dim3 grid((unsigned)((k + bm - 1) / bm), (unsigned)((k + bn - 1) / bn), (unsigned)S);
     {
       ProfScope prof(KC_GEMM, st, 2.0 * (double)k * k * m, 8.0 * (2.0 * m * k + k * k));
-      if (half)
+      if (quarter)
+        hipLaunchKernelGGL((k_gemm<double, T, double, true, false, 32, 32>), grid, dim3(256), 0, st, (int)k, (int)k,
+                           (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws);
+      else if (half)
         hipLaunchKernelGGL((k_gemm<double, T, double, true, false, 32>), grid, dim3(256), 0, st, (int)k, (int)k, (int)m,
                            kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws);
       else
