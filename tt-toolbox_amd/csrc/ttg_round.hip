@@ -304,14 +304,21 @@ template <class T> struct Rounder {
     scale_by_inv<double>(st, R, kk * k, nrm + i + 1);
     // core_{i+1} <- R core_{i+1}   (round.m:45-47)
     const int64_t N1 = n[i + 1] * r[i + 2];
-    to_T(RT, R, kk * k);
+    // R in the cores' type (the FP64 form reads R itself, no copy)
+    const T *Rt;
+    if constexpr (std::is_same<T, double>::value) {
+      Rt = R;
+    } else {
+      to_T(RT, R, kk * k);
+      Rt = RT;
+    }
     T *dst = buf[1] + ofs[i + 1];
     if (fmul && *fmul) {
-      (*fmul)(i + 1, RT, kk, dst);  // core_{i+1} generated on chip inside the product
+      (*fmul)(i + 1, Rt, kk, dst);  // core_{i+1} generated on chip inside the product
     } else {
       T *src = buf[0] + ofs[i + 1];
       if (gen) gen(i + 1, src);
-      g(false, false, kk, N1, k, RT, kk, src, k, dst, kk);
+      g(false, false, kk, N1, k, Rt, kk, src, k, dst, kk);
     }
     cp[i + 1] = dst;
     r[i + 1] = kk;  // round.m:48
