@@ -60,10 +60,12 @@ def _chol(gpu_ctx, G, shift=0.0):
     return R, int(flag[0]), stats
 
 
-@pytest.mark.parametrize("k", [5, 40, 65, 100, 256, 320, 480, 512])
+@pytest.mark.parametrize("k", [5, 32, 33, 40, 65, 96, 100, 224, 255, 256, 320, 480, 512])
 def test_cholesky(gpu_ctx, k):
     """Against LAPACK, with the diagonal-spread and max|R - I| statistics
-    (k <= 480: fused pivot/panel kernel; 512: the panel-after-pivot kernel)."""
+    (k <= 256: the kernel with the overlapped block-row update, incl. one
+    block, a last block of one column and full blocks; k <= 480: fused
+    pivot/panel kernel; 512: the panel-after-pivot kernel)."""
     rng = np.random.default_rng(k)
     A = rng.standard_normal((3 * k, k))
     G = A.T @ A

@@ -768,6 +768,23 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__r
     __syncthreads();
     if (dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
     if (sh_fail) break;
+    // block J's rows: the last chunk of block row J1's update, from the
+    // panel in LDS (Ps columns 0.. are block row J1's columns); before the
+    // stores, which the other waves issue meanwhile and which nothing waits
+    // for until the next barrier
+    if (w > 0 && upd) {
+#pragma unroll
+      for (int q = 0; q < MAXU; ++q) {
+        const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
+        if (tile < nt1) {
+#pragma unroll
+          for (int kk = 0; kk < CB; kk += 4) {
+            const int t = kk + (ln >> 4);
+            acc[q] = MFf<double>::op(Ps[t * wps + tj * 16 + (ln & 15)], Ps[t * wps + ti * 16 + (ln & 15)], acc[q]);
+          }
+        }
+      }
+    }
     // diagonal block to R (+ statistics)
     for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
       const int t = idx & (CB - 1), l = idx >> 5;
@@ -797,21 +814,6 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__r
       // part of column J0 + t), which the later updates read
       for (int t = wv; t < CB; t += NW)
         for (int c = lane; c < w; c += 64) R[(J0 + jb + c) + (long)(J0 + t) * k] = Ps[t * wps + c];
-      // block J's rows: the last chunk of block row J1's update, from the
-      // panel in LDS (Ps columns 0.. are block row J1's columns)
-      if (upd) {
-#pragma unroll
-        for (int q = 0; q < MAXU; ++q) {
-          const int tile = u0 + NU * q, ti = tile & 1, tj = tile >> 1;
-          if (tile < nt1) {
-#pragma unroll
-            for (int kk = 0; kk < CB; kk += 4) {
-              const int t = kk + (ln >> 4);
-              acc[q] = MFf<double>::op(Ps[t * wps + tj * 16 + (ln & 15)], Ps[t * wps + ti * 16 + (ln & 15)], acc[q]);
-            }
-          }
-        }
-      }
     }
     if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
   }
