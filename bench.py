@@ -189,8 +189,9 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: the TT rounding path has no multi-GPU form in this "
-              "build (see DESIGN.md for how it would shard); run with --gpus 1", flush=True)
+        print("bench.py measures one GPU only: the TT rounding path has no multi-GPU form (it "
+              "would shard by core ranges, passing the bond factors between GPUs); run with --gpus 1",
+              flush=True)
         sys.exit(2)
 
     import numpy as np
