@@ -47,24 +47,12 @@ def _rate(fn, reps, sync):
     return reps / (time.perf_counter() - t0)
 
 
-def other_configs(ttgpu, W, ctx, dev):
-    """Configs 1, 2 and 5 (and the <y, A x> gradient at config-1 shapes), one GPU,
-    inputs resident in HBM; rates are host-timed over back-to-back calls."""
-    import numpy as np
+def mtimes_leg(ttgpu, W, ctx, dev):
+    """Config 2: A*x (d=12, n=m=4, rank_A 5, rank_x 64), unfused, and the fused
+    round(A*x, 1e-6); inputs resident in HBM, host-timed back-to-back calls."""
     import torch
-    from ttgpu import autograd as AG
     sync = torch.cuda.synchronize
     out = {}
-    # config 1: round(X + 0.5 X, 1e-10), d=20 n=4 r=40 -> 20
-    c1 = W.round1()
-    n1, r1, core1 = c1["z"]
-    d1 = torch.from_numpy(core1).to(dev)
-    o1 = torch.empty(ttgpu.capacity(n1, r1), dtype=torch.float64, device=dev)
-    ro = {}
-    rate = _rate(lambda: ro.__setitem__("r", ttgpu.round_dev(ctx, n1, r1, d1, c1["eps"], out=o1)[0]), 20, sync)
-    out["config1_round"] = {"workload": "round(X + 0.5 X, 1e-10), d=20 n=4 r=40, FP64", "rounds_per_s": rate,
-                            "ranks_out_max": int(max(ro["r"]))}
-    # config 2: A*x (d=12, n=m=4, rank_A 5, rank_x 64) and the fused round(A*x, 1e-6)
     c2 = W.matvec2()
     na, ma, ra, acore = c2["a"]
     nx, rx, xcore = c2["x"]
@@ -86,6 +74,27 @@ def other_configs(ttgpu, W, ctx, dev):
     out["config2_fused_mtimes_round"] = {
         "workload": "round(A*x, 1e-6) fused (product cores built on chip), FP64",
         "rounds_per_s": rate, "ranks_out": [int(v) for v in rr["r"]]}
+    return out
+
+
+def other_configs(ttgpu, W, ctx, dev):
+    """Configs 1, 2 and 5 (and the <y, A x> gradient at config-1 shapes), one GPU,
+    inputs resident in HBM; rates are host-timed over back-to-back calls."""
+    import numpy as np
+    import torch
+    from ttgpu import autograd as AG
+    sync = torch.cuda.synchronize
+    out = {}
+    # config 1: round(X + 0.5 X, 1e-10), d=20 n=4 r=40 -> 20
+    c1 = W.round1()
+    n1, r1, core1 = c1["z"]
+    d1 = torch.from_numpy(core1).to(dev)
+    o1 = torch.empty(ttgpu.capacity(n1, r1), dtype=torch.float64, device=dev)
+    ro = {}
+    rate = _rate(lambda: ro.__setitem__("r", ttgpu.round_dev(ctx, n1, r1, d1, c1["eps"], out=o1)[0]), 20, sync)
+    out["config1_round"] = {"workload": "round(X + 0.5 X, 1e-10), d=20 n=4 r=40, FP64", "rounds_per_s": rate,
+                            "ranks_out_max": int(max(ro["r"]))}
+    out.update(mtimes_leg(ttgpu, W, ctx, dev))
     # config 5: completion loss + gradient, 65,536 samples, d=16 n=8, model rank 32
     c5 = W.completion5()
     n5, r5, core5 = c5["model"]
@@ -178,6 +187,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-rounding leg (512 uneven tensors, config 4 recipe)")
     ap.add_argument("--batched-count", type=int, default=512)
+    ap.add_argument("--mtimes-only", action="store_true",
+                    help="only run the config-2 leg (A*x and the fused round(A*x)) and exit (profiling)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched leg (profiling; prints its JSON object)")
     ap.add_argument("--no-other", action="store_true",
@@ -208,6 +219,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": nb, "copy_probe_bytes_written": nb, "launches": 3}), flush=True)
+        return
+    if args.mtimes_only:
+        ctx = ttgpu.Context(0)
+        print(json.dumps(mtimes_leg(ttgpu, W, ctx, torch.device("cuda:0"))), flush=True)
         return
     if args.batched_only:
         ctx = ttgpu.Context(0)
