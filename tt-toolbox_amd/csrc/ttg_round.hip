@@ -257,7 +257,10 @@ template <class T> struct Rounder {
     return level + 1;
   }
   bool qr_accept(const int *hflags, const double *hstats, int level, bool allow_g32) const {
-    if (hflags[0] || hflags[1]) return false;
+    // flags[1] is the second pass's Cholesky, which only levels >= 1 run
+    // (level 0's second pass is the near-identity series): at level 0 it is
+    // not written, and not read
+    if (hflags[0] || (!nearid_level(level) && hflags[1])) return false;
     if (!(hstats[0] < 1e7)) return false;   // diag(R1) spread: kappa estimate
     // Q1 nearly orthonormal: max|Q1^T Q1 - I| (level 0) or max|R2 - I| (level 1)
     const bool g32 = mixed && level == 0 && allow_g32;
@@ -343,7 +346,8 @@ template <class T> struct Rounder {
       transpose<T>(st, W, kw, N, Tt);
       tallA = Tt;
     }
-    TTG_HIP(hipMemsetAsync(fl, 0, 4 * sizeof(int), st));
+    // (fl[0] is written by every first-pass Cholesky, fl[1] by the second
+    // pass's at levels >= 1, the only ones that read it: no clearing needed)
     const bool bound = wide && allow_notrunc && level == 0;
     if (level < 2) {
       qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats, level, false, bound);
