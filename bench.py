@@ -128,6 +128,48 @@ def other_configs(ttgpu, W, ctx, dev):
     return out
 
 
+def verify_batch(tens, out, eps, oracle_count=2):
+    """Checks the batched output before its rate is reported: for every tensor
+    x + x the ranks equal the generic TT ranks of x (closed form), the cores are
+    balanced (C_k C_k^T = nrm0^2 I, k >= 2) and nrm0^d = ||x + x|| = 2 ||x||;
+    the tensors of most QR work are also rounded by the FP64 oracle (ranks
+    identical, relative difference <= 1e-12)."""
+    import numpy as np
+    from oracle import tt_oracle as O
+    bad, worst_bal, worst_nrm = [], 0.0, 0.0
+    works = []
+    for i, ((n, r, core), (r_t, y_t)) in enumerate(zip(tens, out)):
+        z = O.TT(n, r, core)
+        rx = [1] + [int(v) // 2 for v in r[1:-1]] + [1]
+        x = O.TT.from_cores([np.asfortranarray(z.core_k(k)[:rx[k], :, :rx[k + 1]]) for k in range(z.d)])
+        y = O.TT(n, r_t, y_t)
+        nrm0, bres = O.balance_residual(y)
+        nz = 2.0 * O.tt_norm_gram(x)
+        rel = abs(nrm0 ** len(n) - nz) / nz
+        worst_bal, worst_nrm = max(worst_bal, bres), max(worst_nrm, rel)
+        if r_t.tolist() != O.generic_ranks(n, rx) or bres > 1e-12 or rel > 1e-10:
+            bad.append(i)
+        c, rk = 0.0, float(r[0])
+        for j in range(len(n) - 1):
+            m = rk * n[j]
+            k = min(m, float(r[j + 1]))
+            c += m * k * k
+            rk = k
+        works.append(c)
+    worst_ref = 0.0
+    for j in np.argsort(works)[::-1][:oracle_count]:
+        n, r, core = tens[j]
+        ref = O.tt_round(O.TT(n, r, core), eps)
+        r_t, y_t = out[j]
+        err = O.tt_rel_diff(O.TT(n, r_t, y_t), ref) if r_t.tolist() == ref.r.tolist() else float("inf")
+        worst_ref = max(worst_ref, err)
+        if err > 1e-12:
+            bad.append(int(j))
+    return {"ok": not bad, "failed_tensors": bad[:16], "tensors_checked": len(out),
+            "max_balance_residual": worst_bal, "max_rel_norm_error": worst_nrm,
+            "oracle_tensors": oracle_count, "max_rel_diff_vs_oracle": worst_ref}
+
+
 def batched_leg(ttgpu, W, ctx, dev, count):
     """Config 4 recipe: one call for the whole batch (the first call on a context
     also tunes the execution) against a per-tensor loop of round_dev on one stream
@@ -147,9 +189,12 @@ def batched_leg(ttgpu, W, ctx, dev, count):
     t0 = time.perf_counter()
     ttgpu.round_batched_dev(ctx, ns, rs, dall, cofs[:-1], bb["eps"], info=binfo)  # first call tunes
     t1 = time.perf_counter()
-    ttgpu.round_batched_dev(ctx, ns, rs, dall, cofs[:-1], bb["eps"], info=binfo)
+    res = ttgpu.round_batched_dev(ctx, ns, rs, dall, cofs[:-1], bb["eps"], info=binfo)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    check = verify_batch(tens, [(np.asarray(r_t), y_t.cpu().numpy()) for r_t, y_t in res], bb["eps"])
+    del res
+    log("batched output check:", check)
     m = min(32, len(tens))
     for i in range(m):
         ttgpu.round_dev(ctx, ns[i], rs[i], dall[int(cofs[i]):int(cofs[i + 1])], bb["eps"])
@@ -170,7 +215,8 @@ def batched_leg(ttgpu, W, ctx, dev, count):
     return {
         "workload": f"{len(tens)} TT tensors, d in [8,24], n in {{2,4,8,16}}, ranks in [8,128] doubled "
                     f"by self-addition, eps=1e-8, FP64",
-        "tensors_per_s": len(tens) / (t2 - t1), "seconds": t2 - t1, "first_call_seconds_incl_tuning": t1 - t0,
+        "output_check": check,
+        "tensors_per_s": len(tens) / (t2 - t1) if check["ok"] else None, "seconds": t2 - t1, "first_call_seconds_incl_tuning": t1 - t0,
         "per_tensor_loop_tensors_per_s": len(tens) / loop_est,
         "per_tensor_loop_sample": f"first {m} tensors timed, scaled by QR work to the batch",
         "speedup_vs_loop": loop_est / (t2 - t1), "jacobi_sweeps": binfo.get("jacobi_sweeps")}
@@ -437,11 +483,3 @@ def main():
 
 if __name__ == "__main__":
     main()
-    # skip the HIP runtime's exit-time teardown, which has crashed intermittently
-    # in processes that ran the batched rounder's worker streams; the JSON line is
-    # already written and the OS releases the device resources.  Under a profiler
-    # (rocprofv3 writes its results at exit) the process exits normally.
-    if not any(v in os.environ for v in ("ROCPROF_OUTPUT_PATH", "ROCPROFILER_LIBRARY_CTOR")):
-        sys.stdout.flush()
-        sys.stderr.flush()
-        os._exit(0)

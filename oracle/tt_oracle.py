@@ -391,6 +391,56 @@ def tt_dot(a, b):
 
 
 # ---------------------------------------------------------------------------
+# size-independent properties of a rounded TT (checks at full size)
+# ---------------------------------------------------------------------------
+def generic_ranks(n, r):
+    """Ranks ``round`` returns for a tensor of exact generic TT rank when no
+    singular value falls under the threshold: the sweep-1 caps of
+    ``@tt_tensor/round.m:48`` (``r_{i+1} <- min(r_i n_i, r_{i+1})``, left to
+    right), then the sweep-2 caps ``r_i <- min(r_i, n_i r_{i+1})`` (right to
+    left, the SVD of an ``r_i x n_i r_{i+1}`` unfolding has at most that many
+    values).  For a random TT and for ``x + x`` (rank concatenation of a
+    random ``x``) these are the ranks of the dense unfoldings."""
+    n = [int(v) for v in n]
+    rr = [int(v) for v in r]
+    d = len(n)
+    for i in range(d - 1):
+        rr[i + 1] = min(rr[i] * n[i], rr[i + 1])
+    for i in range(d - 1, 0, -1):
+        rr[i] = min(rr[i], n[i] * rr[i + 1])
+    return rr
+
+
+def balance_residual(tt):
+    """Departure of a rounded TT from the balancing of ``@tt_tensor/round.m:90-107``:
+    cores 2..d have orthogonal right unfoldings scaled by ``nrm0``
+    (``C_k C_k^T = nrm0^2 I``) and ``||C_1||_F = nrm0``, with
+    ``nrm0 = ||C_1||_F``.  Returns ``(nrm0, max_k max|C_k C_k^T / nrm0^2 - I|)``."""
+    nrm0 = float(np.linalg.norm(tt.core_k(0)))
+    if nrm0 == 0.0:
+        return 0.0, 0.0
+    worst = 0.0
+    for k in range(1, tt.d):
+        c = tt.core_k(k).reshape((tt.r[k], -1), order="F")
+        g = (c @ c.T) / (nrm0 * nrm0)
+        worst = max(worst, float(np.max(np.abs(g - np.eye(g.shape[0])))))
+    return nrm0, worst
+
+
+def tt_norm_gram(tt):
+    """``||X||_F`` by the left Gram chain ``G_k = sum_j C_k[:,j,:]^T G_{k-1} C_k[:,j,:]``
+    (the interface products of ``@tt_tensor/dot.m:92-111`` with both
+    arguments equal).  Cheap at low rank; accurate to ``sqrt(u)`` relative at worst."""
+    g = np.ones((1, 1))
+    for k in range(tt.d):
+        rk, nk, rk1 = int(tt.r[k]), int(tt.n[k]), int(tt.r[k + 1])
+        c = tt.core_k(k)
+        t = g @ c.reshape((rk, nk * rk1), order="F")                # G_{k-1} x_1 C_k
+        g = c.reshape((rk * nk, rk1), order="F").T @ t.reshape((rk * nk, rk1), order="F")
+    return float(np.sqrt(max(g[0, 0], 0.0)))
+
+
+# ---------------------------------------------------------------------------
 # textbook cross-check: sequential truncated SVD on the dense tensor
 # ---------------------------------------------------------------------------
 def dense_sequential_round(full, n, eps, rmax=None):

@@ -21,24 +21,3 @@ def gpu_ctx():
         pytest.fail("GPU test collected but no GPU is visible")
     import ttgpu
     return ttgpu.default_context(0)
-
-
-_session = {}
-
-
-def pytest_sessionfinish(session, exitstatus):
-    _session["status"] = int(exitstatus)
-
-
-@pytest.hookimpl(trylast=True)
-def pytest_unconfigure(config):
-    """End a session that used the GPU without the HIP runtime's exit-time
-    teardown.  Processes that ran the batched rounder (host worker threads,
-    each with its own HIP stream) have crashed intermittently inside that
-    teardown after every test had passed and all output was written; the
-    session's status is kept and the OS releases the device resources."""
-    torch = sys.modules.get("torch")
-    if "status" in _session and torch is not None and torch.cuda.is_initialized():
-        sys.stdout.flush()
-        sys.stderr.flush()
-        os._exit(_session["status"])

@@ -223,3 +223,50 @@ def test_bilinear_gradient_matches_finite_differences():
             assert abs(gx[k].ravel()[p] - fd) <= 1e-6 * max(1.0, abs(fd))
         for p, fd in _fd_grad(lambda cs: O.bilinear_value_grad(O.TT.from_cores(cs), a, x)[0], yc, k, rng=rng):
             assert abs(gy[k].ravel()[p] - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
+# ---------------------------------------------------------------- full-size property checkers
+def _dense_unfolding_ranks(x):
+    full = O.tt_full(x)
+    n = [int(v) for v in x.n]
+    out = [1]
+    for i in range(1, len(n)):
+        rows = int(np.prod(n[:i]))
+        s = np.linalg.svd(full.reshape((rows, -1), order="F"), compute_uv=False)
+        out.append(int(np.sum(s > 1e-10 * s[0])))
+    return out + [1]
+
+
+@pytest.mark.parametrize("seed,n,r", [
+    (31, [2, 3, 2, 4, 2], [1, 2, 7, 9, 3, 1]),     # caps from both sides
+    (32, [4, 4, 4], [1, 3, 5, 1]),                  # no cap binds
+    (33, [2, 2, 2, 2, 2, 2], [1, 5, 5, 5, 5, 5, 1]),  # every bond capped
+])
+def test_generic_ranks_equal_dense_unfolding_ranks(seed, n, r):
+    rng = np.random.default_rng(seed)
+    x = O.rand_tt(rng, n, r)
+    z = O.tt_plus(x, x)  # exact rank of x, concatenated to 2r
+    want = _dense_unfolding_ranks(x)
+    assert O.generic_ranks(n, r) == want
+    assert O.tt_round(z, 1e-8).r.tolist() == want
+
+
+def test_balance_residual_of_rounded_tt_and_planted_scaling():
+    rng = np.random.default_rng(34)
+    x = O.rand_tt(rng, [3, 4, 3, 2], [1, 3, 6, 4, 1])
+    y = O.tt_round(O.tt_plus(x, x), 1e-10)
+    nrm0, res = O.balance_residual(y)
+    assert res <= 1e-13
+    assert abs(nrm0 ** y.d - O.tt_norm(y)) <= 1e-12 * O.tt_norm(y)
+    bad = y.copy()
+    ps = bad.ps
+    bad.core[ps[2]:ps[3]] *= 1.001  # one core off its balance
+    assert O.balance_residual(bad)[1] > 1e-3
+
+
+def test_tt_norm_gram_matches_qr_norm_and_dense():
+    rng = np.random.default_rng(35)
+    x = O.rand_tt(rng, [3, 2, 4, 3], [1, 3, 5, 2, 1])
+    dense = np.linalg.norm(O.tt_full(x))
+    assert abs(O.tt_norm_gram(x) - dense) <= 1e-13 * dense
+    assert abs(O.tt_norm(x) - dense) <= 1e-13 * dense

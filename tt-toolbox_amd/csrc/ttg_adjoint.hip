@@ -117,6 +117,21 @@ __global__ __launch_bounds__(256) void k_completion_grad(int d, int k, const lon
   }
 }
 
+__global__ void k_fill_zero_int(int *p, int count) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0;
+}
+
+// flag[0] != 0 if some sample index is outside 0 <= idx[s*d+k] < n[k]
+__global__ void k_check_idx(int d, const long *__restrict__ n, long S, const int *__restrict__ idx,
+                            int *__restrict__ flag) {
+  int bad = 0;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < S * d; e += (long)gridDim.x * blockDim.x) {
+    const int v = idx[e];
+    bad |= (v < 0 || v >= n[e % d]);
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
 template <class T>
 __global__ void k_sum_parts(const T *__restrict__ part, int nparts, long size, T *__restrict__ out) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < size; e += (long)gridDim.x * blockDim.x) {
@@ -159,6 +174,7 @@ ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
   z.take<T>(S);
   z.take<double>(nblk);
   z.take<T>(gblk * maxcore);
+  z.take<int>(64);
   ctx->arena.reserve(z.used + 4096);
   ctx->arena.reset();
   long *tab = ctx->arena.take<long>(5 * (d + 1));
@@ -167,6 +183,7 @@ ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
   T *res = ctx->arena.take<T>(S);
   double *lpart = ctx->arena.take<double>(nblk);
   T *part = ctx->arena.take<T>(gblk * maxcore);
+  int *dflag = ctx->arena.take<int>(64);
   std::vector<long> htab(5 * (d + 1));
   for (int64_t k = 0; k < d; ++k) {
     htab[k] = n[k];
@@ -178,6 +195,18 @@ ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
   htab[(d + 1) + d] = r[d];
   TTG_HIP(hipMemcpyAsync(tab, htab.data(), htab.size() * sizeof(long), hipMemcpyHostToDevice, st));
   const long *dn = tab, *dr = tab + (d + 1), *dco = tab + 2 * (d + 1), *dlo = tab + 3 * (d + 1);
+  // every sample index is checked against the mode sizes before any kernel
+  // gathers core entries with it
+  {
+    hipLaunchKernelGGL(k_fill_zero_int, dim3(1), dim3(64), 0, st, dflag, 1);
+    const unsigned cb = (unsigned)std::min<int64_t>((S * d + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_check_idx, dim3(cb), dim3(256), 0, st, (int)d, dn, (long)S, didx, dflag);
+    TTG_HIP(hipGetLastError());
+    int hflag = 0;
+    TTG_HIP(hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipStreamSynchronize(st));
+    TTG_CHECK(hflag == 0, TTG_BAD_ARG, "sample index out of range (needs 0 <= idx[s*d+k] < n[k])");
+  }
   const size_t shm = (size_t)groups * 2 * G * sizeof(T);
   hipLaunchKernelGGL((k_completion_left<T>), dim3((unsigned)nblk), dim3(block), shm, st, (int)d, dn,
                      dr, dco, dlo, dcore, (long)S, didx, dvals, G, Lbuf, res, lpart);

@@ -48,11 +48,39 @@ static ttg_ctx *make_worker(const ttg_ctx *parent) {
   return w;
 }
 
+// Releases every HIP object a context created, in dependency order: the work
+// on its streams is finished first, then worker contexts (recursively), then
+// the captured graph (before the stream it was captured on), the side stream
+// and its events, profiler events, pinned host memory and the owned stream.
+// The device arenas are freed by their destructors when the context is
+// deleted.  Nothing is left for the runtime's exit-time teardown.
+void release_ctx(ttg_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+  destroy_workers(ctx);
+  if (ctx->pgraph) (void)hipGraphExecDestroy(ctx->pgraph);
+  ctx->pgraph = nullptr;
+  ctx->pgraph_key.clear();
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
+  ctx->side = nullptr;
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  ctx->ev_fork = ctx->ev_join = nullptr;
+  for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
+  ctx->prof.pool.clear();
+  ctx->prof.used = 0;
+  if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  ctx->host_pinned = nullptr;
+  ctx->host_pinned_cap = 0;
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  ctx->stream = nullptr;
+}
+
 void destroy_workers(ttg_ctx *ctx) {
   for (ttg_ctx *w : ctx->workers) {
-    (void)hipStreamSynchronize(w->stream);
-    (void)hipStreamDestroy(w->stream);
-    for (auto e : w->prof.pool) (void)hipEventDestroy(e);
+    release_ctx(w);
     delete w;
   }
   ctx->workers.clear();
@@ -161,12 +189,17 @@ static void run_persist(ttg_ctx *ctx, const BatchArgs &a, int64_t count, const s
   TTG_HIP(hipMemcpyAsync(dr, a.r, rtot * sizeof(int64_t), hipMemcpyHostToDevice, s));
   TTG_HIP(hipMemcpyAsync(drout, a.r_out, rtot * sizeof(int64_t), hipMemcpyHostToDevice, s));
   persist_prepare(sh.nt, sh.ycap);
-  const int kSweeps = 60;
-  // the captured launch is replayed while every argument is unchanged
-  const std::vector<uint64_t> key = {(uint64_t)sh.nt, (uint64_t)slots, (uint64_t)m, (uint64_t)(uintptr_t)ddesc,
+  const int kSweeps = jacobi_sweep_cap();
+  // the captured launch is replayed while every argument is unchanged: the
+  // key holds every value and pointer persist_launch bakes into the graph
+  const std::vector<uint64_t> key = {(uint64_t)sh.nt, (uint64_t)slots, (uint64_t)m, (uint64_t)count,
+                                     (uint64_t)(uintptr_t)ddesc, (uint64_t)(uintptr_t)dorder,
+                                     (uint64_t)(uintptr_t)dn, (uint64_t)(uintptr_t)dr,
+                                     (uint64_t)(uintptr_t)drout, (uint64_t)(uintptr_t)dstat,
+                                     (uint64_t)(uintptr_t)dctr,
                                      (uint64_t)(uintptr_t)a.dcore, (uint64_t)(uintptr_t)a.dcore_out,
                                      (uint64_t)(uintptr_t)dws, (uint64_t)slot_core, (uint64_t)slot_kmax,
-                                     (uint64_t)sh.ycap, (uint64_t)std::hash<double>{}(a.eps),
+                                     (uint64_t)sh.ycap, (uint64_t)kSweeps, (uint64_t)std::hash<double>{}(a.eps),
                                      (uint64_t)(uintptr_t)ctx->parena.base};
   if (!ctx->pgraph || ctx->pgraph_key != key) {
     if (ctx->pgraph) {

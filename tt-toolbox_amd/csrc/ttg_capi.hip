@@ -9,6 +9,8 @@ ttg_status round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r
                      int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
                      ttg_info *info);
 ttg_status round_capacity(int64_t d, const int64_t *n, const int64_t *r, int64_t *cap);
+// sizes/ranks of A*x: d >= 1, every mode size and rank >= 1 (throws TTG_BAD_ARG)
+void validate_mv(int64_t d, const int64_t *n, const int64_t *m, const int64_t *ra, const int64_t *rx);
 ttg_status mtimes_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
                       const int64_t *ra, const double *dacore, const int64_t *rx,
                       const double *dxcore, int64_t *ry, double *dycore, int64_t cap,
@@ -95,19 +97,7 @@ ttg_status ttg_ctx_create(int device, void *stream, ttg_ctx **out) {
 
 ttg_status ttg_ctx_destroy(ttg_ctx *ctx) {
   if (!ctx) return TTG_OK;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
-  ttg::destroy_workers(ctx);
-  if (ctx->pgraph) (void)hipGraphExecDestroy(ctx->pgraph);
-  if (ctx->side) {
-    (void)hipStreamSynchronize(ctx->side);
-    (void)hipStreamDestroy(ctx->side);
-  }
-  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
-  if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
-  for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
+  ttg::release_ctx(ctx);
   delete ctx;
   return TTG_OK;
 }
@@ -246,6 +236,8 @@ ttg_status ttg_mtimes(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *
   if (!ctx) return TTG_BAD_ARG;
   TTG_GUARD_BEGIN
   TTG_CHECK(n && m && ra && rx && ry && acore && xcore && ycore && d >= 1, TTG_BAD_ARG, "null pointer argument");
+  // sizes are checked before any size arithmetic or staging allocation
+  validate_mv(d, n, m, ra, rx);
   TTG_HIP(hipSetDevice(ctx->device));
   ctx->last_error.clear();
   int64_t la = 0, lx = 0, ly = 0;
@@ -305,6 +297,7 @@ ttg_status ttg_mtimes_round(ttg_ctx *ctx, int64_t d, const int64_t *n, const int
   if (!ctx) return TTG_BAD_ARG;
   TTG_GUARD_BEGIN
   TTG_CHECK(n && m && ra && rx && r_out && acore && xcore && core_out && d >= 1, TTG_BAD_ARG, "null pointer argument");
+  validate_mv(d, n, m, ra, rx);
   TTG_HIP(hipSetDevice(ctx->device));
   ctx->last_error.clear();
   std::vector<int64_t> rp(d + 1);

@@ -125,6 +125,11 @@ namespace ttg {
 
 // releases the batched-rounding worker contexts of ctx
 void destroy_workers(ttg_ctx *ctx);
+// releases every HIP object of ctx (workers, graph, streams, events, pinned
+// memory) in dependency order; the arenas go with `delete ctx`
+void release_ctx(ttg_ctx *ctx);
+// Jacobi sweep cap (60; TTG_JACOBI_MAX_SWEEPS=1..60 lowers it to force NO_CONVERGENCE)
+int jacobi_sweep_cap();
 
 // one tensor of a persistent batched rounding (device copy)
 struct PTensorDesc {

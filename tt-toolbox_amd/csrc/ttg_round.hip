@@ -33,6 +33,16 @@ namespace ttg {
 static constexpr int kMaxRank = 512;
 static constexpr int kMaxSweeps = 60;
 
+// Jacobi sweep cap: kMaxSweeps, or TTG_JACOBI_MAX_SWEEPS (1..kMaxSweeps) so
+// that the NO_CONVERGENCE status can be forced from the host in tests (the
+// iteration then ends after that many sweeps; every kernel exits normally)
+int jacobi_sweep_cap() {
+  const char *env = std::getenv("TTG_JACOBI_MAX_SWEEPS");
+  if (!env) return kMaxSweeps;
+  const int v = std::atoi(env);
+  return (v >= 1 && v <= kMaxSweeps) ? v : kMaxSweeps;
+}
+
 // capped ranks of the first sweep (round.m:48)
 static std::vector<int64_t> capped_ranks(int64_t d, const int64_t *n, const int64_t *r) {
   std::vector<int64_t> rr(r, r + d + 1);
@@ -391,7 +401,7 @@ template <class T> struct Rounder {
     // stopping cosine^2: plain form 1e-16 (the last sweep's rotations leave
     // O(1e-16) behind); mixed form 1e-8 (O(1e-8) left, far below its 1e-6
     // error budget, one sweep fewer)
-    jacobi_svd(st, pl, R, /*transS=*/wide, Y, jsweeps, bar, jerr, kMaxSweeps, Yws, nullptr,
+    jacobi_svd(st, pl, R, /*transS=*/wide, Y, jsweeps, bar, jerr, jacobi_sweep_cap(), Yws, nullptr,
                mixed ? 1e-8 : 1e-16);
     sort_chop(st, Y, pl.rows, kj, pl.kpad, ep, rmax_i, sv, perm, scal_i, scal_d + 8);
     // one synchronisation: r1, Jacobi status, QR acceptance data
@@ -672,9 +682,10 @@ ttg_status round_capacity(int64_t d, const int64_t *n, const int64_t *r, int64_t
 }
 
 // ---- mtimes (unfused) and fused mtimes + round -----------------------------
-static void validate_mv(int64_t d, const int64_t *n, const int64_t *m, const int64_t *ra,
-                        const int64_t *rx) {
+void validate_mv(int64_t d, const int64_t *n, const int64_t *m, const int64_t *ra,
+                 const int64_t *rx) {
   TTG_CHECK(d >= 1, TTG_BAD_ARG, "d must be >= 1");
+  TTG_CHECK(n && m && ra && rx, TTG_BAD_ARG, "n, m and the ranks are required");
   for (int64_t i = 0; i < d; ++i) TTG_CHECK(n[i] >= 1 && m[i] >= 1, TTG_BAD_ARG, "mode sizes must be >= 1");
   for (int64_t i = 0; i <= d; ++i) TTG_CHECK(ra[i] >= 1 && rx[i] >= 1, TTG_BAD_ARG, "ranks must be >= 1");
 }

@@ -179,14 +179,14 @@ _live = []
 
 
 def _close_all():
-    """At interpreter exit, forget every context instead of destroying it:
-    finalisers that call into the HIP runtime while it is being torn down
-    (worker streams of the batched rounder included) can crash the process,
-    and the operating system reclaims the device resources anyway."""
-    for ref in list(_live):
+    """At interpreter exit, destroy every live context, newest first, before
+    the HIP runtime tears itself down: each destroy waits for the context's
+    streams and releases its worker contexts, captured graph, streams,
+    events, pinned host memory and device workspace (ttg_ctx_destroy)."""
+    for ref in reversed(list(_live)):
         ctx = ref()
         if ctx is not None:
-            ctx.h = None
+            ctx.close()
     _live.clear()
     _default_ctx.clear()
 
