@@ -38,8 +38,11 @@ typedef enum {
 
 typedef enum {
   TTG_PREC_FP64 = 0,      /* plain form: FP64 throughout */
-  TTG_PREC_FP32 = 1       /* mixed form: FP32 MFMA Gram products (shifted CholeskyQR2); FP64 storage,
-                             triangular solves, contractions, r x r Cholesky and Jacobi SVD */
+  TTG_PREC_FP32 = 1       /* mixed form (error budget 1e-6 relative): the Gram-chain form when every
+                             bond is proven not to truncate (FP64 Gram chains, batched r x r
+                             Cholesky, one CholeskyQR pass, rows orthogonal to <= 1e-6); otherwise
+                             the sweeps with FP32 MFMA Gram products in sweep 1's CholeskyQR2 and
+                             FP64 storage, solves, contractions, r x r Cholesky and Jacobi SVD */
 } ttg_precision;
 
 typedef struct ttg_ctx ttg_ctx;
@@ -52,6 +55,11 @@ typedef struct {
   double  seconds;          /* wall time of the call on the host clock */
   int64_t qr_retries;       /* bond QRs redone at a higher level (FP64 / Cholesky pass, Householder) */
   int64_t svd_skipped;      /* sweep-2 bonds proven not to truncate: completed from the QR, no SVD */
+  int64_t path;             /* 0: QR/SVD sweeps; 1: Gram-chain form (every bond proven to keep all
+                               its singular values: result from batched Gram factorisations) */
+  double  err_est;          /* estimated relative Frobenius difference from the FP64 result of the
+                               sweeps (Gram form: u * sum_i kappa_F(R_i); sweeps: u * d) */
+  double  orth_est;         /* estimated max |C_k C_k^T / nrm0^2 - I| of the output cores */
 } ttg_info;
 
 /* Context: owns a HIP stream (or borrows `stream` when non-NULL), device workspace.

@@ -63,6 +63,9 @@ void release_ctx(ttg_ctx *ctx) {
   if (ctx->pgraph) (void)hipGraphExecDestroy(ctx->pgraph);
   ctx->pgraph = nullptr;
   ctx->pgraph_key.clear();
+  if (ctx->ggraph) (void)hipGraphExecDestroy(ctx->ggraph);
+  ctx->ggraph = nullptr;
+  ctx->gkey.clear();
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   ctx->side = nullptr;
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
@@ -282,6 +285,8 @@ bool round_small_persist(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_
   if (len_out) *len_out = o;
   if (info) {
     *info = inf[0];
+    info->err_est = 4.0 * 1.2e-16 * (double)d;
+    info->orth_est = 1.2e-16 * (double)d;
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   *status = st[0];

@@ -574,10 +574,21 @@ __device__ __forceinline__ unsigned chol_pivot_chain(double *Dl, double *Wt, int
 // for k <= 256; MAXU = 5 per wave).
 // ---------------------------------------------------------------------------
 template <int MAXT, int MAXU, int US>
-__global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k, const double *__restrict__ G, long ldg,
-                                                        double shift_rel, double *__restrict__ R,
-                                                        int *flag, double *stats,
-                                                        unsigned long long *dbg, double *Wout, double *fro) {
+__global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k_, const double *__restrict__ G_, long ldg_,
+                                                        double shift_rel, double *__restrict__ R_,
+                                                        int *flag_, double *stats_,
+                                                        unsigned long long *dbg, double *Wout_, double *fro_,
+                                                        const CholJob *__restrict__ jobs) {
+  // jobs != nullptr: workgroup b factorises the matrix of jobs[b] (batched)
+  const CholJob *jb = jobs ? jobs + blockIdx.x : nullptr;
+  const int k = jb ? jb->k : k_;
+  const double *__restrict__ G = jb ? jb->G : G_;
+  const long ldg = jb ? (long)jb->ldg : ldg_;
+  double *__restrict__ R = jb ? jb->R : R_;
+  int *flag = jb ? jb->flag : flag_;
+  double *stats = jb ? jb->stats : stats_;
+  double *Wout = jb ? jb->Wout : Wout_;
+  double *fro = jb ? jb->fro : fro_;
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
   double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
@@ -861,11 +872,29 @@ bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
     TTG_HIP(hipFuncSetAttribute((const void *)k_chol6<MAXT, MAXU, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds6));
     hipLaunchKernelGGL((k_chol6<MAXT, MAXU, US>), dim3(1), dim3(CHOL_THREADS), lds6, s, (int)k, G, (long)ldg,
-                       shift_rel, R, flag, stats, dbg, Wout, fro);
+                       shift_rel, R, flag, stats, dbg, Wout, fro, (const CholJob *)nullptr);
     wrote_w = true;
   }
   TTG_HIP(hipGetLastError());
   return wrote_w;
+}
+
+// Batched k_chol6: one workgroup per job, every job with k <= 256 (no shift).
+void chol_batched(hipStream_t s, const CholJob *jobs_dev, int njobs, int64_t kmax) {
+  if (njobs <= 0) return;
+  TTG_CHECK(kmax >= 1 && kmax <= 256, TTG_BAD_ARG, "batched Cholesky supports k <= 256");
+  constexpr int MAXT = 4, MAXU = 5, US = 225;
+  const int64_t wmax = std::max<int64_t>(kmax - CB, 0);
+  const int64_t wps = ((wmax + 15) & ~(int64_t)15) + 1;
+  const size_t lds = ((size_t)CB * CB + (size_t)CB * wps) * sizeof(double);
+  const size_t lds6 = std::max(lds, (size_t)(CB * CB + CB * US) * sizeof(double));
+  ProfScope prof(KC_CHOL, s, (double)njobs * kmax * kmax * kmax / 3.0, 8.0 * 2.0 * njobs * kmax * kmax);
+  TTG_HIP(hipFuncSetAttribute((const void *)k_chol6<MAXT, MAXU, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds6));
+  hipLaunchKernelGGL((k_chol6<MAXT, MAXU, US>), dim3((unsigned)njobs), dim3(CHOL_THREADS), lds6, s, 0,
+                     (const double *)nullptr, 0L, 0.0, (double *)nullptr, (int *)nullptr, (double *)nullptr,
+                     (unsigned long long *)nullptr, (double *)nullptr, (double *)nullptr, jobs_dev);
+  TTG_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -1014,10 +1043,23 @@ __host__ __device__ constexpr size_t trsm3_lds(int kp) {
   return ((size_t)TRS * kp + (size_t)(kp > CB ? kp - CB : 0) * TRS + 4 * (size_t)CB * TRS) * sizeof(double);
 }
 
-__global__ __launch_bounds__(256) void k_trsm3(int m, int k, const double *A, long lda,
-                                               const double *__restrict__ R, const double *__restrict__ Winv,
-                                               double *__restrict__ X, long ldx) {
+__global__ __launch_bounds__(256) void k_trsm3(int m_, int k_, const double *A_, long lda_,
+                                               const double *__restrict__ R_, const double *__restrict__ Winv_,
+                                               double *__restrict__ X_, long ldx_,
+                                               const TrsmJob *__restrict__ jobs) {
   extern __shared__ __align__(16) char smem[];
+  // jobs != nullptr: blockIdx.y selects the job (batched); row blocks past a
+  // job's m exit at once (the whole workgroup)
+  const TrsmJob *jb = jobs ? jobs + blockIdx.y : nullptr;
+  const int m = jb ? jb->m : m_;
+  const int k = jb ? jb->k : k_;
+  const double *A = jb ? jb->A : A_;
+  const long lda = jb ? (long)jb->lda : lda_;
+  const double *__restrict__ R = jb ? jb->R : R_;
+  const double *__restrict__ Winv = jb ? jb->Winv : Winv_;
+  double *__restrict__ X = jb ? jb->X : X_;
+  const long ldx = jb ? (long)jb->ldx : ldx_;
+  if ((int)blockIdx.x * CB >= m) return;
   typedef v4d_f V;
   const int kp = (k + CB - 1) / CB * CB;
   double *Xs = reinterpret_cast<double *>(smem);  // Xs[col * TRS + row]: A_J, then B_J, then X_J
@@ -1321,7 +1363,7 @@ void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t l
       const size_t lds3 = trsm3_lds((int)kp);
       TTG_HIP(hipFuncSetAttribute((const void *)k_trsm3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3));
       hipLaunchKernelGGL(k_trsm3, dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds3, s, (int)m, (int)k, A, (long)lda,
-                         R, Winv, X, (long)ldx);
+                         R, Winv, X, (long)ldx, (const TrsmJob *)nullptr);
       TTG_HIP(hipGetLastError());
       return;
     }
@@ -1342,6 +1384,21 @@ void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t l
                      A, (long)lda, R, X, (long)ldx, dbg);
   TTG_HIP(hipGetLastError());
 }
+// Batched k_trsm3 (k <= 256 for every job, Winv from k_chol6): grid
+// (row blocks of the tallest job) x njobs.
+void trsm3_batched(hipStream_t s, const TrsmJob *jobs_dev, int njobs, int64_t mmax, int64_t kmax) {
+  if (njobs <= 0 || mmax <= 0) return;
+  const int64_t kp = (kmax + CB - 1) / CB * CB;
+  TTG_CHECK(kp <= 256, TTG_BAD_ARG, "batched triangular solve supports k <= 256");
+  const size_t lds3 = trsm3_lds((int)kp);
+  ProfScope prof(KC_TRTRI, s, (double)njobs * mmax * kmax * kmax, 8.0 * njobs * (2.0 * mmax * kmax + kmax * kmax));
+  TTG_HIP(hipFuncSetAttribute((const void *)k_trsm3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3));
+  hipLaunchKernelGGL(k_trsm3, dim3((unsigned)((mmax + CB - 1) / CB), (unsigned)njobs), dim3(256), lds3, s, 0, 0,
+                     (const double *)nullptr, 0L, (const double *)nullptr, (const double *)nullptr, (double *)nullptr,
+                     0L, jobs_dev);
+  TTG_HIP(hipGetLastError());
+}
+
 template void trsm_right_upper<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const double *, double *, int64_t, unsigned long long *, const double *);
 template void trsm_right_upper<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const double *, float *, int64_t, unsigned long long *, const double *);
 

@@ -1,0 +1,568 @@
+// Gram-chain form of round (@tt_tensor/round.m:20-111) for tensors none of
+// whose bonds truncates.
+//
+// round.m orthogonalises X left to right by QR (sweep 1, round.m:39-53) and
+// then truncates right to left by SVD (sweep 2, round.m:59-89).  When no
+// singular value of any bond falls under the threshold ep*||s||
+// (ep = eps/sqrt(d-1), core/my_chop2.m:28-34), the result is X itself in
+// right-orthogonal form, balanced (round.m:90-107), with the ranks of the
+// capped sweeps.  This form proves that case and builds that result without
+// a sequential chain of factorisations:
+//
+//   caps   where a rank exceeds r_i n_i (left, round.m:48) or n_i r_{i+1}
+//          (right, the size of sweep 2's SVD) the core becomes a square
+//          identity and moves into its neighbour (one GEMM; the result does
+//          not depend on a bond's gauge): afterwards every rank is the rank
+//          round.m returns when nothing truncates;
+//   chains (two streams) right Gram chain H_i = s_i C_i (H_{i+1} (x) I) C_i^T
+//          of the right unfoldings X_{>=i}, left Gram chain
+//          G_{i+1} = s_i C_i^T (G_i (x) I) C_i of the left unfoldings
+//          X_{<=i} (s_i = 1/||C_i||_F^2 keeps them in range), GEMMs only;
+//   batch  every G_i and H_i factorised at once (G = R_L^T R_L,
+//          H = R_R^T R_R, one workgroup each) and inverted (X = I R^{-1});
+//   proof  the singular values of bond i are those of R_L R_R^T, so
+//          sigma_min >= 1/||R_R^{-T} R_L^{-1}||_F, and ||X||_F^2 = trace(G H);
+//          bond i keeps every value if sigma_min >= 2 ep ||X|| (the tail of
+//          my_chop2's reversed cumulative sum then never drops under
+//          (ep ||s||)^2); the bound is used only where it is far above the
+//          Gram's rounding floor (sigma_min^2 >= 1e-12 ||X||^2);
+//   result core_i = sqrt(s_i) R_R,i^{-T} C_i R_R,i+1^T (right-orthogonal
+//          rows), core_0 = C_0 R_R,1^T, all scaled by nrm0 = ||X||^(1/d)
+//          (round.m:102-107's balance), written straight into the output by
+//          two batched GEMMs.  One CholeskyQR pass: its rows are orthogonal
+//          to O(u kappa^2), accepted only if u (||R_R|| ||R_R^{-1}||)_F^2 is
+//          at most the form's orthogonality bound.
+//
+// One host synchronisation at the end reads the verdict; when any bond is
+// not proven (it may truncate), a factorisation fails or the tensor is
+// outside the batched kernels' sizes, the caller runs the QR/SVD sweeps.
+#include "ttg_internal.h"
+#include "ttg_device.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace ttg {
+
+namespace {
+
+// sum of squares of a chunk of core i (grid: chunks x d), fixed order
+__global__ __launch_bounds__(256) void k_core_sumsq(const double *const *__restrict__ cores,
+                                                    const int64_t *__restrict__ sizes, int chunks,
+                                                    double *__restrict__ part) {
+  __shared__ double red[4];
+  const int i = blockIdx.y, c = blockIdx.x;
+  const double *x = cores[i];
+  const long n = (long)sizes[i];
+  const long per = (n + chunks - 1) / chunks;
+  const long b = (long)c * per, e = min(n, b + per);
+  double acc = 0.0;
+  for (long t = b + threadIdx.x; t < e; t += 256) {
+    const double v = x[t];
+    acc += v * v;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[(long)i * chunks + c] = acc;
+}
+
+// s[i] = 1 / ||C_i||_F^2 (0 for a zero or non-finite core: the chain then
+// fails its factorisation and the caller takes the sweeps)
+__global__ void k_core_scale(int d, int chunks, const double *__restrict__ part, double *__restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d) return;
+  double t = 0.0;
+  for (int c = 0; c < chunks; ++c) t += part[(long)i * chunks + c];
+  s[i] = (t > 0.0 && t < INFINITY) ? 1.0 / t : 0.0;
+}
+
+// operands of one bond's certificate (device table, bonds 1..d-1)
+struct GramBond {
+  const double *G, *H, *RinvR, *Z;
+  int k, pad;
+};
+
+// per bond b, part p (grid: parts x bonds): partial sums over a slice of the
+// k x k operands, st[(b * parts + p) * 4 + {0,1,2,3}] = ||Z||_F^2 (Z =
+// R_R^{-T} R_L^{-1}), sum(G o H) = trace(G H), trace(H) = ||R_R||_F^2 and
+// ||R_R^{-1}||_F^2 over the slice; k_gram_final adds the parts in order.
+constexpr int kBondParts = 16;
+__global__ __launch_bounds__(256) void k_gram_bond(const GramBond *__restrict__ bonds, double *__restrict__ st) {
+  __shared__ double red[4];
+  const GramBond b = bonds[blockIdx.y];
+  const long kk = (long)b.k * b.k;
+  const long per = (kk + kBondParts - 1) / kBondParts;
+  const long t0 = (long)blockIdx.x * per, t1 = min(kk, t0 + per);
+  double z = 0.0, gh = 0.0, tr = 0.0, ri = 0.0;
+  for (long t = t0 + threadIdx.x; t < t1; t += 256) {
+    const double zv = b.Z[t], rv = b.RinvR[t];
+    z += zv * zv;
+    gh += b.G[t] * b.H[t];
+    ri += rv * rv;
+    if (t % (b.k + 1) == 0) tr += b.H[t];  // diagonal entry
+  }
+  z = block_sum(z, red);
+  gh = block_sum(gh, red);
+  tr = block_sum(tr, red);
+  ri = block_sum(ri, red);
+  if (threadIdx.x == 0) {
+    double *o = st + ((long)blockIdx.y * kBondParts + blockIdx.x) * 4;
+    o[0] = z;
+    o[1] = gh;
+    o[2] = tr;
+    o[3] = ri;
+  }
+}
+
+// Verdict and output scales (one workgroup).  ok[0] = 1 iff every
+// factorisation succeeded and every bond is proven (see the file header);
+// alpha[0] = nrm0 / ||A_0||, alpha[i] = sqrt(s_i) nrm0 with
+// log ||X|| = log ||A_0|| - sum_{j>=1} log(s_j) / 2 (A_0 = C_0 R_R,1^T).
+// diag[0] = smallest sigma_lb / (2 ep ||X||), diag[1] = largest
+// ||R_R|| ||R_R^{-1}|| (Frobenius), diag[2] = ||X||, diag[3] = first failing bond (0: none).
+__global__ __launch_bounds__(256) void k_gram_final(int d, const double *__restrict__ st,
+                                                    const int *__restrict__ cflags, const double *__restrict__ A0,
+                                                    long a0size, const double *__restrict__ s, double ep,
+                                                    double orth_tol, double *__restrict__ alpha,
+                                                    int *__restrict__ ok, double *__restrict__ diag) {
+  __shared__ double red[4];
+  __shared__ double sh_nrm0;
+  double a = 0.0;
+  for (long t = threadIdx.x; t < a0size; t += 256) a += A0[t] * A0[t];
+  a = block_sum(a, red);
+  // bonds i = 1..d-1, blockDim-strided; per-thread partials, then block reductions
+  double bad = 0.0, mmin = INFINITY, kmax = 0.0, lsum = 0.0, ksum = 0.0;
+  for (int i = 1 + threadIdx.x; i < d; i += blockDim.x) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const double *b = st + (long)(i - 1) * kBondParts * 4;
+    for (int p = 0; p < kBondParts; ++p)
+      for (int q = 0; q < 4; ++q) v[q] += b[4 * p + q];
+    const double rel = v[0] * v[1];  // ||X||^2 / sigma_lb^2
+    const double kap2 = v[2] * v[3];
+    const bool fin = rel > 0.0 && rel < INFINITY && kap2 < INFINITY && s[i] > 0.0;
+    const bool fact = cflags[i - 1] == 0 && cflags[d - 1 + i - 1] == 0;
+    const bool cert = fin && rel * (2.0 * ep) * (2.0 * ep) <= 1.0 && rel <= 1e12;
+    const bool orth = fin && 1.2e-16 * kap2 <= orth_tol;
+    if (!(fact && cert && orth)) bad = fmax(bad, (double)(d - i));  // largest = first failing bond
+    if (fin) {
+      mmin = fmin(mmin, 1.0 / (sqrt(rel) * 2.0 * ep));
+      kmax = fmax(kmax, sqrt(kap2));
+      ksum += sqrt(kap2);
+    }
+    lsum += s[i] > 0.0 ? log(s[i]) : 0.0;
+  }
+  bad = block_max(bad, red);
+  mmin = -block_max(-mmin, red);
+  kmax = block_max(kmax, red);
+  lsum = block_sum(lsum, red);
+  ksum = block_sum(ksum, red);
+  const double na = sqrt(a);
+  const bool bad_norm = !(na > 0.0 && na < INFINITY);
+  const double lx = (na > 0.0 ? log(na) : 0.0) - 0.5 * lsum;
+  const double nrm0 = exp(lx / d);
+  if (threadIdx.x == 0) {
+    sh_nrm0 = nrm0;
+    alpha[0] = na > 0.0 ? nrm0 / na : 0.0;
+    ok[0] = (bad == 0.0 && !bad_norm) ? 1 : 0;
+    diag[0] = mmin;
+    diag[1] = kmax;
+    diag[2] = exp(lx);
+    diag[3] = bad > 0.0 ? (double)d - bad : (bad_norm ? (double)d : 0.0);
+    diag[4] = 1.2e-16 * ksum;         // error estimate: u sum_i kappa_F(R_i)
+    diag[5] = 1.2e-16 * kmax * kmax;  // orthogonality estimate: u max_i kappa_F(R_i)^2
+  }
+  __syncthreads();
+  for (int i = 1 + threadIdx.x; i < d; i += blockDim.x) alpha[i] = sqrt(s[i]) * sh_nrm0;
+}
+
+// dst[t] = (*alpha) src[t]
+__global__ void k_scale_copy(double *__restrict__ dst, const double *__restrict__ src, long count,
+                             const double *__restrict__ alpha) {
+  const double a = alpha[0];
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x)
+    dst[t] = a * src[t];
+}
+
+}  // namespace
+
+// Result of the Gram-chain form: ranks equal to the capped input ranks
+// (no bond truncates).  Returns false when the form does not apply or a
+// bond is not proven; the output buffer may then hold partial results.
+bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore, double eps,
+                const int64_t *rmax, double orth_tol, int64_t *r_out, double *dcore_out, int64_t cap,
+                int64_t *len_out, ttg_info *info) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (d < 2 || eps < 0.0) return false;
+  {
+    const char *env = std::getenv("TTG_GRAM");
+    if (env && env[0] == '0') return false;
+  }
+  // ranks after the caps: left (round.m:48), then right (size of the SVD)
+  std::vector<int64_t> rr(r, r + d + 1);
+  std::vector<int> left_cap(d, 0), right_cap(d, 0);
+  for (int64_t i = 0; i + 1 < d; ++i)
+    if (rr[i + 1] > rr[i] * n[i]) {
+      left_cap[i] = 1;
+      rr[i + 1] = rr[i] * n[i];
+    }
+  for (int64_t i = d - 1; i >= 1; --i)
+    if (rr[i] > n[i] * rr[i + 1]) {
+      right_cap[i] = 1;
+      rr[i] = n[i] * rr[i + 1];
+    }
+  int64_t kmax = 1;
+  for (int64_t i = 1; i < d; ++i) kmax = std::max(kmax, rr[i]);
+  if (kmax > 256) return false;  // batched factorisation kernels: k <= 256
+  if (rmax)
+    for (int64_t i = 1; i < d; ++i)
+      if (rmax[i] < rr[i]) return false;  // a binding rmax truncates
+  // capacity: the QR-capped size (what the sweeps need) bounds ours
+  std::vector<int64_t> rq(r, r + d + 1);
+  for (int64_t i = 0; i + 1 < d; ++i) rq[i + 1] = std::min(rq[i] * n[i], rq[i + 1]);
+  int64_t need = 0, out_len = 0, in_len = 0;
+  for (int64_t i = 0; i < d; ++i) {
+    need += rq[i] * n[i] * rq[i + 1];
+    out_len += rr[i] * n[i] * rr[i + 1];
+    in_len += r[i] * n[i] * r[i + 1];
+  }
+  if (cap < need) return false;
+  (void)in_len;
+
+  hipStream_t st = ctx->stream;
+  const int cus = ctx->num_cus;
+  if (!ctx->side) {
+    TTG_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    TTG_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    TTG_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+  }
+  hipStream_t sd = ctx->side;
+
+  // ---- workspace ------------------------------------------------------------
+  // cores rewritten by the caps get a slot of their final size each
+  std::vector<int64_t> in_ofs(d + 1, 0), cur_r(r, r + d + 1);
+  for (int64_t i = 0; i < d; ++i) in_ofs[i + 1] = in_ofs[i] + r[i] * n[i] * r[i + 1];
+  std::vector<int> modified(d, 0);
+  for (int64_t i = 0; i < d; ++i)
+    if (left_cap[i] || right_cap[i] || (i > 0 && left_cap[i - 1]) || (i + 1 < d && right_cap[i + 1])) modified[i] = 1;
+  // size of a modified core while the caps run: its largest intermediate
+  std::vector<int64_t> slot(d, 0);
+  {
+    std::vector<int64_t> r1(r, r + d + 1);
+    for (int64_t i = 0; i < d; ++i) slot[i] = r1[i] * n[i] * r1[i + 1];
+    for (int64_t i = 0; i + 1 < d; ++i)
+      if (left_cap[i]) {
+        r1[i + 1] = r1[i] * n[i];
+        slot[i] = std::max(slot[i], r1[i] * n[i] * r1[i + 1]);
+        slot[i + 1] = std::max(slot[i + 1], r1[i + 1] * n[i + 1] * r1[i + 2]);
+      }
+    for (int64_t i = d - 1; i >= 1; --i)
+      if (right_cap[i]) {
+        r1[i] = n[i] * r1[i + 1];
+        slot[i] = std::max(slot[i], r1[i] * n[i] * r1[i + 1]);
+        slot[i - 1] = std::max(slot[i - 1], r1[i - 1] * n[i - 1] * r1[i]);
+      }
+  }
+  int64_t max_core = 1, capq = 1;
+  for (int64_t i = 0; i < d; ++i) max_core = std::max(max_core, std::max(slot[i], rr[i] * n[i] * rr[i + 1]));
+  for (int64_t i = 0; i < d; ++i)
+    if (modified[i]) capq = std::max(capq, slot[i]);
+  std::vector<int64_t> sq_ofs(d + 1, 0), out_ofs(d + 1, 0);
+  for (int64_t i = 1; i <= d; ++i) sq_ofs[i] = sq_ofs[i - 1] + rr[i - 1] * rr[i - 1];
+  for (int64_t i = 0; i < d; ++i) out_ofs[i + 1] = out_ofs[i] + rr[i] * n[i] * rr[i + 1];
+  const int64_t sqtot = sq_ofs[d] + 1;
+  const int64_t wjob = (kmax / 32 + 1) * 33 * 32;
+  const int nb = (int)(d - 1);
+  const int chunks = 32;
+  const int64_t gws = std::max<int64_t>(32 * kmax * kmax, 1 << 16);
+  auto layout = [&](auto &a) {
+    for (int64_t i = 0; i < d; ++i) a.template take<double>(modified[i] ? slot[i] : 1);  // cap slots
+    a.template take<double>(capq);            // copy of a neighbour core rewritten in place
+    for (int q = 0; q < 6; ++q) a.template take<double>(sqtot);  // H, G, RR, RL, RinvR, RinvL
+    a.template take<double>(sqtot);           // Z
+    a.template take<double>(2 * (size_t)nb * wjob);  // block inverses
+    a.template take<double>(max_core);        // right chain temporary
+    a.template take<double>(max_core);        // left chain temporary
+    a.template take<double>(gws);             // split-K partials, right chain
+    a.template take<double>(gws);             // split-K partials, left chain
+    a.template take<double>(out_len);         // A_i = C_i R_R,i+1^T
+    a.template take<double>((size_t)d * chunks);  // norm partials
+    a.template take<double>(d + 8);           // s
+    a.template take<double>(d + 8);           // alpha
+    a.template take<double>(4 * (size_t)nb * kBondParts + 8);  // per-bond statistics
+    a.template take<double>(8 * (size_t)nb + 8);  // Cholesky statistics
+    a.template take<double>(16);              // verdict diagnostics
+    a.template take<int>(2 * (size_t)nb + 8);     // Cholesky flags
+    a.template take<int>(16);                 // ok
+    a.template take<char>(4096 + (size_t)d * 64 + (size_t)nb * 512);  // tables
+  };
+  ArenaSizer z;
+  layout(z);
+  // the form keeps its own workspace: the job tables and the captured launch
+  // sequence stay valid while the shapes and pointers of the calls repeat
+  ctx->garena.reserve(z.used + 65536);
+  ctx->garena.reset();
+  Arena &A = ctx->garena;
+  std::vector<double *> cslot(d);
+  for (int64_t i = 0; i < d; ++i) cslot[i] = A.take<double>(modified[i] ? slot[i] : 1);
+  double *tscr = A.take<double>(capq);
+  double *H = A.take<double>(sqtot), *Gm = A.take<double>(sqtot), *RR = A.take<double>(sqtot);
+  double *RL = A.take<double>(sqtot), *RiR = A.take<double>(sqtot), *RiL = A.take<double>(sqtot);
+  double *Zm = A.take<double>(sqtot);
+  double *Wv = A.take<double>(2 * (size_t)nb * wjob);
+  double *Tr = A.take<double>(max_core), *Tl = A.take<double>(max_core);
+  double *wsr = A.take<double>(gws), *wsl = A.take<double>(gws);
+  double *Ab = A.take<double>(out_len);
+  double *part = A.take<double>((size_t)d * chunks);
+  double *sv = A.take<double>(d + 8), *alpha = A.take<double>(d + 8);
+  double *bst = A.take<double>(4 * (size_t)nb * kBondParts + 8), *cst = A.take<double>(8 * (size_t)nb + 8);
+  double *diag = A.take<double>(16);
+  int *cfl = A.take<int>(2 * (size_t)nb + 8), *okd = A.take<int>(16);
+
+  // ---- caps: host plan (launch = false) and device work (launch = true) --------
+  std::vector<const double *> cp(d);
+  auto caps = [&](bool launch) {
+    for (int64_t i = 0; i < d; ++i) cp[i] = dcore + in_ofs[i];
+    for (int64_t i = 0; i <= d; ++i) cur_r[i] = r[i];
+    // a neighbour's product reads its current core and writes its slot; a
+    // core already in its slot is read from the scratch copy instead
+    auto src_of = [&](int64_t i, int64_t count) -> const double * {
+      if (cp[i] != cslot[i]) return cp[i];
+      if (launch) TTG_HIP(hipMemcpyAsync(tscr, cp[i], count * sizeof(double), hipMemcpyDeviceToDevice, st));
+      return tscr;
+    };
+    // A capped bond needs no factorisation: the result of round does not depend
+    // on the gauge of a bond, so the capped core becomes the identity (an
+    // orthogonal square factor) and the whole core moves into its neighbour.
+    for (int64_t i = 0; i + 1 < d; ++i) {
+      if (!left_cap[i]) continue;
+      const int64_t m = cur_r[i] * n[i], k = cur_r[i + 1];  // core_i: m x k, m < k
+      const int64_t N1 = n[i + 1] * cur_r[i + 2];
+      // core_{i+1} <- core_i core_{i+1}  (m x N1)
+      const double *a = cp[i];  // read before the identity is written (stream order)
+      const double *src = src_of(i + 1, k * N1);
+      if (launch) gemm<double>(st, false, false, m, N1, k, 1.0, a, m, src, k, 0.0, cslot[i + 1], m, wsr, gws, cus);
+      cp[i + 1] = cslot[i + 1];
+      if (launch) eye_plus(st, m, cslot[i], nullptr, 0.0);  // core_i <- I (r_i x n_i x m)
+      cp[i] = cslot[i];
+      cur_r[i + 1] = m;
+    }
+    for (int64_t i = d - 1; i >= 1; --i) {
+      if (!right_cap[i]) continue;
+      const int64_t rk = cur_r[i], N = n[i] * cur_r[i + 1];  // core_i: rk x N, rk > N
+      const int64_t mp = cur_r[i - 1] * n[i - 1];
+      // core_{i-1} <- core_{i-1} core_i  (mp x N)
+      const double *w = cp[i];  // read before the identity is written (stream order)
+      const double *src = src_of(i - 1, mp * rk);
+      if (launch) gemm<double>(st, false, false, mp, N, rk, 1.0, src, mp, w, rk, 0.0, cslot[i - 1], mp, wsr, gws, cus);
+      cp[i - 1] = cslot[i - 1];
+      if (launch) eye_plus(st, N, cslot[i], nullptr, 0.0);  // core_i <- I (N x n_i x r_{i+1})
+      cp[i] = cslot[i];
+      cur_r[i] = N;
+    }
+  };
+  caps(false);
+  for (int64_t i = 0; i <= d; ++i)
+    if (cur_r[i] != rr[i]) throw Error(TTG_HIP_ERROR, "internal: rank caps disagree");
+
+  // ---- tables (one upload through pinned memory) ------------------------------
+  const int njc = 2 * nb;
+  std::vector<CholJob> cj(njc);
+  std::vector<TrsmJob> tj(njc);
+  std::vector<GramBond> gb(nb);
+  std::vector<BGemmJob> ja(nb), jz(nb), jb(nb);
+  for (int64_t i = 1; i < d; ++i) {
+    const int b = (int)(i - 1);
+    const int k = (int)rr[i];
+    const int64_t o = sq_ofs[i];
+    cj[b] = CholJob{H + o, RR + o, cfl + b, cst + 4 * b, Wv + (size_t)b * wjob, nullptr, k, k};
+    cj[nb + b] = CholJob{Gm + o, RL + o, cfl + nb + b, cst + 4 * (nb + b), Wv + (size_t)(nb + b) * wjob, nullptr, k, k};
+    tj[b] = TrsmJob{nullptr, RR + o, Wv + (size_t)b * wjob, RiR + o, k, k, 0, k};
+    tj[nb + b] = TrsmJob{nullptr, RL + o, Wv + (size_t)(nb + b) * wjob, RiL + o, k, k, 0, k};
+    gb[b] = GramBond{Gm + o, H + o, RiR + o, Zm + o, k, 0};
+    jz[b] = BGemmJob{RiR + o, RiL + o, Zm + o, nullptr, 1.0, k, k, k, k, k, k, 0, 0};
+    const int64_t N = n[i] * rr[i + 1];
+    const double *src = (i + 1 < d) ? Ab + out_ofs[i] : cp[i];
+    jb[b] = BGemmJob{RiR + o, src, dcore_out + out_ofs[i], alpha + i, 1.0, k, (int)N, k, k, k, k, 0, 0};
+  }
+  for (int64_t i = 0; i + 1 < d; ++i) {
+    const int m = (int)(rr[i] * n[i]), k = (int)rr[i + 1];
+    ja[i] = BGemmJob{cp[i], RR + sq_ofs[i + 1], Ab + out_ofs[i], nullptr, 1.0, m, k, k, m, k, m, 0, 0};
+  }
+  bool qa, qz, qb;
+  double fa, fz, fb;
+  const int ta_ = bgemm_plan(true, ja.data(), nb, &qa, &fa);
+  const int tz_ = bgemm_plan(false, jz.data(), nb, &qz, &fz);
+  const int tb_ = bgemm_plan(false, jb.data(), nb, &qb, &fb);
+  std::vector<int64_t> sizes(d);
+  std::vector<const double *> cptr(d);
+  for (int64_t i = 0; i < d; ++i) {
+    sizes[i] = rr[i] * n[i] * rr[i + 1];
+    cptr[i] = cp[i];
+  }
+  // pack: [cj | tj | gb | ja | jz | jb | cptr | sizes | one]
+  std::vector<size_t> off;
+  size_t tb = 0;
+  auto add = [&](size_t bytes) {
+    off.push_back(tb);
+    tb += (bytes + 63) & ~(size_t)63;
+  };
+  add(njc * sizeof(CholJob));
+  add(njc * sizeof(TrsmJob));
+  add(nb * sizeof(GramBond));
+  add(nb * sizeof(BGemmJob));
+  add(nb * sizeof(BGemmJob));
+  add(nb * sizeof(BGemmJob));
+  add(d * sizeof(double *));
+  add(d * sizeof(int64_t));
+  add(sizeof(double));
+  char *tab = A.take<char>(tb);
+  const size_t readback = 256;
+  if (!ctx->host_pinned || ctx->host_pinned_cap < tb + readback) {
+    TTG_HIP(hipStreamSynchronize(st));
+    if (ctx->host_pinned) TTG_HIP(hipHostFree(ctx->host_pinned));
+    ctx->host_pinned = nullptr;
+    const size_t c2 = std::max<size_t>(tb + readback, 4096);
+    TTG_HIP(hipHostMalloc(&ctx->host_pinned, c2, hipHostMallocDefault));
+    ctx->host_pinned_cap = c2;
+  }
+  char *hp = static_cast<char *>(ctx->host_pinned);
+  std::memcpy(hp + off[0], cj.data(), njc * sizeof(CholJob));
+  std::memcpy(hp + off[1], tj.data(), njc * sizeof(TrsmJob));
+  std::memcpy(hp + off[2], gb.data(), nb * sizeof(GramBond));
+  std::memcpy(hp + off[3], ja.data(), nb * sizeof(BGemmJob));
+  std::memcpy(hp + off[4], jz.data(), nb * sizeof(BGemmJob));
+  std::memcpy(hp + off[5], jb.data(), nb * sizeof(BGemmJob));
+  std::memcpy(hp + off[6], cptr.data(), d * sizeof(double *));
+  std::memcpy(hp + off[7], sizes.data(), d * sizeof(int64_t));
+  const double one = 1.0;
+  std::memcpy(hp + off[8], &one, sizeof(double));
+  // the launch sequence is captured once per shape/pointer set and replayed
+  // (its device tables live in the form's own workspace)
+  const double ep = eps / std::sqrt((double)(d - 1));
+  std::vector<uint64_t> key = {(uint64_t)d, (uint64_t)(uintptr_t)dcore, (uint64_t)(uintptr_t)dcore_out,
+                               (uint64_t)(uintptr_t)ctx->garena.base, (uint64_t)ctx->garena.cap, (uint64_t)cus};
+  {
+    uint64_t b;
+    std::memcpy(&b, &ep, sizeof(b));
+    key.push_back(b);
+    std::memcpy(&b, &orth_tol, sizeof(b));
+    key.push_back(b);
+  }
+  for (int64_t i = 0; i < d; ++i) key.push_back((uint64_t)n[i]);
+  for (int64_t i = 0; i <= d; ++i) key.push_back((uint64_t)r[i]);
+  const bool profiling = g_prof && g_prof->kind != KC_NONE;
+  bool use_graph = !profiling;
+  {
+    const char *env = std::getenv("TTG_GRAPH");
+    if (env && env[0] == '0') use_graph = false;
+  }
+  const bool replay = use_graph && ctx->ggraph && ctx->gkey == key;
+  const CholJob *dcj = reinterpret_cast<const CholJob *>(tab + off[0]);
+  const TrsmJob *dtj = reinterpret_cast<const TrsmJob *>(tab + off[1]);
+  const GramBond *dgb = reinterpret_cast<const GramBond *>(tab + off[2]);
+  const BGemmJob *dja = reinterpret_cast<const BGemmJob *>(tab + off[3]);
+  const BGemmJob *djz = reinterpret_cast<const BGemmJob *>(tab + off[4]);
+  const BGemmJob *djb = reinterpret_cast<const BGemmJob *>(tab + off[5]);
+  const double *const *dcp = reinterpret_cast<const double *const *>(tab + off[6]);
+  const int64_t *dsz = reinterpret_cast<const int64_t *>(tab + off[7]);
+  const double *d_one = reinterpret_cast<const double *>(tab + off[8]);
+
+  auto enqueue = [&]() {
+    caps(true);
+    // ---- scales s_i = 1/||C_i||^2 -----------------------------------------------
+    hipLaunchKernelGGL(k_core_sumsq, dim3(chunks, (unsigned)d), dim3(256), 0, st, dcp, dsz, chunks, part);
+    hipLaunchKernelGGL(k_core_scale, dim3((unsigned)((d + 255) / 256)), dim3(256), 0, st, (int)d, chunks, part, sv);
+    TTG_HIP(hipGetLastError());
+
+    // ---- the two Gram chains, concurrently -------------------------------------
+    TTG_HIP(hipEventRecord(ctx->ev_fork, st));
+    TTG_HIP(hipStreamWaitEvent(sd, ctx->ev_fork, 0));
+    for (int64_t i = d - 1; i >= 1; --i) {  // right: H_i from H_{i+1}
+      const int64_t m = rr[i] * n[i], k = rr[i + 1];
+      const double *Hn = (i + 1 < d) ? H + sq_ofs[i + 1] : d_one;
+      gemm_scaled(st, false, false, m, k, k, sv + i, cp[i], m, Hn, k, Tr, m, wsr, gws, cus);
+      gemm<double>(st, false, true, rr[i], rr[i], n[i] * k, 1.0, Tr, rr[i], cp[i], rr[i], 0.0, H + sq_ofs[i], rr[i],
+                   wsr, gws, cus);
+    }
+    for (int64_t i = 0; i + 1 < d; ++i) {  // left: G_{i+1} from G_i
+      const int64_t k0 = rr[i], N = n[i] * rr[i + 1];
+      const double *Gp = i > 0 ? Gm + sq_ofs[i] : d_one;
+      gemm_scaled(sd, false, false, k0, N, k0, sv + i, Gp, k0, cp[i], k0, Tl, k0, wsl, gws, cus);
+      gemm<double>(sd, true, false, rr[i + 1], rr[i + 1], k0 * n[i], 1.0, cp[i], k0 * n[i], Tl, k0 * n[i], 0.0,
+                   Gm + sq_ofs[i + 1], rr[i + 1], wsl, gws, cus);
+    }
+    TTG_HIP(hipEventRecord(ctx->ev_join, sd));
+    TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+
+    // ---- batched factorisations, inverses, products, verdict ----------------------
+    chol_batched(st, dcj, njc, kmax);
+    trsm3_batched(st, dtj, njc, kmax, kmax);
+    bgemm_run(st, false, true, dja, nb, ta_, qa, fa);   // A_i = C_i R_R,i+1^T
+    bgemm_run(st, true, false, djz, nb, tz_, qz, fz);   // Z_i = R_R,i^{-T} R_L,i^{-1}
+    hipLaunchKernelGGL(k_gram_bond, dim3(kBondParts, (unsigned)nb), dim3(256), 0, st, dgb, bst);
+    const double ep = eps / std::sqrt((double)(d - 1));
+    hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, st, (int)d, bst, cfl, Ab, (long)(rr[0] * n[0] * rr[1]), sv,
+                       ep, orth_tol, alpha, okd, diag);
+    TTG_HIP(hipGetLastError());
+    // the output: core_0 = alpha_0 A_0, core_i = alpha_i R_R,i^{-T} A_i
+    hipLaunchKernelGGL(k_scale_copy, dim3(64), dim3(256), 0, st, dcore_out, Ab, (long)(rr[0] * n[0] * rr[1]), alpha);
+    bgemm_run(st, true, false, djb, nb, tb_, qb, fb);
+    TTG_HIP(hipGetLastError());
+  };
+  if (replay) {
+    TTG_HIP(hipGraphLaunch(ctx->ggraph, st));
+  } else {
+    TTG_HIP(hipMemcpyAsync(tab, hp, tb, hipMemcpyHostToDevice, st));
+    if (ctx->ggraph) {
+      TTG_HIP(hipGraphExecDestroy(ctx->ggraph));
+      ctx->ggraph = nullptr;
+      ctx->gkey.clear();
+    }
+    if (use_graph) {
+      hipGraph_t g = nullptr;
+      TTG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue();
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &g);  // leave the stream usable
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      TTG_HIP(hipStreamEndCapture(st, &g));
+      hipGraphExec_t ge = nullptr;
+      const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      TTG_HIP(ie);
+      ctx->ggraph = ge;
+      ctx->gkey = key;
+      TTG_HIP(hipGraphLaunch(ctx->ggraph, st));
+    } else {
+      enqueue();
+    }
+  }
+  char *hr = hp + tb;
+  TTG_HIP(hipMemcpyAsync(hr, diag, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  TTG_HIP(hipMemcpyAsync(hr + 64, okd, sizeof(int), hipMemcpyDeviceToHost, st));
+  TTG_HIP(hipStreamSynchronize(st));
+  int ok = 0;
+  double dg[6];
+  std::memcpy(dg, hr, sizeof(dg));
+  std::memcpy(&ok, hr + 64, sizeof(int));
+  if (!ok) return false;
+  for (int64_t i = 0; i <= d; ++i) r_out[i] = rr[i];
+  if (len_out) *len_out = out_len;
+  if (info) {
+    info->qr_fallbacks = 0;
+    info->jacobi_sweeps = 0;
+    info->jacobi_maxsweeps = 0;
+    info->qr_retries = 0;
+    info->svd_skipped = d - 1;
+    info->path = 1;
+    info->err_est = dg[4];
+    info->orth_est = dg[5];
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return true;
+}
+
+}  // namespace ttg

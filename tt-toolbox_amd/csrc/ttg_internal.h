@@ -119,6 +119,11 @@ struct ttg_ctx {
   hipGraphExec_t pgraph = nullptr;
   std::vector<uint64_t> pgraph_key;
   int batch_mode = 0;
+  // Gram-chain round: its workspace (job tables included) and its captured
+  // launch sequence with the shapes and pointers it was captured for
+  ttg::Arena garena;
+  hipGraphExec_t ggraph = nullptr;
+  std::vector<uint64_t> gkey;
 };
 
 namespace ttg {
@@ -141,6 +146,11 @@ struct PTensorDesc {
 bool round_small_persist(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
                          double eps, const int64_t *rmax, int64_t *r_out, double *dcore_out, int64_t cap,
                          int64_t *len_out, ttg_info *info, ttg_status *status);
+// Gram-chain round for tensors whose bonds provably keep every singular value
+// (false: not applicable or not proven; the caller runs the sweeps)
+bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore, double eps,
+                const int64_t *rmax, double orth_tol, int64_t *r_out, double *dcore_out, int64_t cap,
+                int64_t *len_out, ttg_info *info);
 // true if one workgroup can round the tensor (d <= 64, ranks <= 512)
 bool persist_supported(int64_t d, const int64_t *n, const int64_t *r);
 // doubles of per-slot workspace for the largest core / bond rank of a batch
@@ -164,6 +174,26 @@ void gemm(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, T al
           T *ws, int64_t ws_elems, int num_cus);
 // bytes of split-K workspace gemm() may use for the given shape
 int64_t gemm_ws_elems(int64_t M, int64_t N, int64_t K, int num_cus);
+// FP64 C = (*alpha_dev) op(A) op(B) (alpha_dev: device scalar)
+void gemm_scaled(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, const double *alpha_dev,
+                 const double *A, int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, double *ws,
+                 int64_t wse, int cus);
+// One problem of a batched FP64 GEMM C = alpha (*alpha_dev) op(A) op(B) (alpha_dev may be null).
+struct BGemmJob {
+  const void *A, *B;
+  void *C;
+  const double *alpha_dev;
+  double alpha;
+  int M, N, K, lda, ldb, ldc;
+  int tile0;  // first grid block of the job (set by bgemm_f64)
+  int pad;
+};
+// All jobs in one launch (same op(A), op(B)).  bgemm_plan fills the jobs'
+// first blocks on the host and returns the grid size; the table is then
+// copied to device memory (jobs_dev) by the caller and bgemm_run launches it.
+int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, bool *quarter, double *flops);
+void bgemm_run(hipStream_t st, bool ta, bool tb, const BGemmJob *jobs_dev, int njobs, int total, bool quarter,
+               double flops);
 
 // y[i] = (T)x[i]
 template <class TO, class TI> void convert(hipStream_t s, TO *y, const TI *x, int64_t count);
@@ -192,6 +222,29 @@ void householder_qr(hipStream_t s, int64_t m, int64_t n, double *A, double *Q, d
 bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double shift_rel,
                  double *R, int *flag, double *stats, unsigned long long *dbg = nullptr,
                  double *Wout = nullptr, double *fro = nullptr);
+// Batched one-workgroup Cholesky factorisations (k <= 256, no shift): job b
+// factorises G (ld ldg) into R (ld k), flags, statistics, the inverses of
+// R's diagonal blocks (Wout, for trsm3_batched) and ||R||_F (fro, optional).
+struct CholJob {
+  const double *G;
+  double *R;
+  int *flag;
+  double *stats;
+  double *Wout;
+  double *fro;
+  int k, ldg;
+};
+void chol_batched(hipStream_t s, const CholJob *jobs_dev, int njobs, int64_t kmax);
+// Batched X = A R^{-1} (A == nullptr: A = I) with the block inverses a
+// batched Cholesky left (k <= 256 for every job).
+struct TrsmJob {
+  const double *A;
+  const double *R;
+  const double *Winv;
+  double *X;
+  int m, k, lda, ldx;
+};
+void trsm3_batched(hipStream_t s, const TrsmJob *jobs_dev, int njobs, int64_t mmax, int64_t kmax);
 // X (m x k) = A R^{-1}, R upper triangular (FP64), blocked substitution.
 // Gram of A (m x k, FP32 MFMA when g32) fused with the near-identity second
 // CholeskyQR pass: X = I - Phi(E), R2 = I + Phi(E), *maxabs = max|E| (G is

@@ -82,11 +82,14 @@ constexpr int GBM = 64, GBN = 64, GBK = 64;
 // TI is the storage type of A and B, T the MFMA type: TI = double, T = float
 // rounds the operands to FP32 while staging them (FP32 Gram products of FP64
 // data).
-template <class TI, class T, class TO, bool TA, bool TB, int BM = GBM, int BN = GBN>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T alpha,
-                                              const TI *__restrict__ A, long lda,
-                                              const TI *__restrict__ B, long ldb, T beta,
-                                              TO *__restrict__ C, long ldc, T *__restrict__ P) {
+// One BM x BN block of C over K range [kbeg, kend) with its top-left corner at
+// (m0, n0).  P != nullptr: the raw partial product goes to P (ld M) instead of
+// C (split-K).  Shared by k_gemm (grid tiles) and k_bgemm (batched jobs).
+template <class TI, class T, class TO, bool TA, bool TB, int BM, int BN>
+__device__ __forceinline__ void gemm_block(int M, int N, int kbeg, int kend, int m0, int n0, T alpha,
+                                           const TI *__restrict__ A, long lda, const TI *__restrict__ B,
+                                           long ldb, T beta, TO *__restrict__ C, long ldc,
+                                           T *__restrict__ P) {
   static_assert(!TB || BN == GBN, "op(B) = B^T stages 64 columns per K tile");
   __shared__ T As[GBK][BM + 4];
   constexpr int MI = BM / 32;            // 16-row MFMA tiles per wave
@@ -97,9 +100,6 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
   typedef typename MF<T>::V V;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-  const int kbeg = blockIdx.z * kchunk;
-  const int kend = min(K, kbeg + kchunk);
   V acc[MI][NJ];
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -166,7 +166,6 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
         for (int j = 0; j < NJ; ++j) acc[i][j] = MF<T>::op(b[j], a[i], acc[i][j]);
     }
   }
-  const bool split = gridDim.z > 1;
   // acc[i][j][v] of lane l: C(m0 + wm*BM/2 + i*16 + l%16, n0 + wn*BN/2 + j*16 + row(l, v))
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -177,8 +176,8 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
         const int gm = m0 + wm * (BM / 2) + i * 16 + (lane & 15);
         const int gn = n0 + wn * (BN / 2) + j * 16 + MF<T>::row(lane, v);
         if (gm < M && gn < N) {
-          if (split) {
-            P[(long)blockIdx.z * M * N + gm + (long)gn * M] = acc[i][j][v];
+          if (P) {
+            P[gm + (long)gn * M] = acc[i][j][v];
           } else {
             T val = alpha * acc[i][j][v];
             if (beta != T(0)) val += beta * T(C[gm + (long)gn * ldc]);
@@ -188,10 +187,79 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
       }
 }
 
+// alpha_dev (optional device scalar) multiplies alpha.
+template <class TI, class T, class TO, bool TA, bool TB, int BM = GBM, int BN = GBN>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T alpha,
+                                              const TI *__restrict__ A, long lda,
+                                              const TI *__restrict__ B, long ldb, T beta,
+                                              TO *__restrict__ C, long ldc, T *__restrict__ P,
+                                              const T *__restrict__ alpha_dev) {
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  const T a = alpha_dev ? alpha * alpha_dev[0] : alpha;
+  T *part = gridDim.z > 1 ? P + (long)blockIdx.z * M * N : nullptr;
+  gemm_block<TI, T, TO, TA, TB, BM, BN>(M, N, kbeg, kend, blockIdx.x * BM, blockIdx.y * BN, a, A, lda, B, ldb,
+                                        beta, C, ldc, part);
+}
+
+// Batched GEMM: job j owns grid blocks [tile0_j, tile0_{j+1}); every job has
+// its own shape, operands and (optional) device scale; full K per block.
+template <class T, bool TA, bool TB, int BM, int BN>
+__global__ __launch_bounds__(256) void k_bgemm(const BGemmJob *__restrict__ jobs, int njobs) {
+  int j = 0;
+  while (j + 1 < njobs && jobs[j + 1].tile0 <= (int)blockIdx.x) ++j;
+  const BGemmJob jb = jobs[j];
+  const int t = (int)blockIdx.x - jb.tile0;
+  const int tm = (jb.M + BM - 1) / BM;
+  const T a = jb.alpha_dev ? T(jb.alpha * jb.alpha_dev[0]) : T(jb.alpha);
+  gemm_block<T, T, T, TA, TB, BM, BN>(jb.M, jb.N, 0, jb.K, (t % tm) * BM, (t / tm) * BN, a,
+                                      static_cast<const T *>(jb.A), jb.lda, static_cast<const T *>(jb.B), jb.ldb,
+                                      T(0), static_cast<T *>(jb.C), jb.ldc, nullptr);
+}
+
+int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, bool *quarter, double *flops) {
+  // 32 x 64 blocks; 32 x 32 when op(B) = B and the batch has few blocks
+  int tiles_all = 0;
+  for (int j = 0; j < njobs; ++j) tiles_all += ((jobs[j].M + 31) / 32) * ((jobs[j].N + 63) / 64);
+  const bool q = !tb && tiles_all < 1024;
+  const int bm = 32, bn = q ? 32 : 64;
+  int total = 0;
+  double fl = 0.0;
+  for (int j = 0; j < njobs; ++j) {
+    jobs[j].tile0 = total;
+    total += ((jobs[j].M + bm - 1) / bm) * ((jobs[j].N + bn - 1) / bn);
+    fl += 2.0 * jobs[j].M * (double)jobs[j].N * jobs[j].K;
+  }
+  if (quarter) *quarter = q;
+  if (flops) *flops = fl;
+  return total;
+}
+
+void bgemm_run(hipStream_t st, bool ta, bool tb, const BGemmJob *jobs_dev, int njobs, int total, bool quarter,
+               double flops) {
+  if (njobs <= 0 || total <= 0) return;
+  ProfScope prof(KC_GEMM, st, flops, 0.0);
+  const dim3 grid((unsigned)total);
+#define TTG_BGEMM(TA_, TB_)                                                                            \
+  do {                                                                                                 \
+    if (quarter)                                                                                       \
+      hipLaunchKernelGGL((k_bgemm<double, TA_, false, 32, 32>), grid, dim3(256), 0, st, jobs_dev, njobs); \
+    else                                                                                               \
+      hipLaunchKernelGGL((k_bgemm<double, TA_, TB_, 32, 64>), grid, dim3(256), 0, st, jobs_dev, njobs);   \
+  } while (0)
+  if (!ta && !tb) TTG_BGEMM(false, false);
+  else if (ta && !tb) TTG_BGEMM(true, false);
+  else if (!ta && tb) TTG_BGEMM(false, true);
+  else TTG_BGEMM(true, true);
+#undef TTG_BGEMM
+  TTG_HIP(hipGetLastError());
+}
+
 // Fixed-order sum of the split-K partials (accumulated in double).
 template <class T, class TO>
 __global__ void k_splitk_reduce(int M, int N, int S, const T *__restrict__ P, double alpha,
-                                double beta, TO *__restrict__ C, long ldc) {
+                                double beta, TO *__restrict__ C, long ldc, const T *__restrict__ alpha_dev) {
+  if (alpha_dev) alpha *= (double)alpha_dev[0];
   const long total = (long)M * N;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
        idx += (long)gridDim.x * blockDim.x) {
@@ -234,7 +302,7 @@ int64_t gemm_ws_elems(int64_t M, int64_t N, int64_t K, int num_cus) {
 template <class TI, class T, class TO>
 static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, int64_t K, T alpha,
                       const TI *A, int64_t lda, const TI *B, int64_t ldb, T beta, TO *C, int64_t ldc,
-                      T *ws, int64_t ws_elems, int num_cus) {
+                      T *ws, int64_t ws_elems, int num_cus, const T *alpha_dev = nullptr) {
   if (M <= 0 || N <= 0) return;
   int S = splitk_count(M, N, K, num_cus);
   if (S > 1 && (ws == nullptr || (int64_t)S * M * N > ws_elems)) S = 1;
@@ -255,13 +323,13 @@ static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, in
   do {                                                                                                        \
     if (quarter)                                                                                              \
       hipLaunchKernelGGL((k_gemm<TI, T, TO, TA_, false, 32, 32>), grid, dim3(256), 0, st, (int)M, (int)N,     \
-                         (int)K, kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws);          \
+                         (int)K, kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev); \
     else if (half)                                                                                            \
       hipLaunchKernelGGL((k_gemm<TI, T, TO, TA_, TB_, 32>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K,   \
-                         kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws);                  \
+                         kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);       \
     else                                                                                                      \
       hipLaunchKernelGGL((k_gemm<TI, T, TO, TA_, TB_, GBM>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K,  \
-                         kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws);                  \
+                         kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);       \
   } while (0)
   if (!ta && !tb) TTG_GEMM_LAUNCH(false, false);
   else if (ta && !tb) TTG_GEMM_LAUNCH(true, false);
@@ -273,7 +341,7 @@ static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, in
     const long total = (long)M * N;
     const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
     hipLaunchKernelGGL((k_splitk_reduce<T, TO>), dim3(blocks), dim3(256), 0, st, (int)M, (int)N,
-                       S, (const T *)ws, (double)alpha, (double)beta, C, (long)ldc);
+                       S, (const T *)ws, (double)alpha, (double)beta, C, (long)ldc, alpha_dev);
     TTG_HIP(hipGetLastError());
   }
 }
@@ -289,6 +357,11 @@ void gemm<float>(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t 
                  const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
                  int64_t ldc, float *ws, int64_t wse, int cus) {
   gemm_impl<float, float, float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ws, wse, cus);
+}
+void gemm_scaled(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, const double *alpha_dev,
+                 const double *A, int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, double *ws,
+                 int64_t wse, int cus) {
+  gemm_impl<double, double, double>(s, ta, tb, M, N, K, 1.0, A, lda, B, ldb, 0.0, C, ldc, ws, wse, cus, alpha_dev);
 }
 // FP32 operands, FP64 result (Gram matrices of the mixed form).
 void gemm_f32_f64out(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K,
@@ -356,13 +429,13 @@ static void gram_nearid_impl(hipStream_t st, int64_t m, int64_t k, const double 
       ProfScope prof(KC_GEMM, st, 2.0 * (double)k * k * m, 8.0 * (2.0 * m * k + k * k));
       if (quarter)
         hipLaunchKernelGGL((k_gemm<double, T, double, true, false, 32, 32>), grid, dim3(256), 0, st, (int)k, (int)k,
-                           (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws);
+                           (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, nullptr);
       else if (half)
         hipLaunchKernelGGL((k_gemm<double, T, double, true, false, 32>), grid, dim3(256), 0, st, (int)k, (int)k, (int)m,
-                           kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws);
+                           kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, nullptr);
       else
         hipLaunchKernelGGL((k_gemm<double, T, double, true, false, GBM>), grid, dim3(256), 0, st, (int)k, (int)k,
-                           (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws);
+                           (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, nullptr);
       TTG_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL((k_gram_nearid<T>), dim3(blocks), dim3(256), 0, st, (int)k, S, (const T *)ws, X, R2, maxabs);

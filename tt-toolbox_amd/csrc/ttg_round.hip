@@ -651,6 +651,11 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
     info->jacobi_maxsweeps = R.sweeps_max;
     info->qr_retries = R.retries;
     info->svd_skipped = R.svd_skipped;
+    info->path = 0;
+    // backward-stable sweeps: O(d u); the mixed form's FP32-Gram CholeskyQR2
+    // bonds are accepted with Q orthogonal to 1e-10
+    info->err_est = mixed ? 1e-10 : 4.0 * 1.2e-16 * (double)d;
+    info->orth_est = mixed ? 1e-10 : 1.2e-16 * (double)d;
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return R.noconv ? TTG_NO_CONVERGENCE : TTG_OK;
@@ -661,14 +666,18 @@ ttg_status round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r
                      int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
                      ttg_info *info) {
   TTG_CHECK(prec == TTG_PREC_FP64 || prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  validate_tt(d, n, r);
+  TTG_CHECK(eps >= 0.0 && std::isfinite(eps), TTG_BAD_ARG, "eps must be finite and >= 0");
+  if (rmax)
+    for (int64_t i = 1; i < d; ++i) TTG_CHECK(rmax[i] >= 1, TTG_BAD_ARG, "rmax entries must be >= 1");
   if (prec == TTG_PREC_FP64) {
-    validate_tt(d, n, r);
-    TTG_CHECK(eps >= 0.0 && std::isfinite(eps), TTG_BAD_ARG, "eps must be finite and >= 0");
-    if (rmax)
-      for (int64_t i = 1; i < d; ++i) TTG_CHECK(rmax[i] >= 1, TTG_BAD_ARG, "rmax entries must be >= 1");
     ttg_status st = TTG_OK;
     if (round_small_persist(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info, &st))
       return st;
+  } else {
+    // mixed form: the Gram-chain round when every bond is proven to keep all
+    // its singular values (rows orthogonal to 1e-6 at worst); otherwise the sweeps
+    if (round_gram(ctx, d, n, r, dcore, eps, rmax, 1e-6, r_out, dcore_out, cap, len_out, info)) return TTG_OK;
   }
   return round_impl<double>(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info,
                             nullptr, nullptr, prec == TTG_PREC_FP32);

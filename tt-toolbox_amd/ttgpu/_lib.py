@@ -57,6 +57,9 @@ class TTGInfo(C.Structure):
         ("seconds", C.c_double),
         ("qr_retries", C.c_int64),
         ("svd_skipped", C.c_int64),
+        ("path", C.c_int64),
+        ("err_est", C.c_double),
+        ("orth_est", C.c_double),
     ]
 
     def as_dict(self):
