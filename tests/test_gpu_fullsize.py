@@ -34,13 +34,18 @@ def cfg3():
     return x, c["eps"], ref
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
-def test_config3_full_size(gpu_ctx, cfg3, prec, tol):
+@pytest.mark.parametrize("prec,tol,gram", [("fp64", 1e-12, "1"), ("fp32", 1e-6, "1"), ("fp64", 1e-12, "0"),
+                                           ("fp32", 1e-6, "0")])
+def test_config3_full_size(gpu_ctx, cfg3, monkeypatch, prec, tol, gram):
+    """Both forms, through the Gram form (every bond is proven to keep its
+    singular values) and through the QR/SVD sweeps (TTG_GRAM=0)."""
     x, eps, ref = cfg3
-    print(f"config 3 d=32 n=8 r=256 eps=1e-6 form {prec}", flush=True)
+    monkeypatch.setenv("TTG_GRAM", gram)
+    print(f"config 3 d=32 n=8 r=256 eps=1e-6 form {prec} gram {gram}", flush=True)
     info = {}
     y = ttgpu.round(ttgpu.TTTensor(x.n, x.r, x.core), eps, precision=prec, info=info)
     print("ranks", y.r.tolist(), "info", info, flush=True)
+    assert info["path"] == int(gram)
     assert y.r.tolist() == ref.r.tolist()
     yo = O.TT(y.n, y.r, y.core)
     err = O.tt_rel_diff(yo, ref)

@@ -111,8 +111,10 @@ def test_mixed_small(gpu_ctx):
     assert err <= 1e-6
 
 
-def test_cooperative_jacobi_path(gpu_ctx):
-    """Rank 128 bond: the SVD runs on several cooperating workgroups."""
+def test_cooperative_jacobi_path(gpu_ctx, monkeypatch):
+    """Rank 128 bond: the SVD runs on several cooperating workgroups (the
+    sweeps; the Gram form is turned off)."""
+    monkeypatch.setenv("TTG_GRAM", "0")
     rng = np.random.default_rng(9)
     x = O.rand_tt(rng, [16, 16, 16, 16], [1, 16, 128, 16, 1])
     check_plain(x, 1e-9)
@@ -123,6 +125,7 @@ def test_no_truncation_shortcut(gpu_ctx, monkeypatch, eps):
     """Bonds whose smallest singular value provably clears my_chop2's threshold
     are completed from their QR factors without an SVD; the result must equal
     the full SVD path's and the oracle's (ranks exact, 1e-12 relative)."""
+    monkeypatch.setenv("TTG_GRAM", "0")  # the shortcut of the sweeps, not the Gram form
     rng = np.random.default_rng(21)
     x = O.rand_tt(rng, [4] * 8, [1, 4, 16, 64, 80, 80, 16, 4, 1])  # ranks > 64: multi-CU path
     ref = O.tt_round(x, eps)

@@ -68,12 +68,14 @@ __global__ __launch_bounds__(256) void k_core_sumsq(const double *const *__restr
 
 // s[i] = 1 / ||C_i||_F^2 (0 for a zero or non-finite core: the chain then
 // fails its factorisation and the caller takes the sweeps)
-__global__ void k_core_scale(int d, int chunks, const double *__restrict__ part, double *__restrict__ s) {
+__global__ void k_core_scale(int d, int chunks, const double *__restrict__ part, double *__restrict__ s,
+                             double *__restrict__ sq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d) return;
   double t = 0.0;
   for (int c = 0; c < chunks; ++c) t += part[(long)i * chunks + c];
   s[i] = (t > 0.0 && t < INFINITY) ? 1.0 / t : 0.0;
+  sq[i] = sqrt(s[i]);
 }
 
 // operands of one bond's certificate (device table, bonds 1..d-1)
@@ -123,8 +125,9 @@ __global__ __launch_bounds__(256) void k_gram_bond(const GramBond *__restrict__ 
 __global__ __launch_bounds__(256) void k_gram_final(int d, const double *__restrict__ st,
                                                     const int *__restrict__ cflags, const double *__restrict__ A0,
                                                     long a0size, const double *__restrict__ s, double ep,
-                                                    double orth_tol, double *__restrict__ alpha,
-                                                    int *__restrict__ ok, double *__restrict__ diag) {
+                                                    double orth_tol, const double *__restrict__ nearid,
+                                                    double *__restrict__ alpha, int *__restrict__ ok,
+                                                    double *__restrict__ diag) {
   __shared__ double red[4];
   __shared__ double sh_nrm0;
   double a = 0.0;
@@ -142,7 +145,9 @@ __global__ __launch_bounds__(256) void k_gram_final(int d, const double *__restr
     const bool fin = rel > 0.0 && rel < INFINITY && kap2 < INFINITY && s[i] > 0.0;
     const bool fact = cflags[i - 1] == 0 && cflags[d - 1 + i - 1] == 0;
     const bool cert = fin && rel * (2.0 * ep) * (2.0 * ep) <= 1.0 && rel <= 1e12;
-    const bool orth = fin && 1.2e-16 * kap2 <= orth_tol;
+    // one-pass form: the CholeskyQR orthogonality bound; exact form: the
+    // near-identity second pass of bond i was accepted (max|Q1^T Q1 - I| < 1e-8)
+    const bool orth = fin && (nearid ? nearid[i] < 1e-8 : 1.2e-16 * kap2 <= orth_tol);
     if (!(fact && cert && orth)) bad = fmax(bad, (double)(d - i));  // largest = first failing bond
     if (fin) {
       mmin = fmin(mmin, 1.0 / (sqrt(rel) * 2.0 * ep));
@@ -163,6 +168,7 @@ __global__ __launch_bounds__(256) void k_gram_final(int d, const double *__restr
   if (threadIdx.x == 0) {
     sh_nrm0 = nrm0;
     alpha[0] = na > 0.0 ? nrm0 / na : 0.0;
+    alpha[d] = nrm0;
     ok[0] = (bad == 0.0 && !bad_norm) ? 1 : 0;
     diag[0] = mmin;
     diag[1] = kmax;
@@ -173,6 +179,13 @@ __global__ __launch_bounds__(256) void k_gram_final(int d, const double *__restr
   }
   __syncthreads();
   for (int i = 1 + threadIdx.x; i < d; i += blockDim.x) alpha[i] = sqrt(s[i]) * sh_nrm0;
+}
+
+// x[t] *= (*alpha)
+__global__ void k_scale_inplace(double *__restrict__ x, long count, const double *__restrict__ alpha) {
+  const double a = alpha[0];
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x)
+    x[t] *= a;
 }
 
 // dst[t] = (*alpha) src[t]
@@ -189,13 +202,16 @@ __global__ void k_scale_copy(double *__restrict__ dst, const double *__restrict_
 // (no bond truncates).  Returns false when the form does not apply or a
 // bond is not proven; the output buffer may then hold partial results.
 bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore, double eps,
-                const int64_t *rmax, double orth_tol, int64_t *r_out, double *dcore_out, int64_t cap,
+                const int64_t *rmax, bool exact, double orth_tol, int64_t *r_out, double *dcore_out, int64_t cap,
                 int64_t *len_out, ttg_info *info) {
   auto t0 = std::chrono::steady_clock::now();
   if (d < 2 || eps < 0.0) return false;
   {
+    // TTG_GRAM=0 turns the form off; TTG_SVD_ALWAYS=1 asks for an SVD at every bond
     const char *env = std::getenv("TTG_GRAM");
     if (env && env[0] == '0') return false;
+    const char *sv_always = std::getenv("TTG_SVD_ALWAYS");
+    if (sv_always && sv_always[0] == '1') return false;
   }
   // ranks after the caps: left (round.m:48), then right (size of the SVD)
   std::vector<int64_t> rr(r, r + d + 1);
@@ -288,6 +304,10 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     a.template take<double>((size_t)d * chunks);  // norm partials
     a.template take<double>(d + 8);           // s
     a.template take<double>(d + 8);           // alpha
+    a.template take<double>(d + 8);           // sqrt(s)
+    a.template take<double>(d + 8);           // near-identity departures (exact form)
+    for (int q = 0; q < 4; ++q) a.template take<double>(exact ? max_core : 1);  // exact sweep: V, V^T, Q1, Q
+    for (int q = 0; q < 5; ++q) a.template take<double>(exact ? kmax * kmax : 1);  // G, R2^{-1}, R2, two R
     a.template take<double>(4 * (size_t)nb * kBondParts + 8);  // per-bond statistics
     a.template take<double>(8 * (size_t)nb + 8);  // Cholesky statistics
     a.template take<double>(16);              // verdict diagnostics
@@ -314,6 +334,12 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   double *Ab = A.take<double>(out_len);
   double *part = A.take<double>((size_t)d * chunks);
   double *sv = A.take<double>(d + 8), *alpha = A.take<double>(d + 8);
+  double *sq = A.take<double>(d + 8), *nid = A.take<double>(d + 8);
+  double *Vb = A.take<double>(exact ? max_core : 1), *Vt = A.take<double>(exact ? max_core : 1);
+  double *Q1 = A.take<double>(exact ? max_core : 1), *Qt = A.take<double>(exact ? max_core : 1);
+  double *Gt = A.take<double>(exact ? kmax * kmax : 1), *Xi = A.take<double>(exact ? kmax * kmax : 1);
+  double *R2 = A.take<double>(exact ? kmax * kmax : 1);
+  double *Rq[2] = {A.take<double>(exact ? kmax * kmax : 1), A.take<double>(exact ? kmax * kmax : 1)};
   double *bst = A.take<double>(4 * (size_t)nb * kBondParts + 8), *cst = A.take<double>(8 * (size_t)nb + 8);
   double *diag = A.take<double>(16);
   int *cfl = A.take<int>(2 * (size_t)nb + 8), *okd = A.take<int>(16);
@@ -448,6 +474,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     std::memcpy(&b, &orth_tol, sizeof(b));
     key.push_back(b);
   }
+  key.push_back(exact ? 1 : 0);
   for (int64_t i = 0; i < d; ++i) key.push_back((uint64_t)n[i]);
   for (int64_t i = 0; i <= d; ++i) key.push_back((uint64_t)r[i]);
   const bool profiling = g_prof && g_prof->kind != KC_NONE;
@@ -471,7 +498,8 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     caps(true);
     // ---- scales s_i = 1/||C_i||^2 -----------------------------------------------
     hipLaunchKernelGGL(k_core_sumsq, dim3(chunks, (unsigned)d), dim3(256), 0, st, dcp, dsz, chunks, part);
-    hipLaunchKernelGGL(k_core_scale, dim3((unsigned)((d + 255) / 256)), dim3(256), 0, st, (int)d, chunks, part, sv);
+    hipLaunchKernelGGL(k_core_scale, dim3((unsigned)((d + 255) / 256)), dim3(256), 0, st, (int)d, chunks, part, sv,
+                       sq);
     TTG_HIP(hipGetLastError());
 
     // ---- the two Gram chains, concurrently -------------------------------------
@@ -497,16 +525,50 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     // ---- batched factorisations, inverses, products, verdict ----------------------
     chol_batched(st, dcj, njc, kmax);
     trsm3_batched(st, dtj, njc, kmax, kmax);
-    bgemm_run(st, false, true, dja, nb, ta_, qa, fa);   // A_i = C_i R_R,i+1^T
+    if (!exact) bgemm_run(st, false, true, dja, nb, ta_, qa, fa);   // A_i = C_i R_R,i+1^T
     bgemm_run(st, true, false, djz, nb, tz_, qz, fz);   // Z_i = R_R,i^{-T} R_L,i^{-1}
     hipLaunchKernelGGL(k_gram_bond, dim3(kBondParts, (unsigned)nb), dim3(256), 0, st, dgb, bst);
+    if (exact) {
+      // Right-to-left CholeskyQR2 sweep of V_i = sqrt(s_i) C_i R_{i+1}^T (tall
+      // orientation V^T): the first pass is the triangular solve with the
+      // batched Gram factor R~_i (backward stable, block inverses from the
+      // batched Cholesky), the second the near-identity pass on Q1; then
+      // V_i = R_i^T Q_i with R_i = R2 R~_i, core_i = Q_i^T.
+      const double *Rn = nullptr;
+      for (int64_t i = d - 1; i >= 1; --i) {
+        const int64_t kw = rr[i], k = rr[i + 1], N = n[i] * k, m = kw * n[i];
+        if (i + 1 < d) {
+          gemm_scaled(st, false, true, m, k, k, sq + i, cp[i], m, Rn, k, Vb, m, wsr, gws, cus);
+        } else {
+          hipLaunchKernelGGL(k_scale_copy, dim3(64), dim3(256), 0, st, Vb, cp[i], (long)(m * k), sq + i);
+        }
+        transpose<double>(st, Vb, kw, N, Vt);
+        trsm_right_upper<double>(st, N, kw, Vt, N, RR + sq_ofs[i], Q1, N, nullptr, Wv + (size_t)(i - 1) * wjob);
+        gram_nearid(st, N, kw, Q1, false, Gt, Xi, R2, nid + i, wsr, gws, cus);
+        gemm<double>(st, false, false, N, kw, kw, 1.0, Q1, N, Xi, kw, 0.0, Qt, N, wsr, gws, cus);
+        double *Rc = Rq[i & 1];
+        gemm<double>(st, false, false, kw, kw, kw, 1.0, R2, kw, RR + sq_ofs[i], kw, 0.0, Rc, kw, nullptr, 0, cus);
+        transpose<double>(st, Qt, N, kw, dcore_out + out_ofs[i]);
+        Rn = Rc;
+      }
+      // A_0 = C_0 R_1^T
+      gemm<double>(st, false, true, rr[0] * n[0], rr[1], rr[1], 1.0, cp[0], rr[0] * n[0], Rn, rr[1], 0.0, Ab,
+                   rr[0] * n[0], wsr, gws, cus);
+    }
     const double ep = eps / std::sqrt((double)(d - 1));
     hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, st, (int)d, bst, cfl, Ab, (long)(rr[0] * n[0] * rr[1]), sv,
-                       ep, orth_tol, alpha, okd, diag);
+                       ep, orth_tol, exact ? nid : nullptr, alpha, okd, diag);
     TTG_HIP(hipGetLastError());
-    // the output: core_0 = alpha_0 A_0, core_i = alpha_i R_R,i^{-T} A_i
+    // the output: core_0 = alpha_0 A_0; one-pass form: core_i = alpha_i R_R,i^{-T} A_i,
+    // exact form: cores 1..d-1 (already Q_i^T) times nrm0
     hipLaunchKernelGGL(k_scale_copy, dim3(64), dim3(256), 0, st, dcore_out, Ab, (long)(rr[0] * n[0] * rr[1]), alpha);
-    bgemm_run(st, true, false, djb, nb, tb_, qb, fb);
+    if (!exact) {
+      bgemm_run(st, true, false, djb, nb, tb_, qb, fb);
+    } else {
+      const long cnt = (long)(out_len - out_ofs[1]);
+      const unsigned blocks = (unsigned)std::min<long>((cnt + 255) / 256, 8192);
+      hipLaunchKernelGGL(k_scale_inplace, dim3(blocks), dim3(256), 0, st, dcore_out + out_ofs[1], cnt, alpha + d);
+    }
     TTG_HIP(hipGetLastError());
   };
   if (replay) {
@@ -558,8 +620,8 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     info->qr_retries = 0;
     info->svd_skipped = d - 1;
     info->path = 1;
-    info->err_est = dg[4];
-    info->orth_est = dg[5];
+    info->err_est = exact ? 4.0 * 1.2e-16 * (double)d : dg[4];
+    info->orth_est = exact ? 1e-8 : dg[5];
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return true;

@@ -149,7 +149,7 @@ bool round_small_persist(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_
 // Gram-chain round for tensors whose bonds provably keep every singular value
 // (false: not applicable or not proven; the caller runs the sweeps)
 bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore, double eps,
-                const int64_t *rmax, double orth_tol, int64_t *r_out, double *dcore_out, int64_t cap,
+                const int64_t *rmax, bool exact, double orth_tol, int64_t *r_out, double *dcore_out, int64_t cap,
                 int64_t *len_out, ttg_info *info);
 // true if one workgroup can round the tensor (d <= 64, ranks <= 512)
 bool persist_supported(int64_t d, const int64_t *n, const int64_t *r);

@@ -410,10 +410,15 @@ __global__ __launch_bounds__(256) void k_gram_nearid(int k, int S, const TP *__r
     atomicMax(reinterpret_cast<unsigned long long *>(maxabs), (unsigned long long)__double_as_longlong(mx));
 }
 
+__global__ void k_zero_d(double *p, int count) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0.0;
+}
+
 template <class T>
 static void gram_nearid_impl(hipStream_t st, int64_t m, int64_t k, const double *A, double *G, double *X,
                              double *R2, double *maxabs, T *ws, int64_t ws_elems, int cus) {
-  TTG_HIP(hipMemsetAsync(maxabs, 0, sizeof(double), st));
+  // cleared by a kernel, not a memset node: the call may be captured in a graph
+  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, maxabs, 1);
   int S = splitk_count(k, k, m, cus);
   if (S > 1 && (ws == nullptr || (int64_t)S * k * k > ws_elems)) S = 1;
   const long total = (long)k * k;

@@ -674,10 +674,14 @@ ttg_status round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r
     ttg_status st = TTG_OK;
     if (round_small_persist(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info, &st))
       return st;
+    // plain form: when every bond is proven to keep all its singular values,
+    // one exact right-to-left CholeskyQR2 sweep preconditioned by the Gram
+    // factors (no left sweep, no SVD); otherwise the sweeps
+    if (round_gram(ctx, d, n, r, dcore, eps, rmax, true, 0.0, r_out, dcore_out, cap, len_out, info)) return TTG_OK;
   } else {
     // mixed form: the Gram-chain round when every bond is proven to keep all
     // its singular values (rows orthogonal to 1e-6 at worst); otherwise the sweeps
-    if (round_gram(ctx, d, n, r, dcore, eps, rmax, 1e-6, r_out, dcore_out, cap, len_out, info)) return TTG_OK;
+    if (round_gram(ctx, d, n, r, dcore, eps, rmax, false, 1e-6, r_out, dcore_out, cap, len_out, info)) return TTG_OK;
   }
   return round_impl<double>(ctx, d, n, r, dcore, eps, rmax, r_out, dcore_out, cap, len_out, info,
                             nullptr, nullptr, prec == TTG_PREC_FP32);
