@@ -158,6 +158,22 @@ ttg_status ttg_adjoint_bilinear_f32(ttg_ctx *ctx, int64_t d, const int64_t *n,
                                     const int64_t *rx, const float *dxcore, double *value,
                                     float *dgx, float *dgy);
 
+/* ttg_adjoint_bilinear_f64 without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (the interface chains, 2 sum_k ry_k ra_k rx_k + 2 max_k(core
+ * product) doubles) comes from a static device array of 8 MiB per device,
+ * set up when the first context on the device is created; larger ranks
+ * return TTG_BAD_ARG.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (on different streams or contexts) race.  Same results
+ * as ttg_adjoint_bilinear_f64. */
+ttg_status ttg_adjoint_bilinear_f64_noalloc(ttg_ctx *ctx, int64_t d, const int64_t *n,
+                                            const int64_t *m, const int64_t *ry,
+                                            const double *dycore, const int64_t *ra,
+                                            const double *dacore, const int64_t *rx,
+                                            const double *dxcore, double *value, double *dgx,
+                                            double *dgy);
+
 /* Kernel-class timing (diagnostic; no reference counterpart).  Between
  * begin and end, every launch of the chosen class on this context is
  * bracketed by HIP events on the context's stream; end synchronises and

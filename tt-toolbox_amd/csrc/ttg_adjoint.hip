@@ -375,7 +375,9 @@ template <class T>
 ttg_status adjoint_bilinear(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
                             const int64_t *ry, const T *dycore, const int64_t *ra,
                             const T *dacore, const int64_t *rx, const T *dxcore, double *value,
-                            T *dgx, T *dgy) {
+                            T *dgx, T *dgy, void *ext_ws, size_t ext_bytes) {
+  // ext_ws != nullptr: the scratch comes from caller-provided device memory of
+  // ext_bytes (no allocation); otherwise from the context's workspace
   TTG_CHECK(d >= 1 && n && m && ry && ra && rx && dycore && dacore && dxcore, TTG_BAD_ARG,
             "bad bilinear arguments");
   TTG_CHECK(ry[0] == 1 && ra[0] == 1 && rx[0] == 1 && ry[d] == 1 && ra[d] == 1 && rx[d] == 1,
@@ -397,12 +399,24 @@ ttg_status adjoint_bilinear(ttg_ctx *ctx, int64_t d, const int64_t *n, const int
   z.take<T>(oi[d + 1]);
   z.take<T>(tmax);
   z.take<T>(tmax);
-  ctx->arena.reserve(z.used + 4096);
-  ctx->arena.reset();
-  T *phi = ctx->arena.take<T>(oi[d + 1]);
-  T *psi = ctx->arena.take<T>(oi[d + 1]);
-  T *t2 = ctx->arena.take<T>(tmax);
-  T *w = ctx->arena.take<T>(tmax);
+  T *phi, *psi, *t2, *w;
+  if (ext_ws) {
+    TTG_CHECK(z.used <= ext_bytes, TTG_BAD_ARG,
+              "ranks too large for the no-allocation scratch (" + std::to_string(z.used) + " bytes needed, " +
+                  std::to_string(ext_bytes) + " available)");
+    ArenaView v{static_cast<char *>(ext_ws), ext_bytes, 0};
+    phi = v.take<T>(oi[d + 1]);
+    psi = v.take<T>(oi[d + 1]);
+    t2 = v.take<T>(tmax);
+    w = v.take<T>(tmax);
+  } else {
+    ctx->arena.reserve(z.used + 4096);
+    ctx->arena.reset();
+    phi = ctx->arena.take<T>(oi[d + 1]);
+    psi = ctx->arena.take<T>(oi[d + 1]);
+    t2 = ctx->arena.take<T>(tmax);
+    w = ctx->arena.take<T>(tmax);
+  }
   auto shape = [&](int64_t k) {
     BiShape s;
     s.ry0 = ry[k]; s.ry1 = ry[k + 1]; s.ra0 = ra[k]; s.ra1 = ra[k + 1];
@@ -446,8 +460,8 @@ ttg_status adjoint_bilinear(ttg_ctx *ctx, int64_t d, const int64_t *n, const int
   if (value) *value = (double)hv;
   return TTG_OK;
 }
-template ttg_status adjoint_bilinear<double>(ttg_ctx *, int64_t, const int64_t *, const int64_t *, const int64_t *, const double *, const int64_t *, const double *, const int64_t *, const double *, double *, double *, double *);
-template ttg_status adjoint_bilinear<float>(ttg_ctx *, int64_t, const int64_t *, const int64_t *, const int64_t *, const float *, const int64_t *, const float *, const int64_t *, const float *, double *, float *, float *);
+template ttg_status adjoint_bilinear<double>(ttg_ctx *, int64_t, const int64_t *, const int64_t *, const int64_t *, const double *, const int64_t *, const double *, const int64_t *, const double *, double *, double *, double *, void *, size_t);
+template ttg_status adjoint_bilinear<float>(ttg_ctx *, int64_t, const int64_t *, const int64_t *, const int64_t *, const float *, const int64_t *, const float *, const int64_t *, const float *, double *, float *, float *, void *, size_t);
 
 // ---------------------------------------------------------------------------
 // batched rounding: per-tensor sweep on the shared stream (first version)

@@ -2,6 +2,7 @@
 #include "ttg_internal.h"
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 
 namespace ttg {
 ttg_status round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
@@ -33,7 +34,32 @@ template <class T>
 ttg_status adjoint_bilinear(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
                             const int64_t *ry, const T *dycore, const int64_t *ra,
                             const T *dacore, const int64_t *rx, const T *dxcore, double *value,
-                            T *dgx, T *dgy);
+                            T *dgx, T *dgy, void *ext_ws, size_t ext_bytes);
+
+// Scratch of the no-allocation entry point: one copy of this array lives on
+// every device the library runs on (until the process ends).  Host code never
+// passes the array's name to a kernel (it is a host-side stand-in there); it
+// uses the device address hipGetSymbolAddress returns, looked up once per
+// device when the first context on that device is created.
+constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on `device` (the device must be current)
+static ttg_status scratch_lookup(int device) {
+  if (device < 0 || device >= kMaxDevices) return TTG_NO_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return TTG_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return TTG_HIP_ERROR;
+  }
+  g_scratch_ptr[device] = p;
+  return TTG_OK;
+}
 }  // namespace ttg
 
 using namespace ttg;
@@ -78,6 +104,8 @@ ttg_status ttg_ctx_create(int device, void *stream, ttg_ctx **out) {
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return TTG_NO_DEVICE;
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return TTG_NO_DEVICE;
   if (hipSetDevice(device) != hipSuccess) return TTG_NO_DEVICE;
+  // the no-allocation entry point's scratch address on this device
+  if (ttg::scratch_lookup(device) != TTG_OK) return TTG_HIP_ERROR;
   ttg_ctx *ctx = new ttg_ctx();
   ctx->device = device;
   ctx->num_cus = prop.multiProcessorCount;
@@ -388,7 +416,21 @@ ttg_status ttg_adjoint_bilinear_f64(ttg_ctx *ctx, int64_t d, const int64_t *n, c
   TTG_GUARD_BEGIN
   TTG_HIP(hipSetDevice(ctx->device));
   ctx->last_error.clear();
-  return adjoint_bilinear<double>(ctx, d, n, m, ry, dycore, ra, dacore, rx, dxcore, value, dgx, dgy);
+  return adjoint_bilinear<double>(ctx, d, n, m, ry, dycore, ra, dacore, rx, dxcore, value, dgx, dgy, nullptr, 0);
+  TTG_GUARD_END(ctx)
+}
+ttg_status ttg_adjoint_bilinear_f64_noalloc(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
+                                            const int64_t *ry, const double *dycore, const int64_t *ra,
+                                            const double *dacore, const int64_t *rx, const double *dxcore,
+                                            double *value, double *dgx, double *dgy) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  void *ws = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+  TTG_CHECK(ws != nullptr, TTG_HIP_ERROR, "no-allocation scratch not set up on this device");
+  return adjoint_bilinear<double>(ctx, d, n, m, ry, dycore, ra, dacore, rx, dxcore, value, dgx, dgy, ws,
+                                  kNoAllocScratchBytes);
   TTG_GUARD_END(ctx)
 }
 ttg_status ttg_adjoint_bilinear_f32(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
@@ -399,7 +441,7 @@ ttg_status ttg_adjoint_bilinear_f32(ttg_ctx *ctx, int64_t d, const int64_t *n, c
   TTG_GUARD_BEGIN
   TTG_HIP(hipSetDevice(ctx->device));
   ctx->last_error.clear();
-  return adjoint_bilinear<float>(ctx, d, n, m, ry, dycore, ra, dacore, rx, dxcore, value, dgx, dgy);
+  return adjoint_bilinear<float>(ctx, d, n, m, ry, dycore, ra, dacore, rx, dxcore, value, dgx, dgy, nullptr, 0);
   TTG_GUARD_END(ctx)
 }
 

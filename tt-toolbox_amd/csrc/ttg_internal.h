@@ -57,6 +57,19 @@ struct Arena {
   }
 };
 
+// Bump allocation inside memory the caller owns (no allocation, no free).
+struct ArenaView {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  template <class T> T *take(size_t count) {
+    size_t bytes = ((count * sizeof(T)) + 255) & ~size_t(255);
+    if (used + bytes > cap) throw Error(TTG_BAD_ARG, "workspace too small");
+    T *p = reinterpret_cast<T *>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+
 // Size-only twin of Arena used to compute the workspace requirement.
 struct ArenaSizer {
   size_t used = 0;

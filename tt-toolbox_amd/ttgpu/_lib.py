@@ -34,7 +34,7 @@ EXPORTS = [
     "ttg_mtimes", "ttg_mtimes_dev", "ttg_mtimes_round", "ttg_mtimes_round_dev",
     "ttg_round_batched_dev",
     "ttg_adjoint_completion_f64", "ttg_adjoint_completion_f32",
-    "ttg_adjoint_bilinear_f64", "ttg_adjoint_bilinear_f32",
+    "ttg_adjoint_bilinear_f64", "ttg_adjoint_bilinear_f32", "ttg_adjoint_bilinear_f64_noalloc",
     "ttg_profile_begin", "ttg_profile_end",
 ]
 # kernel-level test hooks (include/ttgpu_testing.h)
@@ -103,6 +103,7 @@ def load(path: Path | str | None = None):
         "ttg_adjoint_completion_f32": (st, [P, I64, I64P, I64P, P, I64, P, P, DP, P]),
         "ttg_adjoint_bilinear_f64": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, I64P, P, DP, P, P]),
         "ttg_adjoint_bilinear_f32": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, I64P, P, DP, P, P]),
+        "ttg_adjoint_bilinear_f64_noalloc": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, I64P, P, DP, P, P]),
         "ttg_profile_begin": (st, [P, C.c_int]),
         "ttg_profile_end": (st, [P, I64P, DP, DP, DP]),
         "ttg_test_gemm": (st, [P, C.c_int, C.c_int, C.c_int, I64, I64, I64, P, I64, P, I64, P, I64]),
