@@ -33,7 +33,8 @@ typedef enum {
   TTG_NONFINITE = 3,      /* NaN/Inf met in the input or produced by a factorisation */
   TTG_NO_CONVERGENCE = 4, /* Jacobi sweep cap reached (analogue of the svd->svds warning, round.m:64-74) */
   TTG_HIP_ERROR = 5,      /* a HIP runtime call failed */
-  TTG_NO_DEVICE = 6       /* no usable gfx950 device */
+  TTG_NO_DEVICE = 6,      /* no usable gfx950 device */
+  TTG_IO_ERROR = 7        /* a file cannot be opened, read or written (fmex/tt_read.c:29-34) */
 } ttg_status;
 
 typedef enum {
@@ -129,6 +130,75 @@ ttg_status ttg_round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d,
                                  const int64_t *r, const int64_t *cofs, const double *dcore,
                                  double eps, int prec, int64_t *r_out,
                                  const int64_t *cofs_out, double *dcore_out, ttg_info *info);
+
+/* Z = a*X + b*Y — replaces @tt_tensor/plus.m:41-86 (rank concatenation: X's cores in the
+ * leading blocks, Y's in the trailing blocks) with the scalar products of
+ * @tt_tensor/mtimes.m:10-27 (a and b scale core 1; minus.m:18 is a = 1, b = -1).
+ * Boundary ranks must agree (plus.m:50-59, else TTG_BAD_ARG); rz = rx + ry inside.
+ * cap >= sum_k rz[k]*n[k]*rz[k+1].  All cores are assembled by one launch.  Device pointers. */
+ttg_status ttg_plus_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx,
+                        const double *dx, double a, const int64_t *ry, const double *dy,
+                        double b, int64_t *rz, double *dz, int64_t cap, int64_t *len_out);
+
+/* W = round(a*X + b*Y, eps, rmax) fused — replaces the composition plus.m:41-86 ->
+ * @tt_tensor/round.m.  When the sum can take the small-tensor or the Gram-chain form
+ * its cores are assembled once on the device; otherwise the QR/SVD sweeps run with
+ * the sum generated inside the first sweep's products (zero blocks never stored or
+ * multiplied).  Output and capacity as ttg_round_dev for ranks rx + ry. */
+ttg_status ttg_plus_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx,
+                              const double *dx, double a, const int64_t *ry, const double *dy,
+                              double b, double eps, const int64_t *rmax, int prec,
+                              int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
+                              ttg_info *info);
+
+/* ||X||_F — replaces @tt_tensor/norm.m:25-41 (left-to-right QR sweep, product of the
+ * R-factor norms and the last core's norm).  Device pointer. */
+ttg_status ttg_norm_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
+                        const double *dcore, double *nrm);
+
+/* ||a*X + b*Y||_F — replaces norm(X - Y) (@tt_tensor/minus.m:18 then norm.m:25-41) without
+ * storing the sum: the QR sweep runs over it with its cores generated inside the sweep's
+ * products, so differences of nearly equal tensors keep their digits.  Device pointers. */
+ttg_status ttg_plus_norm_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx,
+                             const double *dx, double a, const int64_t *ry, const double *dy,
+                             double b, double *nrm);
+
+/* <X, Y> — replaces @tt_tensor/dot.m:92-111 (interface chain, boundary ranks 1;
+ * other boundary ranks return TTG_BAD_ARG).  Device pointers. */
+ttg_status ttg_dot_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx,
+                       const double *dx, const int64_t *ry, const double *dy, double *value);
+
+/* X = tt_tensor(full, eps) (TT-SVD) — replaces @tt_tensor/tt_tensor.m:175-244: left to
+ * right, the unfolding of r_i n_i rows is factored by a QR (CholeskyQR2 under acceptance
+ * tests, Householder otherwise) and the one-sided Jacobi SVD of its R factor; the rank is
+ * my_chop2(s, eps/sqrt(d-1) * ||s||) (core/my_chop2.m), capped by rmax (NULL: no cap, as
+ * the reference).  dfull: prod(n) doubles, column-major (first index fastest), device
+ * memory.  Needs min(r_i n_i, prod(n)/(r_i n_i)) <= 512 at every step (else TTG_BAD_ARG).
+ * d = 1 returns the vector as one core.  Output as ttg_round_dev (*len_out = used length;
+ * TTG_NEED_SPACE when cap is short). */
+ttg_status ttg_full_to_tt_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const double *dfull,
+                              double eps, const int64_t *rmax, int64_t *r_out, double *dcore_out,
+                              int64_t cap, int64_t *len_out, ttg_info *info);
+
+/* SDV TT files — replace fmex/tt_read.c:13-103 (tt_tensor(filename),
+ * @tt_tensor/tt_tensor.m:149-165) and fmex/tt_write.c:21-79 (@tt_tensor/tt_save.m:22).
+ * Format: 128-byte header {"TT      ", ver {1,0}, inf {128, dtype, 0, 0}, comment[64],
+ * i {1, 0, d, 0..}}, int32 l, m (d = m - l + 1), int32 n[d], int32 r[d+1], then the
+ * flat core as double (dtype 0) or float (dtype 2; widened on reading).  Complex
+ * files (dtype 1, 3) return TTG_BAD_ARG.  ttg_sdv_info reports d, the core length and
+ * dtype; the readers need n[dcap >= d], r[dcap + 1] and cap >= core length (else
+ * TTG_NEED_SPACE, with *d and *len_out set).  Writers: precision 0 double, 1 float.
+ * The _dev forms stream the core between the file and device memory through two
+ * pinned 16 MiB buffers (disk and PCIe overlap). */
+ttg_status ttg_sdv_info(const char *path, int64_t *d, int64_t *core_len, int *dtype);
+ttg_status ttg_sdv_read(const char *path, int64_t dcap, int64_t *d, int64_t *n, int64_t *r,
+                        double *core, int64_t cap, int64_t *len_out);
+ttg_status ttg_sdv_read_dev(ttg_ctx *ctx, const char *path, int64_t dcap, int64_t *d, int64_t *n,
+                            int64_t *r, double *dcore, int64_t cap, int64_t *len_out);
+ttg_status ttg_sdv_write(const char *path, int64_t d, const int64_t *n, const int64_t *r,
+                         const double *core, int precision);
+ttg_status ttg_sdv_write_dev(ttg_ctx *ctx, const char *path, int64_t d, const int64_t *n,
+                             const int64_t *r, const double *dcore, int precision);
 
 /* Adjoint of the sampled-entry completion loss 0.5*||P_Omega(X) - y||^2.
  * No reference counterpart (TT-Toolbox has no gradients); entry evaluation

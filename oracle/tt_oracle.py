@@ -30,7 +30,7 @@ import numpy as np
 __all__ = [
     "TT", "TTM", "my_chop2", "tt_round", "ttm_round", "ttm_mtimes",
     "tt_plus", "tt_scal", "tt_minus", "tt_norm", "tt_full", "ttm_full",
-    "tt_rel_diff", "tt_dot", "rand_tt", "dense_sequential_round",
+    "tt_rel_diff", "tt_dot", "rand_tt", "dense_sequential_round", "tt_from_full",
     "completion_value_grad", "bilinear_value_grad", "make_config",
 ]
 
@@ -303,6 +303,10 @@ def tt_plus(b, c):
     if b.d != c.d or not np.array_equal(b.n, c.n):
         raise ValueError("inconsistent sizes")
     d = b.d
+    if d == 1:  # :30-33: one core each, the cores add
+        if not np.array_equal(b.r, c.r):
+            raise ValueError("inconsistent sizes")
+        return TT(b.n.copy(), b.r.copy(), b.core + c.core)
     r1, r2 = b.r, c.r
     r = r1 + r2
     if r1[0] != r2[0]:
@@ -388,6 +392,34 @@ def tt_dot(a, b):
         ca, cb = a.core_k(i), b.core_k(i)
         p = np.einsum("ab,aic,bie->ce", p, ca, cb)
     return float(p[0, 0])
+
+
+def tt_from_full(b, eps=1e-14, n=None, svd_driver="gesdd"):
+    """``@tt_tensor/tt_tensor.m:175-244`` (TT-SVD, boundary ranks 1): left to
+    right, ``m = n(i) r(i)``, ``[u,s,v] = svd(c,'econ')``,
+    ``r1 = my_chop2(s, ep*norm(s))`` with ``ep = eps/sqrt(d-1)``, core ``u``,
+    ``c = (v*diag(s))'``; the last core is what remains.  ``b`` is the flat
+    column-major tensor (``b(:)``), ``n`` its mode sizes."""
+    b = np.asarray(b, dtype=np.float64).reshape(-1, order="F")
+    n = np.asarray(n, dtype=np.int64)
+    d = n.size
+    if d == 1:
+        return TT(n.copy(), np.array([1, 1]), b.copy())
+    r = np.ones(d + 1, dtype=np.int64)
+    ep = eps / math.sqrt(d - 1)
+    c = b
+    cores = []
+    for i in range(d - 1):
+        m = int(n[i] * r[i])
+        c = c.reshape((m, c.size // m), order="F")
+        u, s, v = _svd_econ(c, svd_driver)
+        r1 = my_chop2(s, ep * np.linalg.norm(s))
+        u, s, v = u[:, :r1], s[:r1], v[:, :r1]
+        r[i + 1] = r1
+        cores.append(u.reshape((r[i], n[i], r1), order="F"))
+        c = (v * s[None, :]).T
+    cores.append(c.reshape((r[d - 1], n[d - 1], 1), order="F"))
+    return TT.from_cores(cores)
 
 
 # ---------------------------------------------------------------------------

@@ -171,6 +171,10 @@ def test_plus_scal_norm_full_consistency():
     np.testing.assert_allclose(O.tt_full(O.tt_scal(-2.5, a)), -2.5 * O.tt_full(a), atol=1e-13)
     assert abs(O.tt_norm(a) - np.linalg.norm(O.tt_full(a))) <= 1e-13 * np.linalg.norm(O.tt_full(a))
     assert abs(O.tt_dot(a, b) - O.tt_full(a) @ O.tt_full(b)) <= 1e-12
+    # d = 1 (plus.m:30-33): the cores add
+    u = O.rand_tt(rng, [5], [1, 1])
+    v = O.rand_tt(rng, [5], [1, 1])
+    np.testing.assert_allclose(O.tt_full(O.tt_plus(u, v)), O.tt_full(u) + O.tt_full(v), atol=1e-15)
 
 
 # ---------------------------------------------------------------- adjoints
@@ -270,3 +274,59 @@ def test_tt_norm_gram_matches_qr_norm_and_dense():
     dense = np.linalg.norm(O.tt_full(x))
     assert abs(O.tt_norm_gram(x) - dense) <= 1e-13 * dense
     assert abs(O.tt_norm(x) - dense) <= 1e-13 * dense
+
+
+# ---------------------------------------------------------------- TT-SVD
+def _dense_left_projection(full, n, eps):
+    """TT-SVD in exact arithmetic: bond i (left to right) projects the
+    (i_1..i_i) x (rest) unfolding of the current dense tensor onto its
+    leading r_i left singular vectors, r_i = my_chop2(s, eps/sqrt(d-1) ||s||)."""
+    d = len(n)
+    y = np.asarray(full, dtype=np.float64).copy()
+    ranks = [1] * (d + 1)
+    ep = eps / math.sqrt(d - 1)
+    for i in range(1, d):
+        rows = int(np.prod(n[:i]))
+        mat = y.reshape((rows, -1), order="F")
+        u, s, vh = np.linalg.svd(mat, full_matrices=False)
+        r1 = O.my_chop2(s, np.linalg.norm(s) * ep)
+        ranks[i] = r1
+        y = (u[:, :r1] @ (u[:, :r1].T @ mat)).reshape(-1, order="F")
+    return y, ranks
+
+
+@pytest.mark.parametrize("seed,n,r,eps", [
+    (31, [3, 4, 2, 3, 2], [1, 3, 5, 4, 2, 1], 1e-12),
+    (32, [2, 3, 4, 3], [1, 2, 6, 3, 1], 0.05),
+    (33, [4, 4, 4], [1, 4, 4, 1], 0.2),
+])
+def test_tt_from_full_matches_dense_left_projection(seed, n, r, eps):
+    rng = np.random.default_rng(seed)
+    x = O.rand_tt(rng, n, r)
+    x = O.tt_plus(x, O.tt_scal(0.1, O.rand_tt(rng, n, r)))
+    full = O.tt_full(x)
+    t = O.tt_from_full(full, eps, n)
+    dense, ranks = _dense_left_projection(full, n, eps)
+    assert t.r.tolist() == ranks
+    np.testing.assert_allclose(O.tt_full(t), dense, rtol=0, atol=1e-12 * np.linalg.norm(full))
+    assert np.linalg.norm(O.tt_full(t) - full) <= eps * np.linalg.norm(full) * (1 + 1e-12)
+    if eps == 1e-12:  # nothing truncates: the generic ranks of a sum of two rank-r TTs
+        assert t.r.tolist() == O.generic_ranks(n, [1] + [2 * v for v in r[1:-1]] + [1])
+
+
+def test_planted_right_to_left_ttsvd_is_caught():
+    """A right-to-left TT-SVD (a plausible mistake: round.m's direction) gives
+    other ranks on a tensor whose two directions disagree."""
+    rng = np.random.default_rng(34)
+    n = [2, 3, 4, 3]
+    x = O.rand_tt(rng, n, [1, 2, 6, 3, 1])
+    x = O.tt_plus(x, O.tt_scal(0.1, O.rand_tt(rng, n, [1, 2, 6, 3, 1])))
+    full = O.tt_full(x)
+    found = False
+    for eps in (0.26, 0.3):
+        _, left = _dense_left_projection(full, n, eps)
+        _, right = O.dense_sequential_round(full, n, eps)
+        if left != right:
+            found = True
+            assert O.tt_from_full(full, eps, n).r.tolist() == left
+    assert found

@@ -26,6 +26,31 @@ ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, cons
                              const int64_t *cofs, const double *dcore, double eps, int prec,
                              int64_t *r_out, const int64_t *cofs_out, double *dcore_out,
                              ttg_info *info);
+ttg_status plus_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx, double a,
+                    const int64_t *ry, const double *dy, double b, int64_t *rz_out, double *dz, int64_t cap,
+                    int64_t *len_out);
+ttg_status plus_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx,
+                          double a, const int64_t *ry, const double *dy, double b, double eps,
+                          const int64_t *rmax, int prec, int64_t *r_out, double *dcore_out, int64_t cap,
+                          int64_t *len_out, ttg_info *info);
+ttg_status norm_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
+                    double *nrm);
+ttg_status plus_norm_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx,
+                         double a, const int64_t *ry, const double *dy, double b, double *nrm);
+ttg_status dot_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx,
+                   const int64_t *ry, const double *dy, double *value);
+ttg_status full_to_tt_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const double *dfull, double eps,
+                          const int64_t *rmax, int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
+                          ttg_info *info);
+ttg_status sdv_info(const char *path, int64_t *d, int64_t *core_len, int *dtype);
+ttg_status sdv_read(const char *path, int64_t dcap, int64_t *d, int64_t *n, int64_t *r, double *core, int64_t cap,
+                    int64_t *len_out);
+ttg_status sdv_read_dev(ttg_ctx *ctx, const char *path, int64_t dcap, int64_t *d, int64_t *n, int64_t *r,
+                        double *dcore, int64_t cap, int64_t *len_out);
+ttg_status sdv_write(const char *path, int64_t d, const int64_t *n, const int64_t *r, const double *core,
+                     int precision);
+ttg_status sdv_write_dev(ttg_ctx *ctx, const char *path, int64_t d, const int64_t *n, const int64_t *r,
+                         const double *dcore, int precision);
 template <class T>
 ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
                               const T *dcore, int64_t S, const int32_t *didx, const T *dvals,
@@ -446,3 +471,116 @@ ttg_status ttg_adjoint_bilinear_f32(ttg_ctx *ctx, int64_t d, const int64_t *n, c
 }
 
 }  // extern "C"
+
+// ---- sums, norms and inner products ---------------------------------------
+ttg_status ttg_plus_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx, double a,
+                        const int64_t *ry, const double *dy, double b, int64_t *rz, double *dz, int64_t cap,
+                        int64_t *len_out) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_CHECK(dx && dy && rz && dz, TTG_BAD_ARG, "null pointer argument");
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return plus_dev(ctx, d, n, rx, dx, a, ry, dy, b, rz, dz, cap, len_out);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_plus_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx,
+                              double a, const int64_t *ry, const double *dy, double b, double eps,
+                              const int64_t *rmax, int prec, int64_t *r_out, double *dcore_out, int64_t cap,
+                              int64_t *len_out, ttg_info *info) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return plus_round_dev(ctx, d, n, rx, dx, a, ry, dy, b, eps, rmax, prec, r_out, dcore_out, cap, len_out, info);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_norm_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
+                        double *nrm) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return norm_dev(ctx, d, n, r, dcore, nrm);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_plus_norm_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx,
+                             double a, const int64_t *ry, const double *dy, double b, double *nrm) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return plus_norm_dev(ctx, d, n, rx, dx, a, ry, dy, b, nrm);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_dot_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx,
+                       const int64_t *ry, const double *dy, double *value) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return dot_dev(ctx, d, n, rx, dx, ry, dy, value);
+  TTG_GUARD_END(ctx)
+}
+
+// ---- SDV files ---------------------------------------------------------------
+ttg_status ttg_sdv_info(const char *path, int64_t *d, int64_t *core_len, int *dtype) {
+  ttg_ctx *ctx = nullptr;
+  TTG_GUARD_BEGIN
+  TTG_CHECK(d, TTG_BAD_ARG, "null pointer argument");
+  return sdv_info(path, d, core_len, dtype);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_sdv_read(const char *path, int64_t dcap, int64_t *d, int64_t *n, int64_t *r, double *core,
+                        int64_t cap, int64_t *len_out) {
+  ttg_ctx *ctx = nullptr;
+  TTG_GUARD_BEGIN
+  return sdv_read(path, dcap, d, n, r, core, cap, len_out);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_sdv_read_dev(ttg_ctx *ctx, const char *path, int64_t dcap, int64_t *d, int64_t *n, int64_t *r,
+                            double *dcore, int64_t cap, int64_t *len_out) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return sdv_read_dev(ctx, path, dcap, d, n, r, dcore, cap, len_out);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_sdv_write(const char *path, int64_t d, const int64_t *n, const int64_t *r, const double *core,
+                         int precision) {
+  ttg_ctx *ctx = nullptr;
+  TTG_GUARD_BEGIN
+  return sdv_write(path, d, n, r, core, precision);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_sdv_write_dev(ttg_ctx *ctx, const char *path, int64_t d, const int64_t *n, const int64_t *r,
+                             const double *dcore, int precision) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return sdv_write_dev(ctx, path, d, n, r, dcore, precision);
+  TTG_GUARD_END(ctx)
+}
+
+// ---- TT-SVD --------------------------------------------------------------------
+ttg_status ttg_full_to_tt_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const double *dfull, double eps,
+                              const int64_t *rmax, int64_t *r_out, double *dcore_out, int64_t cap,
+                              int64_t *len_out, ttg_info *info) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return full_to_tt_dev(ctx, d, n, dfull, eps, rmax, r_out, dcore_out, cap, len_out, info);
+  TTG_GUARD_END(ctx)
+}
+

@@ -198,6 +198,24 @@ __global__ void k_scale_copy(double *__restrict__ dst, const double *__restrict_
 
 }  // namespace
 
+bool gram_shape_ok(int64_t d, const int64_t *n, const int64_t *r, const int64_t *rmax) {
+  if (d < 2) return false;
+  // TTG_GRAM=0 turns the form off; TTG_SVD_ALWAYS=1 asks for an SVD at every bond
+  const char *env = std::getenv("TTG_GRAM");
+  if (env && env[0] == '0') return false;
+  const char *sv_always = std::getenv("TTG_SVD_ALWAYS");
+  if (sv_always && sv_always[0] == '1') return false;
+  // ranks after the caps: left (round.m:48), then right (size of the SVD)
+  std::vector<int64_t> rr(r, r + d + 1);
+  for (int64_t i = 0; i + 1 < d; ++i) rr[i + 1] = std::min(rr[i + 1], rr[i] * n[i]);
+  for (int64_t i = d - 1; i >= 1; --i) rr[i] = std::min(rr[i], n[i] * rr[i + 1]);
+  for (int64_t i = 1; i < d; ++i) {
+    if (rr[i] > 256) return false;                 // batched factorisation kernels: k <= 256
+    if (rmax && rmax[i] < rr[i]) return false;     // a binding rmax truncates
+  }
+  return true;
+}
+
 // Result of the Gram-chain form: ranks equal to the capped input ranks
 // (no bond truncates).  Returns false when the form does not apply or a
 // bond is not proven; the output buffer may then hold partial results.
@@ -205,14 +223,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
                 const int64_t *rmax, bool exact, double orth_tol, int64_t *r_out, double *dcore_out, int64_t cap,
                 int64_t *len_out, ttg_info *info) {
   auto t0 = std::chrono::steady_clock::now();
-  if (d < 2 || eps < 0.0) return false;
-  {
-    // TTG_GRAM=0 turns the form off; TTG_SVD_ALWAYS=1 asks for an SVD at every bond
-    const char *env = std::getenv("TTG_GRAM");
-    if (env && env[0] == '0') return false;
-    const char *sv_always = std::getenv("TTG_SVD_ALWAYS");
-    if (sv_always && sv_always[0] == '1') return false;
-  }
+  if (eps < 0.0 || !gram_shape_ok(d, n, r, rmax)) return false;
   // ranks after the caps: left (round.m:48), then right (size of the SVD)
   std::vector<int64_t> rr(r, r + d + 1);
   std::vector<int> left_cap(d, 0), right_cap(d, 0);
@@ -228,10 +239,6 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     }
   int64_t kmax = 1;
   for (int64_t i = 1; i < d; ++i) kmax = std::max(kmax, rr[i]);
-  if (kmax > 256) return false;  // batched factorisation kernels: k <= 256
-  if (rmax)
-    for (int64_t i = 1; i < d; ++i)
-      if (rmax[i] < rr[i]) return false;  // a binding rmax truncates
   // capacity: the QR-capped size (what the sweeps need) bounds ours
   std::vector<int64_t> rq(r, r + d + 1);
   for (int64_t i = 0; i + 1 < d; ++i) rq[i + 1] = std::min(rq[i] * n[i], rq[i + 1]);

@@ -137,6 +137,9 @@ struct ttg_ctx {
   ttg::Arena garena;
   hipGraphExec_t ggraph = nullptr;
   std::vector<uint64_t> gkey;
+  // cores of an assembled sum a X + b Y (plus fused with round), kept apart
+  // from the rounding workspaces that read them
+  ttg::Arena sarena;
 };
 
 namespace ttg {
@@ -164,6 +167,9 @@ bool round_small_persist(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_
 bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore, double eps,
                 const int64_t *rmax, bool exact, double orth_tol, int64_t *r_out, double *dcore_out, int64_t cap,
                 int64_t *len_out, ttg_info *info);
+// the shape tests of round_gram (d >= 2, capped ranks <= 256, no binding
+// rmax, the form not switched off): true when the form may be tried
+bool gram_shape_ok(int64_t d, const int64_t *n, const int64_t *r, const int64_t *rmax);
 // true if one workgroup can round the tensor (d <= 64, ranks <= 512)
 bool persist_supported(int64_t d, const int64_t *n, const int64_t *r);
 // doubles of per-slot workspace for the largest core / bond rank of a batch
@@ -335,6 +341,32 @@ void mtimes_core(hipStream_t s, int64_t ra0, int64_t n, int64_t m, int64_t ra1, 
 void gemm_mtimes(hipStream_t s, int64_t M, const double *R, int64_t ldr, int64_t ra0, int64_t n,
                  int64_t m, int64_t ra1, const double *A, int64_t rx0, int64_t rx1, const double *X,
                  double *C, int64_t ldc);
+
+// One core of Z = a X + b Y in the rank concatenation of
+// @tt_tensor/plus.m:41-86: X's core in the leading block, Y's in the
+// trailing block, zeros elsewhere (sum != 0: d = 1, the cores add).
+struct PlusCore {
+  const double *x, *y;
+  double *z;
+  int64_t rx0, rx1, ry0, ry1, rz0, rz1, n;
+  double a, b;
+  int sum, pad;
+};
+// assembles ncores cores (table in device memory) in one launch
+void plus_assemble(hipStream_t s, const PlusCore *tab_dev, int ncores, int64_t max_core);
+void plus_assemble_one(hipStream_t s, const PlusCore &c);
+// ranks of a X + b Y (boundary ranks must agree; throws TTG_BAD_ARG)
+std::vector<int64_t> plus_ranks(int64_t d, const int64_t *n, const int64_t *rx, const int64_t *ry);
+// uploads the table of cores [k0, k1) of a X + b Y (destinations zdst[k - k0])
+// to tab_dev through pinned memory on ctx's stream; returns the largest core
+int64_t plus_table(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx, double a,
+                   const int64_t *ry, const double *dy, double b, const std::vector<int64_t> &rz,
+                   double *const *zdst, int64_t k0, int64_t k1, PlusCore *tab_dev);
+
+// TSQR: Rt (m x m) = R^T of the QR of A^T, A (m x M, ld m).  W, Sa, Sb:
+// scratch of tsqr_ws_elems(m, M, &level) doubles (W) and `level` doubles each.
+int64_t tsqr_ws_elems(int64_t m, int64_t M, int64_t *level_elems);
+void tsqr_rt(hipStream_t s, const double *A, int64_t m, int64_t M, double *W, double *Sa, double *Sb, double *Rt);
 
 // check for non-finite values; sets *flag nonzero if any
 void check_finite(hipStream_t s, const double *x, int64_t count, int *flag);

@@ -86,6 +86,8 @@ template <class T> struct Rounder {
   bool allow_notrunc = true;
   int next_level = 2;  // QR level after the last rejected svd_bond
   bool noconv = false;
+  bool sweep1_only = false;  // norm: the left QR sweep alone (norm.m:25-41)
+  double *tsW = nullptr, *tsA = nullptr, *tsB = nullptr;  // TSQR scratch (TT-SVD)
 
   // a buffer region for core k that is not the current one
   T *other(int64_t k) const {
@@ -430,6 +432,92 @@ template <class T> struct Rounder {
     }
   }
 
+  // ---- TT-SVD of a full tensor (@tt_tensor/tt_tensor.m:216-233) -----------
+  // Step i: the unfolding c (m x M, m = r_i n_i), then the one-sided Jacobi
+  // SVD of a k x k R factor, k = min(m, M), and my_chop2 at ep * ||s||
+  // (sort_chop):
+  //   wide (M >= m): c^T = Q R by TSQR (R only, Householder tree: stable on
+  //     the numerically low-rank unfoldings TT-SVD is for), R J = Y:
+  //     U = J(:, perm), S V^T = U^T c (one GEMM over c; Q is never formed)
+  //   tall (M < m, small): c = Qs R (CholeskyQR2 under the acceptance tests,
+  //     Householder where they fail), R^T J = Y: U = Qs J(:, perm),
+  //     S V^T = Y(:, perm)^T
+  // core_i = U (m x r_{i+1}); the next unfolding is S V^T seen as
+  // (r_{i+1} n_{i+1}) x (M / n_{i+1}).  Cores go to `out`; returns its length.
+  int64_t ttsvd(const double *full, int64_t N, const int64_t *nn, double eps, const int64_t *rmax,
+                int64_t *rr, double *out) {
+    // step 0 reads the caller's tensor; later steps alternate buf[0] / buf[1]
+    const double *c = full;
+    const double ep = eps / std::sqrt((double)(d - 1));
+    rr[0] = 1;
+    int64_t o = 0, len = N;  // len: entries of the current unfolding
+    for (int64_t i = 0; i + 1 < d; ++i) {
+      const int64_t m = rr[i] * nn[i], M = len / m;
+      const int64_t kj = std::min(m, M);
+      TTG_CHECK(kj <= kMaxRank, TTG_BAD_ARG, "TT-SVD unfoldings need min(rows, columns) <= 512");
+      const bool wide = M >= m;
+      const int64_t mt = m;
+      const double *At = c;
+      int level = wide ? 3 : 0;
+      if (wide) tsqr_rt(st, c, m, M, tsW, tsA, tsB, Rprod);  // Rprod = R^T
+      for (; level < 3;) {
+        if (level >= 2) {
+          qr_householder(At, mt, kj, Qs, Rprod);
+          break;
+        }
+        qr_tall_fast(At, mt, kj, Qs, Rprod, iflag, bstats, level, false, false);
+        double *hp = pinned();
+        int *hf = reinterpret_cast<int *>(hp + 8);
+        TTG_HIP(hipMemcpyAsync(hp, bstats, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        TTG_HIP(hipMemcpyAsync(hf, iflag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        TTG_HIP(hipStreamSynchronize(st));
+        double h_st[4];
+        int h_fl[2];
+        std::memcpy(h_st, hp, sizeof(h_st));
+        std::memcpy(h_fl, hf, sizeof(h_fl));
+        if (qr_accept(h_fl, h_st, level, false)) break;
+        level = qr_next(h_fl, h_st, level, false);
+        ++retries;
+      }
+      JacobiPlan pl = jacobi_plan(kj);
+      // wide: S = R^T transposed (R J = Y); tall: S = R transposed (R^T J = Y)
+      jacobi_svd(st, pl, Rprod, /*transS=*/true, Y, jsweeps, bar, jerr, jacobi_sweep_cap(), Yws, nullptr, 1e-16);
+      sort_chop(st, Y, pl.rows, kj, pl.kpad, ep, rmax ? rmax[i + 1] : INT64_MAX, sv, perm, scal_i, scal_d + 8);
+      int h_i[3];
+      {
+        int *hp = reinterpret_cast<int *>(pinned());
+        TTG_HIP(hipMemcpyAsync(hp, scal_i, sizeof(int), hipMemcpyDeviceToHost, st));
+        TTG_HIP(hipMemcpyAsync(hp + 1, jerr, sizeof(int), hipMemcpyDeviceToHost, st));
+        TTG_HIP(hipMemcpyAsync(hp + 2, jsweeps, sizeof(int), hipMemcpyDeviceToHost, st));
+        TTG_HIP(hipStreamSynchronize(st));
+        std::memcpy(h_i, hp, sizeof(h_i));
+      }
+      if (h_i[1] == 2) throw Error(TTG_HIP_ERROR, "Jacobi grid barrier timed out");
+      if (h_i[1] == 3) throw Error(TTG_HIP_ERROR, "Jacobi round barrier did not complete");
+      if (h_i[1] == 1) noconv = true;
+      sweeps_total += h_i[2];
+      sweeps_max = std::max<int64_t>(sweeps_max, h_i[2]);
+      const int64_t r1 = std::max(1, h_i[0]);
+      double *cn = buf[i & 1];
+      if (wide) {
+        gather_cols<double>(st, Y, pl.rows, kj, kj, perm, r1, out + o, m);  // U = J(:, perm)
+        g(true, false, r1, M, m, out + o, m, c, m, cn, r1);                 // S V^T = U^T c
+      } else {
+        gather_cols<double>(st, Y, pl.rows, kj, kj, perm, r1, Q1, kj);      // J(:, perm)
+        g(false, false, m, r1, M, Qs, m, Q1, kj, out + o, m);                // U = Qs J_p
+        gather_cols_t<double>(st, Y, pl.rows, 0, kj, perm, r1, cn);         // S V^T = Y_p^T
+      }
+      rr[i + 1] = r1;
+      o += m * r1;
+      len = r1 * M;
+      c = cn;
+    }
+    // the last core is what remains (tt_tensor.m:234)
+    TTG_HIP(hipMemcpyAsync(out + o, c, len * sizeof(double), hipMemcpyDeviceToDevice, st));
+    rr[d] = 1;
+    return o + len;
+  }
+
   void sweep2_bond(int64_t i, double ep, int64_t rmax_i) {
     int64_t r1, kj;
     bool ok, wide, skipped;
@@ -520,6 +608,7 @@ template <class T> struct Rounder {
         }
       }
     }
+    if (sweep1_only) return;
     const double ep = d > 1 ? eps / std::sqrt((double)(d - 1)) : INFINITY;
     for (int64_t i = d - 1; i >= 1; --i) {
       const int64_t rm = rmax ? rmax[i] : INT64_MAX;
@@ -541,7 +630,8 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
                              double *dcore_out, int64_t cap, int64_t *len_out, ttg_info *info,
                              const std::function<void(int64_t, T *)> &gen,
                              const std::vector<int64_t> *gen_ranks, bool mixed,
-                             const std::function<void(int64_t, const T *, int64_t, T *)> *fmul = nullptr) {
+                             const std::function<void(int64_t, const T *, int64_t, T *)> *fmul = nullptr,
+                             double *norm_out = nullptr) {
   auto t0 = std::chrono::steady_clock::now();
   validate_tt(d, n, r);
   if (rmax)
@@ -578,7 +668,7 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
   for (int64_t i = 0; i <= d; ++i) kmax = std::max(kmax, rc[i]);
   for (int64_t i = 0; i < d; ++i) kmax = std::max(kmax, std::min(rc[i + 1], n[i] * rc[i + 1]));
   TTG_CHECK(kmax <= kMaxRank, TTG_BAD_ARG, "TT ranks above 512 are not supported");
-  const int64_t cap_needed = packed_size(d, n, rc.data());
+  const int64_t cap_needed = norm_out ? 0 : packed_size(d, n, rc.data());
   if (cap < cap_needed) {
     if (len_out) *len_out = cap_needed;
     return TTG_NEED_SPACE;
@@ -618,7 +708,32 @@ static ttg_status round_impl(ttg_ctx *ctx, int64_t d, const int64_t *n, const in
   R.cp.resize(d);
   for (int64_t i = 0; i < d; ++i) R.cp[i] = R.buf[0] + R.ofs[i];
   if (gen) gen(0, R.cp[0]);
+  R.sweep1_only = norm_out != nullptr;
   R.run(eps, rmax, gen);
+  if (norm_out) {
+    // ||X|| = prod_i nrm(i) * ||last core||  (norm.m:38-40): the factors are
+    // multiplied with their binary exponents split off, so the product
+    // neither overflows nor loses digits on the way
+    fro_norm<T>(st, R.cp[d - 1], R.r[d - 1] * R.n[d - 1] * R.r[d], R.nrm + 0);
+    double *hn = R.pinned((d + 1) * sizeof(double));
+    TTG_HIP(hipMemcpyAsync(hn, R.nrm, d * sizeof(double), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipStreamSynchronize(st));
+    double mant = 1.0;
+    long ex = 0;
+    for (int64_t i = 0; i < d; ++i) {
+      int e = 0;
+      mant = std::frexp(mant * hn[i], &e);
+      ex += e;
+    }
+    *norm_out = mant == 0.0 ? 0.0 : std::ldexp(mant, (int)std::max<long>(std::min<long>(ex, 4096), -4096));
+    if (info) {
+      *info = ttg_info{};
+      info->qr_fallbacks = R.fallbacks;
+      info->qr_retries = R.retries;
+      info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return TTG_OK;
+  }
   // finalise: nrm(1) = ||core_1||, balance, pack   (round.m:90-107)
   const int64_t s0 = R.r[0] * R.n[0] * R.r[1];
   fro_norm<T>(st, R.cp[0], s0, R.nrm + 0);
@@ -754,6 +869,276 @@ ttg_status mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int
       };
   return round_impl<double>(ctx, d, n, rp.data(), nullptr, eps, rmax, r_out, dcore_out, cap,
                             len_out, info, gen, &rp, prec == TTG_PREC_FP32, &fmul);
+}
+
+// ---- sums: a X + b Y fused with the round; norms of tensors and differences
+namespace {
+// Rounding-sweep hooks for Z = a X + b Y (@tt_tensor/plus.m:41-86, scalars on
+// core 1 as @tt_tensor/mtimes.m:10-27): core 1 of Z is assembled (it is the
+// first QR's input); every later core only feeds sweep 1's R * core product,
+// and R * Z_k splits by the block structure of Z_k into
+//   [R(:, 0:rx_k) X_k, R(:, rx_k:rz_k) Y_k]    (column ranges of the result)
+// for inner cores and R(:, 0:rx) X_k + R(:, rx:rz) Y_k for the last core, so
+// the zero blocks are neither stored nor multiplied.
+struct PlusHooks {
+  ttg_ctx *ctx;
+  int64_t d;
+  const int64_t *n, *rx, *ry;
+  const double *dx, *dy;
+  double a, b;
+  std::vector<int64_t> rz, ox, oy;
+  std::function<void(int64_t, double *)> gen;
+  std::function<void(int64_t, const double *, int64_t, double *)> fmul;
+
+  PlusHooks(ttg_ctx *c, int64_t d_, const int64_t *n_, const int64_t *rx_, const double *dx_, double a_,
+            const int64_t *ry_, const double *dy_, double b_)
+      : ctx(c), d(d_), n(n_), rx(rx_), ry(ry_), dx(dx_), dy(dy_), a(a_), b(b_) {
+    rz = plus_ranks(d, n, rx, ry);
+    ox.assign(d + 1, 0);
+    oy.assign(d + 1, 0);
+    for (int64_t k = 0; k < d; ++k) {
+      ox[k + 1] = ox[k] + rx[k] * n[k] * rx[k + 1];
+      oy[k + 1] = oy[k] + ry[k] * n[k] * ry[k + 1];
+    }
+    gen = [this](int64_t k, double *dst) {
+      PlusCore c{};
+      c.x = dx + ox[k];
+      c.y = dy + oy[k];
+      c.z = dst;
+      c.rx0 = rx[k];
+      c.rx1 = rx[k + 1];
+      c.ry0 = ry[k];
+      c.ry1 = ry[k + 1];
+      c.rz0 = rz[k];
+      c.rz1 = rz[k + 1];
+      c.n = n[k];
+      c.a = k == 0 ? a : 1.0;
+      c.b = k == 0 ? b : 1.0;
+      c.sum = d == 1;
+      plus_assemble_one(ctx->stream, c);
+    };
+    fmul = [this](int64_t k, const double *Rm, int64_t kk, double *dst) {
+      hipStream_t st = ctx->stream;
+      const int cus = ctx->num_cus;
+      const int64_t nk = n[k];
+      if (k + 1 < d) {
+        gemm<double>(st, false, false, kk, nk * rx[k + 1], rx[k], 1.0, Rm, kk, dx + ox[k], rx[k], 0.0, dst, kk,
+                     nullptr, 0, cus);
+        gemm<double>(st, false, false, kk, nk * ry[k + 1], ry[k], 1.0, Rm + kk * rx[k], kk, dy + oy[k], ry[k],
+                     0.0, dst + kk * nk * rx[k + 1], kk, nullptr, 0, cus);
+      } else {
+        gemm<double>(st, false, false, kk, nk * rz[d], rx[k], 1.0, Rm, kk, dx + ox[k], rx[k], 0.0, dst, kk,
+                     nullptr, 0, cus);
+        gemm<double>(st, false, false, kk, nk * rz[d], ry[k], 1.0, Rm + kk * rx[k], kk, dy + oy[k], ry[k], 1.0,
+                     dst, kk, nullptr, 0, cus);
+      }
+    };
+  }
+
+  // NaN/Inf in either summand (the fused sweep never copies the input)
+  void check_inputs() {
+    hipStream_t st = ctx->stream;
+    ctx->arena.reserve(4096);
+    ctx->arena.reset();
+    int *nf = ctx->arena.take<int>(16);
+    TTG_HIP(hipMemsetAsync(nf, 0, sizeof(int), st));
+    check_finite(st, dx, ox[d], nf);
+    check_finite(st, dy, oy[d], nf);
+    int hnf = 0;
+    TTG_HIP(hipMemcpyAsync(&hnf, nf, sizeof(int), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipStreamSynchronize(st));
+    if (hnf) throw Error(TTG_NONFINITE, "input core contains NaN or Inf");
+  }
+};
+}  // namespace
+
+// ||X|| by the left QR sweep of @tt_tensor/norm.m:25-41 (FP64 CholeskyQR2 with
+// the sweep's acceptance tests, Householder where they fail).
+ttg_status norm_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
+                    double *nrm) {
+  validate_tt(d, n, r);
+  TTG_CHECK(dcore && nrm, TTG_BAD_ARG, "null pointer argument");
+  std::vector<int64_t> ro(d + 1);
+  return round_impl<double>(ctx, d, n, r, dcore, 0.0, nullptr, ro.data(), nullptr, INT64_MAX, nullptr, nullptr,
+                            nullptr, nullptr, false, nullptr, nrm);
+}
+
+// ||a X + b Y|| (minus.m:18 with a = 1, b = -1, then norm.m:25-41): the QR
+// sweep runs over the sum with the sum's cores generated inside sweep 1's
+// products, so the difference of two nearly equal tensors keeps its digits
+// (no cancellation of norms or inner products).
+ttg_status plus_norm_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx,
+                         double a, const int64_t *ry, const double *dy, double b, double *nrm) {
+  TTG_CHECK(dx && dy && nrm, TTG_BAD_ARG, "null pointer argument");
+  PlusHooks h(ctx, d, n, rx, dx, a, ry, dy, b);
+  std::vector<int64_t> ro(d + 1);
+  h.check_inputs();
+  return round_impl<double>(ctx, d, n, h.rz.data(), nullptr, 0.0, nullptr, ro.data(), nullptr, INT64_MAX,
+                            nullptr, nullptr, h.gen, &h.rz, false, &h.fmul, nrm);
+}
+
+// round(a X + b Y, eps, rmax) (plus.m:41-86 then round.m).  When the sum can
+// take the small-tensor or the Gram-chain form (both read every input core
+// more than once), its cores are assembled once into ctx->sarena (one launch,
+// one HBM pass) and rounded like any tensor; otherwise the QR/SVD sweeps run
+// with the sum generated inside sweep 1's products (never assembled).
+ttg_status plus_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx,
+                          double a, const int64_t *ry, const double *dy, double b, double eps,
+                          const int64_t *rmax, int prec, int64_t *r_out, double *dcore_out, int64_t cap,
+                          int64_t *len_out, ttg_info *info) {
+  TTG_CHECK(dx && dy && r_out && dcore_out, TTG_BAD_ARG, "null pointer argument");
+  TTG_CHECK(prec == TTG_PREC_FP64 || prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  TTG_CHECK(eps >= 0.0 && std::isfinite(eps), TTG_BAD_ARG, "eps must be finite and >= 0");
+  PlusHooks h(ctx, d, n, rx, dx, a, ry, dy, b);
+  const int64_t *rz = h.rz.data();
+  if (rmax)
+    for (int64_t i = 1; i < d; ++i) TTG_CHECK(rmax[i] >= 1, TTG_BAD_ARG, "rmax entries must be >= 1");
+  int64_t need = 0;
+  round_capacity(d, n, rz, &need);
+  if (cap < need) {
+    if (len_out) *len_out = need;
+    return TTG_NEED_SPACE;
+  }
+  bool small = false;
+  if (prec == TTG_PREC_FP64) {
+    const char *env = std::getenv("TTG_SMALL_PERSIST");
+    std::vector<int64_t> rc = capped_ranks(d, n, rz);
+    small = !(env && env[0] == '0') && persist_supported(d, n, rz);
+    for (int64_t i = 0; i <= d && small; ++i) small = rc[i] <= 64;
+    for (int64_t i = 1; i < d && small && rmax; ++i) small = rmax[i] >= rc[i];
+  }
+  if (small || gram_shape_ok(d, n, rz, rmax)) {
+    int64_t zlen = 0;
+    for (int64_t k = 0; k < d; ++k) zlen += rz[k] * n[k] * rz[k + 1];
+    ctx->sarena.reserve(zlen * sizeof(double) + d * sizeof(PlusCore) + 4096);
+    ctx->sarena.reset();
+    double *z = ctx->sarena.take<double>(zlen);
+    PlusCore *tab = ctx->sarena.take<PlusCore>(d);
+    std::vector<double *> zdst(d);
+    for (int64_t k = 0, o = 0; k < d; ++k) {
+      zdst[k] = z + o;
+      o += rz[k] * n[k] * rz[k + 1];
+    }
+    const int64_t mx = plus_table(ctx, d, n, rx, dx, a, ry, dy, b, h.rz, zdst.data(), 0, d, tab);
+    plus_assemble(ctx->stream, tab, (int)d, mx);
+    // the table's upload read the pinned buffer, which the round reuses
+    TTG_HIP(hipStreamSynchronize(ctx->stream));
+    return round_dev(ctx, d, n, rz, z, eps, rmax, prec, r_out, dcore_out, cap, len_out, info);
+  }
+  h.check_inputs();
+  return round_impl<double>(ctx, d, n, rz, nullptr, eps, rmax, r_out, dcore_out, cap, len_out, info, h.gen,
+                            &h.rz, prec == TTG_PREC_FP32, &h.fmul);
+}
+
+// TT-SVD: tt_tensor(full, eps) of @tt_tensor/tt_tensor.m:175-244, left to
+// right, ep = eps / sqrt(d - 1), r_{i+1} = my_chop2(s, ep * ||s||), capped by
+// rmax (the reference's constructor has no cap: rmax = NULL matches it).
+ttg_status full_to_tt_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const double *dfull, double eps,
+                          const int64_t *rmax, int64_t *r_out, double *dcore_out, int64_t cap, int64_t *len_out,
+                          ttg_info *info) {
+  auto t0 = std::chrono::steady_clock::now();
+  TTG_CHECK(d >= 1 && n && dfull && r_out && dcore_out, TTG_BAD_ARG, "null or empty argument");
+  TTG_CHECK(eps >= 0.0 && std::isfinite(eps), TTG_BAD_ARG, "eps must be finite and >= 0");
+  int64_t N = 1;
+  for (int64_t i = 0; i < d; ++i) {
+    TTG_CHECK(n[i] >= 1, TTG_BAD_ARG, "mode sizes must be >= 1");
+    TTG_CHECK(N <= (int64_t(1) << 40) / n[i], TTG_BAD_ARG, "full tensor above 2^40 entries");
+    N *= n[i];
+  }
+  if (rmax)
+    for (int64_t i = 1; i < d; ++i) TTG_CHECK(rmax[i] >= 1, TTG_BAD_ARG, "rmax entries must be >= 1");
+  hipStream_t st = ctx->stream;
+  // rank bounds: r_i <= min(prod n[<i], prod n[>=i]) (and rmax): output size bound
+  std::vector<int64_t> rb(d + 1, 1);
+  {
+    int64_t left = 1;
+    for (int64_t i = 1; i < d; ++i) {
+      left = std::min<int64_t>(left * n[i - 1], int64_t(1) << 40);
+      int64_t right = 1;
+      for (int64_t j = i; j < d && right < left; ++j) right = std::min<int64_t>(right * n[j], int64_t(1) << 40);
+      rb[i] = std::min(left, right);
+      if (rmax) rb[i] = std::min(rb[i], rmax[i]);
+    }
+  }
+  int64_t outb = 0, kmax = 1;
+  for (int64_t i = 0; i < d; ++i) {
+    outb += rb[i] * n[i] * rb[i + 1];
+    if (i + 1 < d) kmax = std::max(kmax, std::min<int64_t>(rb[i] * n[i], N / std::max<int64_t>(rb[i] * n[i], 1)));
+  }
+  kmax = std::max<int64_t>(1, std::min<int64_t>(kmax, kMaxRank));
+  const int64_t total = std::max(N, outb);
+  if (d == 1) {  // one core: the tensor itself (tt_tensor.m:199-211)
+    if (len_out) *len_out = N;
+    if (cap < N) return TTG_NEED_SPACE;
+    ctx->arena.reserve(4096);
+    ctx->arena.reset();
+    int *nf = ctx->arena.take<int>(16);
+    TTG_HIP(hipMemsetAsync(nf, 0, sizeof(int), st));
+    copy_check_finite(st, dcore_out, dfull, N, nf);
+    int hnf = 0;
+    TTG_HIP(hipMemcpyAsync(&hnf, nf, sizeof(int), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipStreamSynchronize(st));
+    if (hnf) throw Error(TTG_NONFINITE, "input contains NaN or Inf");
+    r_out[0] = r_out[1] = 1;
+    if (info) *info = ttg_info{};
+    return TTG_OK;
+  }
+  Rounder<double> R;
+  R.ctx = ctx;
+  R.st = st;
+  R.cus = ctx->num_cus;
+  R.d = d;
+  R.mixed = false;
+  R.max_core = N;
+  R.kmax = kmax;
+  R.rsq = kmax * kmax;
+  if (!ctx->side) {
+    TTG_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    TTG_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    TTG_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+  }
+  // TSQR scratch for the largest wide unfolding (m <= kMaxRank rows)
+  // (blocks of B = max(256, 2m) <= 1024 columns: W <= (M + B) m <= N + 1024 m,
+  // one level's R stack <= (M / B + 1) m^2 <= N / 2 + m^2)
+  const int64_t tsw = N + 1024 * kMaxRank, tsl = N / 2 + kMaxRank * kMaxRank;
+  ArenaSizer z;
+  Rounder<double>::layout(z, d, total, N, kmax, R.rsq);
+  z.take<int>(16);
+  z.take<double>(tsw);
+  z.take<double>(tsl);
+  z.take<double>(tsl);
+  ctx->arena.reserve(z.used + 4096);
+  ctx->arena.reset();
+  R.carve(ctx->arena, total);
+  int *nf = ctx->arena.take<int>(16);
+  R.tsW = ctx->arena.take<double>(tsw);
+  R.tsA = ctx->arena.take<double>(tsl);
+  R.tsB = ctx->arena.take<double>(tsl);
+  TTG_HIP(hipMemsetAsync(nf, 0, sizeof(int), st));
+  check_finite(st, dfull, N, nf);
+  {
+    int *hnf = reinterpret_cast<int *>(R.pinned());
+    TTG_HIP(hipMemcpyAsync(hnf, nf, sizeof(int), hipMemcpyDeviceToHost, st));
+    TTG_HIP(hipStreamSynchronize(st));
+    if (*hnf) throw Error(TTG_NONFINITE, "input contains NaN or Inf");
+  }
+  std::vector<int64_t> rr(d + 1, 1);
+  const int64_t len = R.ttsvd(dfull, N, n, eps, rmax, rr.data(), R.buf[2]);
+  if (len_out) *len_out = len;
+  if (cap < len) return TTG_NEED_SPACE;
+  TTG_HIP(hipMemcpyAsync(dcore_out, R.buf[2], len * sizeof(double), hipMemcpyDeviceToDevice, st));
+  TTG_HIP(hipStreamSynchronize(st));
+  for (int64_t i = 0; i <= d; ++i) r_out[i] = rr[i];
+  if (info) {
+    *info = ttg_info{};
+    info->qr_fallbacks = R.fallbacks;
+    info->qr_retries = R.retries;
+    info->jacobi_sweeps = R.sweeps_total;
+    info->jacobi_maxsweeps = R.sweeps_max;
+    info->err_est = 4.0 * 1.2e-16 * (double)d;
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return R.noconv ? TTG_NO_CONVERGENCE : TTG_OK;
 }
 
 }  // namespace ttg

@@ -22,7 +22,7 @@ TTG_HIP_ERROR = 5
 TTG_NO_DEVICE = 6
 STATUS_NAMES = {
     0: "TTG_OK", 1: "TTG_BAD_ARG", 2: "TTG_NEED_SPACE", 3: "TTG_NONFINITE",
-    4: "TTG_NO_CONVERGENCE", 5: "TTG_HIP_ERROR", 6: "TTG_NO_DEVICE",
+    4: "TTG_NO_CONVERGENCE", 5: "TTG_HIP_ERROR", 6: "TTG_NO_DEVICE", 7: "TTG_IO_ERROR",
 }
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -35,6 +35,8 @@ EXPORTS = [
     "ttg_round_batched_dev",
     "ttg_adjoint_completion_f64", "ttg_adjoint_completion_f32",
     "ttg_adjoint_bilinear_f64", "ttg_adjoint_bilinear_f32", "ttg_adjoint_bilinear_f64_noalloc",
+    "ttg_plus_dev", "ttg_plus_round_dev", "ttg_norm_dev", "ttg_plus_norm_dev", "ttg_dot_dev",
+    "ttg_full_to_tt_dev", "ttg_sdv_info", "ttg_sdv_read", "ttg_sdv_read_dev", "ttg_sdv_write", "ttg_sdv_write_dev",
     "ttg_profile_begin", "ttg_profile_end",
 ]
 # kernel-level test hooks (include/ttgpu_testing.h)
@@ -104,6 +106,18 @@ def load(path: Path | str | None = None):
         "ttg_adjoint_bilinear_f64": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, I64P, P, DP, P, P]),
         "ttg_adjoint_bilinear_f32": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, I64P, P, DP, P, P]),
         "ttg_adjoint_bilinear_f64_noalloc": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, I64P, P, DP, P, P]),
+        "ttg_plus_dev": (st, [P, I64, I64P, I64P, P, C.c_double, I64P, P, C.c_double, I64P, P, I64, I64P]),
+        "ttg_plus_round_dev": (st, [P, I64, I64P, I64P, P, C.c_double, I64P, P, C.c_double, C.c_double, I64P,
+                                    C.c_int, I64P, P, I64, I64P, P]),
+        "ttg_norm_dev": (st, [P, I64, I64P, I64P, P, DP]),
+        "ttg_plus_norm_dev": (st, [P, I64, I64P, I64P, P, C.c_double, I64P, P, C.c_double, DP]),
+        "ttg_dot_dev": (st, [P, I64, I64P, I64P, P, I64P, P, DP]),
+        "ttg_full_to_tt_dev": (st, [P, I64, I64P, P, C.c_double, I64P, I64P, P, I64, I64P, P]),
+        "ttg_sdv_info": (st, [C.c_char_p, I64P, I64P, C.POINTER(C.c_int)]),
+        "ttg_sdv_read": (st, [C.c_char_p, I64, I64P, I64P, I64P, P, I64, I64P]),
+        "ttg_sdv_read_dev": (st, [P, C.c_char_p, I64, I64P, I64P, I64P, P, I64, I64P]),
+        "ttg_sdv_write": (st, [C.c_char_p, I64, I64P, I64P, P, C.c_int]),
+        "ttg_sdv_write_dev": (st, [P, C.c_char_p, I64, I64P, I64P, P, C.c_int]),
         "ttg_profile_begin": (st, [P, C.c_int]),
         "ttg_profile_end": (st, [P, I64P, DP, DP, DP]),
         "ttg_test_gemm": (st, [P, C.c_int, C.c_int, C.c_int, I64, I64, I64, P, I64, P, I64, P, I64]),

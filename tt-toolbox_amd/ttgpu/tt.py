@@ -282,3 +282,223 @@ def round_batched_dev(ctx, tensors_n, tensors_r, dcore, cofs, eps, precision="fp
         ln = int(_ps(tensors_n[t], r_t)[-1])
         res.append((r_t, out[int(cofs_out[t]):int(cofs_out[t]) + ln]))
     return res
+
+
+# ---------------------------------------------------------------------------
+# sums, norms and inner products (device pointers; host forms stage through torch)
+# ---------------------------------------------------------------------------
+def _plus_ranks(rx, ry):
+    rx = np.asarray(rx, dtype=np.int64)
+    ry = np.asarray(ry, dtype=np.int64)
+    rz = rx + ry
+    rz[0], rz[-1] = rx[0], rx[-1]
+    return rz
+
+
+def plus_dev(ctx, n, rx, dx, ry, dy, a=1.0, b=1.0, out=None):
+    """``a*X + b*Y`` in the rank concatenation of ``@tt_tensor/plus.m:41-86``
+    (scalars on core 1, ``@tt_tensor/mtimes.m:10-27``); returns (rz, core view)."""
+    import torch
+    d = len(n)
+    cap = int(_ps(n, _plus_ranks(rx, ry))[-1])
+    if out is None:
+        out = torch.empty(max(cap, 1), dtype=torch.float64, device=dx.device)
+    r_out = (C.c_int64 * (d + 1))()
+    used = C.c_int64()
+    s = ctx.lib.ttg_plus_dev(ctx.h, d, i64(n), i64(rx), _ptr(dx), float(a), i64(ry), _ptr(dy), float(b),
+                             r_out, _ptr(out), int(out.numel()), C.byref(used))
+    ctx.check(s)
+    return np.array(r_out[:], dtype=np.int64), out[:used.value]
+
+
+def plus_round_dev(ctx, n, rx, dx, ry, dy, eps, a=1.0, b=1.0, rmax=None, precision="fp64", out=None,
+                   info=None):
+    """``round(a*X + b*Y, eps, rmax)`` with the sum never stored as a TT by the
+    sweeps; returns (r_out, core view)."""
+    import torch
+    d = len(n)
+    cap = capacity(n, _plus_ranks(rx, ry))
+    if out is None:
+        out = torch.empty(max(cap, 1), dtype=torch.float64, device=dx.device)
+    r_out = (C.c_int64 * (d + 1))()
+    used = C.c_int64()
+    inf = TTGInfo()
+    s = ctx.lib.ttg_plus_round_dev(ctx.h, d, i64(n), i64(rx), _ptr(dx), float(a), i64(ry), _ptr(dy), float(b),
+                                   float(eps), _rmax_arg(rmax, d), _prec(precision), r_out, _ptr(out),
+                                   int(out.numel()), C.byref(used), C.byref(inf))
+    ctx.check(s, allow=(TTG_OK, TTG_NO_CONVERGENCE))
+    if info is not None:
+        info.update(inf.as_dict())
+        info["status"] = s
+    return np.array(r_out[:], dtype=np.int64), out[:used.value]
+
+
+def norm_dev(ctx, n, r, dcore):
+    """``norm(X)`` (``@tt_tensor/norm.m:25-41``, QR sweep) of a device-resident TT."""
+    v = C.c_double()
+    ctx.check(ctx.lib.ttg_norm_dev(ctx.h, len(n), i64(n), i64(r), _ptr(dcore), C.byref(v)))
+    return float(v.value)
+
+
+def plus_norm_dev(ctx, n, rx, dx, ry, dy, a=1.0, b=-1.0):
+    """``norm(a*X + b*Y)`` without storing the sum (default: ``norm(X - Y)``)."""
+    v = C.c_double()
+    ctx.check(ctx.lib.ttg_plus_norm_dev(ctx.h, len(n), i64(n), i64(rx), _ptr(dx), float(a), i64(ry), _ptr(dy),
+                                        float(b), C.byref(v)))
+    return float(v.value)
+
+
+def dot_dev(ctx, n, rx, dx, ry, dy):
+    """``dot(X, Y)`` (``@tt_tensor/dot.m:92-111``) of two device-resident TTs."""
+    v = C.c_double()
+    ctx.check(ctx.lib.ttg_dot_dev(ctx.h, len(n), i64(n), i64(rx), _ptr(dx), i64(ry), _ptr(dy), C.byref(v)))
+    return float(v.value)
+
+
+def _staged(*tts):
+    """Device copies (cuda:0) of host TT cores; torch's stream is drained so
+    that a context on its own stream sees them."""
+    import torch
+    out = [torch.from_numpy(t.core).to("cuda:0") for t in tts]
+    torch.cuda.synchronize()
+    return out
+
+
+def plus(x: TTTensor, y: TTTensor, a=1.0, b=1.0, ctx=None) -> TTTensor:
+    """``a*x + b*y`` (``@tt_tensor/plus.m:41-86``) on the GPU."""
+    ctx = ctx or default_context()
+    dx, dy = _staged(x, y)
+    rz, z = plus_dev(ctx, x.n, x.r, dx, y.r, dy, a, b)
+    return TTTensor(x.n.copy(), rz, z.cpu().numpy().copy())
+
+
+def plus_round(x: TTTensor, y: TTTensor, eps, a=1.0, b=1.0, rmax=None, precision="fp64", ctx=None,
+               info=None) -> TTTensor:
+    """``round(a*x + b*y, eps, rmax)`` on the GPU, the sum fused into the round."""
+    ctx = ctx or default_context()
+    dx, dy = _staged(x, y)
+    r, z = plus_round_dev(ctx, x.n, x.r, dx, y.r, dy, eps, a, b, rmax, precision, info=info)
+    return TTTensor(x.n.copy(), r, z.cpu().numpy().copy())
+
+
+def norm(x: TTTensor, ctx=None) -> float:
+    """``norm(x)`` (``@tt_tensor/norm.m:25-41``) on the GPU."""
+    ctx = ctx or default_context()
+    (dx,) = _staged(x)
+    return norm_dev(ctx, x.n, x.r, dx)
+
+
+def dot(x: TTTensor, y: TTTensor, ctx=None) -> float:
+    """``dot(x, y)`` (``@tt_tensor/dot.m:92-111``) on the GPU."""
+    ctx = ctx or default_context()
+    dx, dy = _staged(x, y)
+    return dot_dev(ctx, x.n, x.r, dx, y.r, dy)
+
+
+def rel_diff(y: TTTensor, yref: TTTensor, ctx=None) -> float:
+    """``norm(y - yref) / norm(yref)`` on the GPU (the sum is never stored)."""
+    ctx = ctx or default_context()
+    dy, dr = _staged(y, yref)
+    den = norm_dev(ctx, yref.n, yref.r, dr)
+    num = plus_norm_dev(ctx, y.n, y.r, dy, yref.r, dr, 1.0, -1.0)
+    return num / den if den > 0 else num
+
+
+# ---------------------------------------------------------------------------
+# TT-SVD (tt_tensor(full, eps), @tt_tensor/tt_tensor.m:175-244)
+# ---------------------------------------------------------------------------
+def full_to_tt_dev(ctx, n, dfull, eps=1e-14, rmax=None, info=None):
+    """TT-SVD of a column-major full tensor in device memory; returns (r, core view)."""
+    import torch
+    n = np.asarray(n, dtype=np.int64)
+    d = int(n.size)
+    r_out = (C.c_int64 * (d + 1))()
+    used = C.c_int64()
+    inf = TTGInfo()
+    cap = max(int(dfull.numel()), 1)
+    for _ in range(2):
+        out = torch.empty(cap, dtype=torch.float64, device=dfull.device)
+        s = ctx.lib.ttg_full_to_tt_dev(ctx.h, d, i64(n), _ptr(dfull), float(eps), _rmax_arg(rmax, d), r_out,
+                                       _ptr(out), cap, C.byref(used), C.byref(inf))
+        if s != _lib.TTG_NEED_SPACE:
+            break
+        cap = int(used.value)
+    ctx.check(s, allow=(TTG_OK, TTG_NO_CONVERGENCE))
+    if info is not None:
+        info.update(inf.as_dict())
+        info["status"] = s
+    return np.array(r_out[:], dtype=np.int64), out[:used.value]
+
+
+def tt_tensor_from_full(full, eps=1e-14, n=None, rmax=None, ctx=None, info=None) -> TTTensor:
+    """``tt_tensor(full, eps, n)`` on the GPU (``full`` a numpy array, column-major
+    order as MATLAB's ``b(:)``; ``n`` defaults to ``full.shape``)."""
+    import torch
+    ctx = ctx or default_context()
+    a = np.asarray(full, dtype=np.float64)
+    n = np.asarray(a.shape if n is None else n, dtype=np.int64).ravel()
+    flat = np.ascontiguousarray(a.reshape(-1, order="F"))
+    if flat.size != int(np.prod(n)):
+        raise ValueError("prod(n) must equal the number of entries")
+    d = torch.from_numpy(flat).to("cuda:0")
+    torch.cuda.synchronize()
+    r, core = full_to_tt_dev(ctx, n, d, eps, rmax, info)
+    return TTTensor(n.copy(), r, core.cpu().numpy().copy())
+
+
+# ---------------------------------------------------------------------------
+# SDV TT files (fmex/tt_read.c, fmex/tt_write.c; tt_tensor(filename), tt_save)
+# ---------------------------------------------------------------------------
+def _path(p):
+    import os
+    return os.fsencode(p)
+
+
+def sdv_info(path):
+    """(d, core length, dtype) of an SDV TT file (dtype 0 double, 2 float)."""
+    lib = _lib.load()
+    d, ln, dt = C.c_int64(), C.c_int64(), C.c_int()
+    s = lib.ttg_sdv_info(_path(path), C.byref(d), C.byref(ln), C.byref(dt))
+    if s != TTG_OK:
+        raise _lib.TTGError(s, f"ttg_sdv_info({path})")
+    return int(d.value), int(ln.value), int(dt.value)
+
+
+def tt_read(path) -> TTTensor:
+    """``tt_tensor(filename)`` (``@tt_tensor/tt_tensor.m:149-165``): host arrays."""
+    lib = _lib.load()
+    d, ln, _ = sdv_info(path)
+    n, r = (C.c_int64 * d)(), (C.c_int64 * (d + 1))()
+    core = np.empty(max(ln, 1), dtype=np.float64)
+    dd, used = C.c_int64(), C.c_int64()
+    s = lib.ttg_sdv_read(_path(path), d, C.byref(dd), n, r, core.ctypes.data_as(C.c_void_p), ln, C.byref(used))
+    if s != TTG_OK:
+        raise _lib.TTGError(s, f"ttg_sdv_read({path})")
+    return TTTensor(np.array(n[:], np.int64), np.array(r[:], np.int64), core[:used.value])
+
+
+def tt_save(path, tt: TTTensor, precision=0):
+    """``tt_save(filename, tt, precision)`` (``@tt_tensor/tt_save.m:22``): 0 double, 1 float."""
+    lib = _lib.load()
+    s = lib.ttg_sdv_write(_path(path), tt.d, i64(tt.n), i64(tt.r), tt.core.ctypes.data_as(C.c_void_p),
+                          int(precision))
+    if s != TTG_OK:
+        raise _lib.TTGError(s, f"ttg_sdv_write({path})")
+
+
+def tt_read_dev(ctx, path, device="cuda:0"):
+    """Read an SDV TT file straight into device memory; returns (n, r, dcore)."""
+    import torch
+    d, ln, _ = sdv_info(path)
+    n, r = (C.c_int64 * d)(), (C.c_int64 * (d + 1))()
+    out = torch.empty(max(ln, 1), dtype=torch.float64, device=device)
+    dd, used = C.c_int64(), C.c_int64()
+    ctx.check(ctx.lib.ttg_sdv_read_dev(ctx.h, _path(path), d, C.byref(dd), n, r, _ptr(out), int(out.numel()),
+                                       C.byref(used)))
+    return np.array(n[:], np.int64), np.array(r[:], np.int64), out[:used.value]
+
+
+def tt_save_dev(ctx, path, n, r, dcore, precision=0):
+    """Write a device-resident TT to an SDV TT file (0 double, 1 float)."""
+    ctx.check(ctx.lib.ttg_sdv_write_dev(ctx.h, _path(path), len(n), i64(n), i64(r), _ptr(dcore), int(precision)))
+
