@@ -47,6 +47,52 @@ def _rate(fn, reps, sync):
     return reps / (time.perf_counter() - t0)
 
 
+def round_flops(n, r_in, r_out):
+    """Algorithmic flops of round.m on a tensor with input ranks r_in and
+    output ranks r_out: sweep 1 QR (4mk^2 - 4k^3/3) and R * core_{i+1}
+    (2 k r n r'), sweep 2 SVD (6MN^2 + 20N^3) and core_{i-1} * U S."""
+    d = len(n)
+    n = [int(v) for v in n]
+    r = [int(v) for v in r_in]
+    rq = list(r)
+    for i in range(d - 1):
+        rq[i + 1] = min(rq[i] * n[i], rq[i + 1])
+    ro = [int(v) for v in r_out]
+    f = 0.0
+    for i in range(d - 1):
+        m_, k_ = rq[i] * n[i], rq[i + 1]
+        if m_ >= k_:
+            f += 4 * m_ * k_ * k_ - 4 * k_ ** 3 / 3
+        f += 2 * k_ * r[i + 1] * n[i + 1] * r[i + 2]
+    for i in range(d - 1, 0, -1):
+        M_, N_ = max(rq[i], n[i] * ro[i + 1]), min(rq[i], n[i] * ro[i + 1])
+        f += 6 * M_ * N_ ** 2 + 20 * N_ ** 3
+        f += 2 * rq[i - 1] * n[i - 1] * rq[i] * ro[i]
+    return f
+
+
+def roofline(work, seconds, bound="mfma", dtype="fp64", what="whole call"):
+    """Achieved rate of `work` (flops, or bytes for bound 'hbm') per call of
+    `seconds`, against the MI355X peak."""
+    if bound == "hbm":
+        a = work / seconds / 1e9
+        return {"bound": "hbm", "achieved": a, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
+                "frac": a * 1e9 / PEAK["hbm"], "traffic": None, "scope": what}
+    a = work / seconds / 1e12
+    return {"bound": "mfma", "achieved": a, "peak": PEAK[dtype] / 1e12, "unit": "TFLOP/s",
+            "frac": a * 1e12 / PEAK[dtype], "traffic": None, "scope": what}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def mtimes_leg(ttgpu, W, ctx, dev):
     """Config 2: A*x (d=12, n=m=4, rank_A 5, rank_x 64), unfused, and the fused
     round(A*x, 1e-6); inputs resident in HBM, host-timed back-to-back calls."""
@@ -65,15 +111,19 @@ def mtimes_leg(ttgpu, W, ctx, dev):
     byt = 8.0 * (ysize + acore.size + xcore.size)
     out["config2_mtimes"] = {"workload": "A*x, d=12 n=m=4 rank_A=5 rank_x=64 -> rank 320, FP64, unfused",
                              "contractions_per_s": rate, "hbm_bytes_per_contraction": byt,
-                             "achieved_GBps_incl_host": byt * rate / 1e9,
-                             "hbm_roofline_frac_incl_host": byt * rate / PEAK["hbm"]}
+                             "roofline": roofline(byt, 1.0 / rate, "hbm",
+                                                  what="whole call incl. host launch overhead")}
     o2 = torch.empty(ttgpu.capacity(na, ry), dtype=torch.float64, device=dev)
     rr = {}
     rate = _rate(lambda: rr.__setitem__("r", ttgpu.mtimes_round_dev(ctx, na, ma, ra, da, rx, dx, 1e-6, out=o2)[0]),
                  5, sync)
+    d2 = len(na)
+    cflops = sum(2.0 * ma[k] * ry[k] * na[k] * ry[k + 1] for k in range(d2))  # product cores
+    fl = round_flops(na, ry, rr["r"]) + cflops
     out["config2_fused_mtimes_round"] = {
         "workload": "round(A*x, 1e-6) fused (product cores built on chip), FP64",
-        "rounds_per_s": rate, "ranks_out": [int(v) for v in rr["r"]]}
+        "rounds_per_s": rate, "ranks_out": [int(v) for v in rr["r"]],
+        "algorithmic_gflop": fl / 1e9, "roofline": roofline(fl, 1.0 / rate)}
     return out
 
 
@@ -92,8 +142,16 @@ def other_configs(ttgpu, W, ctx, dev):
     o1 = torch.empty(ttgpu.capacity(n1, r1), dtype=torch.float64, device=dev)
     ro = {}
     rate = _rate(lambda: ro.__setitem__("r", ttgpu.round_dev(ctx, n1, r1, d1, c1["eps"], out=o1)[0]), 20, sync)
+    fl1 = round_flops(n1, r1, ro["r"])
     out["config1_round"] = {"workload": "round(X + 0.5 X, 1e-10), d=20 n=4 r=40, FP64", "rounds_per_s": rate,
-                            "ranks_out_max": int(max(ro["r"]))}
+                            "ranks_out_max": int(max(ro["r"])), "algorithmic_gflop": fl1 / 1e9,
+                            "roofline": roofline(fl1, 1.0 / rate)}
+    # the same workload from its two summands: a X + b Y fused into the round
+    nx1, rx1, cx1 = c1["x"]
+    dx1 = torch.from_numpy(cx1).to(dev)
+    rate = _rate(lambda: ttgpu.plus_round_dev(ctx, nx1, rx1, dx1, rx1, dx1, c1["eps"], 1.0, 0.5, out=o1), 20, sync)
+    out["config1_plus_round"] = {"workload": "round(X + 0.5 X, 1e-10) from X (d=20 n=4 r=20), sum fused, FP64",
+                                 "rounds_per_s": rate, "roofline": roofline(fl1, 1.0 / rate)}
     out.update(mtimes_leg(ttgpu, W, ctx, dev))
     # config 5: completion loss + gradient, 65,536 samples, d=16 n=8, model rank 32
     c5 = W.completion5()
@@ -106,9 +164,14 @@ def other_configs(ttgpu, W, ctx, dev):
         def step():
             loss = AG.completion_loss(core, n5, r5, idx, vals)
             torch.autograd.grad(loss, core)
+        rate5 = _rate(step, 20, sync)
+        S = int(c5["idx"].shape[0])
+        # per sample: left chain, right chain and the gradient outer products, 2 r_k r_{k+1} flops each
+        fl5 = 6.0 * S * sum(int(r5[k]) * int(r5[k + 1]) for k in range(len(n5)))
         out[f"config5_completion_grad_{name}"] = {
             "workload": "loss 1/2||P(X) - y||^2 and all core gradients, 65,536 samples, d=16 n=8 rank 32",
-            "evaluations_per_s": _rate(step, 20, sync)}
+            "evaluations_per_s": rate5, "algorithmic_gflop": fl5 / 1e9,
+            "roofline": roofline(fl5, 1.0 / rate5, dtype=name)}
     # <y, A x> and its gradients at config-1 shapes
     cb = W.bilinear1()
     nb, mb, rab, acb = cb["a"]
@@ -170,6 +233,81 @@ def verify_batch(tens, out, eps, oracle_count=2):
             "oracle_tensors": oracle_count, "max_rel_diff_vs_oracle": worst_ref}
 
 
+def extra_legs(ttgpu, W, ctx, dev, c, dcore, outs):
+    """On the d=32, n=8, r=256 tensor: the truncating round (eps 0.1; at
+    1e-3 and 1e-2 every singular value of this tensor clears the threshold),
+    the round of the sum from its two rank-128 summands (sum fused), norm, dot
+    and the difference norm of the two forms' results; and the TT-SVD of a
+    16.7M-entry full tensor.  Host-timed back-to-back calls, inputs in HBM."""
+    import numpy as np
+    import torch
+    sync = torch.cuda.synchronize
+    n, r, _ = c["x"]
+    out = {}
+    o = torch.empty(ttgpu.capacity(n, r), dtype=torch.float64, device=dev)
+    for prec in ("fp32", "fp64"):
+        rr, info = {}, {}
+
+        def trunc():
+            rr["r"] = ttgpu.round_dev(ctx, n, r, dcore, 0.1, precision=prec, out=o, info=info)[0]
+        rate = _rate(trunc, 2, sync)
+        fl = round_flops(n, r, rr["r"])
+        out[f"truncating_round_{prec}"] = {
+            "workload": "round d=32 n=8 r=256 eps=0.1 (bonds truncate: QR/SVD sweeps, Jacobi SVD at every bond)",
+            "rounds_per_s": rate, "ranks_out_max": int(max(rr["r"])), "path": info.get("path"),
+            "jacobi_sweeps": info.get("jacobi_sweeps"), "algorithmic_gflop": fl / 1e9,
+            "roofline": roofline(fl, 1.0 / rate)}
+        log(f"truncating round {prec}: {rate:.2f}/s, ranks max {max(rr['r'])}")
+    na, ra, ca = c["xa"]
+    nb, rb, cb = c["xb"]
+    da = torch.from_numpy(ca).to(dev)
+    db = torch.from_numpy(cb).to(dev)
+    sync()
+    for prec in ("fp32", "fp64"):
+        info = {}
+        rate = _rate(lambda: ttgpu.plus_round_dev(ctx, na, ra, da, rb, db, c["eps"], precision=prec, out=o,
+                                                  info=info), 5, sync)
+        out[f"plus_round_{prec}"] = {"workload": "round(Xa + Xb, 1e-6) from the two rank-128 summands "
+                                                 "(sum assembled on the device, then the round)",
+                                     "rounds_per_s": rate, "path": info.get("path")}
+    nrm = {}
+    rate = _rate(lambda: nrm.__setitem__("v", ttgpu.norm_dev(ctx, n, r, dcore)), 5, sync)
+    out["norm"] = {"workload": "norm(X), d=32 n=8 r=256 (left QR sweep, norm.m)", "per_s": rate}
+    rate = _rate(lambda: ttgpu.dot_dev(ctx, n, r, dcore, r, dcore), 5, sync)
+    qk = sum(2.0 * int(r[k]) * int(n[k]) * int(r[k + 1]) * (int(r[k]) + int(r[k + 1])) for k in range(len(n)))
+    out["dot"] = {"workload": "dot(X, X), d=32 n=8 r=256 (interface chain, dot.m)", "per_s": rate,
+                  "roofline": roofline(qk, 1.0 / rate)}
+    (rm, ym), (rp, yp) = outs
+    dn = {}
+    rate = _rate(lambda: dn.__setitem__("v", ttgpu.plus_norm_dev(ctx, n, rm, ym, rp, yp)), 3, sync)
+    out["difference_norm"] = {"workload": "norm(Y_mixed - Y_plain) of the two rounded results (rank 256 + 256, "
+                                          "QR sweep over the fused difference)",
+                              "per_s": rate, "value_relative": dn["v"] / nrm["v"]}
+    # TT-SVD: a 4^12 tensor (16.7M entries) of TT rank 6 plus 1e-9 dense noise, eps 1e-6
+    rng = np.random.default_rng(12)
+    nn = [4] * 12
+    cores = W.rand_cores(rng, nn, [1, 4] + [6] * 9 + [4, 1])
+    full = cores[0][0]  # (n_1, r_1); rows in column-major order of the indices so far
+    for k in range(1, 12):
+        rk, nk, rk1 = cores[k].shape
+        t = (full @ cores[k].reshape(rk, nk * rk1)).reshape(-1, nk, rk1)
+        full = t.transpose(1, 0, 2).reshape(-1, rk1)  # index prev + N_prev * i_k
+    full = np.ascontiguousarray(full[:, 0])
+    full = full + 1e-9 * np.linalg.norm(full) / np.sqrt(full.size) * rng.standard_normal(full.size)
+    dfull = torch.from_numpy(full).to(dev)
+    sync()
+    rs = {}
+    rate = _rate(lambda: rs.__setitem__("r", ttgpu.full_to_tt_dev(ctx, nn, dfull, 1e-6)[0]), 3, sync)
+    out["tt_svd"] = {"workload": "TT-SVD of a 4^12 full tensor (16.7M entries, TT rank 6 + 1e-9 noise), eps 1e-6",
+                     "per_s": rate, "ranks_out": [int(v) for v in rs["r"]],
+                     "roofline": roofline(8.0 * full.size, 1.0 / rate, "hbm",
+                                          what="whole call vs one read of the full tensor")}
+    out["_tt_svd_input"] = (nn, full)
+    del dfull, da, db, o
+    torch.cuda.empty_cache()
+    return out
+
+
 def batched_leg(ttgpu, W, ctx, dev, count):
     """Config 4 recipe: one call for the whole batch (the first call on a context
     also tunes the execution) against a per-tensor loop of round_dev on one stream
@@ -193,6 +331,7 @@ def batched_leg(ttgpu, W, ctx, dev, count):
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     check = verify_batch(tens, [(np.asarray(r_t), y_t.cpu().numpy()) for r_t, y_t in res], bb["eps"])
+    batch_flops = sum(round_flops(n_, r_, r_t) for n_, r_, (r_t, _) in zip(ns, rs, res))
     del res
     log("batched output check:", check)
     m = min(32, len(tens))
@@ -219,7 +358,9 @@ def batched_leg(ttgpu, W, ctx, dev, count):
         "tensors_per_s": len(tens) / (t2 - t1) if check["ok"] else None, "seconds": t2 - t1, "first_call_seconds_incl_tuning": t1 - t0,
         "per_tensor_loop_tensors_per_s": len(tens) / loop_est,
         "per_tensor_loop_sample": f"first {m} tensors timed, scaled by QR work to the batch",
-        "speedup_vs_loop": loop_est / (t2 - t1), "jacobi_sweeps": binfo.get("jacobi_sweeps")}
+        "speedup_vs_loop": loop_est / (t2 - t1), "jacobi_sweeps": binfo.get("jacobi_sweeps"),
+        "algorithmic_gflop_per_batch": batch_flops / 1e9,
+        "roofline": roofline(batch_flops, t2 - t1, what="whole batched call (one persistent launch)")}
 
 
 def main():
@@ -369,27 +510,9 @@ def main():
         del os.environ["TTG_SVD_ALWAYS"]
     log("SVD at every bond:", full_svd)
 
-    # algorithmic work per round at the actual ranks (QR + apply + SVD + apply)
-    d = len(n)
-    rq = list(map(int, r))
-    for i in range(d - 1):
-        rq[i + 1] = min(rq[i] * int(n[i]), rq[i + 1])
+    # algorithmic work per round at the actual ranks (round.m: QR + apply + SVD + apply)
     ro = list(map(int, r_mixed))
-    flops = 0.0
-    byt = 0.0
-    for i in range(d - 1):
-        m_, k_ = rq[i] * int(n[i]), rq[i + 1]
-        if m_ >= k_:
-            flops += 4 * m_ * k_ * k_ - 4 * k_ ** 3 / 3
-        flops += 2 * k_ * r[i + 1] * n[i + 1] * r[i + 2] if i + 2 <= d else 0
-    for i in range(d - 1, 0, -1):
-        M_, N_ = max(rq[i], int(n[i]) * ro[i + 1]), min(rq[i], int(n[i]) * ro[i + 1])
-        flops += 6 * M_ * N_ ** 2 + 20 * N_ ** 3
-        flops += 2 * rq[i - 1] * int(n[i - 1]) * rq[i] * ro[i]
-    size_in = sum(int(r[i]) * int(n[i]) * int(r[i + 1]) for i in range(d))
-    size_q = sum(rq[i] * int(n[i]) * rq[i + 1] for i in range(d))
-    size_out = sum(ro[i] * int(n[i]) * ro[i + 1] for i in range(d))
-    byt = 8.0 * (size_in + 2 * size_q + size_out)  # FP64 storage in both forms
+    flops = round_flops(n, r, ro)
 
     result = {
         "metric": "TT rounds/sec d=32 n=8 r=256: mixed vs FP64 on GPU at rel err <=1e-6",
@@ -437,6 +560,16 @@ def main():
                           "unit": unit, "frac": achieved / peak, "traffic": traffic,
                           "launches_per_round": prof["launches"] / args.steps, "avg_launch_ms": avg_ms}
 
+    # truncating round, the sum fused into the round, norm / dot / difference
+    # norm and TT-SVD (one GPU, inputs in HBM)
+    extra = {}
+    if not args.no_other:
+        ym_d = torch.from_numpy(y_mixed_host).to(dev)
+        yp_d = torch.from_numpy(y_plain_host).to(dev)
+        extra = extra_legs(ttgpu, W, ctx, dev, c, dcore, ((r_mixed, ym_d), (r_plain, yp_d)))
+        del ym_d, yp_d
+        log("extra legs:", {k: v for k, v in extra.items() if not k.startswith("_")})
+
     # batched rounding of uneven tensors (config-4 recipe)
     if not args.no_batched:
         del dcore, out_mixed, out_plain
@@ -447,6 +580,7 @@ def main():
     # the other configurations at one GPU: rounds/s, contractions/s, gradient evaluations/s
     if not args.no_other:
         result["other_configs"] = other_configs(ttgpu, W, ctx, dev)
+        result["other_configs"].update({k: v for k, v in extra.items() if not k.startswith("_")})
         log("other configs:", result["other_configs"])
 
     # parity against the FP64 oracle (CPU) + CPU baseline timing
@@ -466,9 +600,23 @@ def main():
             times.append(time.perf_counter() - t0)
             log(f"cpu oracle round {i + 1}: {times[-1]:.2f} s")
         med = sorted(times)[len(times) // 2]
+        # the same round on one host thread (BLAS/LAPACK limited to one thread)
+        one = None
+        try:
+            from threadpoolctl import threadpool_limits
+            with threadpool_limits(limits=1):
+                t0 = time.perf_counter()
+                O.tt_round(X, eps)
+                one = 1.0 / (time.perf_counter() - t0)
+            log(f"cpu oracle round, 1 thread: {1.0 / one:.2f} s")
+        except Exception as e:  # threadpoolctl missing: report it
+            log("one-thread baseline skipped:", e)
         result["cpu_baseline"] = {"value": 1.0 / med, "unit": "rounds/s", "cores": int(cores), "kind": "port",
                                   "sample": f"{args.cpu_rounds} full rounds of the same d=32 n=8 r=256 tensor "
-                                            f"(numpy/LAPACK restatement of round.m), median"}
+                                            f"(numpy/LAPACK restatement of round.m), median",
+                                  "one_thread": {"value": one, "unit": "rounds/s", "cores": 1,
+                                                 "sample": "1 full round, BLAS/LAPACK limited to one thread"},
+                                  "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
         ym = O.TT(n, r_mixed, y_mixed_host)
         yp = O.TT(n, r_plain, y_plain_host)
         result["rel_err_mixed_vs_oracle"] = O.tt_rel_diff(ym, ref)
@@ -496,6 +644,20 @@ def main():
                 "config1_rounds_per_s": 5 / (t1 - t0),
                 "config2_mtimes_per_s": 1 / (t3 - t2),
                 "config2_mtimes_then_round_per_s": 1 / (t4 - t2)}
+            try:
+                from threadpoolctl import threadpool_limits
+                with threadpool_limits(limits=1):
+                    t0 = time.perf_counter()
+                    for _ in range(5):
+                        O.tt_round(Z, c1["eps"])
+                    result["cpu_baseline_other"]["config1_rounds_per_s_one_thread"] = 5 / (time.perf_counter() - t0)
+            except Exception as e:
+                log("one-thread config-1 baseline skipped:", e)
+            if "_tt_svd_input" in extra:
+                nn, full = extra["_tt_svd_input"]
+                t0 = time.perf_counter()
+                O.tt_from_full(full, 1e-6, nn)
+                result["cpu_baseline_other"]["tt_svd_per_s"] = 1 / (time.perf_counter() - t0)
     print(json.dumps(result), flush=True)
 
 

@@ -122,13 +122,15 @@ ttg_status ttg_mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const
 
 /* Batched rounding of `count` independent TT tensors (uneven d, n, r).
  * Tensor t: d[t], n/r at offsets nofs[t]/rofs[t] of the packed n/r arrays,
- * cores at device offset cofs[t] of dcore.  Outputs: r_out packed like r,
- * cores at device offset cofs_out[t] of dcore_out (capacity from
- * ttg_round_capacity per tensor).  Replaces a MATLAB loop of round() calls. */
+ * cores at device offset cofs[t] of dcore.  rmax: NULL (no cap) or packed like
+ * r (rmax[rofs[t] + i] caps r_i of tensor t, round.m:78).  Outputs: r_out
+ * packed like r, cores at device offset cofs_out[t] of dcore_out (capacity
+ * from ttg_round_capacity per tensor).  Replaces a MATLAB loop of
+ * round(X{t}, eps, rmax{t}) calls. */
 ttg_status ttg_round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d,
                                  const int64_t *nofs, const int64_t *n, const int64_t *rofs,
                                  const int64_t *r, const int64_t *cofs, const double *dcore,
-                                 double eps, int prec, int64_t *r_out,
+                                 double eps, const int64_t *rmax, int prec, int64_t *r_out,
                                  const int64_t *cofs_out, double *dcore_out, ttg_info *info);
 
 /* Z = a*X + b*Y — replaces @tt_tensor/plus.m:41-86 (rank concatenation: X's cores in the

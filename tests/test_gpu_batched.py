@@ -98,3 +98,34 @@ def test_batched_mixed_precision_uses_streams(gpu_ctx):
         ref = O.tt_round(t, eps)
         y = O.TT(t.n, r_t, y_t.cpu().numpy())
         assert O.tt_rel_diff(y, ref) <= 1e-6
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+def test_batched_rmax(gpu_ctx, prec, tol):
+    """rmax per tensor (round.m:78): tensors whose cap binds go to the stream
+    workers, the rest keep the batched launch; every result equals
+    round(X{t}, eps, rmax{t}) of the oracle (ranks identical)."""
+    batch = O.make_batch(count=40, seed=91, dmin=3, dmax=8, nset=(2, 4, 8), rmin=4, rmax=20)
+    eps = 1e-10
+    caps = [3 if i % 3 == 0 else (100 if i % 3 == 1 else 6) for i in range(len(batch))]
+    rms = [np.full(t.d + 1, c, dtype=np.int64) for t, c in zip(batch, caps)]
+    cores = [t.core for t in batch]
+    cofs = np.concatenate([[0], np.cumsum([c.size for c in cores])])
+    dcore = torch.tensor(np.concatenate(cores), device="cuda:0")
+    torch.cuda.synchronize()
+    res = ttgpu.round_batched_dev(gpu_ctx, [t.n for t in batch], [t.r for t in batch], dcore, cofs[:-1], eps,
+                                  precision=prec, rmax=rms)
+    binding = 0
+    for i, (t, (r_t, y_t)) in enumerate(zip(batch, res)):
+        ref = O.tt_round(t, eps, rms[i])
+        assert list(map(int, r_t)) == ref.r.tolist(), f"tensor {i}"
+        assert max(r_t) <= caps[i]
+        binding += int(max(O.tt_round(t, eps).r) > caps[i])
+        y = O.TT(t.n, r_t, y_t.cpu().numpy())
+        assert O.tt_rel_diff(y, ref) <= tol, f"tensor {i}"
+    assert binding >= 10
+    # a scalar cap for the whole batch
+    res2 = ttgpu.round_batched_dev(gpu_ctx, [t.n for t in batch], [t.r for t in batch], dcore, cofs[:-1], eps,
+                                   precision=prec, rmax=5)
+    for i, (t, (r_t, y_t)) in enumerate(zip(batch, res2)):
+        assert list(map(int, r_t)) == O.tt_round(t, eps, 5).r.tolist(), f"tensor {i}"

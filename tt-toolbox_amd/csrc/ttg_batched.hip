@@ -40,6 +40,7 @@ static ttg_ctx *make_worker(const ttg_ctx *parent) {
   ttg_ctx *w = new ttg_ctx();
   w->device = parent->device;
   w->num_cus = parent->num_cus;
+  w->worker = true;
   if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
     delete w;
     throw Error(TTG_HIP_ERROR, "cannot create a worker stream");
@@ -96,6 +97,7 @@ struct BatchArgs {
   int64_t *r_out;
   double eps;
   int prec;
+  const int64_t *rmax = nullptr;  // null, or packed like r
 };
 
 // Rounds tensors order[begin, end) on `nw` workers; per-tensor status and info.
@@ -115,7 +117,7 @@ static void run_batch(ttg_ctx *ctx, const BatchArgs &a, const std::vector<int64_
       try {
         round_capacity(a.d[t], a.n + a.nofs[t], a.r + a.rofs[t], &cap);
         st[t] = round_dev(w, a.d[t], a.n + a.nofs[t], a.r + a.rofs[t], a.dcore + a.cofs[t], a.eps,
-                          nullptr, a.prec, a.r_out + a.rofs[t], a.dcore_out + a.cofs_out[t], cap,
+                          a.rmax ? a.rmax + a.rofs[t] : nullptr, a.prec, a.r_out + a.rofs[t], a.dcore_out + a.cofs_out[t], cap,
                           &used, &inf[t]);
       } catch (const Error &e) {
         st[t] = e.status;
@@ -204,7 +206,11 @@ static void run_persist(ttg_ctx *ctx, const BatchArgs &a, int64_t count, const s
                                      (uint64_t)(uintptr_t)dws, (uint64_t)slot_core, (uint64_t)slot_kmax,
                                      (uint64_t)sh.ycap, (uint64_t)kSweeps, (uint64_t)std::hash<double>{}(a.eps),
                                      (uint64_t)(uintptr_t)ctx->parena.base};
-  if (!ctx->pgraph || ctx->pgraph_key != key) {
+  if (ctx->worker) {
+    // on a stream worker: eager launch (no capture beside the other workers' allocations)
+    persist_launch(s, sh.nt, slots, (int)m, ddesc, dorder, dn, dr, a.dcore, a.dcore_out, drout, dws, slot_core,
+                   slot_kmax, dctr, dstat, a.eps, kSweeps, sh.ycap);
+  } else if (!ctx->pgraph || ctx->pgraph_key != key) {
     if (ctx->pgraph) {
       TTG_HIP(hipGraphExecDestroy(ctx->pgraph));
       ctx->pgraph = nullptr;
@@ -227,7 +233,7 @@ static void run_persist(ttg_ctx *ctx, const BatchArgs &a, int64_t count, const s
     ctx->pgraph = ge;
     ctx->pgraph_key = key;
   }
-  TTG_HIP(hipGraphLaunch(ctx->pgraph, s));
+  if (!ctx->worker) TTG_HIP(hipGraphLaunch(ctx->pgraph, s));
   std::vector<int> hstat(4 * count);
   TTG_HIP(hipMemcpyAsync(a.r_out, drout, rtot * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   TTG_HIP(hipMemcpyAsync(hstat.data(), dstat, 4 * count * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -295,8 +301,8 @@ bool round_small_persist(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_
 
 ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, const int64_t *nofs,
                              const int64_t *n, const int64_t *rofs, const int64_t *r,
-                             const int64_t *cofs, const double *dcore, double eps, int prec,
-                             int64_t *r_out, const int64_t *cofs_out, double *dcore_out,
+                             const int64_t *cofs, const double *dcore, double eps, const int64_t *rmax,
+                             int prec, int64_t *r_out, const int64_t *cofs_out, double *dcore_out,
                              ttg_info *info) {
   TTG_CHECK(count >= 0, TTG_BAD_ARG, "count must be >= 0");
   TTG_CHECK(count == 0 || (d && nofs && n && rofs && r && cofs && dcore && r_out && cofs_out && dcore_out),
@@ -311,6 +317,9 @@ ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, cons
     ttg_status s = round_capacity(d[t], n + nofs[t], r + rofs[t], &cap);
     if (s != TTG_OK) return s;
     const int64_t *nn = n + nofs[t], *rr = r + rofs[t];
+    if (rmax)
+      for (int64_t i = 1; i < d[t]; ++i)
+        TTG_CHECK(rmax[rofs[t] + i] >= 1, TTG_BAD_ARG, "rmax entries must be >= 1");
     double c = 0.0, rk = (double)rr[0];
     for (int64_t i = 0; i + 1 < d[t]; ++i) {
       const double m = rk * (double)nn[i], k = std::min(m, (double)rr[i + 1]);
@@ -324,14 +333,41 @@ ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, cons
   std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
   // the inputs come from the caller's stream
   TTG_HIP(hipStreamSynchronize(ctx->stream));
-  const BatchArgs a{d, nofs, n, rofs, r, cofs, cofs_out, dcore, dcore_out, r_out, eps, prec};
+  BatchArgs a{d, nofs, n, rofs, r, cofs, cofs_out, dcore, dcore_out, r_out, eps, prec};
+  a.rmax = rmax;
   std::vector<ttg_status> st(count, TTG_OK);
   std::vector<ttg_info> inf(count, ttg_info{0, 0, 0, 0.0});
   std::vector<std::string> err(count);
+  // tensors whose rmax binds (an entry below the left-capped rank,
+  // round.m:48) go to the stream workers: the one-workgroup kernel has no
+  // rank cap.  The rest keep the LPT order.
+  std::vector<int64_t> order_w;
+  if (rmax) {
+    std::vector<int64_t> keep;
+    for (int64_t t : order) {
+      bool binds = false;
+      int64_t rk = r[rofs[t]];
+      for (int64_t i = 0; i + 1 < d[t] && !binds; ++i) {
+        rk = std::min(rk * n[nofs[t] + i], r[rofs[t] + i + 1]);
+        binds = rmax[rofs[t] + i + 1] < rk;
+      }
+      (binds ? order_w : keep).push_back(t);
+    }
+    order.swap(keep);
+  }
   // persistent launch for FP64 batches that fit one workgroup per tensor
-  bool persist_ok = (prec == TTG_PREC_FP64);
-  for (int64_t t = 0; t < count && persist_ok; ++t)
+  bool persist_ok = (prec == TTG_PREC_FP64) && !order.empty();
+  for (int64_t t : order) {
+    if (!persist_ok) break;
     persist_ok = persist_supported(d[t], n + nofs[t], r + rofs[t]);
+  }
+  if (!persist_ok) {  // everything goes to the stream workers, in LPT order
+    order.insert(order.end(), order_w.begin(), order_w.end());
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
+    order_w.swap(order);
+    order.clear();
+  }
+  const int64_t np = (int64_t)order.size(), nw = (int64_t)order_w.size();
   if (persist_ok) {
     auto work = [&](const std::vector<int64_t> &sel) {
       double c = 0.0;
@@ -344,12 +380,12 @@ ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, cons
       const char *env = std::getenv("TTG_BATCH_MODE");
       if (env && env[0] >= '1' && env[0] <= '4' && env[1] == '\0') ctx->batch_mode = env[0] - '0';
     }
-    if (ctx->batch_mode == 0 && count >= 64) {
+    if (ctx->batch_mode == 0 && np >= 64) {
       // tune on interleaved slices of the LPT order (every tensor is rounded
       // exactly once): the three persistent shapes on an eighth each, the
       // stream workers on a thirty-second, the rest with the fastest
       std::vector<std::vector<int64_t>> sl(5);
-      for (int64_t j = 0; j < count; ++j) {
+      for (int64_t j = 0; j < np; ++j) {
         const int64_t c8 = j % 8;
         sl[c8 < 3 ? c8 : (j % 32 == 3 ? 3 : 4)].push_back(order[j]);
       }
@@ -375,23 +411,23 @@ ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, cons
       if (mode <= 3) {
         run_persist(ctx, a, count, order, kPShapes[mode - 1], st, inf, err);
       } else {
-        run_batch(ctx, a, order, 0, count, (int)std::min<int64_t>(16, count), st, inf, err);
+        run_batch(ctx, a, order, 0, np, (int)std::min<int64_t>(16, np), st, inf, err);
       }
     }
   }
-  int64_t begin = persist_ok ? count : 0;
-  if (!persist_ok && ctx->batch_workers == 0) {
+  int64_t begin = 0;
+  if (nw > 0 && ctx->batch_workers == 0) {
     // tune the worker count on consecutive slices of the (cost-ordered) batch
     const int cand[3] = {8, 16, 32};
-    const int64_t slice = count / 8;
+    const int64_t slice = nw / 8;
     if (slice >= 4) {
       double best = 1e300;
       int bestw = 16;
-      for (int ci = 0; ci < 3 && begin + slice <= count; ++ci) {
+      for (int ci = 0; ci < 3 && begin + slice <= nw; ++ci) {
         auto s0 = std::chrono::steady_clock::now();
-        run_batch(ctx, a, order, begin, begin + slice, cand[ci], st, inf, err);
+        run_batch(ctx, a, order_w, begin, begin + slice, cand[ci], st, inf, err);
         double cs = 0.0;
-        for (int64_t j = begin; j < begin + slice; ++j) cs += cost[order[j]];
+        for (int64_t j = begin; j < begin + slice; ++j) cs += cost[order_w[j]];
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count();
         const double rate = sec / cs;  // seconds per unit of estimated work
         if (rate < best) {
@@ -405,9 +441,8 @@ ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, cons
       ctx->batch_workers = 16;
     }
   }
-  if (begin < count)
-    run_batch(ctx, a, order, begin, count, (int)std::min<int64_t>(ctx->batch_workers, count - begin), st,
-              inf, err);
+  if (begin < nw)
+    run_batch(ctx, a, order_w, begin, nw, (int)std::min<int64_t>(ctx->batch_workers, nw - begin), st, inf, err);
   ttg_status worst = TTG_OK;
   ttg_info acc{0, 0, 0, 0.0};
   for (int64_t t = 0; t < count; ++t) {

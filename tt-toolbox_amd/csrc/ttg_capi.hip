@@ -23,8 +23,8 @@ ttg_status mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int
                             ttg_info *info);
 ttg_status round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d, const int64_t *nofs,
                              const int64_t *n, const int64_t *rofs, const int64_t *r,
-                             const int64_t *cofs, const double *dcore, double eps, int prec,
-                             int64_t *r_out, const int64_t *cofs_out, double *dcore_out,
+                             const int64_t *cofs, const double *dcore, double eps, const int64_t *rmax,
+                             int prec, int64_t *r_out, const int64_t *cofs_out, double *dcore_out,
                              ttg_info *info);
 ttg_status plus_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx, const double *dx, double a,
                     const int64_t *ry, const double *dy, double b, int64_t *rz_out, double *dz, int64_t cap,
@@ -398,7 +398,7 @@ ttg_status ttg_mtimes_round(ttg_ctx *ctx, int64_t d, const int64_t *n, const int
 ttg_status ttg_round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d,
                                  const int64_t *nofs, const int64_t *n, const int64_t *rofs,
                                  const int64_t *r, const int64_t *cofs, const double *dcore,
-                                 double eps, int prec, int64_t *r_out,
+                                 double eps, const int64_t *rmax, int prec, int64_t *r_out,
                                  const int64_t *cofs_out, double *dcore_out, ttg_info *info) {
   if (!ctx) return TTG_BAD_ARG;
   TTG_GUARD_BEGIN
@@ -406,7 +406,7 @@ ttg_status ttg_round_batched_dev(ttg_ctx *ctx, int64_t count, const int64_t *d,
             TTG_BAD_ARG, "null pointer argument");
   TTG_HIP(hipSetDevice(ctx->device));
   ctx->last_error.clear();
-  return round_batched_dev(ctx, count, d, nofs, n, rofs, r, cofs, dcore, eps, prec, r_out, cofs_out,
+  return round_batched_dev(ctx, count, d, nofs, n, rofs, r, cofs, dcore, eps, rmax, prec, r_out, cofs_out,
                            dcore_out, info);
   TTG_GUARD_END(ctx)
 }

@@ -73,6 +73,7 @@ constexpr int TRS = CB + 1;  // odd LDS row stride of staged 32-row blocks
 // flag = 1-based index of the first non-positive pivot (0 = success).
 constexpr int CHOL_THREADS = 512;
 
+template <bool STAMP>
 __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__restrict__ G, long ldg,
                                                         double shift_rel, double *__restrict__ R,
                                                         int *flag, double *stats,
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__r
   for (int J0 = 0; J0 < k; J0 += CB) {
     const int jb = min(CB, k - J0);
     __syncthreads();
-    if (dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    if (STAMP && dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
     // stage the diagonal block column-major in Dl (identity-padded past jb)
     for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
       const int i = idx & (CB - 1), j = idx >> 5;
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__r
       if (lane == 0) sh_fail = failmask ? J0 + __builtin_ctz(failmask) + 1 : 0;
     }
     __syncthreads();
-    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
+    if (STAMP && dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
     if (sh_fail) break;
     // diagonal block to R (+ statistics)
     for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__r
       }
     }
     __syncthreads();
-    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
+    if (STAMP && dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
     for (int idx = tid; idx < CB * w; idx += CHOL_THREADS) {
       const int t = idx & (CB - 1), c = idx >> 5;
       if (t < jb) R[(J0 + t) + (long)(J0 + jb + c) * k] = Ps[t * wps + c];
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__r
     }
   }
   __syncthreads();
-  if (dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
+  if (STAMP && dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
   const int failed = sh_fail;
   dmax = block_max(dmax, red);
   dmin = -block_max(-dmin, red);
@@ -270,6 +271,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__r
 // the same order per column as k_chol3: identical factors.
 // One panel column per thread of waves 1..7: panels up to 448 columns (k <= 480).
 // ---------------------------------------------------------------------------
+template <bool STAMP>
 __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__restrict__ G, long ldg,
                                                         double shift_rel, double *__restrict__ R,
                                                         int *flag, double *stats,
@@ -296,7 +298,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__r
     const int w = k - J0 - jb;  // > 0 only when jb == CB
     const int wpad = (w + 15) & ~15, wps = wpad + 1;
     __syncthreads();
-    if (dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    if (STAMP && dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
     // diagonal block column-major in Dl (identity-padded past jb); panel rows in Ps
     for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
       const int i = idx & (CB - 1), j = idx >> 5;
@@ -387,7 +389,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__r
       }
     }
     __syncthreads();
-    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
+    if (STAMP && dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
     if (sh_fail) break;
     // diagonal block to R (+ statistics)
     for (int idx = tid; idx < CB * CB; idx += CHOL_THREADS) {
@@ -412,7 +414,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__r
       R[(J0 + t) + (long)(J0 + jb + c) * k] = val;
       dev = fmax(dev, fabs(val));
     }
-    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
+    if (STAMP && dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
     // trailing SYRK on the upper 16x16 tiles (as in k_chol3)
     const int nt = wpad >> 4;
     const int ntiles = nt * (nt + 1) / 2;
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__r
     }
   }
   __syncthreads();
-  if (dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
+  if (STAMP && dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
   const int failed = sh_fail;
   dmax = block_max(dmax, red);
   dmin = -block_max(-dmin, red);
@@ -573,7 +575,7 @@ __device__ __forceinline__ unsigned chol_pivot_chain(double *Dl, double *Wt, int
 // tile = u + 6q, row tile u & 1, column tile u / 2 + 3q (at most 28 tiles
 // for k <= 256; MAXU = 5 per wave).
 // ---------------------------------------------------------------------------
-template <int MAXT, int MAXU, int US>
+template <int MAXT, int MAXU, int US, bool STAMP>
 __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k_, const double *__restrict__ G_, long ldg_,
                                                         double shift_rel, double *__restrict__ R_,
                                                         int *flag_, double *stats_,
@@ -657,7 +659,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k_, const double *__
     int jb, w, wpad, ntiles;
     geom(J0, jb, w, wpad, ntiles);
     __syncthreads();  // the previous block's reads of Dl, Ps are done
-    if (dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
+    if (STAMP && dbg && tid == 0) dbg[3 * (J0 / CB)] = __builtin_amdgcn_s_memrealtime();
     // per-lane index math of this block from an opaque copy of the lane
     // index: left to LICM, the addresses of every tile would be hoisted out
     // of the block loop and spilled
@@ -777,7 +779,7 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k_, const double *__
       }
     }
     __syncthreads();
-    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
+    if (STAMP && dbg && tid == 0) dbg[3 * (J0 / CB) + 1] = __builtin_amdgcn_s_memrealtime();
     if (sh_fail) break;
     // block J's rows: the last chunk of block row J1's update, from the
     // panel in LDS (Ps columns 0.. are block row J1's columns); before the
@@ -826,10 +828,10 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol6(int k_, const double *__
       for (int t = wv; t < CB; t += NW)
         for (int c = lane; c < w; c += 64) R[(J0 + jb + c) + (long)(J0 + t) * k] = Ps[t * wps + c];
     }
-    if (dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
+    if (STAMP && dbg && tid == 0) dbg[3 * (J0 / CB) + 2] = __builtin_amdgcn_s_memrealtime();
   }
   __syncthreads();
-  if (dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
+  if (STAMP && dbg && tid == 0) dbg[3 * ((k + CB - 1) / CB)] = __builtin_amdgcn_s_memrealtime();
   // R's strict lower triangle is zero on return
   for (int j = wv; j < k; j += NW)
     for (int i = j + 1 + lane; i < k; i += 64) R[i + (long)j * k] = 0.0;
@@ -857,11 +859,13 @@ bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
   const size_t lds = ((size_t)CB * CB + (size_t)CB * wps) * sizeof(double);
   ProfScope prof(KC_CHOL, s, (double)k * k * k / 3.0, 8.0 * 2.0 * (double)k * k);
   if (((wmax + 15) & ~(int64_t)15) > CHOL_THREADS - 64) {  // panels beyond 448 columns
-    TTG_HIP(hipFuncSetAttribute((const void *)k_chol3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_chol3, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
+    auto fn = dbg ? k_chol3<true> : k_chol3<false>;
+    TTG_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(fn, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
   } else if (k > 256) {
-    TTG_HIP(hipFuncSetAttribute((const void *)k_chol4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_chol4, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
+    auto fn = dbg ? k_chol4<true> : k_chol4<false>;
+    TTG_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(fn, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
   } else {
     // k <= 256: at most 2 * 16 output tiles (4 per wave), 32 x 224 staged
     // elements (14 per thread), staging and panel row stride 225; Dl + Ps
@@ -869,9 +873,9 @@ bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
     // of the tiles past a block (up to index 7230)
     constexpr int MAXT = 4, MAXU = 5, US = 225;
     const size_t lds6 = std::max(lds, (size_t)(CB * CB + CB * US) * sizeof(double));
-    TTG_HIP(hipFuncSetAttribute((const void *)k_chol6<MAXT, MAXU, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds6));
-    hipLaunchKernelGGL((k_chol6<MAXT, MAXU, US>), dim3(1), dim3(CHOL_THREADS), lds6, s, (int)k, G, (long)ldg,
+    auto fn = dbg ? k_chol6<MAXT, MAXU, US, true> : k_chol6<MAXT, MAXU, US, false>;
+    TTG_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds6));
+    hipLaunchKernelGGL(fn, dim3(1), dim3(CHOL_THREADS), lds6, s, (int)k, G, (long)ldg,
                        shift_rel, R, flag, stats, dbg, Wout, fro, (const CholJob *)nullptr);
     wrote_w = true;
   }
@@ -889,9 +893,9 @@ void chol_batched(hipStream_t s, const CholJob *jobs_dev, int njobs, int64_t kma
   const size_t lds = ((size_t)CB * CB + (size_t)CB * wps) * sizeof(double);
   const size_t lds6 = std::max(lds, (size_t)(CB * CB + CB * US) * sizeof(double));
   ProfScope prof(KC_CHOL, s, (double)njobs * kmax * kmax * kmax / 3.0, 8.0 * 2.0 * njobs * kmax * kmax);
-  TTG_HIP(hipFuncSetAttribute((const void *)k_chol6<MAXT, MAXU, US>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds6));
-  hipLaunchKernelGGL((k_chol6<MAXT, MAXU, US>), dim3((unsigned)njobs), dim3(CHOL_THREADS), lds6, s, 0,
+  TTG_HIP(hipFuncSetAttribute((const void *)k_chol6<MAXT, MAXU, US, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds6));
+  hipLaunchKernelGGL((k_chol6<MAXT, MAXU, US, false>), dim3((unsigned)njobs), dim3(CHOL_THREADS), lds6, s, 0,
                      (const double *)nullptr, 0L, 0.0, (double *)nullptr, (int *)nullptr, (double *)nullptr,
                      (unsigned long long *)nullptr, (double *)nullptr, (double *)nullptr, jobs_dev);
   TTG_HIP(hipGetLastError());
@@ -900,7 +904,7 @@ void chol_batched(hipStream_t s, const CholJob *jobs_dev, int njobs, int64_t kma
 // ---------------------------------------------------------------------------
 // TRSM: X (m x k) = A R^{-1}
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, bool STAMP>
 __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict__ A, long lda,
                                               const double *__restrict__ R, T *__restrict__ X,
                                               long ldx, unsigned long long *dbg) {
@@ -917,7 +921,7 @@ __global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict_
   const int tr = wv & 1, tc = wv >> 1;
   const int r0 = blockIdx.x * CB;
   const int srow = tid >> 3, sq = tid & 7;  // substitution: 8 lanes per row
-  const bool rec = dbg && blockIdx.x == 0 && tid == 0;
+  const bool rec = STAMP && dbg && blockIdx.x == 0 && tid == 0;
   if (rec) dbg[0] = __builtin_amdgcn_s_memrealtime();
   for (int idx = tid; idx < CB * kp; idx += blockDim.x) {
     const int row = idx % CB, col = idx / CB;
@@ -1202,7 +1206,7 @@ __host__ __device__ constexpr size_t trsm2_lds(int kp, size_t tsz) {
   return (size_t)TRS * kp * tsz + (size_t)(kp > CB ? kp - CB : 0) * TRS * tsz + 2 * (size_t)CB * TRS * 8;
 }
 
-template <class T>
+template <class T, bool STAMP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_trsm2(int m, int k, const T *__restrict__ A, long lda,
                                                const double *__restrict__ R, T *__restrict__ X,
                                                long ldx, unsigned long long *dbg) {
@@ -1216,7 +1220,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int tr = wv & 1, tc = wv >> 1;
   const int r0 = blockIdx.x * CB;
-  const bool rec = dbg && blockIdx.x == 0 && tid == 0;
+  const bool rec = STAMP && dbg && blockIdx.x == 0 && tid == 0;
   if (rec) dbg[0] = __builtin_amdgcn_s_memrealtime();
   // stage A_J and R_JJ of block J0 with NTHR threads from thread FIRST on
   // (every load in flight before the first LDS store)
@@ -1371,16 +1375,18 @@ void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t l
   TTG_CHECK(A || kp <= 256, TTG_BAD_ARG, "identity right-hand side needs k <= 256");
   if (kp <= 256) {
     const size_t lds2 = trsm2_lds((int)kp, sizeof(T));
-    TTG_HIP(hipFuncSetAttribute((const void *)k_trsm2<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-    hipLaunchKernelGGL((k_trsm2<T>), dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds2, s, (int)m, (int)k,
+    auto fn = dbg ? k_trsm2<T, true> : k_trsm2<T, false>;
+    TTG_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    hipLaunchKernelGGL(fn, dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds2, s, (int)m, (int)k,
                        A, (long)lda, R, X, (long)ldx, dbg);
     TTG_HIP(hipGetLastError());
     return;
   }
   const size_t lds = ((size_t)CB * kp + (kp <= 256 ? kp : CB) * CB) * sizeof(T) + CB * CB * sizeof(double);
   TTG_CHECK(lds <= 160 * 1024, TTG_BAD_ARG, "rank too large for the triangular solve");
-  TTG_HIP(hipFuncSetAttribute((const void *)k_trsm<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL((k_trsm<T>), dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds, s, (int)m, (int)k,
+  auto fn = dbg ? k_trsm<T, true> : k_trsm<T, false>;
+  TTG_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(fn, dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds, s, (int)m, (int)k,
                      A, (long)lda, R, X, (long)ldx, dbg);
   TTG_HIP(hipGetLastError());
 }

@@ -485,7 +485,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   for (int64_t i = 0; i < d; ++i) key.push_back((uint64_t)n[i]);
   for (int64_t i = 0; i <= d; ++i) key.push_back((uint64_t)r[i]);
   const bool profiling = g_prof && g_prof->kind != KC_NONE;
-  bool use_graph = !profiling;
+  bool use_graph = !profiling && !ctx->worker;  // stream workers launch eagerly
   {
     const char *env = std::getenv("TTG_GRAPH");
     if (env && env[0] == '0') use_graph = false;

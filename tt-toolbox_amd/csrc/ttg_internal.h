@@ -140,6 +140,11 @@ struct ttg_ctx {
   // cores of an assembled sum a X + b Y (plus fused with round), kept apart
   // from the rounding workspaces that read them
   ttg::Arena sarena;
+  // a batched call's stream worker: runs on its own host thread beside other
+  // workers, whose workspace growth (hipMalloc / hipFree / hipHostMalloc /
+  // hipHostFree, which synchronise the device) must not meet a stream
+  // capture in progress, so a worker launches eagerly and never captures
+  bool worker = false;
 };
 
 namespace ttg {

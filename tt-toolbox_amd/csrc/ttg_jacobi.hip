@@ -129,7 +129,7 @@ static __host__ __device__ inline int jacobi_mc_offset(int max_sweeps) { return 
 // then one double: ||S||_F^2 (8-byte aligned: the offset is even)
 static __host__ __device__ inline int jacobi_fro_offset(int max_sweeps) { return jacobi_mc_offset(max_sweeps) + 2 * max_sweeps; }
 // then the J workers' barrier (2 words), the V publication flags (2 x 64, by
-// step parity) and their acknowledgements (64), see k_jacobi4<..., true>
+// step parity) and their acknowledgements (64), see k_jacobi4 with SJ = true
 static __host__ __device__ inline int jacobi_jw_offset(int max_sweeps) { return jacobi_fro_offset(max_sweeps) + 2; }
 constexpr int JW_MAXP = 64;
 int jacobi_bar_words(int max_sweeps) { return jacobi_jw_offset(max_sweeps) + 2 + 3 * JW_MAXP; }
@@ -253,7 +253,7 @@ __device__ __forceinline__ void jacobi_pair_mul(int k, int w, int nw, const doub
 //      only after the rounds.
 // Convergence: a sweep without rotations, or one none of whose rotations had
 // cos^2 > stop_cos2 (they leave cosines of O(cos^2) behind).
-template <int KL, int LC, int NT, bool SJ>
+template <int KL, int LC, int NT, bool SJ, bool STAMP>
 __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict__ Sg, long lds,
                                                 double *__restrict__ Jg, long ldj, double tol,
                                                 double stop_cos2, int max_sweeps, int *sweeps_out,
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
       for (int idx = tid; idx < LC * LC; idx += NT) Vl[idx] = ((idx % LC) == (idx / LC)) ? 1.0 : 0.0;
       if (tid == 0) sh_steprot = 0;
       __syncthreads();
-      const bool rec = dbg && sweep == 0 && w == 0 && tid == 0;
+      const bool rec = STAMP && dbg && sweep == 0 && w == 0 && tid == 0;
       if (rec) dbg[step * 4 + 0] = __builtin_amdgcn_s_memrealtime();
       // 1. Gram: wave j < NTILE * KS forms rows [(j % KS) KR, +KR) of upper tile
       //    j / KS = (ti, tj) (column order); partial tiles are summed in LDS
@@ -624,13 +624,17 @@ static void launch_jacobi_t(hipStream_t s, const JacobiPlan &pl, double *Sg, lon
   if (p > 1) {
     // J columns updated by p separate workgroups (split J) when V can be exchanged
     const bool sj = Vx != nullptr && p <= JW_MAXP;
-    const void *fn = sj ? (const void *)k_jacobi4<KL, LC, NT, true> : (const void *)k_jacobi4<KL, LC, NT, false>;
+    const void *fn = sj ? (dbg ? (const void *)k_jacobi4<KL, LC, NT, true, true>
+                               : (const void *)k_jacobi4<KL, LC, NT, true, false>)
+                        : (dbg ? (const void *)k_jacobi4<KL, LC, NT, false, true>
+                               : (const void *)k_jacobi4<KL, LC, NT, false, false>);
     TTG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
     void *args[] = {&k, &p, &Sg, &lds, &Jg, &ldj, &tol, &stop_cos2, &ms, &sweeps_dev, &bar, &err, &dbg, &Vx, &skip};
     TTG_HIP(hipLaunchCooperativeKernel(fn, dim3(sj ? 2 * p : p), dim3(NT), args, lsz, s));
   } else {
-    TTG_HIP(hipFuncSetAttribute((const void *)k_jacobi4<KL, LC, NT, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
-    hipLaunchKernelGGL((k_jacobi4<KL, LC, NT, false>), dim3(1), dim3(NT), lsz, s, k, p, Sg, lds, Jg, ldj, tol,
+    auto fn = dbg ? k_jacobi4<KL, LC, NT, false, true> : k_jacobi4<KL, LC, NT, false, false>;
+    TTG_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsz));
+    hipLaunchKernelGGL(fn, dim3(1), dim3(NT), lsz, s, k, p, Sg, lds, Jg, ldj, tol,
                        stop_cos2, ms, sweeps_dev, bar, err, dbg, Vx, skip);
     TTG_HIP(hipGetLastError());
   }

@@ -100,7 +100,8 @@ def load(path: Path | str | None = None):
         "ttg_mtimes_dev": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, I64P, P, I64, I64P]),
         "ttg_mtimes_round": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, C.c_double, I64P, C.c_int, I64P, P, I64, I64P, P]),
         "ttg_mtimes_round_dev": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, C.c_double, I64P, C.c_int, I64P, P, I64, I64P, P]),
-        "ttg_round_batched_dev": (st, [P, I64, I64P, I64P, I64P, I64P, I64P, I64P, P, C.c_double, C.c_int, I64P, I64P, P, P]),
+        "ttg_round_batched_dev": (st, [P, I64, I64P, I64P, I64P, I64P, I64P, I64P, P, C.c_double, I64P, C.c_int,
+                                       I64P, I64P, P, P]),
         "ttg_adjoint_completion_f64": (st, [P, I64, I64P, I64P, P, I64, P, P, DP, P]),
         "ttg_adjoint_completion_f32": (st, [P, I64, I64P, I64P, P, I64, P, P, DP, P]),
         "ttg_adjoint_bilinear_f64": (st, [P, I64, I64P, I64P, I64P, P, I64P, P, I64P, P, DP, P, P]),

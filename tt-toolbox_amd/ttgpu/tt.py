@@ -107,6 +107,15 @@ def _rmax_arg(rmax, d):
     return i64(v)
 
 
+def _np_rmax(v, d):
+    v = np.asarray(v, dtype=np.int64).ravel()
+    if v.size == 1:
+        v = np.full(d + 1, int(v[0]), dtype=np.int64)
+    if v.size != d + 1:
+        raise ValueError("rmax must be a scalar or have d+1 entries")
+    return v
+
+
 def capacity(n, r):
     lib = _lib.load()
     cap = C.c_int64()
@@ -254,8 +263,9 @@ def mtimes_round_dev(ctx, n, m, ra, dacore, rx, dxcore, eps, rmax=None, precisio
     return np.array(r_out[:], dtype=np.int64), out[:used.value]
 
 
-def round_batched_dev(ctx, tensors_n, tensors_r, dcore, cofs, eps, precision="fp64", info=None):
-    """Round many TT tensors packed in one CUDA buffer (``cofs[t]`` = start of tensor t)."""
+def round_batched_dev(ctx, tensors_n, tensors_r, dcore, cofs, eps, precision="fp64", info=None, rmax=None):
+    """Round many TT tensors packed in one CUDA buffer (``cofs[t]`` = start of tensor t).
+    ``rmax``: None, a scalar for every bond of every tensor, or one d+1 vector per tensor."""
     import torch
     count = len(tensors_n)
     ds = [len(nn) for nn in tensors_n]
@@ -268,8 +278,14 @@ def round_batched_dev(ctx, tensors_n, tensors_r, dcore, cofs, eps, precision="fp
     out = torch.empty(max(int(cofs_out[-1]), 1), dtype=torch.float64, device=dcore.device)
     r_out = (C.c_int64 * len(rflat))()
     inf = TTGInfo()
+    rm = None
+    if rmax is not None:
+        if np.isscalar(rmax):
+            rm = i64(np.full(len(rflat), int(rmax), dtype=np.int64))
+        else:
+            rm = i64(np.concatenate([_np_rmax(v, dd) for v, dd in zip(rmax, ds)]))
     s = ctx.lib.ttg_round_batched_dev(ctx.h, count, i64(ds), i64(nofs), i64(nflat), i64(rofs),
-                                      i64(rflat), i64(cofs), _ptr(dcore), float(eps),
+                                      i64(rflat), i64(cofs), _ptr(dcore), float(eps), rm,
                                       _prec(precision), r_out, i64(cofs_out[:-1]), _ptr(out),
                                       C.byref(inf))
     ctx.check(s, allow=(TTG_OK, TTG_NO_CONVERGENCE))

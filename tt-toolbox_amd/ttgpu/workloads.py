@@ -111,7 +111,7 @@ def mixed3(d=32, n=8, rank=128, seed=3):
     rng = np.random.default_rng(seed)
     xa = rescale_right_unfoldings(rand_cores(rng, [n] * d, _ranks(d, rank)))
     xb = rescale_right_unfoldings(rand_cores(rng, [n] * d, _ranks(d, rank)))
-    return {"x": flatten(concat(xa, xb)), "eps": 1e-6}
+    return {"x": flatten(concat(xa, xb)), "xa": flatten(xa), "xb": flatten(xb), "eps": 1e-6}
 
 
 def batch4(count=512, seed=4, dmin=8, dmax=24, nset=(2, 4, 8, 16), rmin=8, rmax=128):
