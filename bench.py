@@ -93,7 +93,7 @@ def cpu_model():
     return None
 
 
-def mtimes_leg(ttgpu, W, ctx, dev):
+def mtimes_leg(ttgpu, W, ctx, dev, fused=True):
     """Config 2: A*x (d=12, n=m=4, rank_A 5, rank_x 64), unfused, and the fused
     round(A*x, 1e-6); inputs resident in HBM, host-timed back-to-back calls."""
     import torch
@@ -113,6 +113,8 @@ def mtimes_leg(ttgpu, W, ctx, dev):
                              "contractions_per_s": rate, "hbm_bytes_per_contraction": byt,
                              "roofline": roofline(byt, 1.0 / rate, "hbm",
                                                   what="whole call incl. host launch overhead")}
+    if not fused:
+        return out
     o2 = torch.empty(ttgpu.capacity(na, ry), dtype=torch.float64, device=dev)
     rr = {}
     rate = _rate(lambda: rr.__setitem__("r", ttgpu.mtimes_round_dev(ctx, na, ma, ra, da, rx, dx, 1e-6, out=o2)[0]),
@@ -124,6 +126,34 @@ def mtimes_leg(ttgpu, W, ctx, dev):
         "workload": "round(A*x, 1e-6) fused (product cores built on chip), FP64",
         "rounds_per_s": rate, "ranks_out": [int(v) for v in rr["r"]],
         "algorithmic_gflop": fl / 1e9, "roofline": roofline(fl, 1.0 / rate)}
+    return out
+
+
+def completion_leg(ttgpu, W, dev):
+    """Config 5: completion loss 1/2||P(X) - y||^2 and all core gradients
+    (65,536 samples, d=16, n=8, model rank 32), FP32 and FP64, inputs in HBM."""
+    import torch
+    from ttgpu import autograd as AG
+    sync = torch.cuda.synchronize
+    out = {}
+    c5 = W.completion5()
+    n5, r5, core5 = c5["model"]
+    idx = torch.from_numpy(c5["idx"]).to(dev)
+    vals = torch.from_numpy(c5["vals"]).to(dev)
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        core = torch.tensor(core5, dtype=dt, device=dev, requires_grad=True)
+
+        def step():
+            loss = AG.completion_loss(core, n5, r5, idx, vals)
+            torch.autograd.grad(loss, core)
+        rate5 = _rate(step, 20, sync)
+        S = int(c5["idx"].shape[0])
+        # per sample: left chain, right chain and the gradient outer products, 2 r_k r_{k+1} flops each
+        fl5 = 6.0 * S * sum(int(r5[k]) * int(r5[k + 1]) for k in range(len(n5)))
+        out[f"config5_completion_grad_{name}"] = {
+            "workload": "loss 1/2||P(X) - y||^2 and all core gradients, 65,536 samples, d=16 n=8 rank 32",
+            "evaluations_per_s": rate5, "algorithmic_gflop": fl5 / 1e9,
+            "roofline": roofline(fl5, 1.0 / rate5, dtype=name)}
     return out
 
 
@@ -153,25 +183,7 @@ def other_configs(ttgpu, W, ctx, dev):
     out["config1_plus_round"] = {"workload": "round(X + 0.5 X, 1e-10) from X (d=20 n=4 r=20), sum fused, FP64",
                                  "rounds_per_s": rate, "roofline": roofline(fl1, 1.0 / rate)}
     out.update(mtimes_leg(ttgpu, W, ctx, dev))
-    # config 5: completion loss + gradient, 65,536 samples, d=16 n=8, model rank 32
-    c5 = W.completion5()
-    n5, r5, core5 = c5["model"]
-    idx = torch.from_numpy(c5["idx"]).to(dev)
-    vals = torch.from_numpy(c5["vals"]).to(dev)
-    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
-        core = torch.tensor(core5, dtype=dt, device=dev, requires_grad=True)
-
-        def step():
-            loss = AG.completion_loss(core, n5, r5, idx, vals)
-            torch.autograd.grad(loss, core)
-        rate5 = _rate(step, 20, sync)
-        S = int(c5["idx"].shape[0])
-        # per sample: left chain, right chain and the gradient outer products, 2 r_k r_{k+1} flops each
-        fl5 = 6.0 * S * sum(int(r5[k]) * int(r5[k + 1]) for k in range(len(n5)))
-        out[f"config5_completion_grad_{name}"] = {
-            "workload": "loss 1/2||P(X) - y||^2 and all core gradients, 65,536 samples, d=16 n=8 rank 32",
-            "evaluations_per_s": rate5, "algorithmic_gflop": fl5 / 1e9,
-            "roofline": roofline(fl5, 1.0 / rate5, dtype=name)}
+    out.update(completion_leg(ttgpu, W, dev))
     # <y, A x> and its gradients at config-1 shapes
     cb = W.bilinear1()
     nb, mb, rab, acb = cb["a"]
@@ -376,6 +388,8 @@ def main():
     ap.add_argument("--batched-count", type=int, default=512)
     ap.add_argument("--mtimes-only", action="store_true",
                     help="only run the config-2 leg (A*x and the fused round(A*x)) and exit (profiling)")
+    ap.add_argument("--mtimes-unfused-only", action="store_true",
+                    help="only run the unfused A*x of config 2 and exit (profiling)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched leg (profiling; prints its JSON object)")
     ap.add_argument("--no-other", action="store_true",
@@ -384,6 +398,10 @@ def main():
                     help="skip the comparison rounds with an SVD at every bond (shorter profiles)")
     ap.add_argument("--skip-plain", action="store_true",
                     help="time only the mixed form (profiling)")
+    ap.add_argument("--completion-only", action="store_true",
+                    help="only run the completion-gradient workload (65,536 samples, d=16 n=8 rank 32) and exit")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="time only the plain FP64 form on the metric's tensor and exit (profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -409,9 +427,16 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": nb, "copy_probe_bytes_written": nb, "launches": 3}), flush=True)
         return
+    if args.mtimes_unfused_only:
+        ctx = ttgpu.Context(0)
+        print(json.dumps(mtimes_leg(ttgpu, W, ctx, torch.device("cuda:0"), fused=False)), flush=True)
+        return
     if args.mtimes_only:
         ctx = ttgpu.Context(0)
         print(json.dumps(mtimes_leg(ttgpu, W, ctx, torch.device("cuda:0"))), flush=True)
+        return
+    if args.completion_only:
+        print(json.dumps(completion_leg(ttgpu, W, torch.device("cuda:0"))), flush=True)
         return
     if args.batched_only:
         ctx = ttgpu.Context(0)
@@ -434,6 +459,20 @@ def main():
     def one(prec, out):
         return ttgpu.round_dev(ctx, n, r, dcore, eps, precision=prec, out=out)
 
+    if args.plain_only:
+        for i in range(max(args.warmup, 1)):
+            one("fp64", out_plain)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        info = {}
+        for i in range(args.steps):
+            ttgpu.round_dev(ctx, n, r, dcore, eps, precision="fp64", out=out_plain, info=info)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        print(json.dumps({"plain_ms_per_round": e0.elapsed_time(e1) / args.steps, "info": info}), flush=True)
+        return
     # warm-up (also builds the workspace)
     for i in range(max(args.warmup, 1)):
         one("fp32", out_mixed)

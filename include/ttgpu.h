@@ -120,6 +120,29 @@ ttg_status ttg_mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const
                                 int64_t *r_out, double *dcore_out, int64_t cap,
                                 int64_t *len_out, ttg_info *info);
 
+/* Launch interface of round for repeated rounds of one shape (no reference
+ * counterpart: MATLAB has no asynchronous calls).  It applies to tensors the
+ * Gram-chain form handles (d >= 2, ranks <= 256 after the caps, no rmax), in
+ * the mixed form (prec must be TTG_PREC_FP32; TTG_PREC_FP64 returns TTG_BAD_ARG).
+ * ttg_round_gram_prepare rounds dcore once into dcore_out (it allocates,
+ * uploads its tables, captures its launch sequence for these shapes,
+ * pointers, eps and prec, and synchronises); *proven = 1 when every bond
+ * kept its singular values (the result is the round), 0 otherwise (call
+ * ttg_round_dev).  ttg_round_gram_launch then re-rounds whatever dcore holds
+ * now into dcore_out on the context's stream with no allocation, no host
+ * synchronisation and no host copies, and writes the device int *dstatus
+ * (1: proven, the output is valid; 0: call ttg_round_dev).  Ranks are those
+ * prepare returned.  ttg_round_gram_graph returns the captured graph (a
+ * hipGraph_t, owned by the context, valid until the next prepare, a
+ * Gram-form round of another shape on the context, or ttg_ctx_destroy) for a caller's
+ * hipGraphAddChildGraphNode.  A Gram-form round of another shape on the same
+ * context replaces the prepared one (launch then returns TTG_BAD_ARG). */
+ttg_status ttg_round_gram_prepare(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
+                                  const double *dcore, double eps, int prec, int64_t *r_out,
+                                  double *dcore_out, int64_t cap, int64_t *len_out, int *proven);
+ttg_status ttg_round_gram_launch(ttg_ctx *ctx, int *dstatus);
+ttg_status ttg_round_gram_graph(ttg_ctx *ctx, void **graph);
+
 /* Batched rounding of `count` independent TT tensors (uneven d, n, r).
  * Tensor t: d[t], n/r at offsets nofs[t]/rofs[t] of the packed n/r arrays,
  * cores at device offset cofs[t] of dcore.  rmax: NULL (no cap) or packed like

@@ -128,6 +128,11 @@ int main(int argc, char **argv) {
   CHECK(ttg_adjoint_bilinear_f32(nullptr, 3, n, n, r, fb, r, fb, r, fb, &v, fb, fb) == TTG_BAD_ARG);
   CHECK(ttg_adjoint_bilinear_f64_noalloc(nullptr, 3, n, n, r, buf, r, buf, r, buf, &v, buf, buf) == TTG_BAD_ARG);
   CHECK(ttg_profile_begin(nullptr, 1) == TTG_BAD_ARG);
+  int flag = 0;
+  void *graph = nullptr;
+  CHECK(ttg_round_gram_prepare(nullptr, 3, n, r, buf, 1e-8, 0, ro, buf, 64, &len, &flag) == TTG_BAD_ARG);
+  CHECK(ttg_round_gram_launch(nullptr, &flag) == TTG_BAD_ARG);
+  CHECK(ttg_round_gram_graph(nullptr, &graph) == TTG_BAD_ARG && graph == nullptr);
 
   // SDV files
   sdv_round_trip(argv[1], dir + "/a_copy.sdv");

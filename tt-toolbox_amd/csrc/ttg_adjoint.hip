@@ -16,19 +16,22 @@ namespace ttg {
 // ---------------------------------------------------------------------------
 // Group of G threads per sample (G >= max rank, power of two, <= 256).
 // Lbuf[k] (S x r_k): left interface of core k; the chain is stored for all k.
+// Each step stages the sample's r_k x r_{k+1} slice G_k(:, i_k, :) in LDS
+// (coalesced along r_k, rows padded by one against bank conflicts) and thread
+// t forms entry t of the next interface from it.
 template <class T>
 __global__ void k_completion_left(int d, const long *__restrict__ n, const long *__restrict__ r,
                                   const long *__restrict__ cofs, const long *__restrict__ lofs,
                                   const T *__restrict__ core, long S, const int *__restrict__ idx,
-                                  const T *__restrict__ vals, int G, T *__restrict__ Lbuf,
+                                  const T *__restrict__ vals, int G, int slice_ld, T *__restrict__ Lbuf,
                                   T *__restrict__ res, double *__restrict__ lpart) {
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
-  T *buf = reinterpret_cast<T *>(smem);  // per group: 2 x G
   __shared__ double red[16];
   const int groups = blockDim.x / G;
   const int gid = threadIdx.x / G, t = threadIdx.x % G;
+  T *buf = reinterpret_cast<T *>(smem) + (long)gid * (2 * G + (long)slice_ld * G);  // cur, nxt, slice
   const long s = (long)blockIdx.x * groups + gid;
-  T *cur = buf + gid * 2 * G, *nxt = cur + G;
+  T *cur = buf, *nxt = buf + G, *sl = buf + 2 * G;
   const bool active = s < S;
   if (t == 0) cur[0] = T(1);
   __syncthreads();
@@ -36,10 +39,18 @@ __global__ void k_completion_left(int d, const long *__restrict__ n, const long 
     const long rk = r[k], rk1 = r[k + 1], nk = n[k];
     const int ik = active ? idx[s * d + k] : 0;
     if (active && t < rk) Lbuf[lofs[k] + s * rk + t] = cur[t];
+    if (active) {
+      const T *g = core + cofs[k] + rk * ik;  // G_k(a, ik, b) at g[a + rk nk b]
+      for (long e = t; e < rk * rk1; e += G) {
+        const long a = e % rk, b = e / rk;
+        sl[a + slice_ld * b] = g[a + rk * nk * b];
+      }
+    }
+    __syncthreads();
     if (active && t < rk1) {
-      const T *g = core + cofs[k] + rk * (ik + nk * t);
+      const T *col = sl + slice_ld * t;
       T acc = T(0);
-      for (long a = 0; a < rk; ++a) acc += cur[a] * g[a];
+      for (long a = 0; a < rk; ++a) acc += cur[a] * col[a];
       nxt[t] = acc;
     }
     __syncthreads();
@@ -89,31 +100,63 @@ __global__ void k_completion_right(int d, const long *__restrict__ n, const long
   }
 }
 
-// Partial gradient of core k over a chunk of samples.  Thread t owns the
-// entries e = t, t + blockDim, ... of the n_k x r_k x r_{k+1} core; no races.
+// Partial gradients of every core in one launch: block (x, k) takes samples
+// [x chunk, (x + 1) chunk) for core k and accumulates
+//   acc[i][a][b] += res_s L_k(s)_a R_k(s)_b,  i = i_k(s),
+// in LDS, thread t owning the pairs (a, b) = t, t + blockDim, ... (no two
+// threads touch one entry, and every thread adds its samples in order, so
+// the result is deterministic).  The block's n_k r_k r_{k+1} sums go to
+// part[gblk cofs_k + x csize_k + e], e = a + r_k (i + n_k b) (core layout).
 template <class T>
-__global__ __launch_bounds__(256) void k_completion_grad(int d, int k, const long *__restrict__ n,
-                                                         const long *__restrict__ r, long S,
-                                                         const int *__restrict__ idx,
-                                                         const T *__restrict__ Lbuf, long lofs_k,
-                                                         const T *__restrict__ Rbuf, long rofs_k,
-                                                         const T *__restrict__ res, int chunk,
-                                                         T *__restrict__ part) {
+__global__ __launch_bounds__(256) void k_completion_grad_all(int d, const long *__restrict__ n,
+                                                             const long *__restrict__ r,
+                                                             const long *__restrict__ cofs,
+                                                             const long *__restrict__ lofs,
+                                                             const long *__restrict__ rofs, long S,
+                                                             const int *__restrict__ idx,
+                                                             const T *__restrict__ Lbuf,
+                                                             const T *__restrict__ Rbuf,
+                                                             const T *__restrict__ res, long chunk,
+                                                             T *__restrict__ part) {
+  extern __shared__ __align__(16) char smem[];
+  T *acc = reinterpret_cast<T *>(smem);
+  const int k = blockIdx.y;
   const long rk = r[k], rk1 = r[k + 1], nk = n[k];
-  const long csize = rk * nk * rk1;
-  const long s0 = (long)blockIdx.x * chunk;
-  const long s1 = min(S, s0 + chunk);
-  T *out = part + (long)blockIdx.x * csize;
-  for (long e = threadIdx.x; e < csize; e += blockDim.x) {
-    const long a = e % rk;
-    const long rest = e / rk;
-    const long i = rest % nk, b = rest / nk;
-    T acc = T(0);
-    for (long s = s0; s < s1; ++s) {
-      if (idx[s * d + k] != i) continue;
-      acc += res[s] * Lbuf[lofs_k + s * rk + a] * Rbuf[rofs_k + s * rk1 + b];
+  const long pairs = rk * rk1, csize = rk * nk * rk1;
+  for (long e = threadIdx.x; e < csize; e += blockDim.x) acc[e] = T(0);
+  __syncthreads();
+  const long s0 = (long)blockIdx.x * chunk, s1 = min(S, s0 + chunk);
+  const T *L = Lbuf + lofs[k], *R = Rbuf + rofs[k];
+  for (long s = s0; s < s1; ++s) {
+    const long i = idx[s * d + k];
+    const T w = res[s];
+    T *ai = acc + i * pairs;
+    for (long p = threadIdx.x; p < pairs; p += blockDim.x) {
+      const long a = p % rk, b = p / rk;
+      ai[p] += w * L[s * rk + a] * R[s * rk1 + b];
     }
-    out[e] = acc;
+  }
+  __syncthreads();
+  T *out = part + (long)gridDim.x * cofs[k] + (long)blockIdx.x * csize;
+  for (long e = threadIdx.x; e < csize; e += blockDim.x) {
+    const long a = e % rk, rest = e / rk, i = rest % nk, b = rest / nk;
+    out[e] = acc[i * pairs + a + rk * b];
+  }
+}
+
+// dgrad[e] = sum over the gblk partial blocks of e's core (fixed order).
+template <class T>
+__global__ void k_sum_parts_all(int d, const long *__restrict__ cofs, int gblk, T *__restrict__ part_base,
+                                T *__restrict__ out) {
+  const long total = cofs[d];
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    int k = 0;
+    while (e >= cofs[k + 1]) ++k;
+    const long csize = cofs[k + 1] - cofs[k], off = e - cofs[k];
+    const T *p = part_base + (long)gblk * cofs[k] + off;
+    double acc = 0.0;
+    for (int q = 0; q < gblk; ++q) acc += (double)p[(long)q * csize];
+    out[e] = (T)acc;
   }
 }
 
@@ -132,14 +175,6 @@ __global__ void k_check_idx(int d, const long *__restrict__ n, long S, const int
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
-template <class T>
-__global__ void k_sum_parts(const T *__restrict__ part, int nparts, long size, T *__restrict__ out) {
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < size; e += (long)gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int p = 0; p < nparts; ++p) acc += (double)part[(long)p * size + e];
-    out[e] = (T)acc;
-  }
-}
 
 template <class T>
 ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r,
@@ -162,18 +197,21 @@ ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
     lofs[k + 1] = lofs[k] + S * r[k];
     rofs[k + 1] = rofs[k] + S * r[k + 1];
   }
-  const int chunk = 256;
   const int64_t nblk = (S + groups - 1) / groups;
-  const int64_t gblk = (S + chunk - 1) / chunk;
+  // gradient blocks: about 1024 over all cores, each at least 64 samples
+  const int64_t gblk = std::max<int64_t>(1, std::min<int64_t>((1024 + d - 1) / d, (S + 63) / 64));
+  const int64_t chunk = (S + gblk - 1) / gblk;
   int64_t maxcore = 1;
   for (int64_t k = 0; k < d; ++k) maxcore = std::max(maxcore, r[k] * n[k] * r[k + 1]);
+  TTG_CHECK(maxcore * (int64_t)sizeof(T) <= 64 * 1024, TTG_BAD_ARG,
+            "completion gradient: a core above 64 KB (r_k n_k r_{k+1}) is not supported");
   ArenaSizer z;
   z.take<long>(5 * (d + 1));
   z.take<T>(lofs[d]);
   z.take<T>(rofs[d]);
   z.take<T>(S);
   z.take<double>(nblk);
-  z.take<T>(gblk * maxcore);
+  z.take<T>(gblk * cofs[d]);
   z.take<int>(64);
   ctx->arena.reserve(z.used + 4096);
   ctx->arena.reset();
@@ -182,7 +220,7 @@ ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
   T *Rbuf = ctx->arena.take<T>(rofs[d]);
   T *res = ctx->arena.take<T>(S);
   double *lpart = ctx->arena.take<double>(nblk);
-  T *part = ctx->arena.take<T>(gblk * maxcore);
+  T *part = ctx->arena.take<T>(gblk * cofs[d]);
   int *dflag = ctx->arena.take<int>(64);
   std::vector<long> htab(5 * (d + 1));
   for (int64_t k = 0; k < d; ++k) {
@@ -193,6 +231,8 @@ ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
   }
   for (int64_t k = 0; k <= d; ++k) htab[4 * (d + 1) + k] = rofs[k];
   htab[(d + 1) + d] = r[d];
+  htab[2 * (d + 1) + d] = cofs[d];
+  htab[3 * (d + 1) + d] = lofs[d];
   TTG_HIP(hipMemcpyAsync(tab, htab.data(), htab.size() * sizeof(long), hipMemcpyHostToDevice, st));
   const long *dn = tab, *dr = tab + (d + 1), *dco = tab + 2 * (d + 1), *dlo = tab + 3 * (d + 1);
   // every sample index is checked against the mode sizes before any kernel
@@ -208,22 +248,27 @@ ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
     TTG_CHECK(hflag == 0, TTG_BAD_ARG, "sample index out of range (needs 0 <= idx[s*d+k] < n[k])");
   }
   const size_t shm = (size_t)groups * 2 * G * sizeof(T);
-  hipLaunchKernelGGL((k_completion_left<T>), dim3((unsigned)nblk), dim3(block), shm, st, (int)d, dn,
-                     dr, dco, dlo, dcore, (long)S, didx, dvals, G, Lbuf, res, lpart);
+  const int slice_ld = (int)rmax + 1;
+  const size_t shl = (size_t)groups * (2 * G + (size_t)slice_ld * G) * sizeof(T);
+  TTG_CHECK(shl <= 128 * 1024, TTG_BAD_ARG, "completion: rank too large for the staged slices");
+  TTG_HIP(hipFuncSetAttribute((const void *)k_completion_left<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)shl));
+  hipLaunchKernelGGL((k_completion_left<T>), dim3((unsigned)nblk), dim3(block), shl, st, (int)d, dn,
+                     dr, dco, dlo, dcore, (long)S, didx, dvals, G, slice_ld, Lbuf, res, lpart);
   TTG_HIP(hipGetLastError());
   const long *dro = tab + 4 * (d + 1);
   hipLaunchKernelGGL((k_completion_right<T>), dim3((unsigned)nblk), dim3(block), shm, st, (int)d,
                      dn, dr, dco, dro, dcore, (long)S, didx, G, Rbuf);
   TTG_HIP(hipGetLastError());
-  for (int64_t k = 0; k < d; ++k) {
-    const long csize = r[k] * n[k] * r[k + 1];
-    hipLaunchKernelGGL((k_completion_grad<T>), dim3((unsigned)gblk), dim3(256), 0, st, (int)d, (int)k,
-                       dn, dr, (long)S, didx, Lbuf, (long)lofs[k], Rbuf, (long)rofs[k], res, chunk,
-                       part);
+  {
+    const size_t sha = (size_t)maxcore * sizeof(T);
+    TTG_HIP(hipFuncSetAttribute((const void *)k_completion_grad_all<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)sha));
+    hipLaunchKernelGGL((k_completion_grad_all<T>), dim3((unsigned)gblk, (unsigned)d), dim3(256), sha, st, (int)d,
+                       dn, dr, dco, dlo, dro, (long)S, didx, Lbuf, Rbuf, res, (long)chunk, part);
     TTG_HIP(hipGetLastError());
-    const int blocks = (int)std::min<long>((csize + 255) / 256, 1024);
-    hipLaunchKernelGGL((k_sum_parts<T>), dim3(blocks), dim3(256), 0, st, part, (int)gblk, csize,
-                       dgrad + cofs[k]);
+    const int blocks = (int)std::min<int64_t>((cofs[d] + 255) / 256, 2048);
+    hipLaunchKernelGGL((k_sum_parts_all<T>), dim3(blocks), dim3(256), 0, st, (int)d, dco, (int)gblk, part, dgrad);
     TTG_HIP(hipGetLastError());
   }
   std::vector<double> hl(nblk);

@@ -67,6 +67,9 @@ void release_ctx(ttg_ctx *ctx) {
   if (ctx->ggraph) (void)hipGraphExecDestroy(ctx->ggraph);
   ctx->ggraph = nullptr;
   ctx->gkey.clear();
+  if (ctx->ggraph_src) (void)hipGraphDestroy(ctx->ggraph_src);
+  ctx->ggraph_src = nullptr;
+  ctx->gprep.clear();
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   ctx->side = nullptr;
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);

@@ -584,3 +584,32 @@ ttg_status ttg_full_to_tt_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const d
   TTG_GUARD_END(ctx)
 }
 
+// ---- launch interface of the Gram-chain round ---------------------------------
+ttg_status ttg_round_gram_prepare(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
+                                  double eps, int prec, int64_t *r_out, double *dcore_out, int64_t cap,
+                                  int64_t *len_out, int *proven) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return round_gram_prepare(ctx, d, n, r, dcore, eps, prec, r_out, dcore_out, cap, len_out, proven);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_round_gram_launch(ttg_ctx *ctx, int *dstatus) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  TTG_HIP(hipSetDevice(ctx->device));
+  ctx->last_error.clear();
+  return round_gram_launch(ctx, dstatus);
+  TTG_GUARD_END(ctx)
+}
+
+ttg_status ttg_round_gram_graph(ttg_ctx *ctx, void **graph) {
+  if (!ctx) return TTG_BAD_ARG;
+  TTG_GUARD_BEGIN
+  ctx->last_error.clear();
+  return round_gram_graph(ctx, graph);
+  TTG_GUARD_END(ctx)
+}
+

@@ -580,12 +580,17 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   };
   if (replay) {
     TTG_HIP(hipGraphLaunch(ctx->ggraph, st));
+    ++ctx->gruns;
   } else {
     TTG_HIP(hipMemcpyAsync(tab, hp, tb, hipMemcpyHostToDevice, st));
     if (ctx->ggraph) {
       TTG_HIP(hipGraphExecDestroy(ctx->ggraph));
       ctx->ggraph = nullptr;
       ctx->gkey.clear();
+    }
+    if (ctx->ggraph_src) {
+      TTG_HIP(hipGraphDestroy(ctx->ggraph_src));
+      ctx->ggraph_src = nullptr;
     }
     if (use_graph) {
       hipGraph_t g = nullptr;
@@ -600,11 +605,14 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
       TTG_HIP(hipStreamEndCapture(st, &g));
       hipGraphExec_t ge = nullptr;
       const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) (void)hipGraphDestroy(g);
       TTG_HIP(ie);
       ctx->ggraph = ge;
+      ctx->ggraph_src = g;  // kept for ttg_round_gram_graph (child-node form)
       ctx->gkey = key;
+      ctx->gok = okd;
       TTG_HIP(hipGraphLaunch(ctx->ggraph, st));
+      ++ctx->gruns;
     } else {
       enqueue();
     }
@@ -632,6 +640,67 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return true;
+}
+
+namespace {
+__global__ void k_copy_flag(const int *src, int *dst) { *dst = *src; }
+}  // namespace
+
+// Runs the Gram-chain round once (capturing its launch sequence for these
+// shapes, pointers, eps and form) and keeps it for round_gram_launch.
+ttg_status round_gram_prepare(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
+                              double eps, int prec, int64_t *r_out, double *dcore_out, int64_t cap,
+                              int64_t *len_out, int *proven) {
+  TTG_CHECK(prec == TTG_PREC_FP64 || prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  // the plain form's exact sweep is not offered here: its first run through
+  // this interface did not finish on the GPU and the cause is not known
+  TTG_CHECK(prec == TTG_PREC_FP32, TTG_BAD_ARG, "the launch interface serves the mixed form (TTG_PREC_FP32)");
+  TTG_CHECK(n && r && dcore && r_out && dcore_out && proven, TTG_BAD_ARG, "null pointer argument");
+  TTG_CHECK(d >= 1, TTG_BAD_ARG, "d must be >= 1");
+  for (int64_t i = 0; i < d; ++i) TTG_CHECK(n[i] >= 1, TTG_BAD_ARG, "mode sizes must be >= 1");
+  for (int64_t i = 0; i <= d; ++i) TTG_CHECK(r[i] >= 1, TTG_BAD_ARG, "ranks must be >= 1");
+  TTG_CHECK(eps >= 0.0 && std::isfinite(eps), TTG_BAD_ARG, "eps must be finite and >= 0");
+  {
+    const char *env = std::getenv("TTG_GRAPH");
+    TTG_CHECK(!(env && env[0] == '0'), TTG_BAD_ARG, "the launch interface needs graph capture (TTG_GRAPH=0 is set)");
+  }
+  TTG_CHECK(!ctx->worker, TTG_BAD_ARG, "not on a stream worker");
+  TTG_CHECK(gram_shape_ok(d, n, r, nullptr), TTG_BAD_ARG,
+            "the Gram-chain form does not apply (d < 2 or ranks above 256 after the caps)");
+  ctx->gprep.clear();
+  const uint64_t before = ctx->gruns;
+  const bool exact = prec == TTG_PREC_FP64;
+  const bool ok = round_gram(ctx, d, n, r, dcore, eps, nullptr, exact, exact ? 0.0 : 1e-6, r_out, dcore_out, cap,
+                             len_out, nullptr);
+  TTG_CHECK(ctx->gruns != before && ctx->ggraph, TTG_NEED_SPACE,
+            "the round did not run as a captured launch sequence (output capacity below the QR-capped size "
+            "of ttg_round_capacity, or kernel-class profiling active)");
+  ctx->gprep = ctx->gkey;
+  *proven = ok ? 1 : 0;
+  return TTG_OK;
+}
+
+// Replays the prepared round on ctx's stream: no allocation, no host
+// synchronisation, no host copies.  *dstatus (device) <- 1 if every bond was
+// proven this time (dcore_out holds the rounded tensor), 0 otherwise.
+ttg_status round_gram_launch(ttg_ctx *ctx, int *dstatus) {
+  TTG_CHECK(dstatus, TTG_BAD_ARG, "null status pointer");
+  TTG_CHECK(ctx->ggraph && !ctx->gprep.empty() && ctx->gprep == ctx->gkey, TTG_BAD_ARG,
+            "no prepared Gram-chain round on this context (call ttg_round_gram_prepare; a later round of "
+            "another shape replaces it)");
+  TTG_HIP(hipGraphLaunch(ctx->ggraph, ctx->stream));
+  ++ctx->gruns;
+  hipLaunchKernelGGL(k_copy_flag, dim3(1), dim3(1), 0, ctx->stream, (const int *)ctx->gok, dstatus);
+  TTG_HIP(hipGetLastError());
+  return TTG_OK;
+}
+
+ttg_status round_gram_graph(ttg_ctx *ctx, void **graph) {
+  TTG_CHECK(graph, TTG_BAD_ARG, "null pointer argument");
+  TTG_CHECK(ctx->ggraph_src && !ctx->gprep.empty() && ctx->gprep == ctx->gkey, TTG_BAD_ARG,
+            "no prepared Gram-chain round on this context");
+  *graph = ctx->ggraph_src;
+  return TTG_OK;
 }
 
 }  // namespace ttg

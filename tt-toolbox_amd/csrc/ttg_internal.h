@@ -137,6 +137,10 @@ struct ttg_ctx {
   ttg::Arena garena;
   hipGraphExec_t ggraph = nullptr;
   std::vector<uint64_t> gkey;
+  hipGraph_t ggraph_src = nullptr;  // the captured graph behind ggraph (child-node form)
+  int *gok = nullptr;               // its proof flag (device)
+  uint64_t gruns = 0;               // launches of ggraph so far
+  std::vector<uint64_t> gprep;      // key of the prepared launch (ttg_round_gram_prepare)
   // cores of an assembled sum a X + b Y (plus fused with round), kept apart
   // from the rounding workspaces that read them
   ttg::Arena sarena;
@@ -175,6 +179,13 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
 // the shape tests of round_gram (d >= 2, capped ranks <= 256, no binding
 // rmax, the form not switched off): true when the form may be tried
 bool gram_shape_ok(int64_t d, const int64_t *n, const int64_t *r, const int64_t *rmax);
+// launch interface of the Gram-chain round (prepare allocates and synchronises;
+// launch only enqueues the captured graph and the proof-flag copy)
+ttg_status round_gram_prepare(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
+                              double eps, int prec, int64_t *r_out, double *dcore_out, int64_t cap,
+                              int64_t *len_out, int *proven);
+ttg_status round_gram_launch(ttg_ctx *ctx, int *dstatus);
+ttg_status round_gram_graph(ttg_ctx *ctx, void **graph);
 // true if one workgroup can round the tensor (d <= 64, ranks <= 512)
 bool persist_supported(int64_t d, const int64_t *n, const int64_t *r);
 // doubles of per-slot workspace for the largest core / bond rank of a batch
@@ -341,6 +352,13 @@ void finalize_balance(hipStream_t s, int64_t d, TI *const *src_dev, const int64_
 template <class T>
 void mtimes_core(hipStream_t s, int64_t ra0, int64_t n, int64_t m, int64_t ra1, const double *A,
                  int64_t rx0, int64_t rx1, const double *X, T *Y);
+// every product core of A*x in one launch (table of cores in device memory)
+struct MvCore {
+  const double *A, *X;
+  double *Y;
+  int64_t ra0, n, m, ra1, rx0, rx1;
+};
+void mtimes_all(hipStream_t s, const MvCore *tab_dev, int ncores, int64_t max_cols, int64_t max_rows, int64_t max_a);
 // C (M x n ra1 rx1) = R (M x ra0 rx0, ld ldr) * (product core of A and X), the
 // product core generated on chip (never written to memory).
 void gemm_mtimes(hipStream_t s, int64_t M, const double *R, int64_t ldr, int64_t ra0, int64_t n,

@@ -202,17 +202,34 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
                                         beta, C, ldc, part);
 }
 
-// Batched GEMM: job j owns grid blocks [tile0_j, tile0_{j+1}); every job has
-// its own shape, operands and (optional) device scale; full K per block.
+// Workgroups are dispatched to the 8 XCDs round-robin (block w on XCD w % 8),
+// and each XCD has its own L2.  This bijection gives XCD x the contiguous
+// tile range [x q + min(x, rem), ...) (q = total / 8, rem = total % 8), so the
+// M tiles that share one B panel (consecutive in a job's tile order) run on
+// one XCD and read the panel from HBM once instead of once per XCD.
+__device__ __forceinline__ int xcd_tile(int w, int total) {
+  constexpr int X = 8;
+  const int q = total / X, rem = total % X;
+  const int x = w % X, local = w / X;
+  return x * q + min(x, rem) + local;
+}
+
+// Batched GEMM: job j owns tiles [tile0_j, tile0_{j+1}); every job has its
+// own shape, operands and (optional) device scale; full K per block.
 template <class T, bool TA, bool TB, int BM, int BN>
 __global__ __launch_bounds__(256) void k_bgemm(const BGemmJob *__restrict__ jobs, int njobs) {
+  const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   int j = 0;
-  while (j + 1 < njobs && jobs[j + 1].tile0 <= (int)blockIdx.x) ++j;
+  while (j + 1 < njobs && jobs[j + 1].tile0 <= tile) ++j;
   const BGemmJob jb = jobs[j];
-  const int t = (int)blockIdx.x - jb.tile0;
-  const int tm = (jb.M + BM - 1) / BM;
+  const int t = tile - jb.tile0;
+  const int tm = (jb.M + BM - 1) / BM, tn = (jb.N + BN - 1) / BN;
+  // consecutive tiles (one XCD) share the larger operand's panel: M fastest
+  // when op(B) is the larger, N fastest when op(A) is
+  const bool mfast = jb.N >= jb.M;
+  const int im = mfast ? t % tm : t / tn, in = mfast ? t / tm : t % tn;
   const T a = jb.alpha_dev ? T(jb.alpha * jb.alpha_dev[0]) : T(jb.alpha);
-  gemm_block<T, T, T, TA, TB, BM, BN>(jb.M, jb.N, 0, jb.K, (t % tm) * BM, (t / tm) * BN, a,
+  gemm_block<T, T, T, TA, TB, BM, BN>(jb.M, jb.N, 0, jb.K, im * BM, in * BN, a,
                                       static_cast<const T *>(jb.A), jb.lda, static_cast<const T *>(jb.B), jb.ldb,
                                       T(0), static_cast<T *>(jb.C), jb.ldc, nullptr);
 }
@@ -1039,6 +1056,66 @@ void mtimes_core(hipStream_t s, int64_t ra0, int64_t n, int64_t m, int64_t ra1, 
   TTG_HIP(hipGetLastError());
 }
 template void mtimes_core<double>(hipStream_t, int64_t, int64_t, int64_t, int64_t, const double *, int64_t, int64_t, const double *, double *);
+
+// All d product cores in one launch (unfused A*x).  Block (x, y) covers a
+// contiguous run of columns of core y (a column is (j, b, be), the output's
+// index j + n (b + ra1 be)); the block stages core y's A (ra0 n m ra1
+// doubles, at most 16K) in LDS once, and each thread keeps one output row
+// (a + ra0 al, split once), so per column it reads m values of X (shared by
+// the n ra1 consecutive columns of one be, cached) and m values of A from
+// LDS, and the block writes the column's r0 rows contiguously.  Four
+// columns per step keep their loads in flight together.  Sizes are checked
+// below 2^31 by the caller, so the index arithmetic is 32-bit.
+constexpr int MV_ALDS = 16384;  // doubles of A staged in LDS
+__global__ __launch_bounds__(1024) void k_mtimes_all(const MvCore *__restrict__ tab, int cols_per_block) {
+  extern __shared__ __align__(16) double As[];
+  const MvCore c = tab[blockIdx.y];
+  const unsigned r0 = (unsigned)(c.ra0 * c.rx0), ncol = (unsigned)(c.n * c.ra1 * c.rx1);
+  const unsigned ra0 = (unsigned)c.ra0, rx0 = (unsigned)c.rx0, nn = (unsigned)c.n, mm = (unsigned)c.m;
+  const unsigned ra1 = (unsigned)c.ra1;
+  const unsigned c0 = blockIdx.x * (unsigned)cols_per_block;
+  if (c0 >= ncol) return;  // uniform over the block
+  const unsigned c1 = min(ncol, c0 + (unsigned)cols_per_block);
+  const unsigned asz = ra0 * nn * mm * ra1;
+  for (unsigned e = threadIdx.x; e < asz; e += blockDim.x) As[e] = c.A[e];
+  __syncthreads();
+  const unsigned qa = ra0 * nn;  // stride of the contracted index in A
+  for (unsigned row0 = 0; row0 < r0; row0 += blockDim.x) {
+    const unsigned row = row0 + threadIdx.x;
+    if (row >= r0) continue;
+    const unsigned a = row % ra0, al = row / ra0;
+    const double *xr = c.X + al;
+    for (unsigned col = c0; col < c1; col += 4) {
+      double s[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const unsigned cu = min(col + u, c1 - 1);
+        const unsigned j = cu % nn, rest = cu / nn, b = rest % ra1, be = rest / ra1;
+        const double *ap = As + a + ra0 * (j + nn * mm * b);
+        const double *xp = xr + (long)rx0 * mm * be;
+        double acc = 0.0;
+        for (unsigned q = 0; q < mm; ++q) acc = __builtin_fma(ap[qa * q], xp[(long)rx0 * q], acc);
+        s[u] = acc;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (col + u < c1) c.Y[(long)(col + u) * r0 + row] = s[u];
+    }
+  }
+}
+void mtimes_all(hipStream_t s, const MvCore *tab_dev, int ncores, int64_t max_cols, int64_t max_rows,
+                int64_t max_a) {
+  if (ncores <= 0 || max_cols <= 0) return;
+  TTG_CHECK(max_a <= MV_ALDS, TTG_BAD_ARG, "A*x: a core of A above 16K entries");
+  const int nt = (int)std::min<int64_t>(1024, (max_rows + 63) / 64 * 64);
+  // about 4 blocks per CU for the widest core, at least 8 columns each
+  const int64_t cpb = std::max<int64_t>(8, (max_cols + 1023) / 1024);
+  const int bx = (int)((max_cols + cpb - 1) / cpb);
+  const size_t lds = (size_t)max_a * sizeof(double);
+  TTG_HIP(hipFuncSetAttribute((const void *)k_mtimes_all, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_mtimes_all, dim3(bx, ncores), dim3(nt), lds, s, tab_dev, (int)cpb);
+  TTG_HIP(hipGetLastError());
+}
 
 // C (M x N) = R (M x K) * P, where P is the (ra0 rx0) x (n ra1 rx1) unfolding of
 // the product core of TT-matrix core A (ra0 x n x m x ra1) and TT core X

@@ -828,14 +828,34 @@ ttg_status mtimes_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *
   const int64_t need = packed_size(d, n, rp.data());
   if (len_out) *len_out = need;
   if (cap < need) return TTG_NEED_SPACE;
-  int64_t oa = 0, ox = 0, oy = 0;
+  // one launch for every core (the table goes through pinned memory)
+  std::vector<MvCore> tab(d);
+  int64_t oa = 0, ox = 0, oy = 0, max_cols = 1, max_rows = 1, max_a = 1;
   for (int64_t i = 0; i < d; ++i) {
-    mtimes_core<double>(ctx->stream, ra[i], n[i], m[i], ra[i + 1], dacore + oa, rx[i], rx[i + 1],
-                        dxcore + ox, dycore + oy);
+    tab[i] = MvCore{dacore + oa, dxcore + ox, dycore + oy, ra[i], n[i], m[i], ra[i + 1], rx[i], rx[i + 1]};
+    TTG_CHECK(rp[i] * n[i] * rp[i + 1] < (int64_t(1) << 31) && ra[i] * n[i] * m[i] * ra[i + 1] < (int64_t(1) << 31),
+              TTG_BAD_ARG, "a core of A*x has 2^31 or more entries");
+    max_cols = std::max(max_cols, n[i] * rp[i + 1]);
+    max_rows = std::max(max_rows, rp[i]);
+    max_a = std::max(max_a, ra[i] * n[i] * m[i] * ra[i + 1]);
     oa += ra[i] * n[i] * m[i] * ra[i + 1];
     ox += rx[i] * m[i] * rx[i + 1];
     oy += rp[i] * n[i] * rp[i + 1];
   }
+  ctx->arena.reserve(d * sizeof(MvCore) + 4096);
+  ctx->arena.reset();
+  MvCore *dtab = ctx->arena.take<MvCore>(d);
+  if (!ctx->host_pinned || ctx->host_pinned_cap < d * sizeof(MvCore)) {
+    TTG_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->host_pinned) TTG_HIP(hipHostFree(ctx->host_pinned));
+    ctx->host_pinned = nullptr;
+    const size_t c2 = std::max<size_t>(d * sizeof(MvCore), 4096);
+    TTG_HIP(hipHostMalloc(&ctx->host_pinned, c2, hipHostMallocDefault));
+    ctx->host_pinned_cap = c2;
+  }
+  std::memcpy(ctx->host_pinned, tab.data(), d * sizeof(MvCore));
+  TTG_HIP(hipMemcpyAsync(dtab, ctx->host_pinned, d * sizeof(MvCore), hipMemcpyHostToDevice, ctx->stream));
+  mtimes_all(ctx->stream, dtab, (int)d, max_cols, max_rows, max_a);
   TTG_HIP(hipStreamSynchronize(ctx->stream));
   for (int64_t i = 0; i <= d; ++i) ry[i] = rp[i];
   return TTG_OK;

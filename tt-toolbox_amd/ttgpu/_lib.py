@@ -36,6 +36,7 @@ EXPORTS = [
     "ttg_adjoint_completion_f64", "ttg_adjoint_completion_f32",
     "ttg_adjoint_bilinear_f64", "ttg_adjoint_bilinear_f32", "ttg_adjoint_bilinear_f64_noalloc",
     "ttg_plus_dev", "ttg_plus_round_dev", "ttg_norm_dev", "ttg_plus_norm_dev", "ttg_dot_dev",
+    "ttg_round_gram_prepare", "ttg_round_gram_launch", "ttg_round_gram_graph",
     "ttg_full_to_tt_dev", "ttg_sdv_info", "ttg_sdv_read", "ttg_sdv_read_dev", "ttg_sdv_write", "ttg_sdv_write_dev",
     "ttg_profile_begin", "ttg_profile_end",
 ]
@@ -114,6 +115,10 @@ def load(path: Path | str | None = None):
         "ttg_plus_norm_dev": (st, [P, I64, I64P, I64P, P, C.c_double, I64P, P, C.c_double, DP]),
         "ttg_dot_dev": (st, [P, I64, I64P, I64P, P, I64P, P, DP]),
         "ttg_full_to_tt_dev": (st, [P, I64, I64P, P, C.c_double, I64P, I64P, P, I64, I64P, P]),
+        "ttg_round_gram_prepare": (st, [P, I64, I64P, I64P, P, C.c_double, C.c_int, I64P, P, I64, I64P,
+                                        C.POINTER(C.c_int)]),
+        "ttg_round_gram_launch": (st, [P, P]),
+        "ttg_round_gram_graph": (st, [P, C.POINTER(P)]),
         "ttg_sdv_info": (st, [C.c_char_p, I64P, I64P, C.POINTER(C.c_int)]),
         "ttg_sdv_read": (st, [C.c_char_p, I64, I64P, I64P, I64P, P, I64, I64P]),
         "ttg_sdv_read_dev": (st, [P, C.c_char_p, I64, I64P, I64P, I64P, P, I64, I64P]),

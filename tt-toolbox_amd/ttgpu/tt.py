@@ -421,6 +421,37 @@ def rel_diff(y: TTTensor, yref: TTTensor, ctx=None) -> float:
 
 
 # ---------------------------------------------------------------------------
+# launch interface of the Gram-chain round (repeated rounds of one shape)
+# ---------------------------------------------------------------------------
+def round_gram_prepare(ctx, n, r, dcore, eps, precision="fp32", out=None):
+    """Round once and keep the captured launch sequence; returns (r_out, core view, proven)."""
+    import torch
+    d = len(n)
+    cap = capacity(n, r)
+    if out is None:
+        out = torch.empty(max(cap, 1), dtype=torch.float64, device=dcore.device)
+    r_out = (C.c_int64 * (d + 1))()
+    used = C.c_int64()
+    proven = C.c_int()
+    ctx.check(ctx.lib.ttg_round_gram_prepare(ctx.h, d, i64(n), i64(r), _ptr(dcore), float(eps), _prec(precision),
+                                             r_out, _ptr(out), int(out.numel()), C.byref(used), C.byref(proven)))
+    return np.array(r_out[:], dtype=np.int64), out[:used.value], bool(proven.value)
+
+
+def round_gram_launch(ctx, dstatus):
+    """Re-round the prepared input buffer on the context's stream (no host
+    synchronisation); ``dstatus`` is a one-element int32 device tensor."""
+    ctx.check(ctx.lib.ttg_round_gram_launch(ctx.h, _ptr(dstatus)))
+
+
+def round_gram_graph(ctx):
+    """The prepared round as a hipGraph_t handle (for hipGraphAddChildGraphNode)."""
+    g = C.c_void_p()
+    ctx.check(ctx.lib.ttg_round_gram_graph(ctx.h, C.byref(g)))
+    return g
+
+
+# ---------------------------------------------------------------------------
 # TT-SVD (tt_tensor(full, eps), @tt_tensor/tt_tensor.m:175-244)
 # ---------------------------------------------------------------------------
 def full_to_tt_dev(ctx, n, dfull, eps=1e-14, rmax=None, info=None):
