@@ -512,7 +512,9 @@ def main():
 
     # plain FP64 form on the same input
     if args.skip_plain:
-        print(json.dumps({"mixed_ms_per_round": ms_step, "info": info}), flush=True)
+        print(json.dumps({"mixed_ms_per_round": ms_step, "info": info, "dominant": dominant,
+                          "dominant_launches_per_round": prof["launches"] / args.steps,
+                          "dominant_avg_launch_ms": prof["ms"] / max(prof["launches"], 1)}), flush=True)
         return
     one("fp64", out_plain)
     torch.cuda.synchronize()

@@ -8,6 +8,7 @@
 #include "ttg_internal.h"
 #include "ttg_device.h"
 #include <algorithm>
+#include <cstdlib>
 
 namespace ttg {
 
@@ -19,12 +20,16 @@ namespace ttg {
 // Each step stages the sample's r_k x r_{k+1} slice G_k(:, i_k, :) in LDS
 // (coalesced along r_k, rows padded by one against bank conflicts) and thread
 // t forms entry t of the next interface from it.
+// Cores [k0, k1): the chain starts from Lin (S x r_k0; null: the scalar 1,
+// k0 = 0), stores L_k at Lbuf + lofs[k] when `store`, and ends either with
+// the residuals and loss partials (k1 = d) or by writing L_k1 to Lout.
 template <class T>
 __global__ void k_completion_left(int d, const long *__restrict__ n, const long *__restrict__ r,
                                   const long *__restrict__ cofs, const long *__restrict__ lofs,
                                   const T *__restrict__ core, long S, const int *__restrict__ idx,
                                   const T *__restrict__ vals, int G, int slice_ld, T *__restrict__ Lbuf,
-                                  T *__restrict__ res, double *__restrict__ lpart) {
+                                  T *__restrict__ res, double *__restrict__ lpart, int k0, int k1, int store,
+                                  const T *__restrict__ Lin, T *__restrict__ Lout) {
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   __shared__ double red[16];
   const int groups = blockDim.x / G;
@@ -33,12 +38,16 @@ __global__ void k_completion_left(int d, const long *__restrict__ n, const long 
   const long s = (long)blockIdx.x * groups + gid;
   T *cur = buf, *nxt = buf + G, *sl = buf + 2 * G;
   const bool active = s < S;
-  if (t == 0) cur[0] = T(1);
+  if (Lin) {
+    if (active && t < r[k0]) cur[t] = Lin[s * r[k0] + t];
+  } else if (t == 0) {
+    cur[0] = T(1);
+  }
   __syncthreads();
-  for (int k = 0; k < d; ++k) {
+  for (int k = k0; k < k1; ++k) {
     const long rk = r[k], rk1 = r[k + 1], nk = n[k];
     const int ik = active ? idx[s * d + k] : 0;
-    if (active && t < rk) Lbuf[lofs[k] + s * rk + t] = cur[t];
+    if (store && active && t < rk) Lbuf[lofs[k] + s * rk + t] = cur[t];
     if (active) {
       const T *g = core + cofs[k] + rk * ik;  // G_k(a, ik, b) at g[a + rk nk b]
       for (long e = t; e < rk * rk1; e += G) {
@@ -57,6 +66,10 @@ __global__ void k_completion_left(int d, const long *__restrict__ n, const long 
     T *tmp = cur;
     cur = nxt;
     nxt = tmp;
+  }
+  if (k1 < d) {  // uniform over the grid
+    if (active && t < r[k1]) Lout[s * r[k1] + t] = cur[t];
+    return;
   }
   double l = 0.0;
   if (active && t == 0) {
@@ -107,6 +120,8 @@ __global__ void k_completion_right(int d, const long *__restrict__ n, const long
 // threads touch one entry, and every thread adds its samples in order, so
 // the result is deterministic).  The block's n_k r_k r_{k+1} sums go to
 // part[gblk cofs_k + x csize_k + e], e = a + r_k (i + n_k b) (core layout).
+// single != 0: one core (k0), L is the interface itself (Lbuf, no offset)
+// and the partials go to part + x csize.
 template <class T>
 __global__ __launch_bounds__(256) void k_completion_grad_all(int d, const long *__restrict__ n,
                                                              const long *__restrict__ r,
@@ -117,16 +132,16 @@ __global__ __launch_bounds__(256) void k_completion_grad_all(int d, const long *
                                                              const T *__restrict__ Lbuf,
                                                              const T *__restrict__ Rbuf,
                                                              const T *__restrict__ res, long chunk,
-                                                             T *__restrict__ part) {
+                                                             T *__restrict__ part, int k0, int single) {
   extern __shared__ __align__(16) char smem[];
   T *acc = reinterpret_cast<T *>(smem);
-  const int k = blockIdx.y;
+  const int k = blockIdx.y + k0;
   const long rk = r[k], rk1 = r[k + 1], nk = n[k];
   const long pairs = rk * rk1, csize = rk * nk * rk1;
   for (long e = threadIdx.x; e < csize; e += blockDim.x) acc[e] = T(0);
   __syncthreads();
   const long s0 = (long)blockIdx.x * chunk, s1 = min(S, s0 + chunk);
-  const T *L = Lbuf + lofs[k], *R = Rbuf + rofs[k];
+  const T *L = single ? Lbuf : Lbuf + lofs[k], *R = Rbuf + rofs[k];
   for (long s = s0; s < s1; ++s) {
     const long i = idx[s * d + k];
     const T w = res[s];
@@ -137,7 +152,7 @@ __global__ __launch_bounds__(256) void k_completion_grad_all(int d, const long *
     }
   }
   __syncthreads();
-  T *out = part + (long)gridDim.x * cofs[k] + (long)blockIdx.x * csize;
+  T *out = part + (single ? 0L : (long)gridDim.x * cofs[k]) + (long)blockIdx.x * csize;
   for (long e = threadIdx.x; e < csize; e += blockDim.x) {
     const long a = e % rk, rest = e / rk, i = rest % nk, b = rest / nk;
     out[e] = acc[i * pairs + a + rk * b];
@@ -145,19 +160,25 @@ __global__ __launch_bounds__(256) void k_completion_grad_all(int d, const long *
 }
 
 // dgrad[e] = sum over the gblk partial blocks of e's core (fixed order).
+// konly >= 0: only core konly, whose partials start at part_base.
 template <class T>
 __global__ void k_sum_parts_all(int d, const long *__restrict__ cofs, int gblk, T *__restrict__ part_base,
-                                T *__restrict__ out) {
-  const long total = cofs[d];
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-    int k = 0;
+                                T *__restrict__ out, int konly) {
+  const long e0 = konly >= 0 ? cofs[konly] : 0, total = konly >= 0 ? cofs[konly + 1] : cofs[d];
+  for (long e = e0 + blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    int k = konly >= 0 ? konly : 0;
     while (e >= cofs[k + 1]) ++k;
     const long csize = cofs[k + 1] - cofs[k], off = e - cofs[k];
-    const T *p = part_base + (long)gblk * cofs[k] + off;
+    const T *p = part_base + (konly >= 0 ? 0L : (long)gblk * cofs[k]) + off;
     double acc = 0.0;
     for (int q = 0; q < gblk; ++q) acc += (double)p[(long)q * csize];
     out[e] = (T)acc;
   }
+}
+
+template <class T>
+__global__ void k_fill_one(T *p, long count) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) p[i] = T(1);
 }
 
 __global__ void k_fill_zero_int(int *p, int count) {
@@ -205,22 +226,35 @@ ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
   for (int64_t k = 0; k < d; ++k) maxcore = std::max(maxcore, r[k] * n[k] * r[k + 1]);
   TTG_CHECK(maxcore * (int64_t)sizeof(T) <= 64 * 1024, TTG_BAD_ARG,
             "completion gradient: a core above 64 KB (r_k n_k r_{k+1}) is not supported");
+  // Left interfaces of every core (S sum_k r_k values) are stored when they
+  // fit the budget (TTG_ADJOINT_LBUDGET_MB, default 4096 MiB); otherwise they
+  // are recomputed core by core during the gradient pass from two S x rmax
+  // buffers (same arithmetic, same chunks: bitwise the same result)
+  size_t budget = size_t(4096) << 20;
+  if (const char *env = std::getenv("TTG_ADJOINT_LBUDGET_MB")) budget = size_t(std::max(0L, std::atol(env))) << 20;
+  const bool recompute = (size_t)lofs[d] * sizeof(T) > budget;
   ArenaSizer z;
   z.take<long>(5 * (d + 1));
-  z.take<T>(lofs[d]);
+  if (recompute) {
+    z.take<T>(S * rmax);
+    z.take<T>(S * rmax);
+  } else {
+    z.take<T>(lofs[d]);
+  }
   z.take<T>(rofs[d]);
   z.take<T>(S);
   z.take<double>(nblk);
-  z.take<T>(gblk * cofs[d]);
+  z.take<T>(recompute ? gblk * maxcore : gblk * cofs[d]);
   z.take<int>(64);
   ctx->arena.reserve(z.used + 4096);
   ctx->arena.reset();
   long *tab = ctx->arena.take<long>(5 * (d + 1));
-  T *Lbuf = ctx->arena.take<T>(lofs[d]);
+  T *Lbuf = recompute ? nullptr : ctx->arena.take<T>(lofs[d]);
+  T *Lc[2] = {recompute ? ctx->arena.take<T>(S * rmax) : nullptr, recompute ? ctx->arena.take<T>(S * rmax) : nullptr};
   T *Rbuf = ctx->arena.take<T>(rofs[d]);
   T *res = ctx->arena.take<T>(S);
   double *lpart = ctx->arena.take<double>(nblk);
-  T *part = ctx->arena.take<T>(gblk * cofs[d]);
+  T *part = ctx->arena.take<T>(recompute ? gblk * maxcore : gblk * cofs[d]);
   int *dflag = ctx->arena.take<int>(64);
   std::vector<long> htab(5 * (d + 1));
   for (int64_t k = 0; k < d; ++k) {
@@ -254,22 +288,44 @@ ttg_status adjoint_completion(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
   TTG_HIP(hipFuncSetAttribute((const void *)k_completion_left<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)shl));
   hipLaunchKernelGGL((k_completion_left<T>), dim3((unsigned)nblk), dim3(block), shl, st, (int)d, dn,
-                     dr, dco, dlo, dcore, (long)S, didx, dvals, G, slice_ld, Lbuf, res, lpart);
+                     dr, dco, dlo, dcore, (long)S, didx, dvals, G, slice_ld, Lbuf, res, lpart, 0, (int)d,
+                     recompute ? 0 : 1, (const T *)nullptr, (T *)nullptr);
   TTG_HIP(hipGetLastError());
   const long *dro = tab + 4 * (d + 1);
   hipLaunchKernelGGL((k_completion_right<T>), dim3((unsigned)nblk), dim3(block), shm, st, (int)d,
                      dn, dr, dco, dro, dcore, (long)S, didx, G, Rbuf);
   TTG_HIP(hipGetLastError());
-  {
-    const size_t sha = (size_t)maxcore * sizeof(T);
-    TTG_HIP(hipFuncSetAttribute((const void *)k_completion_grad_all<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)sha));
+  const size_t sha = (size_t)maxcore * sizeof(T);
+  TTG_HIP(hipFuncSetAttribute((const void *)k_completion_grad_all<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)sha));
+  if (!recompute) {
     hipLaunchKernelGGL((k_completion_grad_all<T>), dim3((unsigned)gblk, (unsigned)d), dim3(256), sha, st, (int)d,
-                       dn, dr, dco, dlo, dro, (long)S, didx, Lbuf, Rbuf, res, (long)chunk, part);
+                       dn, dr, dco, dlo, dro, (long)S, didx, Lbuf, Rbuf, res, (long)chunk, part, 0, 0);
     TTG_HIP(hipGetLastError());
     const int blocks = (int)std::min<int64_t>((cofs[d] + 255) / 256, 2048);
-    hipLaunchKernelGGL((k_sum_parts_all<T>), dim3(blocks), dim3(256), 0, st, (int)d, dco, (int)gblk, part, dgrad);
+    hipLaunchKernelGGL((k_sum_parts_all<T>), dim3(blocks), dim3(256), 0, st, (int)d, dco, (int)gblk, part, dgrad,
+                       -1);
     TTG_HIP(hipGetLastError());
+  } else {
+    // L_0 = 1; per core: its gradient from L_k and R_k, then L_{k+1} = L_k G_k
+    hipLaunchKernelGGL((k_fill_one<T>), dim3((unsigned)std::min<int64_t>((S + 255) / 256, 2048)), dim3(256), 0, st,
+                       Lc[0], (long)S);
+    for (int64_t k = 0; k < d; ++k) {
+      T *Lk = Lc[k & 1], *Ln = Lc[(k + 1) & 1];
+      hipLaunchKernelGGL((k_completion_grad_all<T>), dim3((unsigned)gblk, 1u), dim3(256), sha, st, (int)d, dn, dr,
+                         dco, dlo, dro, (long)S, didx, (const T *)Lk, Rbuf, res, (long)chunk, part, (int)k, 1);
+      TTG_HIP(hipGetLastError());
+      const int blocks = (int)std::min<int64_t>((cofs[k + 1] - cofs[k] + 255) / 256, 2048);
+      hipLaunchKernelGGL((k_sum_parts_all<T>), dim3(blocks), dim3(256), 0, st, (int)d, dco, (int)gblk, part, dgrad,
+                         (int)k);
+      TTG_HIP(hipGetLastError());
+      if (k + 1 < d) {
+        hipLaunchKernelGGL((k_completion_left<T>), dim3((unsigned)nblk), dim3(block), shl, st, (int)d, dn, dr, dco,
+                           dlo, dcore, (long)S, didx, dvals, G, slice_ld, (T *)nullptr, res, lpart, (int)k,
+                           (int)k + 1, 0, (const T *)Lk, Ln);
+        TTG_HIP(hipGetLastError());
+      }
+    }
   }
   std::vector<double> hl(nblk);
   TTG_HIP(hipMemcpyAsync(hl.data(), lpart, nblk * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -64,12 +64,7 @@ void release_ctx(ttg_ctx *ctx) {
   if (ctx->pgraph) (void)hipGraphExecDestroy(ctx->pgraph);
   ctx->pgraph = nullptr;
   ctx->pgraph_key.clear();
-  if (ctx->ggraph) (void)hipGraphExecDestroy(ctx->ggraph);
-  ctx->ggraph = nullptr;
-  ctx->gkey.clear();
-  if (ctx->ggraph_src) (void)hipGraphDestroy(ctx->ggraph_src);
-  ctx->ggraph_src = nullptr;
-  ctx->gprep.clear();
+  gram_graph_release(ctx);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   ctx->side = nullptr;
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);

@@ -421,7 +421,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     const int m = (int)(rr[i] * n[i]), k = (int)rr[i + 1];
     ja[i] = BGemmJob{cp[i], RR + sq_ofs[i + 1], Ab + out_ofs[i], nullptr, 1.0, m, k, k, m, k, m, 0, 0};
   }
-  bool qa, qz, qb;
+  int qa, qz, qb;
   double fa, fz, fb;
   const int ta_ = bgemm_plan(true, ja.data(), nb, &qa, &fa);
   const int tz_ = bgemm_plan(false, jz.data(), nb, &qz, &fz);
@@ -490,7 +490,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     const char *env = std::getenv("TTG_GRAPH");
     if (env && env[0] == '0') use_graph = false;
   }
-  const bool replay = use_graph && ctx->ggraph && ctx->gkey == key;
+  const bool replay = use_graph && ctx->ggraph[0] && ctx->gkey == key;
   const CholJob *dcj = reinterpret_cast<const CholJob *>(tab + off[0]);
   const TrsmJob *dtj = reinterpret_cast<const TrsmJob *>(tab + off[1]);
   const GramBond *dgb = reinterpret_cast<const GramBond *>(tab + off[2]);
@@ -501,34 +501,35 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   const int64_t *dsz = reinterpret_cast<const int64_t *>(tab + off[7]);
   const double *d_one = reinterpret_cast<const double *>(tab + off[8]);
 
-  auto enqueue = [&]() {
+  // the launch sequence in four parts: set-up (st), right chain, left chain,
+  // the rest (st); the chains run on st and sd at once
+  auto enq_setup = [&]() {
     caps(true);
     // ---- scales s_i = 1/||C_i||^2 -----------------------------------------------
     hipLaunchKernelGGL(k_core_sumsq, dim3(chunks, (unsigned)d), dim3(256), 0, st, dcp, dsz, chunks, part);
     hipLaunchKernelGGL(k_core_scale, dim3((unsigned)((d + 255) / 256)), dim3(256), 0, st, (int)d, chunks, part, sv,
                        sq);
     TTG_HIP(hipGetLastError());
-
-    // ---- the two Gram chains, concurrently -------------------------------------
-    TTG_HIP(hipEventRecord(ctx->ev_fork, st));
-    TTG_HIP(hipStreamWaitEvent(sd, ctx->ev_fork, 0));
+  };
+  auto enq_right = [&](hipStream_t s) {
     for (int64_t i = d - 1; i >= 1; --i) {  // right: H_i from H_{i+1}
       const int64_t m = rr[i] * n[i], k = rr[i + 1];
       const double *Hn = (i + 1 < d) ? H + sq_ofs[i + 1] : d_one;
-      gemm_scaled(st, false, false, m, k, k, sv + i, cp[i], m, Hn, k, Tr, m, wsr, gws, cus);
-      gemm<double>(st, false, true, rr[i], rr[i], n[i] * k, 1.0, Tr, rr[i], cp[i], rr[i], 0.0, H + sq_ofs[i], rr[i],
+      gemm_scaled(s, false, false, m, k, k, sv + i, cp[i], m, Hn, k, Tr, m, wsr, gws, cus);
+      gemm<double>(s, false, true, rr[i], rr[i], n[i] * k, 1.0, Tr, rr[i], cp[i], rr[i], 0.0, H + sq_ofs[i], rr[i],
                    wsr, gws, cus);
     }
+  };
+  auto enq_left = [&](hipStream_t s) {
     for (int64_t i = 0; i + 1 < d; ++i) {  // left: G_{i+1} from G_i
       const int64_t k0 = rr[i], N = n[i] * rr[i + 1];
       const double *Gp = i > 0 ? Gm + sq_ofs[i] : d_one;
-      gemm_scaled(sd, false, false, k0, N, k0, sv + i, Gp, k0, cp[i], k0, Tl, k0, wsl, gws, cus);
-      gemm<double>(sd, true, false, rr[i + 1], rr[i + 1], k0 * n[i], 1.0, cp[i], k0 * n[i], Tl, k0 * n[i], 0.0,
+      gemm_scaled(s, false, false, k0, N, k0, sv + i, Gp, k0, cp[i], k0, Tl, k0, wsl, gws, cus);
+      gemm<double>(s, true, false, rr[i + 1], rr[i + 1], k0 * n[i], 1.0, cp[i], k0 * n[i], Tl, k0 * n[i], 0.0,
                    Gm + sq_ofs[i + 1], rr[i + 1], wsl, gws, cus);
     }
-    TTG_HIP(hipEventRecord(ctx->ev_join, sd));
-    TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
-
+  };
+  auto enq_rest = [&]() {
     // ---- batched factorisations, inverses, products, verdict ----------------------
     chol_batched(st, dcj, njc, kmax);
     trsm3_batched(st, dtj, njc, kmax, kmax);
@@ -578,41 +579,50 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     }
     TTG_HIP(hipGetLastError());
   };
+  // the whole sequence on st, the left chain forked to sd (eager runs and the
+  // one-graph form)
+  auto enqueue = [&]() {
+    enq_setup();
+    TTG_HIP(hipEventRecord(ctx->ev_fork, st));
+    TTG_HIP(hipStreamWaitEvent(sd, ctx->ev_fork, 0));
+    enq_right(st);
+    enq_left(sd);
+    TTG_HIP(hipEventRecord(ctx->ev_join, sd));
+    TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    enq_rest();
+  };
+  auto capture = [&](auto &&fn) -> hipGraph_t {
+    hipGraph_t g = nullptr;
+    TTG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      fn();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &g);  // leave the stream usable
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    TTG_HIP(hipStreamEndCapture(st, &g));
+    return g;
+  };
   if (replay) {
-    TTG_HIP(hipGraphLaunch(ctx->ggraph, st));
-    ++ctx->gruns;
+    gram_graph_launch(ctx);
   } else {
     TTG_HIP(hipMemcpyAsync(tab, hp, tb, hipMemcpyHostToDevice, st));
-    if (ctx->ggraph) {
-      TTG_HIP(hipGraphExecDestroy(ctx->ggraph));
-      ctx->ggraph = nullptr;
-      ctx->gkey.clear();
-    }
-    if (ctx->ggraph_src) {
-      TTG_HIP(hipGraphDestroy(ctx->ggraph_src));
-      ctx->ggraph_src = nullptr;
-    }
+    gram_graph_release(ctx);
     if (use_graph) {
-      hipGraph_t g = nullptr;
-      TTG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      try {
-        enqueue();
-      } catch (...) {
-        (void)hipStreamEndCapture(st, &g);  // leave the stream usable
-        if (g) (void)hipGraphDestroy(g);
-        throw;
+      hipGraph_t parts[4] = {capture(enq_setup), capture([&] { enq_right(st); }), capture([&] { enq_left(st); }),
+                             capture(enq_rest)};
+      hipError_t ie = hipSuccess;
+      for (int q = 0; q < 4; ++q) {
+        if (ie == hipSuccess) ie = hipGraphInstantiate(&ctx->ggraph[q], parts[q], nullptr, nullptr, 0);
+        (void)hipGraphDestroy(parts[q]);
       }
-      TTG_HIP(hipStreamEndCapture(st, &g));
-      hipGraphExec_t ge = nullptr;
-      const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-      if (ie != hipSuccess) (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) gram_graph_release(ctx);
       TTG_HIP(ie);
-      ctx->ggraph = ge;
-      ctx->ggraph_src = g;  // kept for ttg_round_gram_graph (child-node form)
+      ctx->ggraph_src = capture(enqueue);  // kept for ttg_round_gram_graph (child-node form)
       ctx->gkey = key;
       ctx->gok = okd;
-      TTG_HIP(hipGraphLaunch(ctx->ggraph, st));
-      ++ctx->gruns;
+      gram_graph_launch(ctx);
     } else {
       enqueue();
     }
@@ -646,6 +656,30 @@ namespace {
 __global__ void k_copy_flag(const int *src, int *dst) { *dst = *src; }
 }  // namespace
 
+void gram_graph_launch(ttg_ctx *ctx) {
+  hipStream_t st = ctx->stream, sd = ctx->side;
+  TTG_HIP(hipGraphLaunch(ctx->ggraph[0], st));
+  TTG_HIP(hipEventRecord(ctx->ev_fork, st));
+  TTG_HIP(hipStreamWaitEvent(sd, ctx->ev_fork, 0));
+  TTG_HIP(hipGraphLaunch(ctx->ggraph[2], sd));  // left chain
+  TTG_HIP(hipGraphLaunch(ctx->ggraph[1], st));  // right chain
+  TTG_HIP(hipEventRecord(ctx->ev_join, sd));
+  TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+  TTG_HIP(hipGraphLaunch(ctx->ggraph[3], st));
+  ++ctx->gruns;
+}
+
+void gram_graph_release(ttg_ctx *ctx) {
+  for (auto &g : ctx->ggraph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  ctx->gkey.clear();
+  if (ctx->ggraph_src) (void)hipGraphDestroy(ctx->ggraph_src);
+  ctx->ggraph_src = nullptr;
+  ctx->gprep.clear();
+}
+
 // Runs the Gram-chain round once (capturing its launch sequence for these
 // shapes, pointers, eps and form) and keeps it for round_gram_launch.
 ttg_status round_gram_prepare(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore,
@@ -672,7 +706,7 @@ ttg_status round_gram_prepare(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
   const bool exact = prec == TTG_PREC_FP64;
   const bool ok = round_gram(ctx, d, n, r, dcore, eps, nullptr, exact, exact ? 0.0 : 1e-6, r_out, dcore_out, cap,
                              len_out, nullptr);
-  TTG_CHECK(ctx->gruns != before && ctx->ggraph, TTG_NEED_SPACE,
+  TTG_CHECK(ctx->gruns != before && ctx->ggraph[0], TTG_NEED_SPACE,
             "the round did not run as a captured launch sequence (output capacity below the QR-capped size "
             "of ttg_round_capacity, or kernel-class profiling active)");
   ctx->gprep = ctx->gkey;
@@ -685,11 +719,10 @@ ttg_status round_gram_prepare(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
 // proven this time (dcore_out holds the rounded tensor), 0 otherwise.
 ttg_status round_gram_launch(ttg_ctx *ctx, int *dstatus) {
   TTG_CHECK(dstatus, TTG_BAD_ARG, "null status pointer");
-  TTG_CHECK(ctx->ggraph && !ctx->gprep.empty() && ctx->gprep == ctx->gkey, TTG_BAD_ARG,
+  TTG_CHECK(ctx->ggraph[0] && !ctx->gprep.empty() && ctx->gprep == ctx->gkey, TTG_BAD_ARG,
             "no prepared Gram-chain round on this context (call ttg_round_gram_prepare; a later round of "
             "another shape replaces it)");
-  TTG_HIP(hipGraphLaunch(ctx->ggraph, ctx->stream));
-  ++ctx->gruns;
+  gram_graph_launch(ctx);
   hipLaunchKernelGGL(k_copy_flag, dim3(1), dim3(1), 0, ctx->stream, (const int *)ctx->gok, dstatus);
   TTG_HIP(hipGetLastError());
   return TTG_OK;

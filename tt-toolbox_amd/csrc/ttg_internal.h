@@ -1,6 +1,7 @@
 // Internal declarations shared by the ttgpu HIP sources (gfx950 only).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -76,13 +77,18 @@ struct ArenaSizer {
   template <class T> void take(size_t count) { used += ((count * sizeof(T)) + 255) & ~size_t(255); }
 };
 
-// Kernel-class timing: events around every launch of one class.
+// Kernel-class timing of one class.  Two ways: an event pair recorded on the
+// stream around a scope (begin/end: the time includes the dispatch of the
+// scope's kernels), or, for scopes built with `own`, a pair per kernel
+// passed to hipExtLaunchKernel, which stamps the kernel's own start and end
+// (the execution time a kernel trace reports).
 enum KernelClass { KC_NONE = 0, KC_GEMM = 1, KC_JACOBI = 2, KC_CHOL = 3, KC_TRTRI = 4, KC_HQR = 5 };
 struct Profiler {
   int kind = KC_NONE;
   std::vector<hipEvent_t> pool;  // pairs
   size_t used = 0;               // pairs in use
   std::vector<double> flops, bytes;
+  size_t take_pair();  // a fresh pair (flops and bytes 0); returns its index
   void begin(hipStream_t s);
   void end(hipStream_t s, double fl, double by);
 };
@@ -92,16 +98,45 @@ struct ProfScope {
   Profiler *p;
   hipStream_t s;
   double fl, by;
-  ProfScope(int kind, hipStream_t st, double f, double b) : p(nullptr), s(st), fl(f), by(b) {
+  bool own;
+  size_t first = ~size_t(0);
+  ProfScope(int kind, hipStream_t st, double f, double b, bool own_events = false)
+      : p(nullptr), s(st), fl(f), by(b), own(own_events) {
     if (g_prof && g_prof->kind == kind) {
       p = g_prof;
-      p->begin(s);
+      if (!own) p->begin(s);
     }
   }
+  // events for the scope's next kernel (own mode); the scope's flops and
+  // bytes go to its first kernel
+  void events(hipEvent_t *a, hipEvent_t *b) {
+    const size_t i = p->take_pair();
+    if (first == ~size_t(0)) first = i;
+    *a = p->pool[2 * i];
+    *b = p->pool[2 * i + 1];
+  }
   ~ProfScope() {
-    if (p) p->end(s, fl, by);
+    if (!p) return;
+    if (!own) {
+      p->end(s, fl, by);
+    } else if (first != ~size_t(0)) {
+      p->flops[first] += fl;
+      p->bytes[first] += by;
+    }
   }
 };
+// Launch inside an own-mode ProfScope: with the kernel's own timestamps when
+// the scope is profiling, a plain launch otherwise.
+#define TTG_TIMED_LAUNCH(ps, K, grid, block, shm, st, ...)                      \
+  do {                                                                          \
+    if ((ps).p) {                                                               \
+      hipEvent_t e0_, e1_;                                                      \
+      (ps).events(&e0_, &e1_);                                                  \
+      hipExtLaunchKernelGGL(K, grid, block, shm, st, e0_, e1_, 0, __VA_ARGS__); \
+    } else {                                                                    \
+      hipLaunchKernelGGL(K, grid, block, shm, st, __VA_ARGS__);                 \
+    }                                                                           \
+  } while (0)
 
 }  // namespace ttg
 
@@ -133,13 +168,16 @@ struct ttg_ctx {
   std::vector<uint64_t> pgraph_key;
   int batch_mode = 0;
   // Gram-chain round: its workspace (job tables included) and its captured
-  // launch sequence with the shapes and pointers it was captured for
+  // launch sequence with the shapes and pointers it was captured for, in
+  // four parts (set-up, right chain, left chain, factorisations and output)
+  // whose chains replay on two streams at once: a graph's own branches
+  // replay one after the other on one queue
   ttg::Arena garena;
-  hipGraphExec_t ggraph = nullptr;
+  hipGraphExec_t ggraph[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<uint64_t> gkey;
-  hipGraph_t ggraph_src = nullptr;  // the captured graph behind ggraph (child-node form)
+  hipGraph_t ggraph_src = nullptr;  // the whole sequence as one graph (child-node form)
   int *gok = nullptr;               // its proof flag (device)
-  uint64_t gruns = 0;               // launches of ggraph so far
+  uint64_t gruns = 0;               // replays so far
   std::vector<uint64_t> gprep;      // key of the prepared launch (ttg_round_gram_prepare)
   // cores of an assembled sum a X + b Y (plus fused with round), kept apart
   // from the rounding workspaces that read them
@@ -176,6 +214,10 @@ bool round_small_persist(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_
 bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, const double *dcore, double eps,
                 const int64_t *rmax, bool exact, double orth_tol, int64_t *r_out, double *dcore_out, int64_t cap,
                 int64_t *len_out, ttg_info *info);
+// the Gram-chain round's captured parts: launch on ctx's stream (chains
+// forked to the side stream and joined back), release
+void gram_graph_launch(ttg_ctx *ctx);
+void gram_graph_release(ttg_ctx *ctx);
 // the shape tests of round_gram (d >= 2, capped ranks <= 256, no binding
 // rmax, the form not switched off): true when the form may be tried
 bool gram_shape_ok(int64_t d, const int64_t *n, const int64_t *r, const int64_t *rmax);
@@ -226,8 +268,9 @@ struct BGemmJob {
 // All jobs in one launch (same op(A), op(B)).  bgemm_plan fills the jobs'
 // first blocks on the host and returns the grid size; the table is then
 // copied to device memory (jobs_dev) by the caller and bgemm_run launches it.
-int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, bool *quarter, double *flops);
-void bgemm_run(hipStream_t st, bool ta, bool tb, const BGemmJob *jobs_dev, int njobs, int total, bool quarter,
+// shape: 0 = 32 x 64 blocks, 1 = 32 x 32, 2 = 64 x 64
+int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, int *shape, double *flops);
+void bgemm_run(hipStream_t st, bool ta, bool tb, const BGemmJob *jobs_dev, int njobs, int total, int shape,
                double flops);
 
 // y[i] = (T)x[i]

@@ -21,7 +21,7 @@ namespace ttg {
 
 thread_local Profiler *g_prof = nullptr;
 
-void Profiler::begin(hipStream_t s) {
+size_t Profiler::take_pair() {
   if (pool.size() < 2 * (used + 1)) {
     hipEvent_t a, b;
     TTG_HIP(hipEventCreate(&a));
@@ -29,13 +29,19 @@ void Profiler::begin(hipStream_t s) {
     pool.push_back(a);
     pool.push_back(b);
   }
-  TTG_HIP(hipEventRecord(pool[2 * used], s));
+  flops.push_back(0.0);
+  bytes.push_back(0.0);
+  return used++;
+}
+void Profiler::begin(hipStream_t s) {
+  const size_t i = take_pair();
+  TTG_HIP(hipEventRecord(pool[2 * i], s));
 }
 void Profiler::end(hipStream_t s, double fl, double by) {
-  TTG_HIP(hipEventRecord(pool[2 * used + 1], s));
-  flops.push_back(fl);
-  bytes.push_back(by);
-  ++used;
+  const size_t i = used - 1;
+  TTG_HIP(hipEventRecord(pool[2 * i + 1], s));
+  flops[i] = fl;
+  bytes[i] = by;
 }
 
 // ---------------------------------------------------------------------------
@@ -234,12 +240,16 @@ __global__ __launch_bounds__(256) void k_bgemm(const BGemmJob *__restrict__ jobs
                                       T(0), static_cast<T *>(jb.C), jb.ldc, nullptr);
 }
 
-int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, bool *quarter, double *flops) {
-  // 32 x 64 blocks; 32 x 32 when op(B) = B and the batch has few blocks
-  int tiles_all = 0;
-  for (int j = 0; j < njobs; ++j) tiles_all += ((jobs[j].M + 31) / 32) * ((jobs[j].N + 63) / 64);
-  const bool q = !tb && tiles_all < 1024;
-  const int bm = 32, bn = q ? 32 : 64;
+int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, int *shape, double *flops) {
+  // 64 x 64 blocks when they give at least two per CU (256 CUs), else
+  // 32 x 64; 32 x 32 when op(B) = B and the batch has few blocks
+  int tiles_all = 0, tiles_big = 0;
+  for (int j = 0; j < njobs; ++j) {
+    tiles_all += ((jobs[j].M + 31) / 32) * ((jobs[j].N + 63) / 64);
+    tiles_big += ((jobs[j].M + 63) / 64) * ((jobs[j].N + 63) / 64);
+  }
+  const int sh = tiles_big >= 512 ? 2 : (!tb && tiles_all < 1024 ? 1 : 0);
+  const int bm = sh == 2 ? 64 : 32, bn = sh == 1 ? 32 : 64;
   int total = 0;
   double fl = 0.0;
   for (int j = 0; j < njobs; ++j) {
@@ -247,22 +257,24 @@ int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, bool *quarter, double *flops)
     total += ((jobs[j].M + bm - 1) / bm) * ((jobs[j].N + bn - 1) / bn);
     fl += 2.0 * jobs[j].M * (double)jobs[j].N * jobs[j].K;
   }
-  if (quarter) *quarter = q;
+  if (shape) *shape = sh;
   if (flops) *flops = fl;
   return total;
 }
 
-void bgemm_run(hipStream_t st, bool ta, bool tb, const BGemmJob *jobs_dev, int njobs, int total, bool quarter,
+void bgemm_run(hipStream_t st, bool ta, bool tb, const BGemmJob *jobs_dev, int njobs, int total, int shape,
                double flops) {
   if (njobs <= 0 || total <= 0) return;
-  ProfScope prof(KC_GEMM, st, flops, 0.0);
+  ProfScope prof(KC_GEMM, st, flops, 0.0, true);
   const dim3 grid((unsigned)total);
-#define TTG_BGEMM(TA_, TB_)                                                                            \
-  do {                                                                                                 \
-    if (quarter)                                                                                       \
-      hipLaunchKernelGGL((k_bgemm<double, TA_, false, 32, 32>), grid, dim3(256), 0, st, jobs_dev, njobs); \
-    else                                                                                               \
-      hipLaunchKernelGGL((k_bgemm<double, TA_, TB_, 32, 64>), grid, dim3(256), 0, st, jobs_dev, njobs);   \
+#define TTG_BGEMM(TA_, TB_)                                                                                   \
+  do {                                                                                                        \
+    if (shape == 1)                                                                                           \
+      TTG_TIMED_LAUNCH(prof, (k_bgemm<double, TA_, false, 32, 32>), grid, dim3(256), 0, st, jobs_dev, njobs); \
+    else if (shape == 2)                                                                                      \
+      TTG_TIMED_LAUNCH(prof, (k_bgemm<double, TA_, TB_, 64, 64>), grid, dim3(256), 0, st, jobs_dev, njobs);   \
+    else                                                                                                      \
+      TTG_TIMED_LAUNCH(prof, (k_bgemm<double, TA_, TB_, 32, 64>), grid, dim3(256), 0, st, jobs_dev, njobs);   \
   } while (0)
   if (!ta && !tb) TTG_BGEMM(false, false);
   else if (ta && !tb) TTG_BGEMM(true, false);
@@ -335,18 +347,18 @@ static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, in
   const int bm = half ? 32 : GBM, bn = quarter ? 32 : GBN;
   dim3 grid((unsigned)((M + bm - 1) / bm), (unsigned)((N + bn - 1) / bn), (unsigned)S);
   ProfScope prof(KC_GEMM, st, 2.0 * (double)M * (double)N * (double)K,
-                 (double)sizeof(TI) * ((double)M * K + (double)K * N) + (double)sizeof(TO) * M * N);
-#define TTG_GEMM_LAUNCH(TA_, TB_)                                                                             \
-  do {                                                                                                        \
-    if (quarter)                                                                                              \
-      hipLaunchKernelGGL((k_gemm<TI, T, TO, TA_, false, 32, 32>), grid, dim3(256), 0, st, (int)M, (int)N,     \
-                         (int)K, kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev); \
-    else if (half)                                                                                            \
-      hipLaunchKernelGGL((k_gemm<TI, T, TO, TA_, TB_, 32>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K,   \
-                         kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);       \
-    else                                                                                                      \
-      hipLaunchKernelGGL((k_gemm<TI, T, TO, TA_, TB_, GBM>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K,  \
-                         kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);       \
+                 (double)sizeof(TI) * ((double)M * K + (double)K * N) + (double)sizeof(TO) * M * N, true);
+#define TTG_GEMM_LAUNCH(TA_, TB_)                                                                                \
+  do {                                                                                                           \
+    if (quarter)                                                                                                 \
+      TTG_TIMED_LAUNCH(prof, (k_gemm<TI, T, TO, TA_, false, 32, 32>), grid, dim3(256), 0, st, (int)M, (int)N,    \
+                       (int)K, kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);    \
+    else if (half)                                                                                               \
+      TTG_TIMED_LAUNCH(prof, (k_gemm<TI, T, TO, TA_, TB_, 32>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K,  \
+                       kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);            \
+    else                                                                                                         \
+      TTG_TIMED_LAUNCH(prof, (k_gemm<TI, T, TO, TA_, TB_, GBM>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K, \
+                       kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);            \
   } while (0)
   if (!ta && !tb) TTG_GEMM_LAUNCH(false, false);
   else if (ta && !tb) TTG_GEMM_LAUNCH(true, false);
@@ -357,8 +369,8 @@ static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, in
   if (S > 1) {
     const long total = (long)M * N;
     const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
-    hipLaunchKernelGGL((k_splitk_reduce<T, TO>), dim3(blocks), dim3(256), 0, st, (int)M, (int)N,
-                       S, (const T *)ws, (double)alpha, (double)beta, C, (long)ldc, alpha_dev);
+    TTG_TIMED_LAUNCH(prof, (k_splitk_reduce<T, TO>), dim3(blocks), dim3(256), 0, st, (int)M, (int)N, S,
+                     (const T *)ws, (double)alpha, (double)beta, C, (long)ldc, alpha_dev);
     TTG_HIP(hipGetLastError());
   }
 }
@@ -448,16 +460,16 @@ static void gram_nearid_impl(hipStream_t st, int64_t m, int64_t k, const double 
     const int bm = half ? 32 : GBM, bn = quarter ? 32 : GBN;
     dim3 grid((unsigned)((k + bm - 1) / bm), (unsigned)((k + bn - 1) / bn), (unsigned)S);
     {
-      ProfScope prof(KC_GEMM, st, 2.0 * (double)k * k * m, 8.0 * (2.0 * m * k + k * k));
+      ProfScope prof(KC_GEMM, st, 2.0 * (double)k * k * m, 8.0 * (2.0 * m * k + k * k), true);
       if (quarter)
-        hipLaunchKernelGGL((k_gemm<double, T, double, true, false, 32, 32>), grid, dim3(256), 0, st, (int)k, (int)k,
-                           (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, nullptr);
+        TTG_TIMED_LAUNCH(prof, (k_gemm<double, T, double, true, false, 32, 32>), grid, dim3(256), 0, st, (int)k,
+                         (int)k, (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, (const T *)nullptr);
       else if (half)
-        hipLaunchKernelGGL((k_gemm<double, T, double, true, false, 32>), grid, dim3(256), 0, st, (int)k, (int)k, (int)m,
-                           kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, nullptr);
+        TTG_TIMED_LAUNCH(prof, (k_gemm<double, T, double, true, false, 32>), grid, dim3(256), 0, st, (int)k, (int)k,
+                         (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, (const T *)nullptr);
       else
-        hipLaunchKernelGGL((k_gemm<double, T, double, true, false, GBM>), grid, dim3(256), 0, st, (int)k, (int)k,
-                           (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, nullptr);
+        TTG_TIMED_LAUNCH(prof, (k_gemm<double, T, double, true, false, GBM>), grid, dim3(256), 0, st, (int)k, (int)k,
+                         (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, (const T *)nullptr);
       TTG_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL((k_gram_nearid<T>), dim3(blocks), dim3(256), 0, st, (int)k, S, (const T *)ws, X, R2, maxabs);
@@ -1195,10 +1207,10 @@ void gemm_mtimes(hipStream_t s, int64_t M, const double *R, int64_t ldr, int64_t
   TTG_CHECK(gy <= 65535, TTG_BAD_ARG, "product core has too many columns");
   // algorithmic flops of R * P; bytes: R, C and the two factor cores (P is never stored)
   ProfScope prof(KC_GEMM, s, 2.0 * (double)M * N * K,
-                 8.0 * ((double)M * K + (double)M * N + (double)ra0 * n * m * ra1 + (double)rx0 * m * rx1));
-  hipLaunchKernelGGL(k_gemm_mtimes, dim3((unsigned)((M + MP_TM - 1) / MP_TM), (unsigned)gy), dim3(256), 0, s,
-                     (int)M, (int)N, (int)K, R, (long)ldr, (int)ra0, (int)n, (int)m, (int)ra1, A, (int)rx0, X,
-                     C, (long)ldc);
+                 8.0 * ((double)M * K + (double)M * N + (double)ra0 * n * m * ra1 + (double)rx0 * m * rx1), true);
+  TTG_TIMED_LAUNCH(prof, k_gemm_mtimes, dim3((unsigned)((M + MP_TM - 1) / MP_TM), (unsigned)gy), dim3(256), 0, s,
+                   (int)M, (int)N, (int)K, R, (long)ldr, (int)ra0, (int)n, (int)m, (int)ra1, A, (int)rx0, X, C,
+                   (long)ldc);
   TTG_HIP(hipGetLastError());
 }
 template void mtimes_core<float>(hipStream_t, int64_t, int64_t, int64_t, int64_t, const double *, int64_t, int64_t, const double *, float *);
