@@ -412,14 +412,14 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     tj[b] = TrsmJob{nullptr, RR + o, Wv + (size_t)b * wjob, RiR + o, k, k, 0, k};
     tj[nb + b] = TrsmJob{nullptr, RL + o, Wv + (size_t)(nb + b) * wjob, RiL + o, k, k, 0, k};
     gb[b] = GramBond{Gm + o, H + o, RiR + o, Zm + o, k, 0};
-    jz[b] = BGemmJob{RiR + o, RiL + o, Zm + o, nullptr, 1.0, k, k, k, k, k, k, 0, 0};
+    jz[b] = BGemmJob{RiR + o, RiL + o, Zm + o, nullptr, 1.0, k, k, k, k, k, k, 0, 2 | 4};
     const int64_t N = n[i] * rr[i + 1];
     const double *src = (i + 1 < d) ? Ab + out_ofs[i] : cp[i];
-    jb[b] = BGemmJob{RiR + o, src, dcore_out + out_ofs[i], alpha + i, 1.0, k, (int)N, k, k, k, k, 0, 0};
+    jb[b] = BGemmJob{RiR + o, src, dcore_out + out_ofs[i], alpha + i, 1.0, k, (int)N, k, k, k, k, 0, 2};
   }
   for (int64_t i = 0; i + 1 < d; ++i) {
     const int m = (int)(rr[i] * n[i]), k = (int)rr[i + 1];
-    ja[i] = BGemmJob{cp[i], RR + sq_ofs[i + 1], Ab + out_ofs[i], nullptr, 1.0, m, k, k, m, k, m, 0, 0};
+    ja[i] = BGemmJob{cp[i], RR + sq_ofs[i + 1], Ab + out_ofs[i], nullptr, 1.0, m, k, k, m, k, m, 0, 1};
   }
   int qa, qz, qb;
   double fa, fz, fb;

@@ -263,13 +263,17 @@ struct BGemmJob {
   double alpha;
   int M, N, K, lda, ldb, ldc;
   int tile0;  // first grid block of the job (set by bgemm_f64)
-  int pad;
+  // triangular operands whose vanishing part is exactly zero in memory: the
+  // K tiles it covers are skipped (the sum is bitwise the full product's).
+  // 1: op(B) lower (block column n0 needs K >= n0); 2: op(A) lower (block row
+  // m0 needs K < m0 + BM); 4: op(B) upper (K < n0 + BN); 8: op(A) upper (K >= m0)
+  int tri;
 };
 // All jobs in one launch (same op(A), op(B)).  bgemm_plan fills the jobs'
 // first blocks on the host and returns the grid size; the table is then
 // copied to device memory (jobs_dev) by the caller and bgemm_run launches it.
-// shape: 0 = 32 x 64 blocks, 1 = 32 x 32, 2 = 64 x 64
-int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, int *shape, double *flops);
+// shape: 0 = 32 x 64 blocks, 1 = 32 x 32, 2 = 64 x 64 (force >= 0 imposes one)
+int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, int *shape, double *flops, int force = -1);
 void bgemm_run(hipStream_t st, bool ta, bool tb, const BGemmJob *jobs_dev, int njobs, int total, int shape,
                double flops);
 

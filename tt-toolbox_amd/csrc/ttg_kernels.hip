@@ -88,25 +88,22 @@ constexpr int GBM = 64, GBN = 64, GBK = 64;
 // TI is the storage type of A and B, T the MFMA type: TI = double, T = float
 // rounds the operands to FP32 while staging them (FP32 Gram products of FP64
 // data).
-// One BM x BN block of C over K range [kbeg, kend) with its top-left corner at
-// (m0, n0).  P != nullptr: the raw partial product goes to P (ld M) instead of
-// C (split-K).  Shared by k_gemm (grid tiles) and k_bgemm (batched jobs).
-template <class TI, class T, class TO, bool TA, bool TB, int BM, int BN>
-__device__ __forceinline__ void gemm_block(int M, int N, int kbeg, int kend, int m0, int n0, T alpha,
-                                           const TI *__restrict__ A, long lda, const TI *__restrict__ B,
-                                           long ldb, T beta, TO *__restrict__ C, long ldc,
-                                           T *__restrict__ P) {
+// One BM x BN block of op(A) op(B) over K range [kbeg, kend) with its
+// top-left corner at (m0, n0), accumulated in acc (layout below), staged
+// through the caller's LDS tiles As, Bs.
+template <class TI, class T, bool TA, bool TB, int BM, int BN>
+__device__ __forceinline__ void gemm_acc(int M, int N, int kbeg, int kend, int m0, int n0,
+                                         const TI *__restrict__ A, long lda, const TI *__restrict__ B, long ldb,
+                                         T (*As)[BM + 4], T (*Bs)[BN + 4],
+                                         typename MF<T>::V (&acc)[BM / 32][BN / 32]) {
   static_assert(!TB || BN == GBN, "op(B) = B^T stages 64 columns per K tile");
-  __shared__ T As[GBK][BM + 4];
   constexpr int MI = BM / 32;            // 16-row MFMA tiles per wave
   constexpr int NJ = BN / 32;            // 16-column MFMA tiles per wave
   constexpr int LA = BM * GBK / 256;     // A elements each thread stages per K tile
   constexpr int LB = BN * GBK / 256;     // B elements each thread stages per K tile
-  __shared__ T Bs[GBK][BN + 4];
   typedef typename MF<T>::V V;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  V acc[MI][NJ];
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -172,6 +169,24 @@ __device__ __forceinline__ void gemm_block(int M, int N, int kbeg, int kend, int
         for (int j = 0; j < NJ; ++j) acc[i][j] = MF<T>::op(b[j], a[i], acc[i][j]);
     }
   }
+}
+
+// One BM x BN block of C over K range [kbeg, kend) with its top-left corner at
+// (m0, n0).  P != nullptr: the raw partial product goes to P (ld M) instead of
+// C (split-K).  Shared by k_gemm (grid tiles) and k_bgemm (batched jobs).
+template <class TI, class T, class TO, bool TA, bool TB, int BM, int BN>
+__device__ __forceinline__ void gemm_block(int M, int N, int kbeg, int kend, int m0, int n0, T alpha,
+                                           const TI *__restrict__ A, long lda, const TI *__restrict__ B,
+                                           long ldb, T beta, TO *__restrict__ C, long ldc,
+                                           T *__restrict__ P) {
+  __shared__ T As[GBK][BM + 4];
+  __shared__ T Bs[GBK][BN + 4];
+  constexpr int MI = BM / 32, NJ = BN / 32;
+  typedef typename MF<T>::V V;
+  V acc[MI][NJ];
+  gemm_acc<TI, T, TA, TB, BM, BN>(M, N, kbeg, kend, m0, n0, A, lda, B, ldb, As, Bs, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
   // acc[i][j][v] of lane l: C(m0 + wm*BM/2 + i*16 + l%16, n0 + wn*BN/2 + j*16 + row(l, v))
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -235,12 +250,17 @@ __global__ __launch_bounds__(256) void k_bgemm(const BGemmJob *__restrict__ jobs
   const bool mfast = jb.N >= jb.M;
   const int im = mfast ? t % tm : t / tn, in = mfast ? t / tm : t % tn;
   const T a = jb.alpha_dev ? T(jb.alpha * jb.alpha_dev[0]) : T(jb.alpha);
-  gemm_block<T, T, T, TA, TB, BM, BN>(jb.M, jb.N, 0, jb.K, im * BM, in * BN, a,
+  int kb = 0, ke = jb.K;
+  if (jb.tri & 1) kb = max(kb, in * BN);
+  if (jb.tri & 8) kb = max(kb, im * BM);
+  if (jb.tri & 2) ke = min(ke, im * BM + BM);
+  if (jb.tri & 4) ke = min(ke, in * BN + BN);
+  gemm_block<T, T, T, TA, TB, BM, BN>(jb.M, jb.N, kb, ke, im * BM, in * BN, a,
                                       static_cast<const T *>(jb.A), jb.lda, static_cast<const T *>(jb.B), jb.ldb,
                                       T(0), static_cast<T *>(jb.C), jb.ldc, nullptr);
 }
 
-int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, int *shape, double *flops) {
+int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, int *shape, double *flops, int force) {
   // 64 x 64 blocks when they give at least two per CU (256 CUs), else
   // 32 x 64; 32 x 32 when op(B) = B and the batch has few blocks
   int tiles_all = 0, tiles_big = 0;
@@ -248,7 +268,8 @@ int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, int *shape, double *flops) {
     tiles_all += ((jobs[j].M + 31) / 32) * ((jobs[j].N + 63) / 64);
     tiles_big += ((jobs[j].M + 63) / 64) * ((jobs[j].N + 63) / 64);
   }
-  const int sh = tiles_big >= 512 ? 2 : (!tb && tiles_all < 1024 ? 1 : 0);
+  int sh = tiles_big >= 512 ? 2 : (!tb && tiles_all < 1024 ? 1 : 0);
+  if (force >= 0 && force <= 2 && !(force == 1 && tb)) sh = force;
   const int bm = sh == 2 ? 64 : 32, bn = sh == 1 ? 32 : 64;
   int total = 0;
   double fl = 0.0;
