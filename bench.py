@@ -109,10 +109,19 @@ def mtimes_leg(ttgpu, W, ctx, dev, fused=True):
     oy = torch.empty(ysize, dtype=torch.float64, device=dev)
     rate = _rate(lambda: ttgpu.mtimes_dev(ctx, na, ma, ra, da, rx, dx, out=oy), 50, sync)
     byt = 8.0 * (ysize + acore.size + xcore.size)
+    # the product-core kernel's own duration (HIP events passed to its launch)
+    ctx.profile_begin("mtimes")
+    for _ in range(20):
+        ttgpu.mtimes_dev(ctx, na, ma, ra, da, rx, dx, out=oy)
+    prof = ctx.profile_end()
+    kern_s = prof["ms"] / max(prof["launches"], 1) / 1e3
     out["config2_mtimes"] = {"workload": "A*x, d=12 n=m=4 rank_A=5 rank_x=64 -> rank 320, FP64, unfused",
                              "contractions_per_s": rate, "hbm_bytes_per_contraction": byt,
                              "roofline": roofline(byt, 1.0 / rate, "hbm",
-                                                  what="whole call incl. host launch overhead")}
+                                                  what="whole call incl. host launch overhead"),
+                             "kernel": {"name": "k_mtimes_all", "launches_per_contraction": 1,
+                                        "avg_launch_us": kern_s * 1e6,
+                                        "roofline": roofline(byt, kern_s, "hbm", what="k_mtimes_all launch")}}
     if not fused:
         return out
     o2 = torch.empty(ttgpu.capacity(na, ry), dtype=torch.float64, device=dev)

@@ -274,7 +274,7 @@ ttg_status ttg_adjoint_bilinear_f64_noalloc(ttg_ctx *ctx, int64_t d, const int64
  * bracketed by HIP events on the context's stream; end synchronises and
  * returns the launch count, summed kernel time and the algorithmic flops and
  * bytes of those launches.  Classes: 1 GEMM, 2 Jacobi SVD, 3 Cholesky,
- * 4 triangular inverse, 5 Householder QR. */
+ * 4 triangular inverse, 5 Householder QR, 6 A*x product cores. */
 ttg_status ttg_profile_begin(ttg_ctx *ctx, int kernel_class);
 ttg_status ttg_profile_end(ttg_ctx *ctx, int64_t *launches, double *ms_total,
                            double *flops_total, double *bytes_total);

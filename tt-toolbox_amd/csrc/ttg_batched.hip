@@ -76,8 +76,32 @@ void release_ctx(ttg_ctx *ctx) {
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   ctx->host_pinned = nullptr;
   ctx->host_pinned_cap = 0;
+  if (ctx->pinned_ev) (void)hipEventDestroy(ctx->pinned_ev);
+  ctx->pinned_ev = nullptr;
+  ctx->pinned_pending = false;
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
+}
+
+void *pinned_acquire(ttg_ctx *ctx, size_t bytes) {
+  if (ctx->pinned_pending) {
+    TTG_HIP(hipEventSynchronize(ctx->pinned_ev));
+    ctx->pinned_pending = false;
+  }
+  if (!ctx->host_pinned || ctx->host_pinned_cap < bytes) {
+    if (ctx->host_pinned) TTG_HIP(hipHostFree(ctx->host_pinned));
+    ctx->host_pinned = nullptr;
+    const size_t cap = std::max<size_t>(bytes, 4096);
+    TTG_HIP(hipHostMalloc(&ctx->host_pinned, cap, hipHostMallocDefault));
+    ctx->host_pinned_cap = cap;
+  }
+  return ctx->host_pinned;
+}
+
+void pinned_release(ttg_ctx *ctx, hipStream_t s) {
+  if (!ctx->pinned_ev) TTG_HIP(hipEventCreateWithFlags(&ctx->pinned_ev, hipEventDisableTiming));
+  TTG_HIP(hipEventRecord(ctx->pinned_ev, s));
+  ctx->pinned_pending = true;
 }
 
 void destroy_workers(ttg_ctx *ctx) {

@@ -450,15 +450,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   add(sizeof(double));
   char *tab = A.take<char>(tb);
   const size_t readback = 256;
-  if (!ctx->host_pinned || ctx->host_pinned_cap < tb + readback) {
-    TTG_HIP(hipStreamSynchronize(st));
-    if (ctx->host_pinned) TTG_HIP(hipHostFree(ctx->host_pinned));
-    ctx->host_pinned = nullptr;
-    const size_t c2 = std::max<size_t>(tb + readback, 4096);
-    TTG_HIP(hipHostMalloc(&ctx->host_pinned, c2, hipHostMallocDefault));
-    ctx->host_pinned_cap = c2;
-  }
-  char *hp = static_cast<char *>(ctx->host_pinned);
+  char *hp = static_cast<char *>(pinned_acquire(ctx, tb + readback));
   std::memcpy(hp + off[0], cj.data(), njc * sizeof(CholJob));
   std::memcpy(hp + off[1], tj.data(), njc * sizeof(TrsmJob));
   std::memcpy(hp + off[2], gb.data(), nb * sizeof(GramBond));

@@ -82,7 +82,7 @@ struct ArenaSizer {
 // scope's kernels), or, for scopes built with `own`, a pair per kernel
 // passed to hipExtLaunchKernel, which stamps the kernel's own start and end
 // (the execution time a kernel trace reports).
-enum KernelClass { KC_NONE = 0, KC_GEMM = 1, KC_JACOBI = 2, KC_CHOL = 3, KC_TRTRI = 4, KC_HQR = 5 };
+enum KernelClass { KC_NONE = 0, KC_GEMM = 1, KC_JACOBI = 2, KC_CHOL = 3, KC_TRTRI = 4, KC_HQR = 5, KC_MTIMES = 6 };
 struct Profiler {
   int kind = KC_NONE;
   std::vector<hipEvent_t> pool;  // pairs
@@ -147,8 +147,10 @@ struct ttg_ctx {
   int num_cus = 256;
   ttg::Arena arena;       // per-call scratch
   ttg::Arena persist;     // small persistent device scalars (barrier counters, flags)
-  void *host_pinned = nullptr;
+  void *host_pinned = nullptr;  // staging buffer: take it with ttg::pinned_acquire
   size_t host_pinned_cap = 0;
+  hipEvent_t pinned_ev = nullptr;  // recorded after the last copy enqueued out of it
+  bool pinned_pending = false;
   std::string last_error;
   // diagnostics of the current call
   int64_t qr_fallbacks = 0, jacobi_sweeps = 0, jacobi_maxsweeps = 0;
@@ -193,6 +195,12 @@ namespace ttg {
 
 // releases the batched-rounding worker contexts of ctx
 void destroy_workers(ttg_ctx *ctx);
+// The context's pinned host staging buffer, at least `bytes`: first waits for
+// the asynchronous copies last enqueued out of (or into) it, so the caller may
+// overwrite it; grows it when short.
+void *pinned_acquire(ttg_ctx *ctx, size_t bytes);
+// marks the copies just enqueued on s as the buffer's users
+void pinned_release(ttg_ctx *ctx, hipStream_t s);
 // releases every HIP object of ctx (workers, graph, streams, events, pinned
 // memory) in dependency order; the arenas go with `delete ctx`
 void release_ctx(ttg_ctx *ctx);
@@ -405,7 +413,10 @@ struct MvCore {
   double *Y;
   int64_t ra0, n, m, ra1, rx0, rx1;
 };
-void mtimes_all(hipStream_t s, const MvCore *tab_dev, int ncores, int64_t max_cols, int64_t max_rows, int64_t max_a);
+// max_units: the largest mtimes_units(ra0 rx0, rx1) over the cores
+int64_t mtimes_units(int64_t r0, int64_t rx1);
+// bytes: algorithmic HBM traffic (A, X read, Y written) for kernel-class timing
+void mtimes_all(hipStream_t s, const MvCore *tab_dev, int ncores, int64_t max_units, double bytes);
 // C (M x n ra1 rx1) = R (M x ra0 rx0, ld ldr) * (product core of A and X), the
 // product core generated on chip (never written to memory).
 void gemm_mtimes(hipStream_t s, int64_t M, const double *R, int64_t ldr, int64_t ra0, int64_t n,

@@ -1090,63 +1090,95 @@ void mtimes_core(hipStream_t s, int64_t ra0, int64_t n, int64_t m, int64_t ra1, 
 }
 template void mtimes_core<double>(hipStream_t, int64_t, int64_t, int64_t, int64_t, const double *, int64_t, int64_t, const double *, double *);
 
-// All d product cores in one launch (unfused A*x).  Block (x, y) covers a
-// contiguous run of columns of core y (a column is (j, b, be), the output's
-// index j + n (b + ra1 be)); the block stages core y's A (ra0 n m ra1
-// doubles, at most 16K) in LDS once, and each thread keeps one output row
-// (a + ra0 al, split once), so per column it reads m values of X (shared by
-// the n ra1 consecutive columns of one be, cached) and m values of A from
-// LDS, and the block writes the column's r0 rows contiguously.  Four
-// columns per step keep their loads in flight together.  Sizes are checked
-// below 2^31 by the caller, so the index arithmetic is 32-bit.
-constexpr int MV_ALDS = 16384;  // doubles of A staged in LDS
-__global__ __launch_bounds__(1024) void k_mtimes_all(const MvCore *__restrict__ tab, int cols_per_block) {
-  extern __shared__ __align__(16) double As[];
-  const MvCore c = tab[blockIdx.y];
-  const unsigned r0 = (unsigned)(c.ra0 * c.rx0), ncol = (unsigned)(c.n * c.ra1 * c.rx1);
-  const unsigned ra0 = (unsigned)c.ra0, rx0 = (unsigned)c.rx0, nn = (unsigned)c.n, mm = (unsigned)c.m;
-  const unsigned ra1 = (unsigned)c.ra1;
-  const unsigned c0 = blockIdx.x * (unsigned)cols_per_block;
-  if (c0 >= ncol) return;  // uniform over the block
-  const unsigned c1 = min(ncol, c0 + (unsigned)cols_per_block);
-  const unsigned asz = ra0 * nn * mm * ra1;
-  for (unsigned e = threadIdx.x; e < asz; e += blockDim.x) As[e] = c.A[e];
-  __syncthreads();
-  const unsigned qa = ra0 * nn;  // stride of the contracted index in A
-  for (unsigned row0 = 0; row0 < r0; row0 += blockDim.x) {
-    const unsigned row = row0 + threadIdx.x;
-    if (row >= r0) continue;
-    const unsigned a = row % ra0, al = row / ra0;
-    const double *xr = c.X + al;
-    for (unsigned col = c0; col < c1; col += 4) {
-      double s[4];
+// All d product cores in one launch (unfused A*x), y(row, j, b, be) =
+// sum_q A(a, j, q, b) X(al, q, be) with row = a + ra0 al and output column
+// j + n (b + ra1 be) (mtimes.m's order).  Block (x, y): core y, unit x =
+// (group of MV_BE consecutive be, 64-row block); thread (row, g): one row and
+// the columns (j + n b) = g mod 4.  For a fixed row and column the M entries
+// A(a, j, :, b) are the same for every be, so the thread keeps them in
+// registers (up to 32 per pass) and, per be, loads only its M values of X:
+// about one load per output instead of M + 1.  With the mode size a
+// compile-time constant the loads of a pass are issued together; 64
+// consecutive rows per wave make each column's store one contiguous 512-byte
+// run.  The table's pointers are global memory (global, not flat, loads and
+// stores).  No barrier: threads past the last row leave at once.  Sizes are
+// checked below 2^31 by the caller (32-bit indices).
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+constexpr int MV_BE = 8;  // be per block
+template <int M>
+__device__ __forceinline__ void mtimes_cols(gcdouble *A, gcdouble *X, gdouble *Y, unsigned ra0, unsigned nn,
+                                            unsigned mm, unsigned rx0, unsigned r0, unsigned cnt, unsigned g,
+                                            unsigned nbe, unsigned xs, size_t ys) {
+  const unsigned qs = ra0 * nn;  // stride of the contracted index in A
+  if (M > 0) {
+    constexpr int MM = M > 0 ? M : 1;
+    constexpr int NC = 32 / MM;  // columns per pass (A entries kept: NC M <= 32)
+    for (unsigned cb = g; cb < cnt; cb += 4 * NC) {
+      double av[NC][MM];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const unsigned cu = min(col + u, c1 - 1);
-        const unsigned j = cu % nn, rest = cu / nn, b = rest % ra1, be = rest / ra1;
-        const double *ap = As + a + ra0 * (j + nn * mm * b);
-        const double *xp = xr + (long)rx0 * mm * be;
-        double acc = 0.0;
-        for (unsigned q = 0; q < mm; ++q) acc = __builtin_fma(ap[qa * q], xp[(long)rx0 * q], acc);
-        s[u] = acc;
+      for (int u = 0; u < NC; ++u) {
+        const unsigned cl = min(cb + 4u * u, cnt - 1);  // past the end: repeat the last column, store nothing
+        const unsigned j = cl % nn, b = cl / nn;
+        gcdouble *ap = A + ra0 * (j + nn * mm * b);  // A(a, j, q, b) at ap[ra0 n q]
+#pragma unroll
+        for (int q = 0; q < M; ++q) av[u][q] = ap[qs * q];
       }
+      for (unsigned e = 0; e < nbe; ++e) {
+        double xv[MM];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (col + u < c1) c.Y[(long)(col + u) * r0 + row] = s[u];
+        for (int q = 0; q < M; ++q) xv[q] = X[xs * e + rx0 * q];
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {
+          double acc = 0.0;
+#pragma unroll
+          for (int q = 0; q < M; ++q) acc = __builtin_fma(av[u][q], xv[q], acc);
+          if (cb + 4u * u < cnt) Y[ys * e + (size_t)r0 * (cb + 4u * u)] = acc;
+        }
+      }
     }
+  } else {
+    for (unsigned e = 0; e < nbe; ++e)
+      for (unsigned cl = g; cl < cnt; cl += 4) {
+        const unsigned j = cl % nn, b = cl / nn;
+        gcdouble *ap = A + ra0 * (j + nn * mm * b);
+        double acc = 0.0;
+        for (unsigned q = 0; q < mm; ++q) acc = __builtin_fma(ap[qs * q], X[xs * e + rx0 * q], acc);
+        Y[ys * e + (size_t)r0 * cl] = acc;
+      }
   }
 }
-void mtimes_all(hipStream_t s, const MvCore *tab_dev, int ncores, int64_t max_cols, int64_t max_rows,
-                int64_t max_a) {
-  if (ncores <= 0 || max_cols <= 0) return;
-  TTG_CHECK(max_a <= MV_ALDS, TTG_BAD_ARG, "A*x: a core of A above 16K entries");
-  const int nt = (int)std::min<int64_t>(1024, (max_rows + 63) / 64 * 64);
-  // about 4 blocks per CU for the widest core, at least 8 columns each
-  const int64_t cpb = std::max<int64_t>(8, (max_cols + 1023) / 1024);
-  const int bx = (int)((max_cols + cpb - 1) / cpb);
-  const size_t lds = (size_t)max_a * sizeof(double);
-  TTG_HIP(hipFuncSetAttribute((const void *)k_mtimes_all, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_mtimes_all, dim3(bx, ncores), dim3(nt), lds, s, tab_dev, (int)cpb);
+__global__ __launch_bounds__(256) void k_mtimes_all(const MvCore *__restrict__ tab) {
+  const MvCore c = tab[blockIdx.y];
+  const unsigned ra0 = (unsigned)c.ra0, rx0 = (unsigned)c.rx0, nn = (unsigned)c.n, mm = (unsigned)c.m;
+  const unsigned ra1 = (unsigned)c.ra1, rx1 = (unsigned)c.rx1;
+  const unsigned r0 = ra0 * rx0, rb = (r0 + 63) / 64, nbg = (rx1 + MV_BE - 1) / MV_BE;
+  if (blockIdx.x >= rb * nbg) return;
+  const unsigned bg = blockIdx.x / rb;
+  const unsigned row = (blockIdx.x - bg * rb) * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+  if (row >= r0) return;
+  const unsigned be0 = bg * MV_BE, nbe = min((unsigned)MV_BE, rx1 - be0);
+  const unsigned a = row % ra0, al = row / ra0;
+  const unsigned cnt = nn * ra1;
+  const unsigned xs = rx0 * mm;                   // X(al, q, be) at X[xs be + rx0 q]
+  const size_t ys = (size_t)r0 * nn * ra1;        // column (j + n b) of be at Y[ys be + r0 (j + n b)]
+  gcdouble *A = (gcdouble *)c.A + a;
+  gcdouble *X = (gcdouble *)c.X + al + (size_t)xs * be0;
+  gdouble *Y = (gdouble *)c.Y + row + ys * be0;
+  switch (mm) {
+    case 1: mtimes_cols<1>(A, X, Y, ra0, nn, mm, rx0, r0, cnt, g, nbe, xs, ys); break;
+    case 2: mtimes_cols<2>(A, X, Y, ra0, nn, mm, rx0, r0, cnt, g, nbe, xs, ys); break;
+    case 3: mtimes_cols<3>(A, X, Y, ra0, nn, mm, rx0, r0, cnt, g, nbe, xs, ys); break;
+    case 4: mtimes_cols<4>(A, X, Y, ra0, nn, mm, rx0, r0, cnt, g, nbe, xs, ys); break;
+    case 8: mtimes_cols<8>(A, X, Y, ra0, nn, mm, rx0, r0, cnt, g, nbe, xs, ys); break;
+    default: mtimes_cols<0>(A, X, Y, ra0, nn, mm, rx0, r0, cnt, g, nbe, xs, ys); break;
+  }
+}
+int64_t mtimes_units(int64_t r0, int64_t rx1) { return (r0 + 63) / 64 * ((rx1 + MV_BE - 1) / MV_BE); }
+void mtimes_all(hipStream_t s, const MvCore *tab_dev, int ncores, int64_t max_units, double bytes) {
+  if (ncores <= 0 || max_units <= 0) return;
+  ProfScope prof(KC_MTIMES, s, 0.0, bytes, true);
+  TTG_TIMED_LAUNCH(prof, k_mtimes_all, dim3((unsigned)max_units, (unsigned)ncores), dim3(256), 0, s, tab_dev);
   TTG_HIP(hipGetLastError());
 }
 

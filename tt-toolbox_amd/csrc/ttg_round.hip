@@ -169,18 +169,7 @@ template <class T> struct Rounder {
   // ---- small helpers ------------------------------------------------------
   // 4 KB of pinned host memory for the per-bond read-backs (a plain DMA; the
   // synchronisation then waits only for the GPU)
-  double *pinned(size_t bytes = 4096) {
-    if (!ctx->host_pinned || ctx->host_pinned_cap < bytes) {
-      // (every earlier use of the buffer has been waited for: each read-back
-      // synchronises, and the finalisation reuses it only after the sweeps)
-      if (ctx->host_pinned) TTG_HIP(hipHostFree(ctx->host_pinned));
-      ctx->host_pinned = nullptr;
-      const size_t cap = std::max<size_t>(bytes, 4096);
-      TTG_HIP(hipHostMalloc(&ctx->host_pinned, cap, hipHostMallocDefault));
-      ctx->host_pinned_cap = cap;
-    }
-    return static_cast<double *>(ctx->host_pinned);
-  }
+  double *pinned(size_t bytes = 4096) { return static_cast<double *>(pinned_acquire(ctx, bytes)); }
   void g(bool ta, bool tb, int64_t M, int64_t N, int64_t K, const T *A, int64_t lda, const T *B,
          int64_t ldb, T *C, int64_t ldc) {
     gemm<T>(st, ta, tb, M, N, K, T(1), A, lda, B, ldb, T(0), C, ldc, gws, gws_elems, cus);
@@ -830,14 +819,12 @@ ttg_status mtimes_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *
   if (cap < need) return TTG_NEED_SPACE;
   // one launch for every core (the table goes through pinned memory)
   std::vector<MvCore> tab(d);
-  int64_t oa = 0, ox = 0, oy = 0, max_cols = 1, max_rows = 1, max_a = 1;
+  int64_t oa = 0, ox = 0, oy = 0, max_units = 1;
   for (int64_t i = 0; i < d; ++i) {
     tab[i] = MvCore{dacore + oa, dxcore + ox, dycore + oy, ra[i], n[i], m[i], ra[i + 1], rx[i], rx[i + 1]};
     TTG_CHECK(rp[i] * n[i] * rp[i + 1] < (int64_t(1) << 31) && ra[i] * n[i] * m[i] * ra[i + 1] < (int64_t(1) << 31),
               TTG_BAD_ARG, "a core of A*x has 2^31 or more entries");
-    max_cols = std::max(max_cols, n[i] * rp[i + 1]);
-    max_rows = std::max(max_rows, rp[i]);
-    max_a = std::max(max_a, ra[i] * n[i] * m[i] * ra[i + 1]);
+    max_units = std::max(max_units, mtimes_units(rp[i], rx[i + 1]));
     oa += ra[i] * n[i] * m[i] * ra[i + 1];
     ox += rx[i] * m[i] * rx[i + 1];
     oy += rp[i] * n[i] * rp[i + 1];
@@ -845,17 +832,11 @@ ttg_status mtimes_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *
   ctx->arena.reserve(d * sizeof(MvCore) + 4096);
   ctx->arena.reset();
   MvCore *dtab = ctx->arena.take<MvCore>(d);
-  if (!ctx->host_pinned || ctx->host_pinned_cap < d * sizeof(MvCore)) {
-    TTG_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->host_pinned) TTG_HIP(hipHostFree(ctx->host_pinned));
-    ctx->host_pinned = nullptr;
-    const size_t c2 = std::max<size_t>(d * sizeof(MvCore), 4096);
-    TTG_HIP(hipHostMalloc(&ctx->host_pinned, c2, hipHostMallocDefault));
-    ctx->host_pinned_cap = c2;
-  }
-  std::memcpy(ctx->host_pinned, tab.data(), d * sizeof(MvCore));
-  TTG_HIP(hipMemcpyAsync(dtab, ctx->host_pinned, d * sizeof(MvCore), hipMemcpyHostToDevice, ctx->stream));
-  mtimes_all(ctx->stream, dtab, (int)d, max_cols, max_rows, max_a);
+  void *hp = pinned_acquire(ctx, d * sizeof(MvCore));
+  std::memcpy(hp, tab.data(), d * sizeof(MvCore));
+  TTG_HIP(hipMemcpyAsync(dtab, hp, d * sizeof(MvCore), hipMemcpyHostToDevice, ctx->stream));
+  pinned_release(ctx, ctx->stream);
+  mtimes_all(ctx->stream, dtab, (int)d, max_units, 8.0 * (double)(oa + ox + oy));
   TTG_HIP(hipStreamSynchronize(ctx->stream));
   for (int64_t i = 0; i <= d; ++i) ry[i] = rp[i];
   return TTG_OK;

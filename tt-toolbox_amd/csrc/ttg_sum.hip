@@ -44,19 +44,6 @@ __global__ void k_plus_assemble(const PlusCore *tab) { plus_core_body(tab[blockI
 // one core, its description passed by value (no table upload)
 __global__ void k_plus_assemble_one(PlusCore c) { plus_core_body(c); }
 
-void *pinned_bytes(ttg_ctx *ctx, size_t bytes) {
-  if (!ctx->host_pinned || ctx->host_pinned_cap < bytes) {
-    if (ctx->host_pinned) {
-      TTG_HIP(hipStreamSynchronize(ctx->stream));
-      TTG_HIP(hipHostFree(ctx->host_pinned));
-    }
-    ctx->host_pinned = nullptr;
-    const size_t cap = std::max<size_t>(bytes, 4096);
-    TTG_HIP(hipHostMalloc(&ctx->host_pinned, cap, hipHostMallocDefault));
-    ctx->host_pinned_cap = cap;
-  }
-  return ctx->host_pinned;
-}
 
 }  // namespace
 
@@ -123,9 +110,10 @@ int64_t plus_table(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx,
     oy += ry[k] * n[k] * ry[k + 1];
   }
   const size_t bytes = tab.size() * sizeof(PlusCore);
-  void *hp = pinned_bytes(ctx, bytes);
+  void *hp = pinned_acquire(ctx, bytes);
   std::memcpy(hp, tab.data(), bytes);
   TTG_HIP(hipMemcpyAsync(tab_dev, hp, bytes, hipMemcpyHostToDevice, ctx->stream));
+  pinned_release(ctx, ctx->stream);
   return mx;
 }
 
@@ -183,9 +171,10 @@ ttg_status dot_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *rx,
   double *p[2] = {ctx->arena.take<double>(pmax), ctx->arena.take<double>(pmax)};
   double *T = ctx->arena.take<double>(tmax);
   double *ws = ctx->arena.take<double>(wsmax);
-  double *hp = static_cast<double *>(pinned_bytes(ctx, sizeof(double)));
+  double *hp = static_cast<double *>(pinned_acquire(ctx, sizeof(double)));
   hp[0] = 1.0;
   TTG_HIP(hipMemcpyAsync(p[0], hp, sizeof(double), hipMemcpyHostToDevice, st));
+  pinned_release(ctx, st);
   int64_t ox = 0, oy = 0;
   int cur = 0;
   for (int64_t k = 0; k < d; ++k) {

@@ -43,7 +43,7 @@ EXPORTS = [
 # kernel-level test hooks (include/ttgpu_testing.h)
 TEST_EXPORTS = ["ttg_test_gemm", "ttg_test_householder", "ttg_test_chol", "ttg_test_trsm", "ttg_test_jacobi", "ttg_test_jacobi_timing",
                 "ttg_test_chol_timing", "ttg_test_trsm_timing", "ttg_test_chol_trsm"]
-KERNEL_CLASSES = {"gemm": 1, "jacobi": 2, "cholesky": 3, "trtri": 4, "householder": 5}
+KERNEL_CLASSES = {"gemm": 1, "jacobi": 2, "cholesky": 3, "trtri": 4, "householder": 5, "mtimes": 6}
 
 
 class TTGError(RuntimeError):
