@@ -32,6 +32,27 @@ def test_gemm(gpu_ctx, prec, ta, tb, M, N, K):
     assert np.max(np.abs(Cm - ref)) <= tol * np.max(np.abs(ref)) * np.sqrt(K), np.max(np.abs(Cm - ref))
 
 
+@pytest.mark.parametrize("ta,tb,M,N,K", [(0, 0, 45, 70, 130), (1, 0, 33, 64, 257), (0, 1, 100, 64, 31),
+                                          (1, 1, 70, 65, 96), (0, 0, 2048, 256, 256), (1, 0, 256, 256, 2048)])
+def test_gemm_padded_leading_dims(gpu_ctx, ta, tb, M, N, K):
+    """Operands and result inside larger column-major arrays (leading
+    dimensions past the row counts): the tile loads must read only the
+    window, and the store only touch C's window."""
+    rng = np.random.default_rng(M + 3 * N + 5 * K)
+    ar, ac = (K, M) if ta else (M, K)
+    br, bc = (N, K) if tb else (K, N)
+    Abig = np.asfortranarray(rng.standard_normal((ar + 5, ac)))
+    Bbig = np.asfortranarray(rng.standard_normal((br + 7, bc)))
+    Cbig = np.asfortranarray(np.full((M + 3, N), 7.0))  # the hook clears C on the device first
+    s = gpu_ctx.lib.ttg_test_gemm(gpu_ctx.h, 0, ta, tb, M, N, K, f64p(Abig), ar + 5, f64p(Bbig), br + 7,
+                                  f64p(Cbig), M + 3)
+    gpu_ctx.check(s)
+    A, B = Abig[:ar], Bbig[:br]
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    assert np.max(np.abs(Cbig[:M] - ref)) <= 1e-12 * np.max(np.abs(ref)) * np.sqrt(K)
+    assert np.all(Cbig[M:] == 0.0)
+
+
 @pytest.mark.parametrize("m,n,rank", [(40, 7, 7), (160, 40, 20), (8, 30, 8), (300, 64, 64), (64, 64, 10),
                                          (2048, 256, 128), (1000, 100, 100), (70, 90, 50)])
 def test_householder(gpu_ctx, m, n, rank):

@@ -46,9 +46,13 @@ __device__ __forceinline__ void circle_pair(int players, int round, int idx, int
 
 // Software grid barrier for a cooperative launch: bar[0] arrivals, bar[1]
 // generation.  Bounded spin: after ~4 s a timeout flag is raised and every
-// workgroup leaves the kernel at its next check.
+// workgroup leaves the kernel at its next check.  Every wave first waits for
+// its own stores and writes them back device-wide (agent-scope release: a
+// workgroup barrier alone does not wait for other waves' stores), then one
+// thread arrives.
 __device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen, int *err) {
   __shared__ int sh_ok;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -514,7 +518,8 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
         if (!sh_rot_ok) return;
         double *vdst = Vx + ((long)(g & 1) * p + w) * (LC * LC);
         for (int idx = tid; idx < LC * LC; idx += NT) vdst[idx] = Vl[idx];
-        __syncthreads();  // with thread 0's agent-scope release below, publishes the whole block
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // every wave's part of V
+        __syncthreads();
         if (tid == 0)
           __hip_atomic_store(&vflag[(g & 1) * JW_MAXP + w], 2 * g + (rot > 0 ? 1u : 0u), __ATOMIC_RELEASE,
                              __HIP_MEMORY_SCOPE_AGENT);

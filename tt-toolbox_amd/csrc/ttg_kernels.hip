@@ -70,6 +70,13 @@ template <> struct MF<float> {
 };
 
 constexpr int GBM = 64, GBN = 64, GBK = 64;
+// The tile loads address an operand by 32-bit byte offsets within 64 of its
+// columns: leading dimensions must stay below 2^22 elements.
+constexpr int64_t kGemmMaxLd = int64_t(1) << 22;
+static inline void gemm_check_ld(int64_t lda, int64_t ldb) {
+  TTG_CHECK(lda < kGemmMaxLd && ldb < kGemmMaxLd, TTG_BAD_ARG,
+            "GEMM operand with a leading dimension of 2^22 elements or more");
+}
 
 // C(MxN) = alpha*op(A)*op(B) + beta*C.  Block BM x BN (64 x 64; 32 x 64 when
 // 64 x 64 blocks would not fill the CUs; 32 x 32 when even 32 x 64 blocks
@@ -79,28 +86,50 @@ constexpr int GBM = 64, GBN = 64, GBK = 64;
 // v_mfma_*_16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the
 // accumulator element v of lane l is D[MF<T>::row(l, v)][l%16].  The waves
 // form the transposed tiles (op(B)^T op(A)^T), so that lane l holds rows
-// l%16 of C and 16 lanes store 16 consecutive rows (coalesced).  The next
-// K tile is loaded into registers while the current one is multiplied from
-// LDS; a 64-deep K tile gives each wave 64 MFMAs per tile, enough to cover
-// the load latency of the next one.
+// l%16 of C and 16 lanes store 16 consecutive rows (coalesced).  K tiles
+// go through two LDS buffers (gemm_acc): while one is multiplied, the next
+// tile is written to the other and the one after is loaded into registers.
 // Split-K: blockIdx.z selects K range [z*kchunk, ...); with gridDim.z > 1 the
 // raw partial product goes to P + z*M*N (ld M) and k_splitk_reduce finishes.
 // TI is the storage type of A and B, T the MFMA type: TI = double, T = float
 // rounds the operands to FP32 while staging them (FP32 Gram products of FP64
 // data).
+// One element through a buffer descriptor: raw (stride 0) load at byte offset
+// voff; an offset at or past the descriptor's range reads 0.
+template <class TI> __device__ __forceinline__ TI buffer_load(__amdgpu_buffer_rsrc_t r, unsigned voff);
+template <> __device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+template <> __device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+}
+
+// K tile of a BM x BN block: 64, or 32 for FP64 tiles above 32 x 32 (two
+// LDS buffers of an FP64 64 x 64 tile at K tile 64 would take 133 KB)
+template <class T, int BM, int BN> struct GemmKT {
+  static constexpr int value = (sizeof(T) == 8 && BM * BN > 32 * 32) ? 32 : 64;
+};
+
 // One BM x BN block of op(A) op(B) over K range [kbeg, kend) with its
 // top-left corner at (m0, n0), accumulated in acc (layout below), staged
-// through the caller's LDS tiles As, Bs.
-template <class TI, class T, bool TA, bool TB, int BM, int BN>
+// through the caller's two LDS buffers As[2], Bs[2] (K tile KT).  Tile t is
+// multiplied from buffer t % 2 while tile t+1 (loaded into registers during
+// tile t-1) is written to the other buffer and tile t+2 is loaded: one
+// workgroup barrier per K tile.  Rows of the LDS tiles are padded by one
+// element (odd stride), so the transposed writes of a K-contiguous operand
+// (64 lanes, 64 rows, one column) fall in distinct banks.
+template <class TI, class T, bool TA, bool TB, int BM, int BN, int KT>
 __device__ __forceinline__ void gemm_acc(int M, int N, int kbeg, int kend, int m0, int n0,
                                          const TI *__restrict__ A, long lda, const TI *__restrict__ B, long ldb,
-                                         T (*As)[BM + 4], T (*Bs)[BN + 4],
+                                         T (*As)[KT][BM + 1], T (*Bs)[KT][BN + 1],
                                          typename MF<T>::V (&acc)[BM / 32][BN / 32]) {
-  static_assert(!TB || BN == GBN, "op(B) = B^T stages 64 columns per K tile");
+  static_assert(!TB || BN == 64, "op(B) = B^T stages 64 columns per K row");
+  static_assert(KT == 32 || KT == 64, "K tile 32 or 64");
   constexpr int MI = BM / 32;            // 16-row MFMA tiles per wave
   constexpr int NJ = BN / 32;            // 16-column MFMA tiles per wave
-  constexpr int LA = BM * GBK / 256;     // A elements each thread stages per K tile
-  constexpr int LB = BN * GBK / 256;     // B elements each thread stages per K tile
+  constexpr int LA = BM * KT / 256;      // A elements each thread stages per K tile
+  constexpr int LB = BN * KT / 256;      // B elements each thread stages per K tile
+  constexpr int KQ = 256 / KT;           // threads along M or N when the operand is K-contiguous
   typedef typename MF<T>::V V;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -108,66 +137,99 @@ __device__ __forceinline__ void gemm_acc(int M, int N, int kbeg, int kend, int m
   for (int i = 0; i < MI; ++i)
 #pragma unroll
     for (int j = 0; j < NJ; ++j) acc[i][j] = V{0, 0, 0, 0};
-  // 64 consecutive threads read 64 consecutive elements of the operand's
-  // contiguous dimension (M or K for A, K or N for B)
-  // prefetched in the storage type: a conversion here would stall on the load
+  // consecutive threads read consecutive elements of the operand's contiguous
+  // dimension (M or K for A, K or N for B); prefetched in the storage type (a
+  // conversion here would stall on the load).  Loads are buffer loads from a
+  // descriptor based at the tile's first element: each thread's element
+  // offsets are fixed (computed once), and elements outside the M x K (K x N)
+  // window read 0 through the descriptor's range check (for the slow index of
+  // the operand: the range ends at the window's edge) or through an offset
+  // past any range (for the fast index: that thread's elements, chosen once
+  // for M and N and per tile for K).  No branch and no 64-bit address
+  // arithmetic per element.
   TI ra[LA], rb[LB];
-  // A, M contiguous: BM consecutive threads read BM consecutive rows
-  constexpr int AQ = 256 / BM;
+  constexpr int AQ = 256 / BM;  // A, M contiguous: BM consecutive threads read BM consecutive rows
+  const int kc = tid % KT, kq = tid / KT;
+  constexpr unsigned OOB = 0x80000000u;  // past every range (ranges are below 2^31 bytes)
+  unsigned va[LA], vb[LB];
+#pragma unroll
+  for (int i = 0; i < LA; ++i) {
+    if (!TA) {
+      const bool ok = m0 + tid % BM < M;
+      va[i] = ok ? (unsigned)(tid % BM + (long)(tid / BM + AQ * i) * lda) * (unsigned)sizeof(TI) : OOB;
+    } else {
+      va[i] = (unsigned)(kc + (long)(kq + KQ * i) * lda) * (unsigned)sizeof(TI);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < LB; ++i) {
+    if (!TB) {
+      vb[i] = (unsigned)(kc + (long)(kq + KQ * i) * ldb) * (unsigned)sizeof(TI);
+    } else {
+      const bool ok = n0 + lane < N;
+      vb[i] = ok ? (unsigned)(lane + (long)(wave + 4 * i) * ldb) * (unsigned)sizeof(TI) : OOB;
+    }
+  }
+  auto bytes_ok = [](long b) { return (int)(b <= 0 ? 0 : (b > 0x7fffffffL ? 0x7fffffffL : b)); };
   auto fetch = [&](int k0) {
-    const int c = tid & 63, q = tid >> 6;
+    // A: tile at (m0, k0) of op(A); the range ends at column kend (!TA: K is
+    // the slow index) or at row M of op(A) (TA: M is the slow index)
+    const TI *abase = TA ? A + k0 + (long)m0 * lda : A + m0 + (long)k0 * lda;
+    const long abytes = TA ? (long)(M - m0) * lda * (long)sizeof(TI) : (long)(kend - k0) * lda * (long)sizeof(TI);
+    const __amdgpu_buffer_rsrc_t ar =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<TI *>(abase), (short)0, bytes_ok(abytes), 0x00020000);
+    const TI *bbase = TB ? B + n0 + (long)k0 * ldb : B + k0 + (long)n0 * ldb;
+    const long bbytes = TB ? (long)(kend - k0) * ldb * (long)sizeof(TI) : (long)(N - n0) * ldb * (long)sizeof(TI);
+    const __amdgpu_buffer_rsrc_t br =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<TI *>(bbase), (short)0, bytes_ok(bbytes), 0x00020000);
+    // this thread's K index past kend (K-contiguous operands): every offset
+    // gets the range-breaking top bit (offsets are below 2^31)
+    const unsigned kout = (k0 + kc < kend) ? 0u : OOB;
+#pragma unroll
+    for (int i = 0; i < LA; ++i) ra[i] = buffer_load<TI>(ar, TA ? (va[i] | kout) : va[i]);
+#pragma unroll
+    for (int i = 0; i < LB; ++i) rb[i] = buffer_load<TI>(br, TB ? vb[i] : (vb[i] | kout));
+  };
+  auto stash = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
-      if (!TA) {
-        const int gm = m0 + tid % BM, gk = k0 + tid / BM + AQ * i;
-        ra[i] = (gm < M && gk < kend) ? A[gm + (long)gk * lda] : TI(0);
-      } else {
-        const int gk = k0 + c, gm = m0 + q + 4 * i;
-        ra[i] = (gm < M && gk < kend) ? A[gk + (long)gm * lda] : TI(0);
-      }
+      if (!TA) As[buf][tid / BM + AQ * i][tid % BM] = T(ra[i]);
+      else As[buf][kc][kq + KQ * i] = T(ra[i]);
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
-      if (!TB) {
-        const int gk = k0 + c, gn = n0 + q + 4 * i;
-        rb[i] = (gn < N && gk < kend) ? B[gk + (long)gn * ldb] : TI(0);
-      } else {
-        const int gn = n0 + c, gk = k0 + q + 4 * i;
-        rb[i] = (gn < N && gk < kend) ? B[gn + (long)gk * ldb] : TI(0);
-      }
+      if (!TB) Bs[buf][kc][kq + KQ * i] = T(rb[i]);
+      else Bs[buf][wave + 4 * i][lane] = T(rb[i]);
     }
   };
-  auto stash = [&]() {
-    const int c = tid & 63, q = tid >> 6;
+  auto multiply = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < LA; ++i) {
-      if (!TA) As[tid / BM + AQ * i][tid % BM] = T(ra[i]);
-      else As[c][q + 4 * i] = T(ra[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < LB; ++i) {
-      if (!TB) Bs[c][q + 4 * i] = T(rb[i]);
-      else Bs[q + 4 * i][c] = T(rb[i]);
-    }
-  };
-  if (kbeg < kend) fetch(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += GBK) {
-    __syncthreads();
-    stash();
-    __syncthreads();
-    if (k0 + GBK < kend) fetch(k0 + GBK);
-#pragma unroll
-    for (int kk = 0; kk < GBK; kk += 4) {
+    for (int kk = 0; kk < KT; kk += 4) {
       T a[MI], b[NJ];
 #pragma unroll
-      for (int i = 0; i < MI; ++i) a[i] = As[kk + (lane >> 4)][wm * (BM / 2) + i * 16 + (lane & 15)];
+      for (int i = 0; i < MI; ++i) a[i] = As[buf][kk + (lane >> 4)][wm * (BM / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) b[j] = Bs[kk + (lane >> 4)][wn * (BN / 2) + j * 16 + (lane & 15)];
+      for (int j = 0; j < NJ; ++j) b[j] = Bs[buf][kk + (lane >> 4)][wn * (BN / 2) + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < MI; ++i)
 #pragma unroll
         for (int j = 0; j < NJ; ++j) acc[i][j] = MF<T>::op(b[j], a[i], acc[i][j]);
     }
+  };
+  const int nt = kbeg < kend ? (kend - kbeg + KT - 1) / KT : 0;
+  if (nt == 0) return;
+  fetch(kbeg);
+  __syncthreads();  // the caller's earlier use of the buffers is over
+  stash(0);
+  if (nt > 1) fetch(kbeg + KT);
+  __syncthreads();
+  for (int t = 0; t < nt; ++t) {
+    multiply(t & 1);
+    if (t + 1 < nt) {
+      stash((t + 1) & 1);  // buffer last read by tile t-1 (before the previous barrier)
+      if (t + 2 < nt) fetch(kbeg + (t + 2) * KT);
+    }
+    __syncthreads();
   }
 }
 
@@ -179,12 +241,13 @@ __device__ __forceinline__ void gemm_block(int M, int N, int kbeg, int kend, int
                                            const TI *__restrict__ A, long lda, const TI *__restrict__ B,
                                            long ldb, T beta, TO *__restrict__ C, long ldc,
                                            T *__restrict__ P) {
-  __shared__ T As[GBK][BM + 4];
-  __shared__ T Bs[GBK][BN + 4];
+  constexpr int KT = GemmKT<T, BM, BN>::value;
+  __shared__ T As[2][KT][BM + 1];
+  __shared__ T Bs[2][KT][BN + 1];
   constexpr int MI = BM / 32, NJ = BN / 32;
   typedef typename MF<T>::V V;
   V acc[MI][NJ];
-  gemm_acc<TI, T, TA, TB, BM, BN>(M, N, kbeg, kend, m0, n0, A, lda, B, ldb, As, Bs, acc);
+  gemm_acc<TI, T, TA, TB, BM, BN, KT>(M, N, kbeg, kend, m0, n0, A, lda, B, ldb, As, Bs, acc);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   // acc[i][j][v] of lane l: C(m0 + wm*BM/2 + i*16 + l%16, n0 + wn*BN/2 + j*16 + row(l, v))
@@ -274,6 +337,7 @@ int bgemm_plan(bool tb, BGemmJob *jobs, int njobs, int *shape, double *flops, in
   int total = 0;
   double fl = 0.0;
   for (int j = 0; j < njobs; ++j) {
+    gemm_check_ld(jobs[j].lda, jobs[j].ldb);
     jobs[j].tile0 = total;
     total += ((jobs[j].M + bm - 1) / bm) * ((jobs[j].N + bn - 1) / bn);
     fl += 2.0 * jobs[j].M * (double)jobs[j].N * jobs[j].K;
@@ -354,6 +418,7 @@ static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, in
                       const TI *A, int64_t lda, const TI *B, int64_t ldb, T beta, TO *C, int64_t ldc,
                       T *ws, int64_t ws_elems, int num_cus, const T *alpha_dev = nullptr) {
   if (M <= 0 || N <= 0) return;
+  gemm_check_ld(lda, ldb);
   int S = splitk_count(M, N, K, num_cus);
   if (S > 1 && (ws == nullptr || (int64_t)S * M * N > ws_elems)) S = 1;
   int kchunk = (int)K;
@@ -467,6 +532,7 @@ __global__ void k_zero_d(double *p, int count) {
 template <class T>
 static void gram_nearid_impl(hipStream_t st, int64_t m, int64_t k, const double *A, double *G, double *X,
                              double *R2, double *maxabs, T *ws, int64_t ws_elems, int cus) {
+  gemm_check_ld(m, m);
   // cleared by a kernel, not a memset node: the call may be captured in a graph
   hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, maxabs, 1);
   int S = splitk_count(k, k, m, cus);
