@@ -46,9 +46,13 @@ def test_fused_round_config2_full_size(gpu_ctx):
     ref = O.tt_round(O.ttm_mtimes(a, x), c["eps"])
     A = ttgpu.TTMatrix(a.n, a.m, a.tt.r, a.tt.core)
     X = ttgpu.TTTensor(x.n, x.r, x.core)
-    y = ttgpu.mtimes_round(A, X, c["eps"])
-    print("config 2 ranks", y.r.tolist(), flush=True)
+    info = {}
+    y = ttgpu.mtimes_round(A, X, c["eps"], info=info)
+    print("config 2 ranks", y.r.tolist(), info, flush=True)
     assert y.r.tolist() == ref.r.tolist()
+    # no bond truncates below its cap: every bond (wide and right-capped tall
+    # ones alike) is certified and completed without an SVD
+    assert info["svd_skipped"] == len(a.n) - 1 and info["jacobi_sweeps"] == 0, info
     assert O.tt_rel_diff(O.TT(y.n, y.r, y.core), ref) <= 1e-12
     # unfused path: A*x on the GPU, then round on the GPU
     p = ttgpu.mtimes(A, X)

@@ -381,9 +381,11 @@ void sort_chop(hipStream_t s, const double *Y, int64_t ldy, int64_t k, int64_t k
                double ep, int64_t rmax, double *sv, int *perm, int *out_i, double *out_d,
                const int *skip = nullptr);
 
-// skip = 1 iff 1/nrmX >= 2 ep nrmR, k <= rmax and nrmR > 0 (no truncation at the bond)
-void notrunc_decide(hipStream_t s, const double *nrmR, const double *nrmX, double ep, int64_t k,
-                    int64_t rmax, int *skip);
+// skip = 1 iff 1/x >= 1.05 ep nrmR, k <= rmax and nrmR > 0 (no truncation at
+// the bond), x the smaller upper bound of ||R^{-1}||_2: nrmX = ||R^{-1}||_F or
+// the one from the normalised squarings' norms pw[0..3] (pw may be null)
+void notrunc_decide(hipStream_t s, const double *nrmR, const double *nrmX, const double *pw, double ep,
+                    int64_t k, int64_t rmax, int *skip);
 // X = I + sign * U (k x k; U may be null)
 void eye_plus(hipStream_t s, int64_t k, double *X, const double *U, double sign);
 

@@ -1030,16 +1030,29 @@ void sort_chop(hipStream_t s, const double *Y, int64_t ldy, int64_t k, int64_t k
 // sigma_min^2 >= (ep ||s||)^2 (core/my_chop2.m:25-31: the first reversed
 // cumulative sum is sigma_min^2, compared with strict <).  skip = 1 if the
 // bound clears the threshold with a factor 2, rmax does not bind and R != 0.
-__global__ void k_notrunc_decide(const double *nrmR, const double *nrmX, double ep, long k, long rmax,
-                                 int *skip) {
+// skip[0] = 1 when every singular value of the bond clears my_chop2's
+// threshold ep ||s|| (core/my_chop2.m:28-34): with a = ||R||_F = ||s|| and x
+// an upper bound of ||R^{-1}||_2, sigma_min >= 1/x; the margin 1.05 covers
+// R = R2 R1 (||R2 - I|| <= 5e-6) and the rounding of the norms.  x is the
+// smaller of ||R^{-1}||_F (nrmX) and, when pw is given, ||(M^T M)^8||_F^(1/16)
+// from the normalised squarings' norms pw[0..3]:
+// ||B^8||_F = pw0^8 pw1^4 pw2^2 pw3 (B = M^T M).
+__global__ void k_notrunc_decide(const double *nrmR, const double *nrmX, const double *pw, double ep, long k,
+                                 long rmax, int *skip) {
   if (threadIdx.x == 0) {
-    const double a = nrmR[0], x = nrmX[0];
-    skip[0] = (k <= rmax && a > 0.0 && x > 0.0 && isfinite(x) && 1.0 / x >= 2.0 * ep * a) ? 1 : 0;
+    const double a = nrmR[0];
+    double x = nrmX[0];
+    if (pw && pw[0] > 0.0 && pw[1] > 0.0 && pw[2] > 0.0 && pw[3] > 0.0) {
+      const double lb = (8.0 * log(pw[0]) + 4.0 * log(pw[1]) + 2.0 * log(pw[2]) + log(pw[3])) / 16.0;
+      const double b = exp(lb);
+      if (isfinite(b) && b > 0.0) x = fmin(x, b);
+    }
+    skip[0] = (k <= rmax && a > 0.0 && x > 0.0 && isfinite(x) && 1.0 / x >= 1.05 * ep * a) ? 1 : 0;
   }
 }
-void notrunc_decide(hipStream_t s, const double *nrmR, const double *nrmX, double ep, int64_t k,
-                    int64_t rmax, int *skip) {
-  hipLaunchKernelGGL(k_notrunc_decide, dim3(1), dim3(64), 0, s, nrmR, nrmX, ep, (long)k, (long)rmax, skip);
+void notrunc_decide(hipStream_t s, const double *nrmR, const double *nrmX, const double *pw, double ep,
+                    int64_t k, int64_t rmax, int *skip) {
+  hipLaunchKernelGGL(k_notrunc_decide, dim3(1), dim3(64), 0, s, nrmR, nrmX, pw, ep, (long)k, (long)rmax, skip);
   TTG_HIP(hipGetLastError());
 }
 

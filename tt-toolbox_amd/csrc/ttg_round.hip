@@ -219,10 +219,38 @@ template <class T> struct Rounder {
   // Cholesky + substitution second pass (accepted if its R is within 0.1 of
   // I).  Level 2 (caller): Householder.
   bool nearid_level(int level) const { return level == 0; }
-  // bound (level 0 only): also start, on the side stream, ||R1||_F and
-  // ||R1^{-1}||_F for the no-truncation test (R = R2 R1 with ||R2 - I|| of
-  // order k max|E| <= 5e-6, so the bounds carry over with the test's factor
-  // 2 margin); joined by the caller through ctx->ev_join.
+  // bound (level 0 only): also start, on the side stream, ||R1||_F and two
+  // upper bounds of ||R1^{-1}||_2 for the no-truncation test: ||R1^{-1}||_F
+  // (loose by up to sqrt(k)) and ||(M^T M)^8||_F^(1/16), M = R1^{-1} (loose
+  // by at most k^(1/16): (M^T M)^8 has the eigenvalues ||M||_2^16 >= ... and
+  // its Frobenius norm lies between the largest and sqrt(k) times it), formed
+  // by three squarings, each normalised by its Frobenius norm (scal_d[2..5]).
+  // R = R2 R1 with ||R2 - I|| of order k max|E| <= 5e-6, so the bounds carry
+  // over to R inside the test's margin.  Joined by the caller through
+  // ctx->ev_join.
+  // ||(M^T M)^8||_F^(1/16) for M = Rtmp (k x k): B = M^T M and three
+  // squarings, each normalised by its Frobenius norm (scal_d[2..5]); R4 and
+  // Rtmp are overwritten
+  void inv_power_bound(hipStream_t s, int64_t k) {
+    gemm<double>(s, true, false, k, k, k, 1.0, Rtmp, k, Rtmp, k, 0.0, R4, k, nullptr, 0, cus);  // M^T M
+    double *Bp[2] = {R4, Rtmp};
+    for (int j = 0; j < 3; ++j) {
+      fro_norm<double>(s, Bp[j & 1], k * k, scal_d + 2 + j);
+      scale_by_inv<double>(s, Bp[j & 1], k * k, scal_d + 2 + j);
+      gemm<double>(s, false, false, k, k, k, 1.0, Bp[j & 1], k, Bp[j & 1], k, 0.0, Bp[(j + 1) & 1], k, nullptr, 0,
+                   cus);
+    }
+    fro_norm<double>(s, Bp[1], k * k, scal_d + 5);
+  }
+  // the bounds of the no-truncation test from a finished R (k x k, FP64) on
+  // the main stream: ||R||_F, ||R^{-1}||_F, the powered bound
+  void inv_bounds(const double *Rf, int64_t k) {
+    fro_norm<double>(st, Rf, k * k, scal_d + 0);
+    eye_plus(st, k, R4, nullptr, 0.0);
+    trsm_right_upper<double>(st, k, k, R4, k, Rf, Rtmp, k);  // Rtmp = R^{-1}
+    fro_norm<double>(st, Rtmp, k * k, scal_d + 1);
+    inv_power_bound(st, k);
+  }
   void qr_tall_fast(const T *A, int64_t m, int64_t k, T *Qout, double *Rout, int *flags,
                     double *stt, int level, bool allow_g32, bool bound = false) {
     const bool g32 = mixed && level == 0 && allow_g32;
@@ -239,6 +267,7 @@ template <class T> struct Rounder {
         trsm_right_upper<double>(ctx->side, k, k, R4, k, R1, Rtmp, k);
       }
       fro_norm<double>(ctx->side, Rtmp, k * k, scal_d + 1);
+      inv_power_bound(ctx->side, k);
       TTG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
       if (!w1) fro_norm<double>(st, R1, k * k, scal_d + 0);
     }
@@ -349,27 +378,33 @@ template <class T> struct Rounder {
     }
     // (fl[0] is written by every first-pass Cholesky, fl[1] by the second
     // pass's at levels >= 1, the only ones that read it: no clearing needed)
-    const bool bound = wide && allow_notrunc && level == 0;
+    // the test's bounds: at QR level 0 overlapped with the QR's second pass
+    // (from R1, on the side stream), at the fallback levels from the final R.
+    // Wide and tall bonds alike: the singular values of W are those of R.
+    const bool bound = allow_notrunc;
     if (level < 2) {
-      qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats, level, false, bound);
+      qr_tall_fast(tallA, mt, kj, Qs, R, fl, stats, level, false, bound && level == 0);
     } else {
       qr_householder(tallA, mt, kj, Qs, R);
     }
+    if (bound && level > 0) inv_bounds(R, kj);
     JacobiPlan pl = jacobi_plan(kj);
     int h_fl[4] = {0, 0, 0, 0};
     double h_st[6] = {0, 0, 0, 0, 0, 0};
     skipped = false;
-    // No-truncation shortcut (wide bonds): when 1/||R^{-1}||_F, a lower bound
-    // of the smallest singular value, clears my_chop2's threshold, every
-    // singular value is kept and the SVD would only change the gauge of the
-    // bond, so it is completed from its QR factors, W = R^T Q^T:
-    // core_i = Q^T (orthonormal rows, as V' would have), core_{i-1} <-
-    // core_{i-1} R^T.  Same tensor, same ranks.  The decision and the QR
-    // acceptance are read in one synchronisation; a rejected QR is redone
-    // before any SVD work.
+    // No-truncation shortcut: when 1/||R^{-1}||_2's upper bound
+    // (qr_tall_fast, inv_bounds), a lower bound of the smallest singular
+    // value, clears my_chop2's threshold, every singular value is kept and
+    // the SVD would only change the gauge of the bond, so it is completed
+    // without one (sweep2_bond): wide W = R^T Q^T gives core_i = Q^T
+    // (orthonormal rows, as V' would have) and core_{i-1} <- core_{i-1} R^T;
+    // tall W (rank N = n_i r_{i+1}, the right cap) gives core_i = I_N and
+    // core_{i-1} <- core_{i-1} W.  Same tensor, same ranks.  The decision and
+    // the QR acceptance are read in one synchronisation; a rejected QR is
+    // redone before any SVD work.
     if (bound) {
-      TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
-      notrunc_decide(st, scal_d + 0, scal_d + 1, ep, kj, rmax_i, skip);
+      if (level == 0) TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+      notrunc_decide(st, scal_d + 0, scal_d + 1, scal_d + 2, ep, kj, rmax_i, skip);
       double *h_io = pinned();
       TTG_HIP(hipMemcpyAsync(h_io, stats, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
       TTG_HIP(hipStreamSynchronize(st));
@@ -526,6 +561,10 @@ template <class T> struct Rounder {
       // no truncation: U S = R^T, core_i = Qs^T
       transpose<T>(st, Rprod, kw, kw, US);
       transpose<T>(st, Qs, N, kw, core_i);
+    } else if (skipped) {
+      // tall, no truncation (rank N, the right cap): U S = W, core_i = I_N
+      TTG_HIP(hipMemcpyAsync(US, cp[i], kw * N * sizeof(T), hipMemcpyDeviceToDevice, st));
+      eye_plus(st, N, core_i, nullptr, 0.0);
     } else if (wide) {
       // W^T = Qs R, R^T J = Y:  U S = Y[0:kw, perm],  V = Qs J[:, perm]
       gather_cols<T>(st, Y, pl.rows, 0, kw, perm, r1, US, kw);
