@@ -122,7 +122,7 @@ ttg_status ttg_mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const
 
 /* Launch interface of round for repeated rounds of one shape (no reference
  * counterpart: MATLAB has no asynchronous calls).  It applies to tensors the
- * Gram-chain form handles (d >= 2, ranks <= 256 after the caps, no rmax), in
+ * Gram-chain form handles (d >= 2, ranks <= 480 after the caps, no rmax), in
  * the mixed form (prec must be TTG_PREC_FP32; TTG_PREC_FP64 returns TTG_BAD_ARG).
  * ttg_round_gram_prepare rounds dcore once into dcore_out (it allocates,
  * uploads its tables, captures its launch sequence for these shapes,

@@ -91,8 +91,8 @@ def test_not_proven_is_reported():
     ttgpu.round_gram_launch(ctx, dstat)
     torch.cuda.synchronize()
     assert int(dstat.item()) == 0
-    # a shape outside the form (ranks above 256 after the caps) is refused up front
-    wide = O.rand_tt(rng, [20] * 4, [1, 20, 300, 20, 1])
+    # a shape outside the form (ranks above 480 after the caps) is refused up front
+    wide = O.rand_tt(rng, [30] * 4, [1, 30, 600, 30, 1])
     with pytest.raises(ttgpu.TTGError) as e:
         ttgpu.round_gram_prepare(ctx, wide.n, wide.r, torch.tensor(wide.core, device="cuda:0"), 1e-8, "fp32")
     assert e.value.status == 1

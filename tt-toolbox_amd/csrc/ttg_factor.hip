@@ -270,12 +270,22 @@ __global__ __launch_bounds__(CHOL_THREADS) void k_chol3(int k, const double *__r
 // beside the diagonal factorisation instead of after it.  Same operations in
 // the same order per column as k_chol3: identical factors.
 // One panel column per thread of waves 1..7: panels up to 448 columns (k <= 480).
+// jobs != nullptr: workgroup b factorises the matrix of jobs[b] (batched, no
+// shift; the jobs' Wout and fro are not written).
 // ---------------------------------------------------------------------------
 template <bool STAMP>
-__global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k, const double *__restrict__ G, long ldg,
-                                                        double shift_rel, double *__restrict__ R,
-                                                        int *flag, double *stats,
-                                                        unsigned long long *dbg) {
+__global__ __launch_bounds__(CHOL_THREADS) void k_chol4(int k_, const double *__restrict__ G_, long ldg_,
+                                                        double shift_rel, double *__restrict__ R_,
+                                                        int *flag_, double *stats_,
+                                                        unsigned long long *dbg,
+                                                        const CholJob *__restrict__ jobs = nullptr) {
+  const CholJob *jb = jobs ? jobs + blockIdx.x : nullptr;
+  const int k = jb ? jb->k : k_;
+  const double *__restrict__ G = jb ? jb->G : G_;
+  const long ldg = jb ? (long)jb->ldg : ldg_;
+  double *__restrict__ R = jb ? jb->R : R_;
+  int *flag = jb ? jb->flag : flag_;
+  double *stats = jb ? jb->stats : stats_;
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   double *Dl = reinterpret_cast<double *>(smem);  // CB x CB, Dl[t * CB + l] = R(J0 + t, J0 + l)
   double *Ps = Dl + CB * CB;  // CB x wps, Ps[t * wps + c] = R(J0 + t, J0 + CB + c)
@@ -865,7 +875,8 @@ bool chol_factor(hipStream_t s, int64_t k, const double *G, int64_t ldg, double 
   } else if (k > 256) {
     auto fn = dbg ? k_chol4<true> : k_chol4<false>;
     TTG_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(fn, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg);
+    hipLaunchKernelGGL(fn, dim3(1), dim3(CHOL_THREADS), lds, s, (int)k, G, (long)ldg, shift_rel, R, flag, stats, dbg,
+                       (const CholJob *)nullptr);
   } else {
     // k <= 256: at most 2 * 16 output tiles (4 per wave), 32 x 224 staged
     // elements (14 per thread), staging and panel row stride 225; Dl + Ps
@@ -898,6 +909,21 @@ void chol_batched(hipStream_t s, const CholJob *jobs_dev, int njobs, int64_t kma
   hipLaunchKernelGGL((k_chol6<MAXT, MAXU, US, false>), dim3((unsigned)njobs), dim3(CHOL_THREADS), lds6, s, 0,
                      (const double *)nullptr, 0L, 0.0, (double *)nullptr, (int *)nullptr, (double *)nullptr,
                      (unsigned long long *)nullptr, (double *)nullptr, (double *)nullptr, jobs_dev);
+  TTG_HIP(hipGetLastError());
+}
+
+// Batched Cholesky for 256 < k <= 480 (k_chol4, one workgroup per job).
+void chol4_batched(hipStream_t s, const CholJob *jobs_dev, int njobs, int64_t kmax) {
+  if (njobs <= 0) return;
+  TTG_CHECK(kmax >= 1 && kmax <= 480, TTG_BAD_ARG, "batched Cholesky (k_chol4) supports k <= 480");
+  const int64_t wmax = std::max<int64_t>(kmax - CB, 0);
+  const int64_t wps = ((wmax + 15) & ~(int64_t)15) + 1;
+  const size_t lds = ((size_t)CB * CB + (size_t)CB * wps) * sizeof(double);
+  ProfScope prof(KC_CHOL, s, (double)njobs * kmax * kmax * kmax / 3.0, 8.0 * 2.0 * njobs * kmax * kmax);
+  TTG_HIP(hipFuncSetAttribute((const void *)k_chol4<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_chol4<false>, dim3((unsigned)njobs), dim3(CHOL_THREADS), lds, s, 0, (const double *)nullptr,
+                     0L, 0.0, (double *)nullptr, (int *)nullptr, (double *)nullptr, (unsigned long long *)nullptr,
+                     jobs_dev);
   TTG_HIP(hipGetLastError());
 }
 

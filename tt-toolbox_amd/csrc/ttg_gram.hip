@@ -210,7 +210,7 @@ bool gram_shape_ok(int64_t d, const int64_t *n, const int64_t *r, const int64_t 
   for (int64_t i = 0; i + 1 < d; ++i) rr[i + 1] = std::min(rr[i + 1], rr[i] * n[i]);
   for (int64_t i = d - 1; i >= 1; --i) rr[i] = std::min(rr[i], n[i] * rr[i + 1]);
   for (int64_t i = 1; i < d; ++i) {
-    if (rr[i] > 256) return false;                 // batched factorisation kernels: k <= 256
+    if (rr[i] > 480) return false;                 // batched factorisation kernels: k <= 480
     if (rmax && rmax[i] < rr[i]) return false;     // a binding rmax truncates
   }
   return true;
@@ -293,7 +293,8 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   for (int64_t i = 1; i <= d; ++i) sq_ofs[i] = sq_ofs[i - 1] + rr[i - 1] * rr[i - 1];
   for (int64_t i = 0; i < d; ++i) out_ofs[i + 1] = out_ofs[i] + rr[i] * n[i] * rr[i + 1];
   const int64_t sqtot = sq_ofs[d] + 1;
-  const int64_t wjob = (kmax / 32 + 1) * 33 * 32;
+  // block inverses of the batched factorisation (bonds up to 256 only)
+  const int64_t wjob = (std::min<int64_t>(kmax, 256) / 32 + 1) * 33 * 32;
   const int nb = (int)(d - 1);
   const int chunks = 32;
   const int64_t gws = std::max<int64_t>(32 * kmax * kmax, 1 << 16);
@@ -313,6 +314,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     a.template take<double>(d + 8);           // alpha
     a.template take<double>(d + 8);           // sqrt(s)
     a.template take<double>(d + 8);           // near-identity departures (exact form)
+    a.template take<double>(kmax > 256 ? kmax * kmax : 1);  // identity (inverses of bonds above 256)
     for (int q = 0; q < 4; ++q) a.template take<double>(exact ? max_core : 1);  // exact sweep: V, V^T, Q1, Q
     for (int q = 0; q < 5; ++q) a.template take<double>(exact ? kmax * kmax : 1);  // G, R2^{-1}, R2, two R
     a.template take<double>(4 * (size_t)nb * kBondParts + 8);  // per-bond statistics
@@ -342,6 +344,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   double *part = A.take<double>((size_t)d * chunks);
   double *sv = A.take<double>(d + 8), *alpha = A.take<double>(d + 8);
   double *sq = A.take<double>(d + 8), *nid = A.take<double>(d + 8);
+  double *Ieye = A.take<double>(kmax > 256 ? kmax * kmax : 1);
   double *Vb = A.take<double>(exact ? max_core : 1), *Vt = A.take<double>(exact ? max_core : 1);
   double *Q1 = A.take<double>(exact ? max_core : 1), *Qt = A.take<double>(exact ? max_core : 1);
   double *Gt = A.take<double>(exact ? kmax * kmax : 1), *Xi = A.take<double>(exact ? kmax * kmax : 1);
@@ -398,19 +401,36 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     if (cur_r[i] != rr[i]) throw Error(TTG_HIP_ERROR, "internal: rank caps disagree");
 
   // ---- tables (one upload through pinned memory) ------------------------------
-  const int njc = 2 * nb;
-  std::vector<CholJob> cj(njc);
-  std::vector<TrsmJob> tj(njc);
+  // factorisations of bonds up to 256 in one k_chol6 launch (block inverses
+  // kept for the k_trsm3 inverses), larger ones (up to 480) in one k_chol4
+  // launch with generic triangular solves for their inverses
+  std::vector<CholJob> cj, cjl;
+  std::vector<TrsmJob> tj;
+  struct BigInv {
+    const double *R;
+    double *Rinv;
+    int k;
+  };
+  std::vector<BigInv> big;
+  int64_t kbig = 1;
   std::vector<GramBond> gb(nb);
   std::vector<BGemmJob> ja(nb), jz(nb), jb(nb);
   for (int64_t i = 1; i < d; ++i) {
     const int b = (int)(i - 1);
     const int k = (int)rr[i];
     const int64_t o = sq_ofs[i];
-    cj[b] = CholJob{H + o, RR + o, cfl + b, cst + 4 * b, Wv + (size_t)b * wjob, nullptr, k, k};
-    cj[nb + b] = CholJob{Gm + o, RL + o, cfl + nb + b, cst + 4 * (nb + b), Wv + (size_t)(nb + b) * wjob, nullptr, k, k};
-    tj[b] = TrsmJob{nullptr, RR + o, Wv + (size_t)b * wjob, RiR + o, k, k, 0, k};
-    tj[nb + b] = TrsmJob{nullptr, RL + o, Wv + (size_t)(nb + b) * wjob, RiL + o, k, k, 0, k};
+    if (k <= 256) {
+      cj.push_back(CholJob{H + o, RR + o, cfl + b, cst + 4 * b, Wv + (size_t)b * wjob, nullptr, k, k});
+      cj.push_back(CholJob{Gm + o, RL + o, cfl + nb + b, cst + 4 * (nb + b), Wv + (size_t)(nb + b) * wjob, nullptr, k, k});
+      tj.push_back(TrsmJob{nullptr, RR + o, Wv + (size_t)b * wjob, RiR + o, k, k, 0, k});
+      tj.push_back(TrsmJob{nullptr, RL + o, Wv + (size_t)(nb + b) * wjob, RiL + o, k, k, 0, k});
+    } else {
+      cjl.push_back(CholJob{H + o, RR + o, cfl + b, cst + 4 * b, nullptr, nullptr, k, k});
+      cjl.push_back(CholJob{Gm + o, RL + o, cfl + nb + b, cst + 4 * (nb + b), nullptr, nullptr, k, k});
+      big.push_back(BigInv{RR + o, RiR + o, k});
+      big.push_back(BigInv{RL + o, RiL + o, k});
+      kbig = std::max<int64_t>(kbig, k);
+    }
     gb[b] = GramBond{Gm + o, H + o, RiR + o, Zm + o, k, 0};
     jz[b] = BGemmJob{RiR + o, RiL + o, Zm + o, nullptr, 1.0, k, k, k, k, k, k, 0, 2 | 4};
     const int64_t N = n[i] * rr[i + 1];
@@ -432,7 +452,10 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     sizes[i] = rr[i] * n[i] * rr[i + 1];
     cptr[i] = cp[i];
   }
-  // pack: [cj | tj | gb | ja | jz | jb | cptr | sizes | one]
+  const int njc = (int)cj.size(), njl = (int)cjl.size();
+  int64_t ksmall = 1;
+  for (const CholJob &c : cj) ksmall = std::max<int64_t>(ksmall, c.k);
+  // pack: [cj | tj | gb | ja | jz | jb | cptr | sizes | one | cjl]
   std::vector<size_t> off;
   size_t tb = 0;
   auto add = [&](size_t bytes) {
@@ -448,11 +471,13 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   add(d * sizeof(double *));
   add(d * sizeof(int64_t));
   add(sizeof(double));
+  add(njl * sizeof(CholJob));
   char *tab = A.take<char>(tb);
   const size_t readback = 256;
   char *hp = static_cast<char *>(pinned_acquire(ctx, tb + readback));
   std::memcpy(hp + off[0], cj.data(), njc * sizeof(CholJob));
   std::memcpy(hp + off[1], tj.data(), njc * sizeof(TrsmJob));
+  std::memcpy(hp + off[9], cjl.data(), njl * sizeof(CholJob));
   std::memcpy(hp + off[2], gb.data(), nb * sizeof(GramBond));
   std::memcpy(hp + off[3], ja.data(), nb * sizeof(BGemmJob));
   std::memcpy(hp + off[4], jz.data(), nb * sizeof(BGemmJob));
@@ -492,6 +517,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   const double *const *dcp = reinterpret_cast<const double *const *>(tab + off[6]);
   const int64_t *dsz = reinterpret_cast<const int64_t *>(tab + off[7]);
   const double *d_one = reinterpret_cast<const double *>(tab + off[8]);
+  const CholJob *dcjl = reinterpret_cast<const CholJob *>(tab + off[9]);
 
   // the launch sequence in four parts: set-up (st), right chain, left chain,
   // the rest (st); the chains run on st and sd at once
@@ -523,8 +549,12 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   };
   auto enq_rest = [&]() {
     // ---- batched factorisations, inverses, products, verdict ----------------------
-    chol_batched(st, dcj, njc, kmax);
-    trsm3_batched(st, dtj, njc, kmax, kmax);
+    chol_batched(st, dcj, njc, ksmall);
+    chol4_batched(st, dcjl, njl, kbig);
+    trsm3_batched(st, dtj, njc, ksmall, ksmall);
+    if (!big.empty()) eye_plus(st, kbig, Ieye, nullptr, 0.0);
+    for (const BigInv &bi : big)  // R^{-1} = I R^{-1}, generic blocked substitution
+      trsm_right_upper<double>(st, bi.k, bi.k, Ieye, kbig, bi.R, bi.Rinv, bi.k, nullptr, nullptr);
     if (!exact) bgemm_run(st, false, true, dja, nb, ta_, qa, fa);   // A_i = C_i R_R,i+1^T
     bgemm_run(st, true, false, djz, nb, tz_, qz, fz);   // Z_i = R_R,i^{-T} R_L,i^{-1}
     hipLaunchKernelGGL(k_gram_bond, dim3(kBondParts, (unsigned)nb), dim3(256), 0, st, dgb, bst);
@@ -543,7 +573,8 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
           hipLaunchKernelGGL(k_scale_copy, dim3(64), dim3(256), 0, st, Vb, cp[i], (long)(m * k), sq + i);
         }
         transpose<double>(st, Vb, kw, N, Vt);
-        trsm_right_upper<double>(st, N, kw, Vt, N, RR + sq_ofs[i], Q1, N, nullptr, Wv + (size_t)(i - 1) * wjob);
+        trsm_right_upper<double>(st, N, kw, Vt, N, RR + sq_ofs[i], Q1, N, nullptr,
+                                 kw <= 256 ? Wv + (size_t)(i - 1) * wjob : nullptr);
         gram_nearid(st, N, kw, Q1, false, Gt, Xi, R2, nid + i, wsr, gws, cus);
         gemm<double>(st, false, false, N, kw, kw, 1.0, Q1, N, Xi, kw, 0.0, Qt, N, wsr, gws, cus);
         double *Rc = Rq[i & 1];
@@ -692,7 +723,7 @@ ttg_status round_gram_prepare(ttg_ctx *ctx, int64_t d, const int64_t *n, const i
   }
   TTG_CHECK(!ctx->worker, TTG_BAD_ARG, "not on a stream worker");
   TTG_CHECK(gram_shape_ok(d, n, r, nullptr), TTG_BAD_ARG,
-            "the Gram-chain form does not apply (d < 2 or ranks above 256 after the caps)");
+            "the Gram-chain form does not apply (d < 2 or ranks above 480 after the caps)");
   ctx->gprep.clear();
   const uint64_t before = ctx->gruns;
   const bool exact = prec == TTG_PREC_FP64;

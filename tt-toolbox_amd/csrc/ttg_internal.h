@@ -325,6 +325,8 @@ struct CholJob {
   int k, ldg;
 };
 void chol_batched(hipStream_t s, const CholJob *jobs_dev, int njobs, int64_t kmax);
+// the same for 256 < k <= 480 (no block inverses, no ||R||_F)
+void chol4_batched(hipStream_t s, const CholJob *jobs_dev, int njobs, int64_t kmax);
 // Batched X = A R^{-1} (A == nullptr: A = I) with the block inverses a
 // batched Cholesky left (k <= 256 for every job).
 struct TrsmJob {

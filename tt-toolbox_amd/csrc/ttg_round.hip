@@ -896,6 +896,28 @@ ttg_status mtimes_round_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int
   hipStream_t st = ctx->stream;
   std::vector<int64_t> nn(n, n + d), mm(m, m + d), raa(ra, ra + d + 1), rxx(rx, rx + d + 1);
   TTG_CHECK(prec == TTG_PREC_FP64 || prec == TTG_PREC_FP32, TTG_BAD_ARG, "unknown precision mode");
+  // When the product can take the Gram-chain form (capped ranks up to 480,
+  // no binding rmax), it is formed once by one k_mtimes_all launch and
+  // rounded as a tensor: that form reads every core several times, which the
+  // sweeps' on-chip generation of the product cores would have to repeat.
+  // Otherwise the product cores are built inside the first sweep's products.
+  if (gram_shape_ok(d, n, rp.data(), rmax)) {
+    int64_t need = 0;
+    round_capacity(d, n, rp.data(), &need);
+    if (cap < need) {
+      if (len_out) *len_out = need;
+      return TTG_NEED_SPACE;
+    }
+    const int64_t plen = packed_size(d, n, rp.data());
+    ctx->sarena.reserve(plen * sizeof(double) + 4096);
+    ctx->sarena.reset();
+    double *prod = ctx->sarena.take<double>(plen);
+    std::vector<int64_t> ry(d + 1);
+    int64_t used = 0;
+    const ttg_status ms = mtimes_dev(ctx, d, n, m, ra, dacore, rx, dxcore, ry.data(), prod, plen, &used);
+    TTG_CHECK(ms == TTG_OK, ms, "A*x staging for the round failed");
+    return round_dev(ctx, d, n, rp.data(), prod, eps, rmax, prec, r_out, dcore_out, cap, len_out, info);
+  }
   // core 0 is materialised (it is the first QR's input); every later product
   // core only ever feeds sweep 1's R * core product and is built inside it
   std::function<void(int64_t, double *)> gen = [&](int64_t i, double *dst) {
