@@ -125,16 +125,31 @@ def mtimes_leg(ttgpu, W, ctx, dev, fused=True):
     if not fused:
         return out
     o2 = torch.empty(ttgpu.capacity(na, ry), dtype=torch.float64, device=dev)
-    rr = {}
-    rate = _rate(lambda: rr.__setitem__("r", ttgpu.mtimes_round_dev(ctx, na, ma, ra, da, rx, dx, 1e-6, out=o2)[0]),
-                 5, sync)
     d2 = len(na)
     cflops = sum(2.0 * ma[k] * ry[k] * na[k] * ry[k + 1] for k in range(d2))  # product cores
-    fl = round_flops(na, ry, rr["r"]) + cflops
-    out["config2_fused_mtimes_round"] = {
-        "workload": "round(A*x, 1e-6) fused (product cores built on chip), FP64",
-        "rounds_per_s": rate, "ranks_out": [int(v) for v in rr["r"]],
-        "algorithmic_gflop": fl / 1e9, "roofline": roofline(fl, 1.0 / rate)}
+    # default route: the product (capped ranks <= 480) is formed once and
+    # rounded by the Gram-chain form; TTG_GRAM=0 times the sweeps with every
+    # product core built on chip inside the first sweep's products
+    for key, prec, env, what in [
+            ("config2_fused_mtimes_round", "fp64", None,
+             "round(A*x, 1e-6), plain FP64: product formed once, Gram-chain round"),
+            ("config2_fused_mtimes_round_mixed", "fp32", None,
+             "round(A*x, 1e-6), mixed form: product formed once, Gram-chain round"),
+            ("config2_fused_sweeps", "fp64", "0",
+             "round(A*x, 1e-6), plain FP64, QR/SVD sweeps with the product cores built on chip (never in HBM)")]:
+        rr, info = {}, {}
+        if env is not None:
+            os.environ["TTG_GRAM"] = env
+        try:
+            rate = _rate(lambda: rr.__setitem__("r", ttgpu.mtimes_round_dev(
+                ctx, na, ma, ra, da, rx, dx, 1e-6, out=o2, precision=prec, info=info)[0]), 5, sync)
+        finally:
+            os.environ.pop("TTG_GRAM", None)
+        fl = round_flops(na, ry, rr["r"]) + cflops
+        out[key] = {"workload": what, "rounds_per_s": rate, "ranks_out": [int(v) for v in rr["r"]],
+                    "path": info.get("path"), "svd_skipped": info.get("svd_skipped"),
+                    "jacobi_sweeps": info.get("jacobi_sweeps"),
+                    "algorithmic_gflop": fl / 1e9, "roofline": roofline(fl, 1.0 / rate)}
     return out
 
 

@@ -534,8 +534,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
       const int64_t m = rr[i] * n[i], k = rr[i + 1];
       const double *Hn = (i + 1 < d) ? H + sq_ofs[i + 1] : d_one;
       gemm_scaled(s, false, false, m, k, k, sv + i, cp[i], m, Hn, k, Tr, m, wsr, gws, cus);
-      gemm<double>(s, false, true, rr[i], rr[i], n[i] * k, 1.0, Tr, rr[i], cp[i], rr[i], 0.0, H + sq_ofs[i], rr[i],
-                   wsr, gws, cus);
+      gemm_sym(s, false, true, rr[i], n[i] * k, Tr, rr[i], cp[i], rr[i], H + sq_ofs[i], rr[i], wsr, gws, cus);
     }
   };
   auto enq_left = [&](hipStream_t s) {
@@ -543,8 +542,8 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
       const int64_t k0 = rr[i], N = n[i] * rr[i + 1];
       const double *Gp = i > 0 ? Gm + sq_ofs[i] : d_one;
       gemm_scaled(s, false, false, k0, N, k0, sv + i, Gp, k0, cp[i], k0, Tl, k0, wsl, gws, cus);
-      gemm<double>(s, true, false, rr[i + 1], rr[i + 1], k0 * n[i], 1.0, cp[i], k0 * n[i], Tl, k0 * n[i], 0.0,
-                   Gm + sq_ofs[i + 1], rr[i + 1], wsl, gws, cus);
+      gemm_sym(s, true, false, rr[i + 1], k0 * n[i], cp[i], k0 * n[i], Tl, k0 * n[i], Gm + sq_ofs[i + 1], rr[i + 1],
+               wsl, gws, cus);
     }
   };
   auto enq_rest = [&]() {

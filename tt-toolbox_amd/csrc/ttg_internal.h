@@ -259,6 +259,11 @@ void gemm(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, T al
           T *ws, int64_t ws_elems, int num_cus);
 // bytes of split-K workspace gemm() may use for the given shape
 int64_t gemm_ws_elems(int64_t M, int64_t N, int64_t K, int num_cus);
+// FP64 C (M x M) = op(A) op(B) for a product known to be symmetric (a Gram
+// matrix): with split-K only the blocks on or above the diagonal are formed
+// and the reduction mirrors them (C exactly symmetric); otherwise as gemm
+void gemm_sym(hipStream_t s, bool ta, bool tb, int64_t M, int64_t K, const double *A, int64_t lda, const double *B,
+              int64_t ldb, double *C, int64_t ldc, double *ws, int64_t wse, int cus);
 // FP64 C = (*alpha_dev) op(A) op(B) (alpha_dev: device scalar)
 void gemm_scaled(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, const double *alpha_dev,
                  const double *A, int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, double *ws,

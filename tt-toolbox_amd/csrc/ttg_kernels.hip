@@ -277,7 +277,10 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T
                                               const TI *__restrict__ A, long lda,
                                               const TI *__restrict__ B, long ldb, T beta,
                                               TO *__restrict__ C, long ldc, T *__restrict__ P,
-                                              const T *__restrict__ alpha_dev) {
+                                              const T *__restrict__ alpha_dev, int sym) {
+  // sym: the product is symmetric (M == N) and only blocks that reach the
+  // diagonal or above are formed; k_splitk_reduce mirrors them
+  if (sym && (int)(blockIdx.x * BM) >= (int)(blockIdx.y * BN + BN)) return;
   const int kbeg = blockIdx.z * kchunk;
   const int kend = min(K, kbeg + kchunk);
   const T a = alpha_dev ? alpha * alpha_dev[0] : alpha;
@@ -370,20 +373,25 @@ void bgemm_run(hipStream_t st, bool ta, bool tb, const BGemmJob *jobs_dev, int n
 }
 
 // Fixed-order sum of the split-K partials (accumulated in double).
+// sym (M == N, only the blocks on or above the diagonal were formed): an
+// entry below the diagonal takes its mirror's sums, so C is exactly symmetric.
 template <class T, class TO>
 __global__ void k_splitk_reduce(int M, int N, int S, const T *__restrict__ P, double alpha,
-                                double beta, TO *__restrict__ C, long ldc, const T *__restrict__ alpha_dev) {
+                                double beta, TO *__restrict__ C, long ldc, const T *__restrict__ alpha_dev,
+                                int sym) {
   if (alpha_dev) alpha *= (double)alpha_dev[0];
   const long total = (long)M * N;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
        idx += (long)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int z = 0; z < S; ++z) s += (double)P[(long)z * total + idx];
     const int gm = (int)(idx % M);
     const long gn = idx / M;
+    if (sym && gm > gn) continue;  // written as the mirror of (gn, gm)
+    double s = 0.0;
+    for (int z = 0; z < S; ++z) s += (double)P[(long)z * total + idx];
     double v = alpha * s;
     if (beta != 0.0) v += beta * (double)C[gm + gn * ldc];
     C[gm + gn * ldc] = (TO)v;
+    if (sym && gm < gn) C[gn + gm * ldc] = (TO)v;
   }
 }
 
@@ -416,11 +424,14 @@ int64_t gemm_ws_elems(int64_t M, int64_t N, int64_t K, int num_cus) {
 template <class TI, class T, class TO>
 static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, int64_t K, T alpha,
                       const TI *A, int64_t lda, const TI *B, int64_t ldb, T beta, TO *C, int64_t ldc,
-                      T *ws, int64_t ws_elems, int num_cus, const T *alpha_dev = nullptr) {
+                      T *ws, int64_t ws_elems, int num_cus, const T *alpha_dev = nullptr, bool sym = false) {
   if (M <= 0 || N <= 0) return;
   gemm_check_ld(lda, ldb);
   int S = splitk_count(M, N, K, num_cus);
   if (S > 1 && (ws == nullptr || (int64_t)S * M * N > ws_elems)) S = 1;
+  // symmetric products skip their lower blocks only when split-K's reduction
+  // (which mirrors them) runs
+  const int sy = (sym && S > 1 && M == N) ? 1 : 0;
   int kchunk = (int)K;
   if (S > 1) {
     kchunk = (int)(((K + S - 1) / S + GBK - 1) / GBK * GBK);
@@ -438,13 +449,13 @@ static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, in
   do {                                                                                                           \
     if (quarter)                                                                                                 \
       TTG_TIMED_LAUNCH(prof, (k_gemm<TI, T, TO, TA_, false, 32, 32>), grid, dim3(256), 0, st, (int)M, (int)N,    \
-                       (int)K, kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);    \
+                       (int)K, kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev, sy);    \
     else if (half)                                                                                               \
       TTG_TIMED_LAUNCH(prof, (k_gemm<TI, T, TO, TA_, TB_, 32>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K,  \
-                       kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);            \
+                       kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev, sy);            \
     else                                                                                                         \
       TTG_TIMED_LAUNCH(prof, (k_gemm<TI, T, TO, TA_, TB_, GBM>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K, \
-                       kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev);            \
+                       kchunk, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, ws, alpha_dev, sy);            \
   } while (0)
   if (!ta && !tb) TTG_GEMM_LAUNCH(false, false);
   else if (ta && !tb) TTG_GEMM_LAUNCH(true, false);
@@ -456,7 +467,7 @@ static void gemm_impl(hipStream_t st, bool ta, bool tb, int64_t M, int64_t N, in
     const long total = (long)M * N;
     const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
     TTG_TIMED_LAUNCH(prof, (k_splitk_reduce<T, TO>), dim3(blocks), dim3(256), 0, st, (int)M, (int)N, S,
-                     (const T *)ws, (double)alpha, (double)beta, C, (long)ldc, alpha_dev);
+                     (const T *)ws, (double)alpha, (double)beta, C, (long)ldc, alpha_dev, sy);
     TTG_HIP(hipGetLastError());
   }
 }
@@ -472,6 +483,11 @@ void gemm<float>(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t 
                  const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
                  int64_t ldc, float *ws, int64_t wse, int cus) {
   gemm_impl<float, float, float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ws, wse, cus);
+}
+void gemm_sym(hipStream_t s, bool ta, bool tb, int64_t M, int64_t K, const double *A, int64_t lda, const double *B,
+              int64_t ldb, double *C, int64_t ldc, double *ws, int64_t wse, int cus) {
+  gemm_impl<double, double, double>(s, ta, tb, M, M, K, 1.0, A, lda, B, ldb, 0.0, C, ldc, ws, wse, cus, nullptr,
+                                    true);
 }
 void gemm_scaled(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, const double *alpha_dev,
                  const double *A, int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, double *ws,
@@ -550,13 +566,13 @@ static void gram_nearid_impl(hipStream_t st, int64_t m, int64_t k, const double 
       ProfScope prof(KC_GEMM, st, 2.0 * (double)k * k * m, 8.0 * (2.0 * m * k + k * k), true);
       if (quarter)
         TTG_TIMED_LAUNCH(prof, (k_gemm<double, T, double, true, false, 32, 32>), grid, dim3(256), 0, st, (int)k,
-                         (int)k, (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, (const T *)nullptr);
+                         (int)k, (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, (const T *)nullptr, 0);
       else if (half)
         TTG_TIMED_LAUNCH(prof, (k_gemm<double, T, double, true, false, 32>), grid, dim3(256), 0, st, (int)k, (int)k,
-                         (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, (const T *)nullptr);
+                         (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, (const T *)nullptr, 0);
       else
         TTG_TIMED_LAUNCH(prof, (k_gemm<double, T, double, true, false, GBM>), grid, dim3(256), 0, st, (int)k, (int)k,
-                         (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, (const T *)nullptr);
+                         (int)m, kchunk, T(1), A, (long)m, A, (long)m, T(0), G, (long)k, ws, (const T *)nullptr, 0);
       TTG_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL((k_gram_nearid<T>), dim3(blocks), dim3(256), 0, st, (int)k, S, (const T *)ws, X, R2, maxabs);
