@@ -931,9 +931,21 @@ void chol4_batched(hipStream_t s, const CholJob *jobs_dev, int njobs, int64_t km
 // TRSM: X (m x k) = A R^{-1}
 // ---------------------------------------------------------------------------
 template <class T, bool STAMP>
-__global__ __launch_bounds__(256) void k_trsm(int m, int k, const T *__restrict__ A, long lda,
-                                              const double *__restrict__ R, T *__restrict__ X,
-                                              long ldx, unsigned long long *dbg) {
+__global__ __launch_bounds__(256) void k_trsm(int m_, int k_, const T *__restrict__ A_, long lda_,
+                                              const double *__restrict__ R_, T *__restrict__ X_,
+                                              long ldx_, unsigned long long *dbg,
+                                              const TrsmJob *__restrict__ jobs = nullptr) {
+  // jobs != nullptr (T = double): blockIdx.y selects the job (batched); row
+  // blocks past a job's m leave at once (the whole workgroup)
+  const TrsmJob *jb = jobs ? jobs + blockIdx.y : nullptr;
+  const int m = jb ? jb->m : m_;
+  const int k = jb ? jb->k : k_;
+  const T *__restrict__ A = jb ? reinterpret_cast<const T *>(jb->A) : A_;
+  const long lda = jb ? (long)jb->lda : lda_;
+  const double *__restrict__ R = jb ? jb->R : R_;
+  T *__restrict__ X = jb ? reinterpret_cast<T *>(jb->X) : X_;
+  const long ldx = jb ? (long)jb->ldx : ldx_;
+  if (jb && (int)blockIdx.x * CB >= m) return;
   extern __shared__ __align__(16) char smem[];  // 16-byte aligned: FP64 LDS accesses must not straddle
   typedef typename MFf<T>::V V;
   const int kp = (k + CB - 1) / CB * CB;
@@ -1413,7 +1425,22 @@ void trsm_right_upper(hipStream_t s, int64_t m, int64_t k, const T *A, int64_t l
   auto fn = dbg ? k_trsm<T, true> : k_trsm<T, false>;
   TTG_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(fn, dim3((unsigned)((m + CB - 1) / CB)), dim3(256), lds, s, (int)m, (int)k,
-                     A, (long)lda, R, X, (long)ldx, dbg);
+                     A, (long)lda, R, X, (long)ldx, dbg, (const TrsmJob *)nullptr);
+  TTG_HIP(hipGetLastError());
+}
+// Batched generic k_trsm (FP64, every job's A given; any k whose staging fits
+// LDS): grid (row blocks of the tallest job) x njobs.
+void trsm_batched(hipStream_t s, const TrsmJob *jobs_dev, int njobs, int64_t mmax, int64_t kmax) {
+  if (njobs <= 0 || mmax <= 0) return;
+  const int64_t kp = (kmax + CB - 1) / CB * CB;
+  const size_t lds = ((size_t)CB * kp + (kp <= 256 ? kp : CB) * CB) * sizeof(double) + CB * CB * sizeof(double);
+  TTG_CHECK(lds <= 160 * 1024, TTG_BAD_ARG, "rank too large for the triangular solve");
+  ProfScope prof(KC_TRTRI, s, (double)njobs * mmax * kmax * kmax, 8.0 * njobs * (2.0 * mmax * kmax + kmax * kmax));
+  TTG_HIP(hipFuncSetAttribute((const void *)k_trsm<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  hipLaunchKernelGGL((k_trsm<double, false>), dim3((unsigned)((mmax + CB - 1) / CB), (unsigned)njobs), dim3(256), lds,
+                     s, 0, 0, (const double *)nullptr, 0L, (const double *)nullptr, (double *)nullptr, 0L,
+                     (unsigned long long *)nullptr, jobs_dev);
   TTG_HIP(hipGetLastError());
 }
 // Batched k_trsm3 (k <= 256 for every job, Winv from k_chol6): grid

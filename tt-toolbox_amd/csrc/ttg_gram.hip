@@ -406,12 +406,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   // launch with generic triangular solves for their inverses
   std::vector<CholJob> cj, cjl;
   std::vector<TrsmJob> tj;
-  struct BigInv {
-    const double *R;
-    double *Rinv;
-    int k;
-  };
-  std::vector<BigInv> big;
+  std::vector<TrsmJob> big;  // R^{-1} = I R^{-1} of bonds above 256 (A = the identity)
   int64_t kbig = 1;
   std::vector<GramBond> gb(nb);
   std::vector<BGemmJob> ja(nb), jz(nb), jb(nb);
@@ -427,8 +422,8 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     } else {
       cjl.push_back(CholJob{H + o, RR + o, cfl + b, cst + 4 * b, nullptr, nullptr, k, k});
       cjl.push_back(CholJob{Gm + o, RL + o, cfl + nb + b, cst + 4 * (nb + b), nullptr, nullptr, k, k});
-      big.push_back(BigInv{RR + o, RiR + o, k});
-      big.push_back(BigInv{RL + o, RiL + o, k});
+      big.push_back(TrsmJob{nullptr, RR + o, nullptr, RiR + o, k, k, 0, k});
+      big.push_back(TrsmJob{nullptr, RL + o, nullptr, RiL + o, k, k, 0, k});
       kbig = std::max<int64_t>(kbig, k);
     }
     gb[b] = GramBond{Gm + o, H + o, RiR + o, Zm + o, k, 0};
@@ -452,10 +447,14 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     sizes[i] = rr[i] * n[i] * rr[i + 1];
     cptr[i] = cp[i];
   }
-  const int njc = (int)cj.size(), njl = (int)cjl.size();
+  const int njc = (int)cj.size(), njl = (int)cjl.size(), njb = (int)big.size();
+  for (TrsmJob &t : big) {
+    t.A = Ieye;
+    t.lda = (int)kbig;
+  }
   int64_t ksmall = 1;
   for (const CholJob &c : cj) ksmall = std::max<int64_t>(ksmall, c.k);
-  // pack: [cj | tj | gb | ja | jz | jb | cptr | sizes | one | cjl]
+  // pack: [cj | tj | gb | ja | jz | jb | cptr | sizes | one | cjl | big]
   std::vector<size_t> off;
   size_t tb = 0;
   auto add = [&](size_t bytes) {
@@ -472,12 +471,14 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   add(d * sizeof(int64_t));
   add(sizeof(double));
   add(njl * sizeof(CholJob));
+  add(njb * sizeof(TrsmJob));
   char *tab = A.take<char>(tb);
   const size_t readback = 256;
   char *hp = static_cast<char *>(pinned_acquire(ctx, tb + readback));
   std::memcpy(hp + off[0], cj.data(), njc * sizeof(CholJob));
   std::memcpy(hp + off[1], tj.data(), njc * sizeof(TrsmJob));
   std::memcpy(hp + off[9], cjl.data(), njl * sizeof(CholJob));
+  std::memcpy(hp + off[10], big.data(), njb * sizeof(TrsmJob));
   std::memcpy(hp + off[2], gb.data(), nb * sizeof(GramBond));
   std::memcpy(hp + off[3], ja.data(), nb * sizeof(BGemmJob));
   std::memcpy(hp + off[4], jz.data(), nb * sizeof(BGemmJob));
@@ -518,6 +519,7 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   const int64_t *dsz = reinterpret_cast<const int64_t *>(tab + off[7]);
   const double *d_one = reinterpret_cast<const double *>(tab + off[8]);
   const CholJob *dcjl = reinterpret_cast<const CholJob *>(tab + off[9]);
+  const TrsmJob *dbig = reinterpret_cast<const TrsmJob *>(tab + off[10]);
 
   // the launch sequence in four parts: set-up (st), right chain, left chain,
   // the rest (st); the chains run on st and sd at once
@@ -551,9 +553,8 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     chol_batched(st, dcj, njc, ksmall);
     chol4_batched(st, dcjl, njl, kbig);
     trsm3_batched(st, dtj, njc, ksmall, ksmall);
-    if (!big.empty()) eye_plus(st, kbig, Ieye, nullptr, 0.0);
-    for (const BigInv &bi : big)  // R^{-1} = I R^{-1}, generic blocked substitution
-      trsm_right_upper<double>(st, bi.k, bi.k, Ieye, kbig, bi.R, bi.Rinv, bi.k, nullptr, nullptr);
+    if (njb > 0) eye_plus(st, kbig, Ieye, nullptr, 0.0);
+    trsm_batched(st, dbig, njb, kbig, kbig);  // R^{-1} = I R^{-1}, generic blocked substitution
     if (!exact) bgemm_run(st, false, true, dja, nb, ta_, qa, fa);   // A_i = C_i R_R,i+1^T
     bgemm_run(st, true, false, djz, nb, tz_, qz, fz);   // Z_i = R_R,i^{-T} R_L,i^{-1}
     hipLaunchKernelGGL(k_gram_bond, dim3(kBondParts, (unsigned)nb), dim3(256), 0, st, dgb, bst);

@@ -342,6 +342,9 @@ struct TrsmJob {
   int m, k, lda, ldx;
 };
 void trsm3_batched(hipStream_t s, const TrsmJob *jobs_dev, int njobs, int64_t mmax, int64_t kmax);
+// Batched X = A R^{-1} through the generic blocked substitution (every job's A
+// given, Winv unused; for k above 256)
+void trsm_batched(hipStream_t s, const TrsmJob *jobs_dev, int njobs, int64_t mmax, int64_t kmax);
 // X (m x k) = A R^{-1}, R upper triangular (FP64), blocked substitution.
 // Gram of A (m x k, FP32 MFMA when g32) fused with the near-identity second
 // CholeskyQR pass: X = I - Phi(E), R2 = I + Phi(E), *maxabs = max|E| (G is

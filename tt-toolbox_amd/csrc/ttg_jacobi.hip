@@ -44,15 +44,18 @@ __device__ __forceinline__ void circle_pair(int players, int round, int idx, int
   }
 }
 
+// Every wave of the workgroup waits until its own stores have reached L2
+// (s_waitcnt vmcnt(0); a workgroup barrier alone does not wait for them), so
+// that one thread's agent-scope release after the barrier (the L2 write-back)
+// publishes the whole workgroup's stores device-wide.
+__device__ __forceinline__ void wave_stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Software grid barrier for a cooperative launch: bar[0] arrivals, bar[1]
 // generation.  Bounded spin: after ~4 s a timeout flag is raised and every
-// workgroup leaves the kernel at its next check.  Every wave first waits for
-// its own stores and writes them back device-wide (agent-scope release: a
-// workgroup barrier alone does not wait for other waves' stores), then one
-// thread arrives.
+// workgroup leaves the kernel at its next check.
 __device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen, int *err) {
   __shared__ int sh_ok;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  wave_stores_done();
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -518,7 +521,7 @@ __global__ __launch_bounds__(NT) void k_jacobi4(int k, int p, double *__restrict
         if (!sh_rot_ok) return;
         double *vdst = Vx + ((long)(g & 1) * p + w) * (LC * LC);
         for (int idx = tid; idx < LC * LC; idx += NT) vdst[idx] = Vl[idx];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // every wave's part of V
+        wave_stores_done();  // every wave's part of V, published by thread 0's release below
         __syncthreads();
         if (tid == 0)
           __hip_atomic_store(&vflag[(g & 1) * JW_MAXP + w], 2 * g + (rot > 0 ? 1u : 0u), __ATOMIC_RELEASE,
