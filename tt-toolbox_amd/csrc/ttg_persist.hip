@@ -359,15 +359,35 @@ __device__ __forceinline__ void p_jrot(double al, double be, double ga, double &
   sn = cs * t;
 }
 
+// Sum over a group of G consecutive lanes (G = 8: lanes 8j..8j+7, G = 16: a
+// DPP row, G = 64: the wave), every lane of the group getting the
+// bitwise-identical result; all 64 lanes must execute it.
+template <int G> __device__ __forceinline__ double group_sum(double v) {
+  if constexpr (G == 64) {
+    return wave_sum(v);
+  } else {
+    v += dpp_d<0xB1>(v);   // quad_perm xor 1
+    v += dpp_d<0x4E>(v);   // quad_perm xor 2
+    v += dpp_d<0x141>(v);  // row_half_mirror: the 8-lane half-row
+    if constexpr (G == 16) v += dpp_d<0x140>(v);  // row_mirror: the 16-lane row
+    return v;
+  }
+}
+
 // One-sided cyclic Jacobi on Y = [S; J] (2k x k, ld 2k; S k x k, J = I on
-// entry): round-robin parallel ordering, k/2 disjoint pairs per round (one
-// wave per pair), dots over S's k rows, rotation if |g| > tol sqrt(a b) and
-// neither column is numerically zero (||s||^2 <= k (2u)^2 ||S||_F^2); a sweep
-// none of whose rotations had cos^2 > 1e-16 ends the iteration.  Returns the
-// number of sweeps (max_sweeps + 1 when not converged).
-template <int NT>
-__device__ __noinline__ int p_jacobi(double *Y, int k, int max_sweeps, double *red, int *cnt) {
+// entry): round-robin parallel ordering, k/2 disjoint pairs per round, dots
+// over S's k rows, rotation if |g| > tol sqrt(a b) and neither column is
+// numerically zero (||s||^2 <= k (2u)^2 ||S||_F^2); a sweep none of whose
+// rotations had cos^2 > 1e-16 ends the iteration.  Returns the number of
+// sweeps (max_sweeps + 1 when not converged).  Each pair is handled by a
+// group of G lanes (64 / G pairs per wave, so that the k/2 pairs of a round
+// take as few passes as possible for small k: G = 8 for k <= 16, 16 for
+// k <= 64, the whole wave above).
+template <int NT, int G>
+__device__ __noinline__ int p_jacobi_g(double *Y, int k, int max_sweeps, double *red, int *cnt) {
+  constexpr int PW = 64 / G;  // pairs per wave
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gl = lane % G, ps = lane / G;
   const long ld = 2L * k;
   double fro = 0.0;
   for (long idx = threadIdx.x; idx < (long)k * k; idx += NT) {
@@ -384,48 +404,61 @@ __device__ __noinline__ int p_jacobi(double *Y, int k, int max_sweeps, double *r
     __syncthreads();
     int mybig = 0;
     for (int rnd = 0; rnd < P - 1; ++rnd) {
-      for (int q = wv; q < P / 2; q += NT / 64) {
-        int a, b;
-        if (q == 0) {
-          a = P - 1;
-          b = rnd;
-        } else {
-          a = (rnd + q) % (P - 1);
-          b = (rnd - q + P - 1) % (P - 1);
+      // the group loop is uniform over the wave (every lane runs the sums)
+      for (int q0 = wv * PW; q0 < P / 2; q0 += (NT / 64) * PW) {
+        const int q = q0 + ps;
+        int a = P, b = P;
+        if (q < P / 2) {
+          if (q == 0) {
+            a = P - 1;
+            b = rnd;
+          } else {
+            a = (rnd + q) % (P - 1);
+            b = (rnd - q + P - 1) % (P - 1);
+          }
         }
-        if (a >= k || b >= k) continue;
-        double *ya = Y + (long)a * ld, *yb = Y + (long)b * ld;
+        const bool act = a < k && b < k;
+        double *ya = Y + (long)(act ? a : 0) * ld, *yb = Y + (long)(act ? b : 0) * ld;
         double al = 0.0, be = 0.0, ga = 0.0;
-        for (int i = lane; i < k; i += 64) {
-          const double x = ya[i], y = yb[i];
-          al = __builtin_fma(x, x, al);
-          be = __builtin_fma(y, y, be);
-          ga = __builtin_fma(x, y, ga);
+        if (act) {
+          for (int i = gl; i < k; i += G) {
+            const double x = ya[i], y = yb[i];
+            al = __builtin_fma(x, x, al);
+            be = __builtin_fma(y, y, be);
+            ga = __builtin_fma(x, y, ga);
+          }
         }
-        al = wave_sum(al);
-        be = wave_sum(be);
-        ga = wave_sum(ga);
+        al = group_sum<G>(al);
+        be = group_sum<G>(be);
+        ga = group_sum<G>(ga);
         const double g2 = ga * ga, ab = al * be;
-        const bool rot = g2 > tol2 * ab && ab >= DBL_MIN && al > small2 && be > small2;
-        if (!rot) continue;
-        mybig += (g2 > 1e-16 * ab) ? 1 : 0;
-        double cs, sn;
-        p_jrot(al, be, ga, cs, sn);
-        for (long i = lane; i < ld; i += 64) {
-          const double x = ya[i], y = yb[i];
-          ya[i] = cs * x - sn * y;
-          yb[i] = sn * x + cs * y;
+        const bool rot = act && g2 > tol2 * ab && ab >= DBL_MIN && al > small2 && be > small2;
+        if (rot) {
+          if (gl == 0) mybig += (g2 > 1e-16 * ab) ? 1 : 0;
+          double cs, sn;
+          p_jrot(al, be, ga, cs, sn);
+          for (long i = gl; i < ld; i += G) {
+            const double x = ya[i], y = yb[i];
+            ya[i] = cs * x - sn * y;
+            yb[i] = sn * x + cs * y;
+          }
         }
       }
       __syncthreads();
     }
-    if (lane == 0 && mybig > 0) atomicAdd(cnt, mybig);
+    if (gl == 0 && mybig > 0) atomicAdd(cnt, mybig);
     __syncthreads();
     const int big = cnt[0];
     __syncthreads();
     if (big == 0) return sweep + 1;
   }
   return max_sweeps + 1;
+}
+template <int NT>
+__device__ int p_jacobi(double *Y, int k, int max_sweeps, double *red, int *cnt) {
+  if (k <= 16) return p_jacobi_g<NT, 8>(Y, k, max_sweeps, red, cnt);
+  if (k <= 64) return p_jacobi_g<NT, 16>(Y, k, max_sweeps, red, cnt);
+  return p_jacobi_g<NT, 64>(Y, k, max_sweeps, red, cnt);
 }
 
 // The same iteration for a Y = [S; J] (global, ld 2k) too large for LDS:
