@@ -272,23 +272,6 @@ __device__ __forceinline__ void gemm_block(int M, int N, int kbeg, int kend, int
 }
 
 // alpha_dev (optional device scalar) multiplies alpha.
-template <class TI, class T, class TO, bool TA, bool TB, int BM = GBM, int BN = GBN>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T alpha,
-                                              const TI *__restrict__ A, long lda,
-                                              const TI *__restrict__ B, long ldb, T beta,
-                                              TO *__restrict__ C, long ldc, T *__restrict__ P,
-                                              const T *__restrict__ alpha_dev, int sym) {
-  // sym: the product is symmetric (M == N) and only blocks that reach the
-  // diagonal or above are formed; k_splitk_reduce mirrors them
-  if (sym && (int)(blockIdx.x * BM) >= (int)(blockIdx.y * BN + BN)) return;
-  const int kbeg = blockIdx.z * kchunk;
-  const int kend = min(K, kbeg + kchunk);
-  const T a = alpha_dev ? alpha * alpha_dev[0] : alpha;
-  T *part = gridDim.z > 1 ? P + (long)blockIdx.z * M * N : nullptr;
-  gemm_block<TI, T, TO, TA, TB, BM, BN>(M, N, kbeg, kend, blockIdx.x * BM, blockIdx.y * BN, a, A, lda, B, ldb,
-                                        beta, C, ldc, part);
-}
-
 // Workgroups are dispatched to the 8 XCDs round-robin (block w on XCD w % 8),
 // and each XCD has its own L2.  This bijection gives XCD x the contiguous
 // tile range [x q + min(x, rem), ...) (q = total / 8, rem = total % 8), so the
@@ -299,6 +282,33 @@ __device__ __forceinline__ int xcd_tile(int w, int total) {
   const int q = total / X, rem = total % X;
   const int x = w % X, local = w / X;
   return x * q + min(x, rem) + local;
+}
+
+template <class TI, class T, class TO, bool TA, bool TB, int BM = GBM, int BN = GBN>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, T alpha,
+                                              const TI *__restrict__ A, long lda,
+                                              const TI *__restrict__ B, long ldb, T beta,
+                                              TO *__restrict__ C, long ldc, T *__restrict__ P,
+                                              const T *__restrict__ alpha_dev, int sym) {
+  // XCD-aware placement: the grid (M tiles, N tiles, K chunks) is walked as
+  // one tile index, K chunk slowest, and within a chunk M fastest when op(B)
+  // is the larger operand (N >= M), N fastest otherwise, so each XCD's
+  // contiguous run of tiles shares the larger operand's panels (and a split-K
+  // product gives each XCD its own K range)
+  const int gx = (int)gridDim.x, gy = (int)gridDim.y, gxy = gx * gy;
+  const int t = xcd_tile((int)(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z)), gxy * (int)gridDim.z);
+  const int tz = t / gxy, tr = t - tz * gxy;
+  const bool mfast = N >= M;
+  const int im = mfast ? tr % gx : tr / gy, in = mfast ? tr / gx : tr % gy;
+  // sym: the product is symmetric (M == N) and only blocks that reach the
+  // diagonal or above are formed; k_splitk_reduce mirrors them
+  if (sym && im * BM >= in * BN + BN) return;
+  const int kbeg = tz * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  const T a = alpha_dev ? alpha * alpha_dev[0] : alpha;
+  T *part = gridDim.z > 1 ? P + (long)tz * M * N : nullptr;
+  gemm_block<TI, T, TO, TA, TB, BM, BN>(M, N, kbeg, kend, im * BM, in * BN, a, A, lda, B, ldb,
+                                        beta, C, ldc, part);
 }
 
 // Batched GEMM: job j owns tiles [tile0_j, tile0_{j+1}); every job has its
