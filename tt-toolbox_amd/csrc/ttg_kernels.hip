@@ -104,10 +104,12 @@ template <> __device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
 }
 
-// K tile of a BM x BN block: 64, or 32 for FP64 tiles above 32 x 32 (two
-// LDS buffers of an FP64 64 x 64 tile at K tile 64 would take 133 KB)
+// K tile of a BM x BN block: 32 for FP64 (two LDS buffers of a 32 x 32 tile
+// take 33 KB, so four workgroups fit a CU's 160 KB and the two Gram chains'
+// products, launched on two streams, run side by side; at K tile 64 a 32 x 32
+// tile takes 68 KB and one chain's product fills every CU), 64 for FP32
 template <class T, int BM, int BN> struct GemmKT {
-  static constexpr int value = (sizeof(T) == 8 && BM * BN > 32 * 32) ? 32 : 64;
+  static constexpr int value = sizeof(T) == 8 ? 32 : 64;
 };
 
 // One BM x BN block of op(A) op(B) over K range [kbeg, kend) with its
