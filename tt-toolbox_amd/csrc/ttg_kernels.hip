@@ -115,9 +115,10 @@ template <class T, int BM, int BN> struct GemmKT {
 // One BM x BN block of op(A) op(B) over K range [kbeg, kend) with its
 // top-left corner at (m0, n0), accumulated in acc (layout below), staged
 // through the caller's two LDS buffers As[2], Bs[2] (K tile KT).  Tile t is
-// multiplied from buffer t % 2 while tile t+1 (loaded into registers during
-// tile t-1) is written to the other buffer and tile t+2 is loaded: one
-// workgroup barrier per K tile.  Rows of the LDS tiles are padded by one
+// multiplied from buffer t % 2 while tile t+1 (in registers since tile t-2)
+// is written to the other buffer and tile t+3 is loaded into the register
+// stage it frees (two register stages: a load has two tiles' time to
+// arrive): one workgroup barrier per K tile.  Rows of the LDS tiles are padded by one
 // element (odd stride), so the transposed writes of a K-contiguous operand
 // (64 lanes, 64 rows, one column) fall in distinct banks.
 template <class TI, class T, bool TA, bool TB, int BM, int BN, int KT>
@@ -149,7 +150,8 @@ __device__ __forceinline__ void gemm_acc(int M, int N, int kbeg, int kend, int m
   // past any range (for the fast index: that thread's elements, chosen once
   // for M and N and per tile for K).  No branch and no 64-bit address
   // arithmetic per element.
-  TI ra[LA], rb[LB];
+  constexpr int NS = 2;  // register stages: K tiles in flight while one is multiplied
+  TI ra[NS][LA], rb[NS][LB];
   constexpr int AQ = 256 / BM;  // A, M contiguous: BM consecutive threads read BM consecutive rows
   const int kc = tid % KT, kq = tid / KT;
   constexpr unsigned OOB = 0x80000000u;  // past every range (ranges are below 2^31 bytes)
@@ -173,7 +175,7 @@ __device__ __forceinline__ void gemm_acc(int M, int N, int kbeg, int kend, int m
     }
   }
   auto bytes_ok = [](long b) { return (int)(b <= 0 ? 0 : (b > 0x7fffffffL ? 0x7fffffffL : b)); };
-  auto fetch = [&](int k0) {
+  auto fetch = [&](int st, int k0) {
     // A: tile at (m0, k0) of op(A); the range ends at column kend (!TA: K is
     // the slow index) or at row M of op(A) (TA: M is the slow index)
     const TI *abase = TA ? A + k0 + (long)m0 * lda : A + m0 + (long)k0 * lda;
@@ -188,20 +190,20 @@ __device__ __forceinline__ void gemm_acc(int M, int N, int kbeg, int kend, int m
     // gets the range-breaking top bit (offsets are below 2^31)
     const unsigned kout = (k0 + kc < kend) ? 0u : OOB;
 #pragma unroll
-    for (int i = 0; i < LA; ++i) ra[i] = buffer_load<TI>(ar, TA ? (va[i] | kout) : va[i]);
+    for (int i = 0; i < LA; ++i) ra[st][i] = buffer_load<TI>(ar, TA ? (va[i] | kout) : va[i]);
 #pragma unroll
-    for (int i = 0; i < LB; ++i) rb[i] = buffer_load<TI>(br, TB ? vb[i] : (vb[i] | kout));
+    for (int i = 0; i < LB; ++i) rb[st][i] = buffer_load<TI>(br, TB ? vb[i] : (vb[i] | kout));
   };
-  auto stash = [&](int buf) {
+  auto stash = [&](int buf, int st) {
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
-      if (!TA) As[buf][tid / BM + AQ * i][tid % BM] = T(ra[i]);
-      else As[buf][kc][kq + KQ * i] = T(ra[i]);
+      if (!TA) As[buf][tid / BM + AQ * i][tid % BM] = T(ra[st][i]);
+      else As[buf][kc][kq + KQ * i] = T(ra[st][i]);
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
-      if (!TB) Bs[buf][kc][kq + KQ * i] = T(rb[i]);
-      else Bs[buf][wave + 4 * i][lane] = T(rb[i]);
+      if (!TB) Bs[buf][kc][kq + KQ * i] = T(rb[st][i]);
+      else Bs[buf][wave + 4 * i][lane] = T(rb[st][i]);
     }
   };
   auto multiply = [&](int buf) {
@@ -218,20 +220,30 @@ __device__ __forceinline__ void gemm_acc(int M, int N, int kbeg, int kend, int m
         for (int j = 0; j < NJ; ++j) acc[i][j] = MF<T>::op(b[j], a[i], acc[i][j]);
     }
   };
+  // tile t sits in register stage t % NS until it is written to LDS buffer
+  // t % 2; tiles t+1 .. t+NS are in flight while tile t is multiplied
   const int nt = kbeg < kend ? (kend - kbeg + KT - 1) / KT : 0;
   if (nt == 0) return;
-  fetch(kbeg);
+#pragma unroll
+  for (int u = 0; u < NS; ++u)
+    if (u < nt) fetch(u, kbeg + u * KT);
   __syncthreads();  // the caller's earlier use of the buffers is over
-  stash(0);
-  if (nt > 1) fetch(kbeg + KT);
+  stash(0, 0);
+  if (NS < nt) fetch(0, kbeg + NS * KT);
   __syncthreads();
-  for (int t = 0; t < nt; ++t) {
-    multiply(t & 1);
-    if (t + 1 < nt) {
-      stash((t + 1) & 1);  // buffer last read by tile t-1 (before the previous barrier)
-      if (t + 2 < nt) fetch(kbeg + (t + 2) * KT);
+  for (int t0 = 0; t0 < nt; t0 += NS) {
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int t = t0 + u;
+      if (t >= nt) break;
+      multiply(t & 1);
+      if (t + 1 < nt) {
+        const int st = (u + 1) % NS;
+        stash((t + 1) & 1, st);  // buffer last read by tile t-1 (before the previous barrier)
+        if (t + 1 + NS < nt) fetch(st, kbeg + (t + 1 + NS) * KT);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
