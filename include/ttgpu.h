@@ -106,9 +106,12 @@ ttg_status ttg_mtimes_dev(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64
                           const double *dxcore, int64_t *ry, double *dycore, int64_t cap,
                           int64_t *len_out);
 
-/* y = round(A*x, eps, rmax) fused: product cores are generated on the device
- * and consumed by the rounding sweep without being stored as a TT —
- * replaces the composition @tt_matrix/mtimes.m:38-65 -> @tt_tensor/round.m. */
+/* y = round(A*x, eps, rmax) as one call — replaces the composition
+ * @tt_matrix/mtimes.m:38-65 -> @tt_tensor/round.m.  When the product's ranks
+ * fit the Gram-chain forms (d >= 2, ranks <= 480 after the caps, no binding
+ * rmax) the product is formed once in the context's workspace and rounded by
+ * them; otherwise the product cores are generated on the device inside the
+ * rounding sweep's first products and never stored as a TT. */
 ttg_status ttg_mtimes_round(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *m,
                             const int64_t *ra, const double *acore, const int64_t *rx,
                             const double *xcore, double eps, const int64_t *rmax, int prec,
