@@ -57,11 +57,21 @@ __global__ __launch_bounds__(256) void k_core_sumsq(const double *const *__restr
   const long n = (long)sizes[i];
   const long per = (n + chunks - 1) / chunks;
   const long b = (long)c * per, e = min(n, b + per);
-  double acc = 0.0;
-  for (long t = b + threadIdx.x; t < e; t += 256) {
-    const double v = x[t];
-    acc += v * v;
+  // four independent loads in flight per thread (the chunk is read once)
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  long t = b + threadIdx.x;
+  for (; t + 768 < e; t += 1024) {
+    const double v0 = x[t], v1 = x[t + 256], v2 = x[t + 512], v3 = x[t + 768];
+    a0 = __builtin_fma(v0, v0, a0);
+    a1 = __builtin_fma(v1, v1, a1);
+    a2 = __builtin_fma(v2, v2, a2);
+    a3 = __builtin_fma(v3, v3, a3);
   }
+  for (; t < e; t += 256) {
+    const double v = x[t];
+    a0 = __builtin_fma(v, v, a0);
+  }
+  double acc = (a0 + a1) + (a2 + a3);
   acc = block_sum(acc, red);
   if (threadIdx.x == 0) part[(long)i * chunks + c] = acc;
 }

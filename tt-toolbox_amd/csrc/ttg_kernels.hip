@@ -224,6 +224,28 @@ __device__ __forceinline__ void gemm_acc(int M, int N, int kbeg, int kend, int m
   // t % 2; tiles t+1 .. t+NS are in flight while tile t is multiplied
   const int nt = kbeg < kend ? (kend - kbeg + KT - 1) / KT : 0;
   if (nt == 0) return;
+  if (nt == 8) {
+    // the chain products' K range (256 at K tile 32), fully unrolled: with no
+    // loop-carried or conditional loads the compiler's waits before each
+    // stash are exact (the newer stage stays in flight) instead of draining
+    // every outstanding load
+#pragma unroll
+    for (int u = 0; u < NS; ++u) fetch(u, kbeg + u * KT);
+    __syncthreads();  // the caller's earlier use of the buffers is over
+    stash(0, 0);
+    fetch(0, kbeg + NS * KT);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      multiply(t & 1);
+      if (t + 1 < 8) {
+        stash((t + 1) & 1, (t + 1) % NS);
+        if (t + 1 + NS < 8) fetch((t + 1) % NS, kbeg + (t + 1 + NS) * KT);
+      }
+      __syncthreads();
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < NS; ++u)
     if (u < nt) fetch(u, kbeg + u * KT);
