@@ -206,18 +206,36 @@ __device__ __forceinline__ void gemm_acc(int M, int N, int kbeg, int kend, int m
       else Bs[buf][wave + 4 * i][lane] = T(rb[st][i]);
     }
   };
+  // software-pipelined over the tile's K steps: the operands of step s+1
+  // are read from LDS before step s's MFMAs issue (two register sets), and
+  // the scheduling groups keep that order (reads of s+1, then MFMAs of s), so
+  // an MFMA waits for reads issued one step earlier instead of just before it
   auto multiply = [&](int buf) {
+    constexpr int KS = KT / 4;
+    T a[2][MI], b[2][NJ];
 #pragma unroll
-    for (int kk = 0; kk < KT; kk += 4) {
-      T a[MI], b[NJ];
+    for (int i = 0; i < MI; ++i) a[0][i] = As[buf][(lane >> 4)][wm * (BM / 2) + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < MI; ++i) a[i] = As[buf][kk + (lane >> 4)][wm * (BM / 2) + i * 16 + (lane & 15)];
+    for (int j = 0; j < NJ; ++j) b[0][j] = Bs[buf][(lane >> 4)][wn * (BN / 2) + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) b[j] = Bs[buf][kk + (lane >> 4)][wn * (BN / 2) + j * 16 + (lane & 15)];
+    for (int s = 0; s < KS; ++s) {
+      const int c = s & 1, nx = c ^ 1;
+      if (s + 1 < KS) {
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+          a[nx][i] = As[buf][4 * (s + 1) + (lane >> 4)][wm * (BM / 2) + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+          b[nx][j] = Bs[buf][4 * (s + 1) + (lane >> 4)][wn * (BN / 2) + j * 16 + (lane & 15)];
+      }
 #pragma unroll
       for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) acc[i][j] = MF<T>::op(b[j], a[i], acc[i][j]);
+        for (int j = 0; j < NJ; ++j) acc[i][j] = MF<T>::op(b[c][j], a[c][i], acc[i][j]);
+      if (s + 1 < KS) {
+        __builtin_amdgcn_sched_group_barrier(0x100, MI + NJ, 0);  // LDS reads of step s+1
+        __builtin_amdgcn_sched_group_barrier(0x008, MI * NJ, 0);  // MFMAs of step s
+      }
     }
   };
   // tile t sits in register stage t % NS until it is written to LDS buffer
