@@ -558,16 +558,27 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
                wsl, gws, cus);
     }
   };
-  auto enq_rest = [&]() {
-    // ---- batched factorisations, inverses, products, verdict ----------------------
+  // the tail in three parts around a second fork: the batched factorisations
+  // (st); then the inverses, Z_i and the per-bond statistics (the side
+  // stream) beside the one-pass form's first products A_i or the exact form's
+  // sweep (st), which need only the factors; then the verdict and the output
+  // (st).  The two branches share no buffer.
+  auto enq_rest_a = [&]() {
+    // ---- batched factorisations ---------------------------------------------------
     chol_batched(st, dcj, njc, ksmall);
     chol4_batched(st, dcjl, njl, kbig);
-    trsm3_batched(st, dtj, njc, ksmall, ksmall);
-    if (njb > 0) eye_plus(st, kbig, Ieye, nullptr, 0.0);
-    trsm_batched(st, dbig, njb, kbig, kbig);  // R^{-1} = I R^{-1}, generic blocked substitution
-    if (!exact) bgemm_run(st, false, true, dja, nb, ta_, qa, fa);   // A_i = C_i R_R,i+1^T
-    bgemm_run(st, true, false, djz, nb, tz_, qz, fz);   // Z_i = R_R,i^{-T} R_L,i^{-1}
-    hipLaunchKernelGGL(k_gram_bond, dim3(kBondParts, (unsigned)nb), dim3(256), 0, st, dgb, bst);
+  };
+  auto enq_side = [&](hipStream_t s) {
+    // ---- inverses, Z_i = R_R,i^{-T} R_L,i^{-1}, per-bond statistics ------------------
+    trsm3_batched(s, dtj, njc, ksmall, ksmall);
+    if (njb > 0) eye_plus(s, kbig, Ieye, nullptr, 0.0);
+    trsm_batched(s, dbig, njb, kbig, kbig);  // R^{-1} = I R^{-1}, generic blocked substitution
+    bgemm_run(s, true, false, djz, nb, tz_, qz, fz);
+    hipLaunchKernelGGL(k_gram_bond, dim3(kBondParts, (unsigned)nb), dim3(256), 0, s, dgb, bst);
+    TTG_HIP(hipGetLastError());
+  };
+  auto enq_main = [&]() {
+    if (!exact) bgemm_run(st, false, true, dja, nb, ta_, qa, fa);  // A_i = C_i R_R,i+1^T
     if (exact) {
       // Right-to-left CholeskyQR2 sweep of V_i = sqrt(s_i) C_i R_{i+1}^T (tall
       // orientation V^T): the first pass is the triangular solve with the
@@ -596,6 +607,10 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
       gemm<double>(st, false, true, rr[0] * n[0], rr[1], rr[1], 1.0, cp[0], rr[0] * n[0], Rn, rr[1], 0.0, Ab,
                    rr[0] * n[0], wsr, gws, cus);
     }
+    TTG_HIP(hipGetLastError());
+  };
+  auto enq_rest_b = [&]() {
+    // ---- verdict and output --------------------------------------------------------
     const double ep = eps / std::sqrt((double)(d - 1));
     hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, st, (int)d, bst, cfl, Ab, (long)(rr[0] * n[0] * rr[1]), sv,
                        ep, orth_tol, exact ? nid : nullptr, alpha, okd, diag);
@@ -622,7 +637,14 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     enq_left(sd);
     TTG_HIP(hipEventRecord(ctx->ev_join, sd));
     TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
-    enq_rest();
+    enq_rest_a();
+    TTG_HIP(hipEventRecord(ctx->ev_fork, st));
+    TTG_HIP(hipStreamWaitEvent(sd, ctx->ev_fork, 0));
+    enq_side(sd);
+    enq_main();
+    TTG_HIP(hipEventRecord(ctx->ev_join, sd));
+    TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    enq_rest_b();
   };
   auto capture = [&](auto &&fn) -> hipGraph_t {
     hipGraph_t g = nullptr;
@@ -643,10 +665,12 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     TTG_HIP(hipMemcpyAsync(tab, hp, tb, hipMemcpyHostToDevice, st));
     gram_graph_release(ctx);
     if (use_graph) {
-      hipGraph_t parts[4] = {capture(enq_setup), capture([&] { enq_right(st); }), capture([&] { enq_left(st); }),
-                             capture(enq_rest)};
+      hipGraph_t parts[kGramParts] = {capture(enq_setup),         capture([&] { enq_right(st); }),
+                                      capture([&] { enq_left(st); }), capture(enq_rest_a),
+                                      capture(enq_main),           capture([&] { enq_side(st); }),
+                                      capture(enq_rest_b)};
       hipError_t ie = hipSuccess;
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < kGramParts; ++q) {
         if (ie == hipSuccess) ie = hipGraphInstantiate(&ctx->ggraph[q], parts[q], nullptr, nullptr, 0);
         (void)hipGraphDestroy(parts[q]);
       }
@@ -691,14 +715,21 @@ __global__ void k_copy_flag(const int *src, int *dst) { *dst = *src; }
 
 void gram_graph_launch(ttg_ctx *ctx) {
   hipStream_t st = ctx->stream, sd = ctx->side;
-  TTG_HIP(hipGraphLaunch(ctx->ggraph[0], st));
+  TTG_HIP(hipGraphLaunch(ctx->ggraph[0], st));  // set-up
   TTG_HIP(hipEventRecord(ctx->ev_fork, st));
   TTG_HIP(hipStreamWaitEvent(sd, ctx->ev_fork, 0));
   TTG_HIP(hipGraphLaunch(ctx->ggraph[2], sd));  // left chain
   TTG_HIP(hipGraphLaunch(ctx->ggraph[1], st));  // right chain
   TTG_HIP(hipEventRecord(ctx->ev_join, sd));
   TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
-  TTG_HIP(hipGraphLaunch(ctx->ggraph[3], st));
+  TTG_HIP(hipGraphLaunch(ctx->ggraph[3], st));  // batched factorisations
+  TTG_HIP(hipEventRecord(ctx->ev_fork, st));
+  TTG_HIP(hipStreamWaitEvent(sd, ctx->ev_fork, 0));
+  TTG_HIP(hipGraphLaunch(ctx->ggraph[5], sd));  // inverses, Z, statistics
+  TTG_HIP(hipGraphLaunch(ctx->ggraph[4], st));  // A_i products or the exact sweep
+  TTG_HIP(hipEventRecord(ctx->ev_join, sd));
+  TTG_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+  TTG_HIP(hipGraphLaunch(ctx->ggraph[6], st));  // verdict and output
   ++ctx->gruns;
 }
 

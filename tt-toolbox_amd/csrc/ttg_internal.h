@@ -140,6 +140,8 @@ struct ProfScope {
 
 }  // namespace ttg
 
+constexpr int kGramParts = 7;  // captured parts of the Gram-chain round
+
 struct ttg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -171,11 +173,12 @@ struct ttg_ctx {
   int batch_mode = 0;
   // Gram-chain round: its workspace (job tables included) and its captured
   // launch sequence with the shapes and pointers it was captured for, in
-  // four parts (set-up, right chain, left chain, factorisations and output)
-  // whose chains replay on two streams at once: a graph's own branches
-  // replay one after the other on one queue
+  // seven parts (set-up; right chain | left chain; factorisations; products
+  // or exact sweep | inverses and statistics; verdict and output) whose
+  // branches replay on two streams at once: a graph's own branches replay
+  // one after the other on one queue
   ttg::Arena garena;
-  hipGraphExec_t ggraph[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraphExec_t ggraph[kGramParts] = {};
   std::vector<uint64_t> gkey;
   hipGraph_t ggraph_src = nullptr;  // the whole sequence as one graph (child-node form)
   int *gok = nullptr;               // its proof flag (device)
