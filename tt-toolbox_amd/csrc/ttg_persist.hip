@@ -53,6 +53,21 @@ __device__ double p_block_sum(double v, double *red) {
   return s;
 }
 
+// Sum over a group of G consecutive lanes (G = 8: lanes 8j..8j+7, G = 16: a
+// DPP row, G = 64: the wave), every lane of the group getting the
+// bitwise-identical result; all 64 lanes must execute it.
+template <int G> __device__ __forceinline__ double group_sum(double v) {
+  if constexpr (G == 64) {
+    return wave_sum(v);
+  } else {
+    v += dpp_d<0xB1>(v);   // quad_perm xor 1
+    v += dpp_d<0x4E>(v);   // quad_perm xor 2
+    v += dpp_d<0x141>(v);  // row_half_mirror: the 8-lane half-row
+    if constexpr (G == 16) v += dpp_d<0x140>(v);  // row_mirror: the 16-lane row
+    return v;
+  }
+}
+
 template <int NT>
 __device__ void p_copy(double *__restrict__ dst, const double *__restrict__ src, long count) {
   for (long i = threadIdx.x; i < count; i += NT) dst[i] = src[i];
@@ -172,10 +187,35 @@ __device__ __noinline__ void p_gemm(bool ta, bool tb, long M, long N, long K, co
 // (dgeqr2).  Then Q (m x kk, ld m, kk = min(m, k)) explicitly (dorg2r:
 // Q = H_0 ... H_{kk-1} [I; 0], applied backwards).  One wave per column in
 // the updates.
+// H_j = I - t v v^T (v = [1; A(j+1:m, j)]) applied to the columns c0 .. k-1
+// of X (ld m), rows j..m-1: each column by a group of G lanes (64 / G
+// columns per wave; the group loop is uniform over the wave).
+template <int NT, int G>
+__device__ __forceinline__ void p_apply_reflector(const double *__restrict__ vj, double tj, long j, long m,
+                                                  double *__restrict__ X, long c0, long k) {
+  constexpr int PW = 64 / G;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gl = lane % G, ps = lane / G;
+  for (long cb = c0 + (long)wv * PW; cb < k; cb += (long)(NT / 64) * PW) {
+    const long c = cb + ps;
+    const bool act = c < k;
+    double *xc = X + (act ? c : c0) * m;
+    double w = (act && gl == 0) ? xc[j] : 0.0;
+    if (act)
+      for (long i = j + 1 + gl; i < m; i += G) w = __builtin_fma(vj[i], xc[i], w);
+    w = group_sum<G>(w) * tj;
+    if (act) {
+      if (gl == 0) xc[j] -= w;
+      for (long i = j + 1 + gl; i < m; i += G) xc[i] = __builtin_fma(-w, vj[i], xc[i]);
+    }
+  }
+}
+
 template <int NT>
 __device__ __noinline__ void p_householder(double *__restrict__ A, long m, long k, double *__restrict__ tau,
                               double *__restrict__ Q, double *red, double *bcast) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // columns of up to 256 rows take 16 lanes each (4 per wave), longer ones a wave
+  const bool grp = m <= 256;
   const long kk = m < k ? m : k;
   for (long j = 0; j < kk; ++j) {
     double *aj = A + j * m;
@@ -199,14 +239,8 @@ __device__ __noinline__ void p_householder(double *__restrict__ A, long m, long 
     }
     // H_j on the trailing columns j+1 .. k-1
     if (tj != 0.0) {
-      for (long c = j + 1 + wv; c < k; c += NT / 64) {
-        double *ac = A + c * m;
-        double w = (lane == 0) ? ac[j] : 0.0;
-        for (long i = j + 1 + lane; i < m; i += 64) w = __builtin_fma(aj[i], ac[i], w);
-        w = wave_sum(w) * tj;
-        if (lane == 0) ac[j] -= w;
-        for (long i = j + 1 + lane; i < m; i += 64) ac[i] = __builtin_fma(-w, aj[i], ac[i]);
-      }
+      if (grp) p_apply_reflector<NT, 16>(aj, tj, j, m, A, j + 1, k);
+      else p_apply_reflector<NT, 64>(aj, tj, j, m, A, j + 1, k);
     }
     __syncthreads();
   }
@@ -221,14 +255,8 @@ __device__ __noinline__ void p_householder(double *__restrict__ A, long m, long 
     const double tj = tau[j];
     const double *vj = A + j * m;
     if (tj != 0.0) {
-      for (long c = j + wv; c < kk; c += NT / 64) {
-        double *qc = Q + c * m;
-        double w = (lane == 0) ? qc[j] : 0.0;
-        for (long i = j + 1 + lane; i < m; i += 64) w = __builtin_fma(vj[i], qc[i], w);
-        w = wave_sum(w) * tj;
-        if (lane == 0) qc[j] -= w;
-        for (long i = j + 1 + lane; i < m; i += 64) qc[i] = __builtin_fma(-w, vj[i], qc[i]);
-      }
+      if (grp) p_apply_reflector<NT, 16>(vj, tj, j, m, Q, j, kk);
+      else p_apply_reflector<NT, 64>(vj, tj, j, m, Q, j, kk);
     }
     __syncthreads();
   }
@@ -357,21 +385,6 @@ __device__ __forceinline__ void p_jrot(double al, double be, double ga, double &
   const double t = (dd >= 0.0 ? g2 : -g2) / (fabs(dd) + sqrt(__builtin_fma(dd, dd, g2 * g2)));
   cs = 1.0 / sqrt(__builtin_fma(t, t, 1.0));
   sn = cs * t;
-}
-
-// Sum over a group of G consecutive lanes (G = 8: lanes 8j..8j+7, G = 16: a
-// DPP row, G = 64: the wave), every lane of the group getting the
-// bitwise-identical result; all 64 lanes must execute it.
-template <int G> __device__ __forceinline__ double group_sum(double v) {
-  if constexpr (G == 64) {
-    return wave_sum(v);
-  } else {
-    v += dpp_d<0xB1>(v);   // quad_perm xor 1
-    v += dpp_d<0x4E>(v);   // quad_perm xor 2
-    v += dpp_d<0x141>(v);  // row_half_mirror: the 8-lane half-row
-    if constexpr (G == 16) v += dpp_d<0x140>(v);  // row_mirror: the 16-lane row
-    return v;
-  }
 }
 
 // One-sided cyclic Jacobi on Y = [S; J] (2k x k, ld 2k; S k x k, J = I on
