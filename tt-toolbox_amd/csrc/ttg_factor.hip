@@ -1139,8 +1139,16 @@ __global__ __launch_bounds__(256) void k_trsm3(int m_, int k_, const double *A_,
       pr[e] = (idx < tot && n < jb) ? R[t + (long)(J0 + n) * k] : 0.0;
     }
   };
-  fetch(0);
-  for (int J0 = 0; J0 < k; J0 += CB) {
+  // A = I (an inverse): row block r0 of X = R^{-1} is zero left of column
+  // block r0 (X_J = (0 - X_<J R_<J,J) W_J = 0 for J < r0 by induction), so
+  // those column blocks are written as zeros and skipped
+  const int jstart = A ? 0 : min(r0, kp);
+  for (int idx = tid; idx < CB * jstart; idx += 256) {
+    const int row = idx & (CB - 1), col = idx >> 5;
+    if (r0 + row < m && col < k) X[(r0 + row) + (long)col * ldx] = 0.0;
+  }
+  fetch(jstart);
+  for (int J0 = jstart; J0 < k; J0 += CB) {
     const int jb = min(CB, k - J0);
     __syncthreads();  // the previous block is done with Ds, Wb, Rs
 #pragma unroll
@@ -1168,8 +1176,8 @@ __global__ __launch_bounds__(256) void k_trsm3(int m_, int k_, const double *A_,
     if (J0 + CB < k) fetch(J0 + CB);
     // acc[v] = C(tr*16 + lane%16, tc*16 + lane/16 + 4v); Bv = this lane's entries of B
     V acc = {0, 0, 0, 0};
-    if (J0 > 0) {
-      for (int kc = 0; kc < J0; kc += CB) {
+    if (J0 > jstart) {
+      for (int kc = jstart; kc < J0; kc += CB) {  // X's columns left of jstart are zero
 #pragma unroll
         for (int kk = 0; kk < CB; kk += 4) {
           const int t = kc + kk + (lane >> 4);
