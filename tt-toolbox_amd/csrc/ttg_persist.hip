@@ -262,6 +262,69 @@ __device__ __noinline__ void p_householder(double *__restrict__ A, long m, long 
   }
 }
 
+// The same factorisation with one workgroup barrier per column (m <= 256):
+// the reflector of column j+1 is formed right after H_j reaches that column,
+// by the 16-lane group that applied H_j to it (the first group of wave 0),
+// while the other groups still apply H_j to the columns after it; the next
+// step starts after the barrier.  (p_householder forms each reflector with
+// a block-wide reduction between barriers: three or four per column.)  Same
+// output as p_householder.
+__device__ __forceinline__ void p_wave_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+template <int NT>
+__device__ __noinline__ void p_householder_la(double *__restrict__ A, long m, long k, double *__restrict__ tau,
+                                              double *__restrict__ Q) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gl = lane & 15, ps = lane >> 4;
+  const long kk = m < k ? m : k;
+  // reflector j from column j (rows j..m-1) by lanes 0..15 of wave 0 (the
+  // whole wave runs the group sum)
+  auto prep = [&](long j) {
+    double *aj = A + j * m;
+    double s = 0.0;
+    if (ps == 0)
+      for (long i = j + 1 + gl; i < m; i += 16) s = __builtin_fma(aj[i], aj[i], s);
+    s = group_sum<16>(s);
+    if (ps == 0) {
+      const double al = aj[j];
+      double tj = 0.0, scale = 0.0, beta = al;
+      if (s > 0.0) {
+        const double nrm = sqrt(__builtin_fma(al, al, s));
+        beta = al >= 0.0 ? -nrm : nrm;
+        tj = (beta - al) / beta;
+        scale = 1.0 / (al - beta);
+        for (long i = j + 1 + gl; i < m; i += 16) aj[i] *= scale;
+      }
+      p_wave_sync();  // every lane of the group has read aj[j]
+      if (gl == 0) {
+        aj[j] = beta;
+        tau[j] = tj;
+      }
+    }
+  };
+  if (wv == 0) prep(0);
+  __syncthreads();
+  for (long j = 0; j < kk; ++j) {
+    const double tj = tau[j];
+    if (tj != 0.0) p_apply_reflector<NT, 16>(A + j * m, tj, j, m, A, j + 1, k);
+    if (j + 1 < kk && wv == 0) {
+      p_wave_sync();  // column j+1 updated (its rows by several lanes of the group)
+      prep(j + 1);
+    }
+    __syncthreads();
+  }
+  // Q = [I; 0] then H_j, j = kk-1 .. 0, on rows j.., columns j..kk-1
+  for (long idx = threadIdx.x; idx < m * kk; idx += NT) {
+    const long c = idx / m, i = idx - c * m;
+    Q[idx] = (i == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (long j = kk - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj != 0.0) p_apply_reflector<NT, 16>(A + j * m, tj, j, m, Q, j, kk);
+    __syncthreads();
+  }
+}
+
 // Blocked Householder QR (dgeqrf/dorgqr structure, compact WY), same output
 // as p_householder: 32-column panels factorised column by column (updates
 // confined to the panel), then with the panel's explicit reflector block V
@@ -742,7 +805,8 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
         const double *Af = Wa;
         if (k <= 64 && m * (k + kk) <= ycap) {
           p_copy<NT>(ylds, Wa, m * k);
-          p_householder<NT>(ylds, m, k, tau, ylds + m * k, red, nullptr);
+          if (m <= 256) p_householder_la<NT>(ylds, m, k, tau, ylds + m * k);
+          else p_householder<NT>(ylds, m, k, tau, ylds + m * k, red, nullptr);
           p_copy<NT>(xout + wofs[i], ylds + m * k, m * kk);
           Af = ylds;
         } else if (k <= 64) {
@@ -794,7 +858,8 @@ __global__ __launch_bounds__(NT) void k_round_persist(int count, const PTensorDe
         // R (kj x kj) -> Ra, in LDS for small bonds as in sweep 1
         if (kj <= 64 && mt * 2 * kj <= ycap) {
           p_copy<NT>(ylds, Wa, mt * kj);
-          p_householder<NT>(ylds, mt, kj, tau, ylds + mt * kj, red, nullptr);
+          if (mt <= 256) p_householder_la<NT>(ylds, mt, kj, tau, ylds + mt * kj);
+          else p_householder<NT>(ylds, mt, kj, tau, ylds + mt * kj, red, nullptr);
           p_copy<NT>(Wq, ylds + mt * kj, mt * kj);
           for (long idx = threadIdx.x; idx < kj * kj; idx += NT) {
             const long c = idx / kj, r = idx - c * kj;
