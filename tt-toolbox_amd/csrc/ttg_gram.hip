@@ -563,16 +563,23 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
   // stream) beside the one-pass form's first products A_i or the exact form's
   // sweep (st), which need only the factors; then the verdict and the output
   // (st).  The two branches share no buffer.
-  auto enq_rest_a = [&]() {
-    // ---- batched factorisations ---------------------------------------------------
-    chol_batched(st, dcj, njc, ksmall);
-    chol4_batched(st, dcjl, njl, kbig);
-  };
-  auto enq_side = [&](hipStream_t s) {
-    // ---- inverses, Z_i = R_R,i^{-T} R_L,i^{-1}, per-bond statistics ------------------
+  auto enq_inverses = [&](hipStream_t s) {
     trsm3_batched(s, dtj, njc, ksmall, ksmall);
     if (njb > 0) eye_plus(s, kbig, Ieye, nullptr, 0.0);
     trsm_batched(s, dbig, njb, kbig, kbig);  // R^{-1} = I R^{-1}, generic blocked substitution
+  };
+  auto enq_rest_a = [&]() {
+    // ---- batched factorisations (and, for the one-pass form, the inverses: run
+    // beside the first output products they would take the CUs from them and
+    // take about four times as long) -------------------------------------------------
+    chol_batched(st, dcj, njc, ksmall);
+    chol4_batched(st, dcjl, njl, kbig);
+    if (!exact) enq_inverses(st);
+  };
+  auto enq_side = [&](hipStream_t s) {
+    // ---- inverses (exact form: beside its sweep), Z_i = R_R,i^{-T} R_L,i^{-1},
+    // per-bond statistics -------------------------------------------------------------
+    if (exact) enq_inverses(s);
     bgemm_run(s, true, false, djz, nb, tz_, qz, fz);
     hipLaunchKernelGGL(k_gram_bond, dim3(kBondParts, (unsigned)nb), dim3(256), 0, s, dgb, bst);
     TTG_HIP(hipGetLastError());
