@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_gram_final(int d, const double *__restr
                                                     long a0size, const double *__restrict__ s, double ep,
                                                     double orth_tol, const double *__restrict__ nearid,
                                                     double *__restrict__ alpha, int *__restrict__ ok,
-                                                    double *__restrict__ diag) {
+                                                    double *__restrict__ diag, double *__restrict__ out0) {
   __shared__ double red[4];
   __shared__ double sh_nrm0;
   double a = 0.0;
@@ -189,6 +189,9 @@ __global__ __launch_bounds__(256) void k_gram_final(int d, const double *__restr
   }
   __syncthreads();
   for (int i = 1 + threadIdx.x; i < d; i += blockDim.x) alpha[i] = sqrt(s[i]) * sh_nrm0;
+  // the output's first core, alpha_0 A_0 (A_0 = C_0 R_1^T: r_0 n_0 r_1 entries)
+  const double a0 = na > 0.0 ? nrm0 / na : 0.0;
+  for (long t = threadIdx.x; t < a0size; t += blockDim.x) out0[t] = a0 * A0[t];
 }
 
 // x[t] *= (*alpha)
@@ -620,11 +623,10 @@ bool round_gram(ttg_ctx *ctx, int64_t d, const int64_t *n, const int64_t *r, con
     // ---- verdict and output --------------------------------------------------------
     const double ep = eps / std::sqrt((double)(d - 1));
     hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, st, (int)d, bst, cfl, Ab, (long)(rr[0] * n[0] * rr[1]), sv,
-                       ep, orth_tol, exact ? nid : nullptr, alpha, okd, diag);
+                       ep, orth_tol, exact ? nid : nullptr, alpha, okd, diag, dcore_out);
     TTG_HIP(hipGetLastError());
-    // the output: core_0 = alpha_0 A_0; one-pass form: core_i = alpha_i R_R,i^{-T} A_i,
-    // exact form: cores 1..d-1 (already Q_i^T) times nrm0
-    hipLaunchKernelGGL(k_scale_copy, dim3(64), dim3(256), 0, st, dcore_out, Ab, (long)(rr[0] * n[0] * rr[1]), alpha);
+    // the output: core_0 = alpha_0 A_0 (written by k_gram_final); one-pass form:
+    // core_i = alpha_i R_R,i^{-T} A_i, exact form: cores 1..d-1 (already Q_i^T) times nrm0
     if (!exact) {
       bgemm_run(st, true, false, djb, nb, tb_, qb, fb);
     } else {
