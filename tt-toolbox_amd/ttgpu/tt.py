@@ -210,13 +210,17 @@ def round_dev(ctx, n, r, dcore, eps, rmax=None, precision="fp64", out=None, info
     """Round a TT whose flat core is a CUDA float64 tensor; returns (r_out, core_out view)."""
     import torch
     d = len(n)
-    cap = capacity(n, r)
-    if out is None:
-        out = torch.empty(max(cap, 1), dtype=torch.float64, device=dcore.device)
+    n_arr, r_arr = i64(n), i64(r)  # marshalled once per call
+    if out is None:  # the capacity is needed only to allocate the output
+        cap = C.c_int64()
+        sc = ctx.lib.ttg_round_capacity(d, n_arr, r_arr, C.byref(cap))
+        if sc != TTG_OK:
+            raise _lib.TTGError(sc, "ttg_round_capacity")
+        out = torch.empty(max(int(cap.value), 1), dtype=torch.float64, device=dcore.device)
     r_out = (C.c_int64 * (d + 1))()
     used = C.c_int64()
     inf = TTGInfo()
-    s = ctx.lib.ttg_round_dev(ctx.h, d, i64(n), i64(r), _ptr(dcore), float(eps), _rmax_arg(rmax, d),
+    s = ctx.lib.ttg_round_dev(ctx.h, d, n_arr, r_arr, _ptr(dcore), float(eps), _rmax_arg(rmax, d),
                               _prec(precision), r_out, _ptr(out), int(out.numel()), C.byref(used),
                               C.byref(inf))
     ctx.check(s, allow=(TTG_OK, TTG_NO_CONVERGENCE))
